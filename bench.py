@@ -1,0 +1,207 @@
+"""Benchmark: advection apply GDoF/s, 3D+3D, k=3, FP64 and FP32-mixed.
+
+Workload: periodic [0,1]^6, 8^3 x 8^3 = 262,144 cells, k=3 (4096 DoF per
+cell, 1,073,741,824 DoF, 8.6 GB per FP64 vector), velocity a_i ~ U(-1,1)
+(counter-based SplitMix64, seed 20261202), f0 = prod sin(2 pi x_i) +
+1e-3 U(-1,1) noise (seed 20261203).  Inputs are generated on the GPU and are
+resident in HBM before timing.
+
+A step is one operator apply (src -> dst) over the whole vector, the
+reference's operator benchmark unit (performance/operators_advection_01.
+likwid.cc:341-348: GDoF/s = n_dofs * n_applies / time).  `value` is the FP64
+apply rate; the FP32-mixed apply and the fused LSRK stage rates are reported
+next to it.  Kernel times come from HIP events recorded on the stream the
+kernels are launched on.
+
+cpu_baseline: the CPU oracle (a restatement of hyper.deal's ECL quadrature
+algorithm, OpenMP over macro cells) timed on this host on a bounded sample of
+the same workload: 3+3, k=3, 4^3 x 8^3 cells.
+"""
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+SEED_VELOCITY = 20261202
+SEED_NOISE = 20261203
+
+
+def splitmix_uniform(seed, i):
+    m = (1 << 64) - 1
+    z = (seed + (i + 1) * 0x9E3779B97F4A7C15) & m
+    z = ((z ^ (z >> 30)) * 0xBF58476D1CE4E5B9) & m
+    z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & m
+    z = z ^ (z >> 31)
+    return (z >> 11) * (2.0 / 2 ** 53) - 1.0
+
+
+def timed(fn, steps, warmup, stream):
+    import torch
+    for _ in range(warmup):
+        fn()
+    torch.cuda.synchronize()
+    start = torch.cuda.Event(enable_timing=True)
+    stop = torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for _ in range(steps):
+        fn()
+    stop.record(stream)
+    torch.cuda.synchronize()
+    return start.elapsed_time(stop) / 1000.0 / steps  # seconds per call
+
+
+def cpu_baseline(velocity):
+    """Time the oracle's ECL apply on 3+3, k=3, 4^3 x 8^3 cells."""
+    import numpy as np
+
+    import oracle as O
+    lib_path = os.path.join("/tmp", "hd_oracle_native_%d" % os.getpid(),
+                            "libhd_oracle.so")
+    try:
+        O.build(native_out=lib_path)
+        lib = O.load(lib_path)
+        arch = "native"
+    except Exception:  # fall back to the portable build in the tree
+        lib = O.load()
+        arch = "x86-64-v3"
+    n_cells = [4, 4, 4, 8, 8, 8]
+    P = O.Problem(3, 3, 3, n_cells, [0.0] * 6, [1.0] * 6, velocity, lib=lib,
+                  func=O.FUNC_PRODUCT_SINES, func_params=[0.0] * 6)
+    u = P.interpolate()
+    P.add_noise(u, SEED_NOISE, 1e-3)
+    dst = P.vector()
+    P.apply(u, dst=dst)  # warm-up
+    n_apply = 0
+    t0 = time.perf_counter()
+    while True:
+        P.apply(u, dst=dst)
+        n_apply += 1
+        el = time.perf_counter() - t0
+        if el > 10.0 or n_apply >= 20:
+            break
+    gdofs = P.n_dofs * n_apply / el / 1e9
+    cores = lib.hdo_num_threads()
+    return {"value": round(gdofs, 4), "unit": "GDoF/s", "cores": cores,
+            "kind": "port",
+            "sample": "oracle ECL apply (restated hyper.deal quadrature "
+                      "algorithm), 3+3 k=3, 4^3 x 8^3 cells = %d DoF, %d "
+                      "applies after 1 warm-up, %s build, %d OpenMP threads"
+                      % (P.n_dofs, n_apply, arch, cores)}
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=50)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--only", choices=["all", "fp64", "fp32", "stage"],
+                    default="all", help="restrict the timed kernels "
+                    "(profiling runs)")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print("bench.py measures one GPU: this build has no multi-GPU path "
+              "(the operator would shard by x/v phase-space blocks with face "
+              "halos; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        sys.exit(2)
+
+    import torch
+
+    import hyperdeal_amd as H
+
+    torch.cuda.set_device(0)
+    stream = torch.cuda.current_stream()
+    n_cells = [8] * 6
+    velocity = [splitmix_uniform(SEED_VELOCITY, i) for i in range(6)]
+    mf = H.MatrixFree(3, 3, 3, n_cells, [0.0] * 6, [1.0] * 6)
+    op = H.AdvectionOperation(mf, velocity)
+    n = mf.n_dofs()
+    results = {}
+
+    u = torch.empty(n, dtype=torch.float64, device="cuda")
+    H.VectorTools.interpolate(mf, u, H.FUNC_PRODUCT_SINES, [0.0] * 6)
+    H.VectorTools.add_noise(mf, u, SEED_NOISE, 1e-3)
+
+    if args.only in ("all", "fp64"):
+        dst = torch.empty_like(u)
+        t = timed(lambda: op.apply(dst, u, 0.0, H.FP64, stream), args.steps,
+                  args.warmup, stream)
+        results["fp64"] = t
+        del dst
+    if args.only in ("all", "fp32"):
+        u32 = u.float()
+        d32 = torch.empty_like(u32)
+        t = timed(lambda: op.apply(d32, u32, 0.0, H.FP32, stream), args.steps,
+                  args.warmup, stream)
+        results["fp32"] = t
+        del u32, d32
+    if args.only in ("all", "stage"):
+        # fused LSRK stage, FP64 (Ti, sol in FP64) and FP32-mixed (Ti FP32)
+        for prec, dtype in ((H.FP64, torch.float64), (H.FP32, torch.float32)):
+            sol = u.clone()
+            ta = u.to(dtype)
+            tb = torch.empty_like(ta)
+            t = timed(lambda: op.stage(ta, tb, sol, 0.0, 1e-4, 1e-4, prec,
+                                       stream), args.steps, args.warmup,
+                      stream)
+            results["stage_fp64" if prec == H.FP64 else "stage_fp32"] = t
+            del sol, ta, tb
+    torch.cuda.synchronize()
+
+    def gdofs(t):
+        return n / t / 1e9
+
+    line = {
+        "metric": "advection apply GDoF/s 3D+3D k=3, FP64 vs FP32-mixed; "
+                  "% of HBM roofline",
+        "value": None, "unit": "GDoF/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": None, "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f64",
+        "data": "synthetic (seeded product of sines + 1e-3 SplitMix64 noise, "
+                "generated on the GPU)",
+        "config": {"workload": "3D+3D linear advection apply, periodic "
+                               "[0,1]^6, 8^3 x 8^3 cells, k=3 (4096 DoF/cell,"
+                               " 1073741824 DoF), Gauss k+1, upwind, skew 0",
+                   "algorithm": ["auto", "quadrature", "collapsed"][
+                       op.algorithm(H.FP64)],
+                   "global_batch": n, "parallelism": "single GPU"},
+    }
+    if "fp64" in results:
+        t = results["fp64"]
+        line["value"] = round(gdofs(t), 3)
+        line["ms_per_step"] = round(t * 1e3, 4)
+        achieved = 16.0 * n / t / 1e9
+        line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1),
+                            "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                            "frac": round(achieved / HBM_PEAK_GBS, 4),
+                            "traffic": None}
+    if "fp32" in results:
+        t = results["fp32"]
+        line["fp32_mixed"] = {
+            "value": round(gdofs(t), 3), "unit": "GDoF/s",
+            "ms_per_apply": round(t * 1e3, 4),
+            "achieved_GBs": round(8.0 * n / t / 1e9, 1),
+            "frac_hbm": round(8.0 * n / t / 1e9 / HBM_PEAK_GBS, 4)}
+        if "fp64" in results:
+            line["fp32_mixed"]["ratio_vs_fp64"] = round(
+                results["fp64"] / t, 3)
+    for key, bpd in (("stage_fp64", 32.0), ("stage_fp32", 24.0)):
+        if key in results:
+            t = results[key]
+            line[key] = {"value": round(gdofs(t), 3), "unit": "GDoF/s",
+                         "ms_per_stage": round(t * 1e3, 4),
+                         "bytes_per_dof": bpd,
+                         "achieved_GBs": round(bpd * n / t / 1e9, 1)}
+    if not args.no_cpu_baseline:
+        line["cpu_baseline"] = cpu_baseline(velocity)
+    print(json.dumps(line), flush=True)
+
+
+if __name__ == "__main__":
+    main()

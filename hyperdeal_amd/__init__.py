@@ -1,0 +1,340 @@
+"""hyperdeal_amd -- MI355X (gfx950) phase-space DG advection operator.
+
+Python mirror of the hyper.deal interface for the hot path, over the C ABI
+in include/hyperdeal_amd.h (libhyperdeal_amd.so, built in-tree):
+
+    hyper.deal                                      here
+    ----------------------------------------------  -------------------------
+    hyperdeal::MatrixFree (reinit, n_dofs)           MatrixFree
+    advection::AdvectionOperation (reinit, apply)    AdvectionOperation
+    LowStorageRungeKuttaIntegrator                   LowStorageRungeKuttaIntegrator
+      (perform_time_step, n_stages)
+    VectorTools::interpolate / norm_and_error        VectorTools
+    advection::compute_critical_time_step            compute_critical_time_step
+
+Vectors are torch tensors on the GPU (torch is only the allocator here);
+every compute call goes to the HIP library and raises HyperdealError on a
+non-zero status.  There is no CPU fallback: if the library is missing the
+import of the binding fails loudly.
+"""
+
+import ctypes
+import os
+
+__all__ = ["MatrixFree", "AdvectionOperation", "LowStorageRungeKuttaIntegrator",
+           "VectorTools", "HyperdealError", "FP64", "FP32", "library_path",
+           "load_library", "compute_critical_time_step"]
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+FP64, FP32 = 0, 1
+ALGO_AUTO, ALGO_QUADRATURE, ALGO_COLLAPSED = 0, 1, 2
+BC_HOMOGENEOUS, BC_INHOMOGENEOUS = 0, 1
+FUNC_ZERO, FUNC_HYPERRECTANGLE, FUNC_CONSTANT_ONE, FUNC_SIN_COS, \
+    FUNC_PRODUCT_SINES = range(5)
+
+# every symbol include/hyperdeal_amd.h declares
+EXPORTED = [
+    "hd_last_error", "hd_version", "hd_matrix_free_create",
+    "hd_matrix_free_destroy", "hd_matrix_free_n_dofs",
+    "hd_matrix_free_n_cells", "hd_matrix_free_dofs_per_cell",
+    "hd_advection_create", "hd_advection_destroy",
+    "hd_advection_set_algorithm", "hd_advection_algorithm",
+    "hd_advection_apply", "hd_lsrk_stage", "hd_lsrk_coefficients",
+    "hd_lsrk_step", "hd_interpolate", "hd_add_noise", "hd_norm_and_error",
+    "hd_vector_convert", "hd_critical_time_step", "hd_device_malloc",
+    "hd_device_free", "hd_memcpy_htod", "hd_memcpy_dtoh",
+    "hd_device_synchronize",
+]
+
+
+class HyperdealError(RuntimeError):
+    pass
+
+
+class _MeshDesc(ctypes.Structure):
+    _fields_ = [("dim_x", ctypes.c_int), ("dim_v", ctypes.c_int),
+                ("n_cells", ctypes.c_int * 6), ("lo", ctypes.c_double * 6),
+                ("hi", ctypes.c_double * 6), ("periodic_x", ctypes.c_int),
+                ("periodic_v", ctypes.c_int)]
+
+
+_lib = None
+_vp = ctypes.c_void_p
+_dp = ctypes.POINTER(ctypes.c_double)
+
+
+def library_path():
+    return os.path.join(_HERE, "libhyperdeal_amd.so")
+
+
+def load_library(path=None):
+    """Load libhyperdeal_amd.so (built by `make -C hyperdeal_amd`)."""
+    global _lib
+    if _lib is not None and path is None:
+        return _lib
+    p = path or library_path()
+    if not os.path.exists(p):
+        raise HyperdealError(
+            "libhyperdeal_amd.so not built: run __graft_entry__.build() or "
+            "`make -C hyperdeal_amd`")
+    lib = ctypes.CDLL(p)
+    lib.hd_last_error.restype = ctypes.c_char_p
+    lib.hd_matrix_free_create.argtypes = [ctypes.POINTER(_MeshDesc),
+                                          ctypes.c_int, ctypes.c_int,
+                                          ctypes.c_int, ctypes.POINTER(_vp)]
+    lib.hd_matrix_free_destroy.argtypes = [_vp]
+    for f in ("hd_matrix_free_n_dofs", "hd_matrix_free_n_cells"):
+        getattr(lib, f).restype = ctypes.c_longlong
+        getattr(lib, f).argtypes = [_vp]
+    lib.hd_matrix_free_dofs_per_cell.argtypes = [_vp]
+    lib.hd_advection_create.argtypes = [_vp, _dp, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_int, _dp,
+                                        ctypes.POINTER(_vp)]
+    lib.hd_advection_destroy.argtypes = [_vp]
+    lib.hd_advection_set_algorithm.argtypes = [_vp, ctypes.c_int]
+    lib.hd_advection_algorithm.argtypes = [_vp, ctypes.c_int]
+    lib.hd_advection_apply.argtypes = [_vp, _vp, _vp, ctypes.c_int,
+                                       ctypes.c_double, _vp]
+    lib.hd_lsrk_stage.argtypes = [_vp, ctypes.c_int, _vp, _vp, _vp,
+                                  ctypes.c_double, ctypes.c_double,
+                                  ctypes.c_double, _vp]
+    lib.hd_lsrk_coefficients.argtypes = [ctypes.c_char_p, _dp, _dp]
+    lib.hd_lsrk_step.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int, _vp, _vp,
+                                 _vp, ctypes.c_double, ctypes.c_double, _vp]
+    lib.hd_interpolate.argtypes = [_vp, ctypes.c_int, _dp, ctypes.c_double,
+                                   _vp, _vp]
+    lib.hd_add_noise.argtypes = [_vp, _vp, ctypes.c_uint64, ctypes.c_double,
+                                 _vp]
+    lib.hd_norm_and_error.argtypes = [_vp, _vp, ctypes.c_int, _dp,
+                                      ctypes.c_double, _dp, _vp]
+    lib.hd_vector_convert.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+    lib.hd_critical_time_step.restype = ctypes.c_double
+    lib.hd_critical_time_step.argtypes = [_vp, _dp]
+    lib.hd_device_synchronize.argtypes = []
+    if path is None:
+        _lib = lib
+    return lib
+
+
+def _check(rc, what):
+    if rc < 0:
+        msg = load_library().hd_last_error().decode()
+        raise HyperdealError("%s failed (%d): %s" % (what, rc, msg))
+    return rc
+
+
+def _darr(vals):
+    vals = list(vals)
+    return (ctypes.c_double * max(len(vals), 1))(*vals)
+
+
+def _ptr(t, dtype, n):
+    """Device pointer of a contiguous torch tensor, with checks."""
+    import torch
+    if not isinstance(t, torch.Tensor) or not t.is_cuda:
+        raise HyperdealError("vectors must be torch tensors on the GPU")
+    if t.dtype != dtype:
+        raise HyperdealError("expected %s, got %s" % (dtype, t.dtype))
+    if not t.is_contiguous() or t.numel() != n:
+        raise HyperdealError("expected a contiguous vector of %d values" % n)
+    return ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    if stream is None:
+        import torch
+        stream = torch.cuda.current_stream()
+    return ctypes.c_void_p(stream.cuda_stream if hasattr(stream, "cuda_stream")
+                           else int(stream))
+
+
+def _torch_dtype(precision):
+    import torch
+    return torch.float64 if precision == FP64 else torch.float32
+
+
+class MatrixFree:
+    """Phase-space matrix-free data of a Cartesian box (hyperdeal::MatrixFree
+    built from two dealii::MatrixFree objects, matrix_free.templates.h:860)."""
+
+    def __init__(self, dim_x, dim_v, degree, n_cells, lo, hi, periodic_x=True,
+                 periodic_v=True, n_points=None, collocation=False):
+        self._lib = load_library()
+        D = dim_x + dim_v
+        if len(n_cells) != D or len(lo) != D or len(hi) != D:
+            raise HyperdealError("n_cells/lo/hi need dim_x + dim_v entries")
+        desc = _MeshDesc()
+        desc.dim_x, desc.dim_v = dim_x, dim_v
+        for d in range(D):
+            desc.n_cells[d] = int(n_cells[d])
+            desc.lo[d], desc.hi[d] = float(lo[d]), float(hi[d])
+        desc.periodic_x, desc.periodic_v = int(periodic_x), int(periodic_v)
+        self.dim_x, self.dim_v, self.dim, self.degree = dim_x, dim_v, D, degree
+        self.n_points = n_points or degree + 1
+        self.n_cells = list(n_cells)
+        self.lo, self.hi = list(lo), list(hi)
+        h = _vp()
+        _check(self._lib.hd_matrix_free_create(ctypes.byref(desc), degree,
+                                               self.n_points, int(collocation),
+                                               ctypes.byref(h)),
+               "hd_matrix_free_create")
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._lib.hd_matrix_free_destroy(self.handle)
+            self.handle = None
+
+    def n_dofs(self):
+        return self._lib.hd_matrix_free_n_dofs(self.handle)
+
+    def n_cells_total(self):
+        return self._lib.hd_matrix_free_n_cells(self.handle)
+
+    def dofs_per_cell(self):
+        return self._lib.hd_matrix_free_dofs_per_cell(self.handle)
+
+    def initialize_dof_vector(self, precision=FP64, device="cuda"):
+        import torch
+        return torch.zeros(self.n_dofs(), dtype=_torch_dtype(precision),
+                           device=device)
+
+
+def compute_critical_time_step(mf, velocity):
+    """dt_crit = min over x and v of 1 / max |J^{-T} a| (cfl.h:94-124)."""
+    return mf._lib.hd_critical_time_step(mf.handle, _darr(velocity))
+
+
+class AdvectionOperation:
+    """advection::AdvectionOperation (advection_operation.h:56-209) with a
+    constant velocity field."""
+
+    def __init__(self, matrix_free, velocity, skew_factor=0.0,
+                 boundary=BC_INHOMOGENEOUS, boundary_function=FUNC_ZERO,
+                 boundary_params=None):
+        self.mf = matrix_free
+        self._lib = matrix_free._lib
+        self.velocity = [float(v) for v in velocity]
+        if len(self.velocity) != matrix_free.dim:
+            raise HyperdealError("velocity needs dim_x + dim_v entries")
+        h = _vp()
+        bp = _darr(boundary_params) if boundary_params is not None else None
+        _check(self._lib.hd_advection_create(matrix_free.handle,
+                                             _darr(self.velocity),
+                                             float(skew_factor), boundary,
+                                             boundary_function, bp,
+                                             ctypes.byref(h)),
+               "hd_advection_create")
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._lib.hd_advection_destroy(self.handle)
+            self.handle = None
+
+    def set_algorithm(self, algorithm):
+        _check(self._lib.hd_advection_set_algorithm(self.handle, algorithm),
+               "hd_advection_set_algorithm")
+
+    def algorithm(self, precision=FP64):
+        return _check(self._lib.hd_advection_algorithm(self.handle, precision),
+                      "hd_advection_algorithm")
+
+    def apply(self, dst, src, time=0.0, precision=FP64, stream=None):
+        """dst = M^{-1}(cell + face integrals)(src) (apply, :134-209)."""
+        n = self.mf.n_dofs()
+        dt = _torch_dtype(precision)
+        _check(self._lib.hd_advection_apply(self.handle, _ptr(src, dt, n),
+                                            _ptr(dst, dt, n), precision,
+                                            float(time), _stream(stream)),
+               "hd_advection_apply")
+        return dst
+
+    def stage(self, ti_in, ti_out, sol, time, a_dt, b_dt, precision=FP64,
+              stream=None):
+        """One fused LSRK stage (time_integrators.templates.h:103-138)."""
+        import torch
+        n = self.mf.n_dofs()
+        dt = _torch_dtype(precision)
+        out = _ptr(ti_out, dt, n) if ti_out is not None else None
+        _check(self._lib.hd_lsrk_stage(self.handle, precision,
+                                       _ptr(ti_in, dt, n), out,
+                                       _ptr(sol, torch.float64, n),
+                                       float(time), float(a_dt), float(b_dt),
+                                       _stream(stream)),
+               "hd_lsrk_stage")
+
+
+class LowStorageRungeKuttaIntegrator:
+    """Two-register LSRK (time_integrators.h:47-115): rk33, rk45, rk47,
+    rk59.  The registers ti_a, ti_b hold `precision` values; the solution is
+    FP64."""
+
+    def __init__(self, rk_type="rk45"):
+        self._lib = load_library()
+        self.type = rk_type
+        a = (ctypes.c_double * 16)()
+        b = (ctypes.c_double * 16)()
+        n = _check(self._lib.hd_lsrk_coefficients(rk_type.encode(), a, b),
+                   "hd_lsrk_coefficients")
+        self.ai = [a[i] for i in range(n - 1)]
+        self.bi = [b[i] for i in range(n)]
+
+    def n_stages(self):
+        return len(self.bi)
+
+    def perform_time_step(self, op, solution, ti_a, ti_b, current_time,
+                          time_step, precision=FP64, stream=None):
+        import torch
+        n = op.mf.n_dofs()
+        dt = _torch_dtype(precision)
+        _check(self._lib.hd_lsrk_step(op.handle, self.type.encode(), precision,
+                                      _ptr(solution, torch.float64, n),
+                                      _ptr(ti_a, dt, n), _ptr(ti_b, dt, n),
+                                      float(current_time), float(time_step),
+                                      _stream(stream)),
+               "hd_lsrk_step")
+
+
+class VectorTools:
+    """numerics/vector_tools.h: interpolate and norm_and_error."""
+
+    @staticmethod
+    def interpolate(mf, dst, function, params=None, time=0.0, stream=None):
+        import torch
+        p = _darr(params) if params is not None else None
+        _check(mf._lib.hd_interpolate(mf.handle, function, p, float(time),
+                                      _ptr(dst, torch.float64, mf.n_dofs()),
+                                      _stream(stream)), "hd_interpolate")
+        return dst
+
+    @staticmethod
+    def add_noise(mf, dst, seed, amplitude, stream=None):
+        import torch
+        _check(mf._lib.hd_add_noise(mf.handle,
+                                    _ptr(dst, torch.float64, mf.n_dofs()),
+                                    int(seed), float(amplitude),
+                                    _stream(stream)), "hd_add_noise")
+        return dst
+
+    @staticmethod
+    def norm_and_error(mf, src, function, params=None, time=0.0, stream=None):
+        import torch
+        out = (ctypes.c_double * 2)()
+        p = _darr(params) if params is not None else None
+        _check(mf._lib.hd_norm_and_error(mf.handle,
+                                         _ptr(src, torch.float64, mf.n_dofs()),
+                                         function, p, float(time), out,
+                                         _stream(stream)), "hd_norm_and_error")
+        return out[0], out[1]
+
+    @staticmethod
+    def convert(mf, src, dst, precision, stream=None):
+        import torch
+        n = mf.n_dofs()
+        _check(mf._lib.hd_vector_convert(mf.handle,
+                                         _ptr(src, torch.float64, n),
+                                         _ptr(dst, _torch_dtype(precision), n),
+                                         precision, _stream(stream)),
+               "hd_vector_convert")
+        return dst

@@ -1,0 +1,608 @@
+// hd_capi.hip -- the extern "C" boundary (include/hyperdeal_amd.h): handles,
+// argument checks, algorithm selection and kernel dispatch.
+#include "../../include/hyperdeal_amd.h"
+#include "hd_device.h"
+#include "hd_functions.h"
+#include "hd_tables.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+namespace
+{
+thread_local std::string g_last_error;
+
+int fail(int code, const std::string &msg)
+{
+  g_last_error = msg;
+  return code;
+}
+
+int hip_fail(hipError_t e, const char *where)
+{
+  return fail(HD_ERR_HIP,
+              std::string(where) + ": " + hipGetErrorString(e));
+}
+
+int ok()
+{
+  return HD_OK;
+}
+} // namespace
+
+struct hd_matrix_free
+{
+  int dim_x = 0, dim_v = 0, D = 0, degree = 0, nd = 0, nq = 0;
+  int collocation = 0;
+  bool periodic[6] = {true, true, true, true, true, true};
+  hd::MeshDev mesh{};
+  hd::Tables1D tables;
+  long long n_dofs = 0;
+  int dofs_cell = 0;
+  double *partial = nullptr; // norm partial sums (2 per cell)
+};
+
+struct hd_advection
+{
+  hd_matrix_free *mf = nullptr;
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  double skew = 0.0;
+  int bc_kind = HD_BC_INHOMOGENEOUS;
+  int bc_function = HD_FUNC_ZERO;
+  double bc_params[8] = {0};
+  int algorithm = HD_ALGO_AUTO;
+  bool collapsed_ok = false;
+  double *coef64 = nullptr;
+  float *coef32 = nullptr;
+  hd::InflowDev inflow{};
+  hd::QuadDev quad{};
+};
+
+namespace
+{
+hipStream_t as_stream(void *s) { return reinterpret_cast<hipStream_t>(s); }
+
+int choose_algorithm(const hd_advection *op, int precision)
+{
+  const hd_matrix_free *mf = op->mf;
+  const bool quad_ok = precision == HD_FP64 && mf->nq == mf->nd &&
+                       hd::quadrature_supported(mf->D, mf->nd);
+  const bool coll_ok = op->collapsed_ok;
+  switch (op->algorithm)
+    {
+      case HD_ALGO_COLLAPSED:
+        return coll_ok ? HD_ALGO_COLLAPSED : -1;
+      case HD_ALGO_QUADRATURE:
+        return quad_ok ? HD_ALGO_QUADRATURE : -1;
+      default:
+        return coll_ok ? HD_ALGO_COLLAPSED : (quad_ok ? HD_ALGO_QUADRATURE : -1);
+    }
+}
+
+int run_operator(hd_advection *op, int mode, int precision, const void *src,
+                 void *dst, const hd::StageDev &stage, double time,
+                 hipStream_t stream)
+{
+  hd_matrix_free *mf = op->mf;
+  const int algo = choose_algorithm(op, precision);
+  if (algo < 0)
+    return fail(HD_ERR_UNSUPPORTED,
+                "no kernel for this operator (dims " + std::to_string(mf->dim_x) +
+                  "+" + std::to_string(mf->dim_v) + ", degree " +
+                  std::to_string(mf->degree) + ", n_points " +
+                  std::to_string(mf->nq) + ", precision " +
+                  std::to_string(precision) + ", algorithm " +
+                  std::to_string(op->algorithm) + ")");
+  hipError_t e;
+  if (algo == HD_ALGO_COLLAPSED)
+    {
+      if (precision == HD_FP64)
+        e = hd::collapsed_apply_f64(mf->D, mf->nd, mode,
+                                    static_cast<const double *>(src),
+                                    static_cast<double *>(dst), op->coef64,
+                                    mf->mesh, op->inflow, stage, stream);
+      else
+        e = hd::collapsed_apply_f32(mf->D, mf->nd, mode,
+                                    static_cast<const float *>(src),
+                                    static_cast<float *>(dst), op->coef32,
+                                    mf->mesh, op->inflow, stage, stream);
+    }
+  else
+    e = hd::quadrature_apply(mf->D, mf->nd, mode,
+                             static_cast<const double *>(src),
+                             static_cast<double *>(dst), op->quad, mf->mesh,
+                             stage, time, mf->dim_x, mf->collocation, stream);
+  if (e != hipSuccess)
+    return hip_fail(e, "kernel launch");
+  return ok();
+}
+
+bool valid_precision(int p) { return p == HD_FP64 || p == HD_FP32; }
+size_t precision_bytes(int p) { return p == HD_FP64 ? 8 : 4; }
+
+bool rk_table(const char *type, std::vector<double> &a, std::vector<double> &b)
+{
+  // Kennedy, Carpenter, Lewis (2000) two-register schemes, the values of
+  // base/time_integrators.templates.h:32-84
+  const std::string t = type ? type : "";
+  if (t == "rk33")
+    {
+      b = {0.245170287303492, 0.184896052186740, 0.569933660509768};
+      a = {0.755726351946097, 0.386954477304099};
+    }
+  else if (t == "rk45")
+    {
+      b = {1153189308089. / 22510343858157., 1772645290293. / 4653164025191.,
+           -1672844663538. / 4480602732383., 2114624349019. / 3568978502595.,
+           5198255086312. / 14908931495163.};
+      a = {970286171893. / 4311952581923., 6584761158862. / 12103376702013.,
+           2251764453980. / 15575788980749.,
+           26877169314380. / 34165994151039.};
+    }
+  else if (t == "rk47")
+    {
+      b = {0.0941840925477795334, 0.149683694803496998, 0.285204742060440058,
+           -0.122201846148053668, 0.0605151571191401122, 0.345986987898399296,
+           0.186627171718797670};
+      const double g[6] = {0.241566650129646868, 0.0423866513027719953,
+                           0.215602732678803776, 0.232328007537583987,
+                           0.256223412574146438, 0.0978694102142697230};
+      a.assign(6, 0.0);
+      for (int i = 0; i < 6; ++i)
+        a[i] = g[i] + b[i];
+    }
+  else if (t == "rk59")
+    {
+      b = {2274579626619. / 23610510767302., 693987741272. / 12394497460941.,
+           -347131529483. / 15096185902911., 1144057200723. / 32081666971178.,
+           1562491064753. / 11797114684756., 13113619727965. / 44346030145118.,
+           393957816125. / 7825732611452., 720647959663. / 6565743875477.,
+           3559252274877. / 14424734981077.};
+      a = {1107026461565. / 5417078080134.,  38141181049399. / 41724347789894.,
+           493273079041. / 11940823631197.,  1851571280403. / 6147804934346.,
+           11782306865191. / 62590030070788., 9452544825720. / 13648368537481.,
+           4435885630781. / 26285702406235., 2357909744247. / 11371140753790.};
+    }
+  else
+    return false;
+  return true;
+}
+} // namespace
+
+extern "C" {
+
+const char *hd_last_error(void) { return g_last_error.c_str(); }
+
+int hd_version(int *major, int *minor)
+{
+  if (major)
+    *major = 0;
+  if (minor)
+    *minor = 1;
+  return ok();
+}
+
+int hd_matrix_free_create(const hd_mesh_desc *mesh, int degree, int n_points,
+                          int collocation, hd_matrix_free **out)
+{
+  if (!mesh || !out)
+    return fail(HD_ERR_INVALID, "hd_matrix_free_create: null argument");
+  *out = nullptr;
+  if (mesh->dim_x < 1 || mesh->dim_x > 3 || mesh->dim_v < 1 || mesh->dim_v > 3)
+    return fail(HD_ERR_INVALID, "dim_x and dim_v must be in 1..3");
+  if (degree < 1 || degree > 7 || n_points < degree + 1 || n_points > 8 ||
+      (collocation && n_points != degree + 1))
+    return fail(HD_ERR_INVALID, "invalid degree / n_points / collocation");
+  auto *mf = new (std::nothrow) hd_matrix_free();
+  if (!mf)
+    return fail(HD_ERR_NOMEM, "out of host memory");
+  mf->dim_x = mesh->dim_x;
+  mf->dim_v = mesh->dim_v;
+  mf->D = mesh->dim_x + mesh->dim_v;
+  mf->degree = degree;
+  mf->nd = degree + 1;
+  mf->nq = n_points;
+  mf->collocation = collocation ? 1 : 0;
+  if (!mf->tables.build(degree, n_points, collocation != 0))
+    {
+      delete mf;
+      return fail(HD_ERR_INVALID, "cannot build 1D tables");
+    }
+  mf->mesh.dim = mf->D;
+  long long nc = 1;
+  for (int d = 0; d < mf->D; ++d)
+    {
+      if (mesh->n_cells[d] < 1 || !(mesh->hi[d] > mesh->lo[d]))
+        {
+          delete mf;
+          return fail(HD_ERR_INVALID, "invalid box or cell count");
+        }
+      mf->mesh.n[d] = mesh->n_cells[d];
+      mf->mesh.stride[d] = nc;
+      mf->mesh.lo[d] = mesh->lo[d];
+      mf->mesh.h[d] = (mesh->hi[d] - mesh->lo[d]) / mesh->n_cells[d];
+      mf->periodic[d] = d < mf->dim_x ? mesh->periodic_x != 0
+                                      : mesh->periodic_v != 0;
+      nc *= mesh->n_cells[d];
+    }
+  if (nc >= (1LL << 31))
+    {
+      delete mf;
+      return fail(HD_ERR_UNSUPPORTED, "more than 2^31 cells");
+    }
+  mf->mesh.n_cells = nc;
+  mf->dofs_cell = hd::ipow(mf->nd, mf->D);
+  mf->n_dofs = nc * mf->dofs_cell;
+  *out = mf;
+  return ok();
+}
+
+int hd_matrix_free_destroy(hd_matrix_free *mf)
+{
+  if (!mf)
+    return ok();
+  if (mf->partial)
+    (void)hipFree(mf->partial);
+  delete mf;
+  return ok();
+}
+
+long long hd_matrix_free_n_dofs(const hd_matrix_free *mf)
+{
+  return mf ? mf->n_dofs : -1;
+}
+long long hd_matrix_free_n_cells(const hd_matrix_free *mf)
+{
+  return mf ? mf->mesh.n_cells : -1;
+}
+int hd_matrix_free_dofs_per_cell(const hd_matrix_free *mf)
+{
+  return mf ? mf->dofs_cell : -1;
+}
+
+int hd_advection_create(hd_matrix_free *mf, const double *velocity,
+                        double skew_factor, int bc_kind, int bc_function,
+                        const double *bc_params, hd_advection **out)
+{
+  if (!mf || !velocity || !out)
+    return fail(HD_ERR_INVALID, "hd_advection_create: null argument");
+  *out = nullptr;
+  if (bc_kind != HD_BC_HOMOGENEOUS && bc_kind != HD_BC_INHOMOGENEOUS)
+    return fail(HD_ERR_INVALID, "invalid boundary kind");
+  auto *op = new (std::nothrow) hd_advection();
+  if (!op)
+    return fail(HD_ERR_NOMEM, "out of host memory");
+  op->mf = mf;
+  op->skew = skew_factor;
+  op->bc_kind = bc_kind;
+  op->bc_function = bc_function;
+  const int D = mf->D, nd = mf->nd, nq = mf->nq;
+  for (int d = 0; d < D; ++d)
+    {
+      op->a[d] = velocity[d];
+      op->bc_params[d] = bc_params ? bc_params[d] : velocity[d];
+    }
+
+  // collapsed form: periodic in every direction
+  bool all_periodic = true;
+  for (int d = 0; d < D; ++d)
+    all_periodic = all_periodic && mf->periodic[d];
+  op->collapsed_ok = all_periodic && hd::collapsed_supported(D, nd);
+  if (op->collapsed_ok)
+    {
+      const int per = nd * nd + nd;
+      std::vector<double> c64(D * per);
+      std::vector<float> c32(D * per);
+      for (int d = 0; d < D; ++d)
+        {
+          const auto c = hd::collapse_direction(mf->tables, op->a[d],
+                                                mf->mesh.h[d], skew_factor);
+          op->inflow.side[d] =
+            op->a[d] == 0.0 ? 0 : (c.inflow_from_left ? 1 : -1);
+          std::copy(c.A.begin(), c.A.end(), c64.begin() + d * per);
+          std::copy(c.b.begin(), c.b.end(), c64.begin() + d * per + nd * nd);
+        }
+      for (size_t i = 0; i < c64.size(); ++i)
+        c32[i] = (float)c64[i];
+      hipError_t e = hipMalloc(&op->coef64, c64.size() * sizeof(double));
+      if (e == hipSuccess)
+        e = hipMalloc(&op->coef32, c32.size() * sizeof(float));
+      if (e == hipSuccess)
+        e = hipMemcpy(op->coef64, c64.data(), c64.size() * sizeof(double),
+                      hipMemcpyHostToDevice);
+      if (e == hipSuccess)
+        e = hipMemcpy(op->coef32, c32.data(), c32.size() * sizeof(float),
+                      hipMemcpyHostToDevice);
+      if (e != hipSuccess)
+        {
+          hd_advection_destroy(op);
+          return hip_fail(e, "hd_advection_create");
+        }
+    }
+
+  // quadrature form: the 1D tables by value
+  if (nq == nd && nd <= 8)
+    {
+      auto &q = op->quad;
+      const auto &t = mf->tables;
+      std::copy(t.S.begin(), t.S.end(), q.S);
+      std::copy(t.Dq.begin(), t.Dq.end(), q.Dq);
+      std::copy(t.F.begin(), t.F.end(), q.F);
+      std::copy(t.Sinv.begin(), t.Sinv.end(), q.Sinv);
+      std::copy(t.qweights.begin(), t.qweights.end(), q.w);
+      std::copy(t.qpoints.begin(), t.qpoints.end(), q.qp);
+      for (int d = 0; d < D; ++d)
+        {
+          q.a[d] = op->a[d];
+          q.periodic[d] = mf->periodic[d] ? 1 : 0;
+          q.bc_params[d] = op->bc_params[d];
+        }
+      q.skew = skew_factor;
+      q.bc_inhomogeneous = bc_kind == HD_BC_INHOMOGENEOUS ? 1 : 0;
+      q.bc_function = bc_function;
+    }
+  *out = op;
+  return ok();
+}
+
+int hd_advection_destroy(hd_advection *op)
+{
+  if (!op)
+    return ok();
+  if (op->coef64)
+    (void)hipFree(op->coef64);
+  if (op->coef32)
+    (void)hipFree(op->coef32);
+  delete op;
+  return ok();
+}
+
+int hd_advection_set_algorithm(hd_advection *op, int algorithm)
+{
+  if (!op || algorithm < HD_ALGO_AUTO || algorithm > HD_ALGO_COLLAPSED)
+    return fail(HD_ERR_INVALID, "invalid algorithm");
+  if (algorithm == HD_ALGO_COLLAPSED && !op->collapsed_ok)
+    return fail(HD_ERR_UNSUPPORTED,
+                "collapsed form needs a periodic mesh in every direction");
+  op->algorithm = algorithm;
+  return ok();
+}
+
+int hd_advection_algorithm(const hd_advection *op, int precision)
+{
+  if (!op || !valid_precision(precision))
+    return fail(HD_ERR_INVALID, "invalid argument");
+  const int a = choose_algorithm(op, precision);
+  return a < 0 ? fail(HD_ERR_UNSUPPORTED, "no kernel") : a;
+}
+
+int hd_advection_apply(hd_advection *op, const void *src, void *dst,
+                       int precision, double time, void *stream)
+{
+  if (!op || !src || !dst)
+    return fail(HD_ERR_INVALID, "hd_advection_apply: null argument");
+  if (!valid_precision(precision))
+    return fail(HD_ERR_INVALID, "invalid precision");
+  if (src == dst)
+    return fail(HD_ERR_INVALID, "src and dst must not alias");
+  hd::StageDev stage{};
+  return run_operator(op, 0, precision, src, dst, stage, time,
+                      as_stream(stream));
+}
+
+int hd_lsrk_stage(hd_advection *op, int precision, const void *ti_in,
+                  void *ti_out, double *sol, double time, double a_dt,
+                  double b_dt, void *stream)
+{
+  if (!op || !ti_in || !sol)
+    return fail(HD_ERR_INVALID, "hd_lsrk_stage: null argument");
+  if (!valid_precision(precision))
+    return fail(HD_ERR_INVALID, "invalid precision");
+  if (ti_out == ti_in || (const void *)sol == ti_in)
+    return fail(HD_ERR_INVALID, "ti_out/sol must not alias ti_in");
+  hd::StageDev stage{};
+  stage.a_dt = a_dt;
+  stage.b_dt = b_dt;
+  stage.sol = sol;
+  stage.ti_out = ti_out;
+  return run_operator(op, 1, precision, ti_in, nullptr, stage, time,
+                      as_stream(stream));
+}
+
+int hd_lsrk_coefficients(const char *type, double *a, double *b)
+{
+  std::vector<double> va, vb;
+  if (!rk_table(type, va, vb))
+    return fail(HD_ERR_INVALID, std::string("unknown RK scheme ") +
+                                  (type ? type : "(null)"));
+  if (a)
+    std::copy(va.begin(), va.end(), a);
+  if (b)
+    std::copy(vb.begin(), vb.end(), b);
+  return (int)vb.size();
+}
+
+int hd_lsrk_step(hd_advection *op, const char *type, int precision,
+                 double *sol, void *ti_a, void *ti_b, double time, double dt,
+                 void *stream)
+{
+  if (!op || !sol || !ti_a || !ti_b || ti_a == ti_b)
+    return fail(HD_ERR_INVALID, "hd_lsrk_step: invalid registers");
+  std::vector<double> a, b;
+  if (!rk_table(type, a, b))
+    return fail(HD_ERR_INVALID, "unknown RK scheme");
+  // the first stage reads a copy of the solution (time_integrators.
+  // templates.h:142-157)
+  int rc = hd_vector_convert(op->mf, sol, ti_a, precision, stream);
+  if (rc != HD_OK)
+    return rc;
+  double sum_b = 0.0;
+  void *in = ti_a, *out = ti_b;
+  const size_t n = b.size();
+  for (size_t s = 0; s < n; ++s)
+    {
+      const double c = s == 0 ? 0.0 : sum_b + a[s - 1];
+      if (s > 0)
+        sum_b += b[s - 1];
+      const bool last = s == n - 1;
+      rc = hd_lsrk_stage(op, precision, in, last ? nullptr : out, sol,
+                         time + c * dt, last ? 0.0 : a[s] * dt, b[s] * dt,
+                         stream);
+      if (rc != HD_OK)
+        return rc;
+      std::swap(in, out);
+    }
+  return ok();
+}
+
+int hd_interpolate(hd_matrix_free *mf, int function, const double *params,
+                   double time, double *dst, void *stream)
+{
+  if (!mf || !dst)
+    return fail(HD_ERR_INVALID, "hd_interpolate: null argument");
+  hd::FuncDev f{};
+  f.id = function;
+  for (int d = 0; d < mf->D; ++d)
+    f.prm[d] = params ? params[d] : 0.0;
+  std::copy(mf->tables.nodes.begin(), mf->tables.nodes.end(), f.nodes);
+  f.nd = mf->nd;
+  f.dim_x = mf->dim_x;
+  const long long n = mf->n_dofs;
+  const hipError_t e = hd::launch_interpolate(dst, mf->mesh, f, time, n,
+                                              mf->dofs_cell, as_stream(stream));
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_interpolate");
+}
+
+int hd_add_noise(hd_matrix_free *mf, double *dst, uint64_t seed,
+                 double amplitude, void *stream)
+{
+  if (!mf || !dst)
+    return fail(HD_ERR_INVALID, "hd_add_noise: null argument");
+  const long long n = mf->n_dofs;
+  const hipError_t e =
+    hd::launch_noise(dst, seed, amplitude, n, as_stream(stream));
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_add_noise");
+}
+
+int hd_vector_convert(hd_matrix_free *mf, const double *src, void *dst,
+                      int precision, void *stream)
+{
+  if (!mf || !src || !dst || !valid_precision(precision))
+    return fail(HD_ERR_INVALID, "hd_vector_convert: invalid argument");
+  const long long n = mf->n_dofs;
+  if (precision == HD_FP64)
+    {
+      const hipError_t e =
+        hipMemcpyAsync(dst, src, n * sizeof(double), hipMemcpyDeviceToDevice,
+                       as_stream(stream));
+      return e == hipSuccess ? ok() : hip_fail(e, "hd_vector_convert");
+    }
+  const hipError_t e = hd::launch_convert_f32(
+    src, static_cast<float *>(dst), n, as_stream(stream));
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_vector_convert");
+}
+
+int hd_norm_and_error(hd_matrix_free *mf, const double *src, int function,
+                      const double *params, double time, double *out,
+                      void *stream)
+{
+  if (!mf || !src || !out)
+    return fail(HD_ERR_INVALID, "hd_norm_and_error: null argument");
+  const long long nc = mf->mesh.n_cells;
+  hipError_t e = hipSuccess;
+  if (!mf->partial)
+    e = hipMalloc(&mf->partial, 2 * nc * sizeof(double));
+  if (e != hipSuccess)
+    return hip_fail(e, "hd_norm_and_error");
+  hd::NormDev nt{};
+  std::copy(mf->tables.S.begin(), mf->tables.S.end(), nt.S);
+  std::copy(mf->tables.qpoints.begin(), mf->tables.qpoints.end(), nt.qp);
+  std::copy(mf->tables.qweights.begin(), mf->tables.qweights.end(), nt.w);
+  nt.nd = mf->nd;
+  nt.nq = mf->nq;
+  hd::FuncDev f{};
+  f.id = function;
+  for (int d = 0; d < mf->D; ++d)
+    f.prm[d] = params ? params[d] : 0.0;
+  f.nd = mf->nd;
+  f.dim_x = mf->dim_x;
+  const int vol = hd::ipow(std::max(mf->nd, mf->nq), mf->D);
+  const size_t lds = (2 * (size_t)vol + 512) * sizeof(double);
+  if (lds > 160 * 1024)
+    return fail(HD_ERR_UNSUPPORTED, "cell too large for hd_norm_and_error");
+  e = hd::launch_norm(src, mf->mesh, nt, f, time, vol, mf->partial,
+                      as_stream(stream));
+  if (e != hipSuccess)
+    return hip_fail(e, "hd_norm_and_error");
+  std::vector<double> host(2 * nc);
+  e = hipMemcpyAsync(host.data(), mf->partial, 2 * nc * sizeof(double),
+                     hipMemcpyDeviceToHost, as_stream(stream));
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(as_stream(stream));
+  if (e != hipSuccess)
+    return hip_fail(e, "hd_norm_and_error");
+  double s0 = 0.0, s1 = 0.0;
+  for (long long c = 0; c < nc; ++c)
+    {
+      s0 += host[2 * c];
+      s1 += host[2 * c + 1];
+    }
+  out[0] = std::sqrt(s0);
+  out[1] = std::sqrt(s1);
+  return ok();
+}
+
+double hd_critical_time_step(const hd_matrix_free *mf, const double *velocity)
+{
+  if (!mf || !velocity)
+    return -1.0;
+  double vx = 0.0, vv = 0.0;
+  for (int d = 0; d < mf->D; ++d)
+    {
+      const double v = std::fabs(velocity[d] / mf->mesh.h[d]);
+      if (d < mf->dim_x)
+        vx = std::max(vx, v);
+      else
+        vv = std::max(vv, v);
+    }
+  return std::min(1.0 / vx, 1.0 / vv);
+}
+
+int hd_device_malloc(void **ptr, uint64_t bytes)
+{
+  if (!ptr)
+    return fail(HD_ERR_INVALID, "null pointer");
+  const hipError_t e = hipMalloc(ptr, bytes);
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_device_malloc");
+}
+
+int hd_device_free(void *ptr)
+{
+  const hipError_t e = hipFree(ptr);
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_device_free");
+}
+
+int hd_memcpy_htod(void *dst, const void *src, uint64_t bytes)
+{
+  const hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_memcpy_htod");
+}
+
+int hd_memcpy_dtoh(void *dst, const void *src, uint64_t bytes)
+{
+  const hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_memcpy_dtoh");
+}
+
+int hd_device_synchronize(void)
+{
+  const hipError_t e = hipDeviceSynchronize();
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_device_synchronize");
+}
+
+} // extern "C"

@@ -1,0 +1,137 @@
+// hd_device.h -- shared device-side definitions for the gfx950 kernels.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+namespace hd
+{
+constexpr int kMaxDim = 6;
+
+__host__ __device__ constexpr int ipow(int b, int e)
+{
+  return e == 0 ? 1 : b * ipow(b, e - 1);
+}
+
+// Cartesian phase-space mesh as the kernels see it: cells numbered
+// lexicographically, direction 0 fastest (x directions first, then v).
+struct MeshDev
+{
+  int dim;
+  int n[kMaxDim];           // cells per direction
+  long long stride[kMaxDim]; // cell-id stride per direction
+  long long n_cells;
+  double lo[kMaxDim], h[kMaxDim];
+};
+
+// Which neighbour feeds the inflow face of each direction (constant
+// velocity): +1 -> the left neighbour's last node, -1 -> the right
+// neighbour's first node, 0 -> no transport in this direction.
+struct InflowDev
+{
+  int side[kMaxDim];
+};
+
+// LSRK stage epilogue: sol += b_dt K; ti_out = sol_old + a_dt K
+struct StageDev
+{
+  double a_dt, b_dt;
+  double *sol;   // FP64 solution, updated in place
+  void *ti_out;  // next stage input (Number), may be null (last stage)
+};
+
+// blocks are dispatched round-robin over the 8 XCDs; give each XCD a
+// contiguous range of logical blocks so that face neighbours share an L2
+__device__ inline long long xcd_chunked_block(long long n_logical)
+{
+  const long long b = blockIdx.x;
+  const long long per = (n_logical + 7) / 8;
+  return (b % 8) * per + b / 8;
+}
+
+// neighbour cell across direction d on the inflow side (periodic)
+__device__ inline long long inflow_neighbour(const MeshDev &m, long long cell,
+                                             int d, int side)
+{
+  const long long s = m.stride[d];
+  const int c = (int)((cell / s) % m.n[d]);
+  if (side > 0) // from the left
+    return c == 0 ? cell + (long long)(m.n[d] - 1) * s : cell - s;
+  return c == m.n[d] - 1 ? cell - (long long)(m.n[d] - 1) * s : cell + s;
+}
+
+// constants of one quadrature-form operator: 1D tables (N x N each, N <= 8),
+// velocity, skew, boundary description
+struct QuadDev
+{
+  double S[64];     // nq x nd
+  double Dq[64];    // nq x nq
+  double F[16];     // 2 x nq
+  double Sinv[64];  // nd x nq
+  double w[8];      // quadrature weights on [0,1]
+  double qp[8];     // quadrature points on [0,1]
+  double a[kMaxDim];
+  double skew;
+  int periodic[kMaxDim];
+  int bc_inhomogeneous; // Dirichlet: 1 -> u+ = -u- + 2 g, 0 -> u+ = -u-
+  int bc_function;
+  double bc_params[kMaxDim + 2];
+};
+
+// an analytic function and the 1D nodes it is interpolated at
+struct FuncDev
+{
+  int id;
+  double prm[kMaxDim + 2];
+  double nodes[16]; // 1D nodal points on [0,1]
+  int nd;
+  int dim_x;
+};
+
+// tables of the L2 norm: nodal values -> quadrature points
+struct NormDev
+{
+  double S[256]; // nq x nd (nq, nd <= 16)
+  double qp[16], w[16];
+  int nd, nq;
+};
+
+// host launchers (one translation unit per kernel family)
+bool collapsed_supported(int D, int N);
+hipError_t collapsed_apply_f64(int D, int N, int mode, const double *src,
+                               double *dst, const double *coef,
+                               const MeshDev &mesh, const InflowDev &inflow,
+                               const StageDev &stage, hipStream_t stream);
+hipError_t collapsed_apply_f32(int D, int N, int mode, const float *src,
+                               float *dst, const float *coef,
+                               const MeshDev &mesh, const InflowDev &inflow,
+                               const StageDev &stage, hipStream_t stream);
+bool quadrature_supported(int D, int N);
+hipError_t quadrature_apply(int D, int N, int mode, const double *src,
+                            double *dst, const QuadDev &q,
+                            const MeshDev &mesh, const StageDev &stage,
+                            double time, int dim_x, int collocation,
+                            hipStream_t stream);
+hipError_t launch_interpolate(double *dst, const MeshDev &mesh,
+                              const FuncDev &f, double t, long long n_dofs,
+                              int dofs_cell, hipStream_t stream);
+hipError_t launch_noise(double *dst, uint64_t seed, double amplitude,
+                        long long n, hipStream_t stream);
+hipError_t launch_convert_f32(const double *src, float *dst, long long n,
+                              hipStream_t stream);
+hipError_t launch_norm(const double *src, const MeshDev &mesh,
+                       const NormDev &nt, const FuncDev &f, double t, int vol,
+                       double *partial, hipStream_t stream);
+
+// SplitMix64 output for counter i (state seed + (i+1) golden increments),
+// mapped to [-1, 1)
+__host__ __device__ inline double splitmix_uniform(uint64_t seed, uint64_t i)
+{
+  uint64_t z = seed + (i + 1ull) * 0x9E3779B97F4A7C15ull;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  z = z ^ (z >> 31);
+  return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+} // namespace hd

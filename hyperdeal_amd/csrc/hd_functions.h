@@ -1,0 +1,45 @@
+// hd_functions.h -- analytic functions evaluated on the device (initial
+// data, Dirichlet data, exact solutions); ids match hd_function in
+// include/hyperdeal_amd.h.
+#pragma once
+
+#include "hd_device.h"
+
+namespace hd
+{
+// analytic functions (same ids as the oracle's) for Dirichlet data
+__host__ __device__ inline double eval_function(int id, const double *prm, int dim_x,
+                                       int D, const double *x, double t)
+{
+  const double pi = 3.14159265358979323846;
+  switch (id)
+    {
+      case 1: // sin(2 pi (x0 - a0 t)) prod cos(2 pi (xd - ad t))
+        {
+          double r = sin(2.0 * pi * (x[0] - t * prm[0]));
+          for (int d = 1; d < D; ++d)
+            r *= cos(2.0 * pi * (x[d] - t * prm[d]));
+          return r;
+        }
+      case 2:
+        return 1.0;
+      case 3:
+        {
+          double r = sin(x[0] * pi);
+          for (int d = 1; d < D; ++d)
+            r *= cos(x[d] * pi);
+          return r;
+        }
+      case 4:
+        {
+          double r = 1.0;
+          for (int d = 0; d < D; ++d)
+            r *= sin(2.0 * pi * (x[d] - t * prm[d]));
+          return r;
+        }
+      default:
+        return 0.0;
+    }
+}
+
+} // namespace hd

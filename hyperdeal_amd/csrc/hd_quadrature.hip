@@ -1,0 +1,404 @@
+// hd_quadrature.hip -- the general ECL apply of the phase-space DG advection
+// operator on gfx950: the reference's quadrature algorithm step by step
+// (operators/advection/advection_operation.h:221-566), for Cartesian meshes
+// with periodic or Dirichlet boundaries, any skew factor, Gauss quadrature
+// with k+1 points or Gauss-Lobatto collocation.
+//
+// One block works on CPB cells held in LDS:
+//   U  <- src cell (nodal), then U <- S^D U             (quadrature values)
+//   R  <- cell integral: -skew (a.grad u) JxW + sum_d D_d^T[(1-skew) u a_d/h_d JxW]
+//   per face f (2D of them): u- = F_s . U along d; u+ = S^{D-1} (neighbour's
+//       face nodes) or the Dirichlet ghost -u- + 2g; R += F_s^T (-flux JxW_f)
+//   R <- Sinv^D (R / JxW)                                 (inverse mass)
+// Every 1D contraction is done line by line: a thread owns whole lines of N
+// values along the current direction, so updates are in place and only a
+// barrier separates consecutive directions.
+#include "hd_device.h"
+#include "hd_functions.h"
+
+#include <cmath>
+
+namespace hd
+{
+namespace quad
+{
+constexpr int kBlock = 256;
+
+template <int D, int N>
+struct Shape
+{
+  static constexpr int CELL = ipow(N, D);
+  static constexpr int FACE = ipow(N, D - 1);
+  static constexpr int CPB = CELL >= 1024 ? 1 : 1024 / CELL;
+};
+
+// base index of line f (0 <= f < N^{D-1}) along direction d
+template <int D, int N>
+__device__ inline int line_base(int f, int d)
+{
+  int base = 0, pw = 1;
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+    {
+      if (j != d)
+        {
+          base += (f % N) * pw;
+          f /= N;
+        }
+      pw *= N;
+    }
+  return base;
+}
+
+// in-place 1D contraction of all lines along d of the CPB cells in buf:
+// line <- M line, M(r, c) = M[r * sr + c * sc]
+template <int D, int N>
+__device__ inline void contract_lines(double *buf, int cellp, int d,
+                                      const double *M, int sr, int sc,
+                                      int ncell)
+{
+  constexpr int FACE = ipow(N, D - 1);
+  const int sd = ipow(N, d);
+  for (int l = threadIdx.x; l < ncell * FACE; l += kBlock)
+    {
+      const int lc = l / FACE;
+      double *p = buf + lc * cellp + line_base<D, N>(l - lc * FACE, d);
+      double in[N];
+#pragma unroll
+      for (int j = 0; j < N; ++j)
+        in[j] = p[j * sd];
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+        {
+          double acc = 0.0;
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            acc += M[i * sr + j * sc] * in[j];
+          p[i * sd] = acc;
+        }
+    }
+}
+
+template <int D, int N>
+__device__ inline double jxw_point(const QuadDev &q, const MeshDev &m, int i)
+{
+  double v = 1.0;
+#pragma unroll
+  for (int j = 0; j < D; ++j)
+    {
+      v *= m.h[j] * q.w[i % N];
+      i /= N;
+    }
+  return v;
+}
+
+// MODE 0: dst = apply(src); MODE 1: LSRK stage epilogue
+template <int D, int N, int MODE>
+__global__ __launch_bounds__(kBlock) void apply_kernel(
+  const double *__restrict__ src, double *__restrict__ dst, QuadDev q,
+  MeshDev mesh, StageDev stage, double time, int dim_x, int collocation)
+{
+  using Sh = Shape<D, N>;
+  constexpr int CELL = Sh::CELL, FACE = Sh::FACE, CPB = Sh::CPB;
+  constexpr int CELLP = CELL + 1;
+  extern __shared__ double smem[];
+  double *U = smem;                 // CPB x CELLP
+  double *R = U + CPB * CELLP;      // CPB x CELLP
+  double *Fm = R + CPB * CELLP;     // CPB x FACE  (u- then flux)
+  double *Fp = Fm + CPB * FACE;     // CPB x FACE  (u+)
+
+  const long long cell0 = (long long)blockIdx.x * CPB;
+  const int ncell = (int)min((long long)CPB, mesh.n_cells - cell0);
+  const int tid = threadIdx.x;
+
+  for (int i = tid; i < ncell * CELL; i += kBlock)
+    {
+      const int lc = i / CELL;
+      U[lc * CELLP + (i - lc * CELL)] = src[cell0 * CELL + i];
+    }
+  __syncthreads();
+
+  // 1) nodal -> quadrature values
+  if (!collocation)
+    for (int d = 0; d < D; ++d)
+      {
+        contract_lines<D, N>(U, CELLP, d, q.S, N, 1, ncell);
+        __syncthreads();
+      }
+
+  // 2) cell integral into R
+  for (int i = tid; i < ncell * CELL; i += kBlock)
+    {
+      const int lc = i / CELL;
+      R[lc * CELLP + (i - lc * CELL)] = 0.0;
+    }
+  __syncthreads();
+  for (int d = 0; d < D; ++d)
+    {
+      const int sd = ipow(N, d);
+      const double c = q.a[d] / mesh.h[d];
+      for (int l = tid; l < ncell * FACE; l += kBlock)
+        {
+          const int lc = l / FACE;
+          const int base = line_base<D, N>(l - lc * FACE, d);
+          const double *u = U + lc * CELLP + base;
+          double *r = R + lc * CELLP + base;
+          double uq[N], jw[N], acc[N];
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            {
+              uq[j] = u[j * sd];
+              jw[j] = jxw_point<D, N>(q, mesh, base + j * sd);
+              acc[j] = 0.0;
+            }
+          if (q.skew != 0.0)
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+              {
+                double g = 0.0;
+#pragma unroll
+                for (int j = 0; j < N; ++j)
+                  g += q.Dq[i * N + j] * uq[j];
+                acc[i] += -q.skew * (g * c) * jw[i];
+              }
+          if (q.skew != 1.0)
+#pragma unroll
+            for (int i = 0; i < N; ++i)
+              {
+                double s = 0.0;
+#pragma unroll
+                for (int j = 0; j < N; ++j)
+                  s += q.Dq[j * N + i] * ((1.0 - q.skew) * uq[j] * c * jw[j]);
+                acc[i] += s;
+              }
+#pragma unroll
+          for (int i = 0; i < N; ++i)
+            r[i * sd] += acc[i];
+        }
+      __syncthreads();
+    }
+
+  // 3) faces
+  for (int f = 0; f < 2 * D; ++f)
+    {
+      const int d = f / 2, side = f % 2;
+      const int sd = ipow(N, d);
+      const double an = q.a[d] * (side ? 1.0 : -1.0);
+      // u- (trace) and the neighbour's face nodes
+      for (int l = tid; l < ncell * FACE; l += kBlock)
+        {
+          const int lc = l / FACE, fi = l - lc * FACE;
+          const int base = line_base<D, N>(fi, d);
+          const double *u = U + lc * CELLP + base;
+          double um = 0.0;
+#pragma unroll
+          for (int p = 0; p < N; ++p)
+            um += q.F[side * N + p] * u[p * sd];
+          Fm[lc * FACE + fi] = um;
+          // neighbour
+          const long long cell = cell0 + lc;
+          const long long s = mesh.stride[d];
+          const int c = (int)((cell / s) % mesh.n[d]);
+          const int cn = c + (side ? 1 : -1);
+          double up = 0.0;
+          if (cn >= 0 && cn < mesh.n[d])
+            up = src[(cell + (side ? s : -s)) * CELL + base +
+                     (side ? 0 : (N - 1)) * sd];
+          else if (q.periodic[d])
+            up = src[(cell + (side ? -(long long)(mesh.n[d] - 1) * s
+                                   : (long long)(mesh.n[d] - 1) * s)) *
+                       CELL +
+                     base + (side ? 0 : (N - 1)) * sd];
+          Fp[lc * FACE + fi] = up;
+        }
+      __syncthreads();
+      // interpolate u+ along the face directions
+      if (!collocation)
+        for (int j = 0; j < D; ++j)
+          {
+            if (j == d)
+              continue;
+            // lines of the face array along j (the face array is the cell
+            // array with direction d of extent 1)
+            const int sj = ipow(N, j < d ? j : j - 1);
+            const int nl = FACE / N;
+            for (int l = tid; l < ncell * nl; l += kBlock)
+              {
+                const int lc = l / nl;
+                int li = l - lc * nl;
+                // base of line li along face-direction jj in N^{D-1}
+                const int jj = j < d ? j : j - 1;
+                int base = 0, pw = 1;
+                for (int k = 0; k < D - 1; ++k)
+                  {
+                    if (k != jj)
+                      {
+                        base += (li % N) * pw;
+                        li /= N;
+                      }
+                    pw *= N;
+                  }
+                double *p = Fp + lc * FACE + base;
+                double in[N];
+#pragma unroll
+                for (int t = 0; t < N; ++t)
+                  in[t] = p[t * sj];
+#pragma unroll
+                for (int i = 0; i < N; ++i)
+                  {
+                    double acc = 0.0;
+#pragma unroll
+                    for (int t = 0; t < N; ++t)
+                      acc += q.S[i * N + t] * in[t];
+                    p[i * sj] = acc;
+                  }
+              }
+            __syncthreads();
+          }
+      // flux at the face points, then lift into R
+      for (int l = tid; l < ncell * FACE; l += kBlock)
+        {
+          const int lc = l / FACE, fi = l - lc * FACE;
+          const int base = line_base<D, N>(fi, d);
+          const long long cell = cell0 + lc;
+          const long long s = mesh.stride[d];
+          const int c = (int)((cell / s) % mesh.n[d]);
+          const bool boundary =
+            !q.periodic[d] && (side ? c == mesh.n[d] - 1 : c == 0);
+          const double um = Fm[lc * FACE + fi];
+          double up = Fp[lc * FACE + fi];
+          // face JxW: product over j != d
+          double jf = 1.0;
+          {
+            int rem = base;
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+              {
+                if (j != d)
+                  jf *= mesh.h[j] * q.w[rem % N];
+                rem /= N;
+              }
+          }
+          if (boundary)
+            {
+              double g = 0.0;
+              if (q.bc_inhomogeneous)
+                {
+                  double x[kMaxDim];
+                  long long cc = cell;
+                  int rem = base;
+                  for (int j = 0; j < D; ++j)
+                    {
+                      const int cj = (int)(cc % mesh.n[j]);
+                      cc /= mesh.n[j];
+                      const int ij = rem % N;
+                      rem /= N;
+                      x[j] = mesh.lo[j] +
+                             (cj + (j == d ? (double)side : q.qp[ij])) *
+                               mesh.h[j];
+                    }
+                  g = eval_function(q.bc_function, q.bc_params, dim_x, D, x,
+                                    time);
+                }
+              up = -um + 2.0 * g;
+            }
+          const double flux =
+            0.5 * ((um + up) * an + fabs(an) * (um - up));
+          const double fd = -(flux - q.skew * um * an) * jf;
+          double *r = R + lc * CELLP + base;
+#pragma unroll
+          for (int p = 0; p < N; ++p)
+            r[p * sd] += q.F[side * N + p] * fd;
+        }
+      __syncthreads();
+    }
+
+  // 4) inverse mass matrix
+  for (int i = tid; i < ncell * CELL; i += kBlock)
+    {
+      const int lc = i / CELL, li = i - lc * CELL;
+      R[lc * CELLP + li] /= jxw_point<D, N>(q, mesh, li);
+    }
+  __syncthreads();
+  if (!collocation)
+    for (int d = D - 1; d >= 0; --d)
+      {
+        contract_lines<D, N>(R, CELLP, d, q.Sinv, N, 1, ncell);
+        __syncthreads();
+      }
+
+  // 5) write
+  for (int i = tid; i < ncell * CELL; i += kBlock)
+    {
+      const int lc = i / CELL;
+      const double k = R[lc * CELLP + (i - lc * CELL)];
+      const long long g = cell0 * CELL + i;
+      if (MODE == 0)
+        dst[g] = k;
+      else
+        {
+          const double s = stage.sol[g];
+          stage.sol[g] = s + stage.b_dt * k;
+          if (stage.ti_out)
+            static_cast<double *>(stage.ti_out)[g] = s + stage.a_dt * k;
+        }
+    }
+}
+
+template <int D, int N>
+hipError_t launch(int mode, const double *src, double *dst,
+                  const QuadDev &q, const MeshDev &mesh,
+                  const StageDev &stage, double time, int dim_x,
+                  int collocation, hipStream_t stream)
+{
+  using Sh = Shape<D, N>;
+  const size_t lds = sizeof(double) *
+                     (2 * Sh::CPB * (Sh::CELL + 1) + 2 * Sh::CPB * Sh::FACE);
+  const long long grid = (mesh.n_cells + Sh::CPB - 1) / Sh::CPB;
+  // gfx950 has 160 KB of LDS per CU; opt in above the 64 KB default
+  hipError_t e = hipFuncSetAttribute(
+    reinterpret_cast<const void *>(mode == 0 ? &apply_kernel<D, N, 0>
+                                             : &apply_kernel<D, N, 1>),
+    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess)
+    return e;
+  if (mode == 0)
+    hipLaunchKernelGGL((apply_kernel<D, N, 0>), dim3((unsigned)grid),
+                       dim3(kBlock), lds, stream, src, dst, q, mesh, stage,
+                       time, dim_x, collocation);
+  else
+    hipLaunchKernelGGL((apply_kernel<D, N, 1>), dim3((unsigned)grid),
+                       dim3(kBlock), lds, stream, src, dst, q, mesh, stage,
+                       time, dim_x, collocation);
+  return hipGetLastError();
+}
+} // namespace quad
+
+bool quadrature_supported(int D, int N)
+{
+  if (D < 2 || D > 6 || N < 2 || N > 8)
+    return false;
+  // both LDS cell buffers must fit in 160 KB
+  const long long cell = (long long)std::pow(N, D);
+  return 2 * (cell + 1) * 8 + 2 * (cell / N) * 8 <= 160 * 1024;
+}
+
+hipError_t quadrature_apply(int D, int N, int mode, const double *src,
+                            double *dst, const QuadDev &q,
+                            const MeshDev &mesh, const StageDev &stage,
+                            double time, int dim_x, int collocation,
+                            hipStream_t stream)
+{
+#define HD_CASE(DD, NN_)                                                     \
+  if (D == DD && N == NN_)                                                   \
+    return quad::launch<DD, NN_>(mode, src, dst, q, mesh, stage, time, dim_x, \
+                                 collocation, stream);
+  HD_CASE(2, 2) HD_CASE(2, 3) HD_CASE(2, 4) HD_CASE(2, 5) HD_CASE(2, 6)
+  HD_CASE(3, 2) HD_CASE(3, 3) HD_CASE(3, 4) HD_CASE(3, 5) HD_CASE(3, 6)
+  HD_CASE(4, 2) HD_CASE(4, 3) HD_CASE(4, 4) HD_CASE(4, 5) HD_CASE(4, 6)
+  HD_CASE(5, 2) HD_CASE(5, 3) HD_CASE(5, 4)
+  HD_CASE(6, 2) HD_CASE(6, 3) HD_CASE(6, 4)
+#undef HD_CASE
+  return hipErrorInvalidValue;
+}
+} // namespace hd

@@ -1,0 +1,193 @@
+/*
+ * hyperdeal_amd.h -- C ABI of the MI355X (gfx950) phase-space DG advection
+ * operator: the drop-in boundary behind hyperdeal::MatrixFree's loops and
+ * hyperdeal::advection::AdvectionOperation.
+ *
+ * Conventions
+ *   - Status: every entry returns HD_OK (0) or a negative HD_ERR_* code; the
+ *     message of the last failure on the calling thread is hd_last_error().
+ *     No C++ exception crosses this boundary.  (The reference raises
+ *     dealii::ExceptionBase through AssertThrow, e.g.
+ *     base/time_integrators.templates.h:87-88.)
+ *   - Vectors are caller-owned device pointers (hipMalloc'd, or any HIP
+ *     device allocation) holding n_dofs values, cell-contiguous: cells are
+ *     numbered lexicographically over the D = dim_x + dim_v directions with
+ *     x0 fastest, each cell holds its (k+1)^D nodal values (Gauss-Lobatto
+ *     nodes) lexicographically with direction 0 fastest -- the in-cell order
+ *     of hyper.deal (i_x + (k+1)^dim_x i_v, matrix_free/
+ *     read_write_operation.h:143-174) and its cell order lid_x + lid_v n_x
+ *     (matrix_free/matrix_free.templates.h:552-562) for a subdivided box.
+ *   - stream: a hipStream_t passed as void*; NULL is the default stream.  All
+ *     work on one handle is stream ordered; a handle is not thread safe
+ *     (hyper.deal's evaluators are not re-entrant either,
+ *     matrix_free/fe_evaluation_base.h:93).
+ */
+#ifndef HYPERDEAL_AMD_H
+#define HYPERDEAL_AMD_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define HD_OK 0
+#define HD_ERR_INVALID -1     /* bad argument */
+#define HD_ERR_UNSUPPORTED -2 /* valid but not implemented for this shape */
+#define HD_ERR_HIP -3         /* HIP runtime failure */
+#define HD_ERR_NOMEM -4
+
+typedef enum hd_precision
+{
+  HD_FP64 = 0, /* vectors and arithmetic in double */
+  HD_FP32 = 1  /* operator vectors and arithmetic in float */
+} hd_precision;
+
+typedef enum hd_algorithm
+{
+  HD_ALGO_AUTO = 0,       /* collapsed where exact, else quadrature */
+  HD_ALGO_QUADRATURE = 1, /* the reference's sum-factorised quadrature steps */
+  HD_ALGO_COLLAPSED = 2   /* exact tensor factorisation (constant velocity,
+                             periodic Cartesian mesh) */
+} hd_algorithm;
+
+/* Boundary data of non-periodic directions (operators/advection/
+ * boundary_descriptor.h:31-115): u+ = -u- (homogeneous) or -u- + 2 g. */
+typedef enum hd_boundary
+{
+  HD_BC_HOMOGENEOUS = 0,
+  HD_BC_INHOMOGENEOUS = 1
+} hd_boundary;
+
+/* Analytic functions for initial data, Dirichlet data and errors. */
+typedef enum hd_function
+{
+  HD_FUNC_ZERO = 0,
+  HD_FUNC_HYPERRECTANGLE = 1, /* sin(2pi(x0-a0 t)) prod cos(2pi(xd-ad t)),
+                                 examples/advection/cases/hyperrectangle.h:33-55 */
+  HD_FUNC_CONSTANT_ONE = 2,
+  HD_FUNC_SIN_COS = 3,        /* sin(pi x0) prod cos(pi xd), tests_functions.h */
+  HD_FUNC_PRODUCT_SINES = 4   /* prod sin(2pi(xd - ad t)) */
+} hd_function;
+
+/* A Cartesian phase-space box: dim_x + dim_v directions (x first). */
+typedef struct hd_mesh_desc
+{
+  int dim_x, dim_v;   /* 1..3 each */
+  int n_cells[6];     /* cells per direction */
+  double lo[6], hi[6];
+  int periodic_x, periodic_v;
+} hd_mesh_desc;
+
+typedef struct hd_matrix_free hd_matrix_free;
+typedef struct hd_advection hd_advection;
+
+/* Thread-local message of the last failing call ("" if none). */
+const char *hd_last_error(void);
+
+/* Library version (major, minor). */
+int hd_version(int *major, int *minor);
+
+/*
+ * Phase-space matrix-free data: replaces hyperdeal::MatrixFree::reinit
+ * (matrix_free/matrix_free.templates.h:860-1235) together with the two
+ * dealii::MatrixFree objects and ShapeInfo::reinit (matrix_free/
+ * shape_info.h:109-231).  n_points is the 1D quadrature size (Gauss, or
+ * Gauss-Lobatto when collocation != 0, then n_points == degree + 1).
+ */
+int hd_matrix_free_create(const hd_mesh_desc *mesh, int degree, int n_points,
+                          int collocation, hd_matrix_free **out);
+int hd_matrix_free_destroy(hd_matrix_free *mf);
+/* n_dofs, n_cells, dofs per cell: MatrixFree::initialize_dof_vector sizes
+ * (matrix_free.templates.h:1367-1413). */
+long long hd_matrix_free_n_dofs(const hd_matrix_free *mf);
+long long hd_matrix_free_n_cells(const hd_matrix_free *mf);
+int hd_matrix_free_dofs_per_cell(const hd_matrix_free *mf);
+
+/*
+ * Advection operator: replaces AdvectionOperation::reinit
+ * (operators/advection/advection_operation.h:97-129) with a constant
+ * transport velocity (ConstantVelocityFieldView, velocity_field_view.h:
+ * 64-166), skew factor (advection_operation_parameters.h:29-42) and the
+ * boundary descriptor.  bc_params: the function's D parameters (its
+ * velocity), may be NULL.
+ */
+int hd_advection_create(hd_matrix_free *mf, const double *velocity,
+                        double skew_factor, int bc_kind, int bc_function,
+                        const double *bc_params, hd_advection **out);
+int hd_advection_destroy(hd_advection *op);
+/* Force an algorithm (HD_ALGO_*); HD_ERR_UNSUPPORTED if not applicable. */
+int hd_advection_set_algorithm(hd_advection *op, int algorithm);
+/* The algorithm apply() runs with the current settings and precision. */
+int hd_advection_algorithm(const hd_advection *op, int precision);
+
+/*
+ * dst = M^{-1} [ cell integrals + upwind face integrals ](src) at time t:
+ * replaces AdvectionOperation::apply (advection_operation.h:134-209), i.e.
+ * MatrixFree::loop_cell_centric with local_apply_advect_and_inverse_mass_
+ * matrix (matrix_free.templates.h:1494-1581, advection_operation.h:221-566).
+ * Periodic neighbours are read straight from src.  src and dst hold double
+ * (HD_FP64) or float (HD_FP32) values and must not overlap.
+ */
+int hd_advection_apply(hd_advection *op, const void *src, void *dst,
+                       int precision, double time, void *stream);
+
+/*
+ * One fused low-storage Runge-Kutta stage: K = apply(ti_in, time), then
+ *   sol += b_dt K;  ti_out = sol_old + a_dt K  (skipped when ti_out == NULL)
+ * -- the stage of LowStorageRungeKuttaIntegrator::perform_time_step
+ * (base/time_integrators.templates.h:103-138) fused into the operator sweep.
+ * sol is FP64; ti_in/ti_out are `precision`.  ti_out must differ from ti_in.
+ */
+int hd_lsrk_stage(hd_advection *op, int precision, const void *ti_in,
+                  void *ti_out, double *sol, double time, double a_dt,
+                  double b_dt, void *stream);
+
+/* Coefficients of rk33/rk45/rk47/rk59 (time_integrators.templates.h:32-84):
+ * a[n_stages-1], b[n_stages].  Returns the number of stages or an error. */
+int hd_lsrk_coefficients(const char *type, double *a, double *b);
+
+/*
+ * A whole LSRK time step (time_integrators.templates.h:91-184) with the
+ * stages fused into the apply: ti_a and ti_b are two `precision` registers
+ * (ping-pong, so neighbours are never read while written).
+ */
+int hd_lsrk_step(hd_advection *op, const char *type, int precision,
+                 double *sol, void *ti_a, void *ti_b, double time, double dt,
+                 void *stream);
+
+/* dst (FP64) = function at the Gauss-Lobatto nodes (numerics/
+ * vector_tools.h:81-135); params: D values or NULL. */
+int hd_interpolate(hd_matrix_free *mf, int function, const double *params,
+                   double time, double *dst, void *stream);
+
+/* dst += amplitude * U(-1, 1), counter-based SplitMix64 keyed by the DoF
+ * index: the seeded noise of the benchmark's initial data. */
+int hd_add_noise(hd_matrix_free *mf, double *dst, uint64_t seed,
+                 double amplitude, void *stream);
+
+/* out[0] = ||u||_L2, out[1] = ||u - f(t)||_L2 at the quadrature points
+ * (numerics/vector_tools.h:144-223); FP64 accumulation, synchronous. */
+int hd_norm_and_error(hd_matrix_free *mf, const double *src, int function,
+                      const double *params, double time, double *out,
+                      void *stream);
+
+/* dst (precision) = src (FP64), element-wise conversion / copy. */
+int hd_vector_convert(hd_matrix_free *mf, const double *src, void *dst,
+                      int precision, void *stream);
+
+/* CFL time step (operators/advection/cfl.h:94-124). */
+double hd_critical_time_step(const hd_matrix_free *mf, const double *velocity);
+
+/* Device memory helpers for hosts without another allocator. */
+int hd_device_malloc(void **ptr, uint64_t bytes);
+int hd_device_free(void *ptr);
+int hd_memcpy_htod(void *dst, const void *src, uint64_t bytes);
+int hd_memcpy_dtoh(void *dst, const void *src, uint64_t bytes);
+int hd_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif

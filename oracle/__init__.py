@@ -67,6 +67,10 @@ def load(path=None):
     lib.hdo_n_cells.argtypes = [ctypes.c_void_p]
     lib.hdo_apply.argtypes = [ctypes.c_void_p, _dp, _dp, ctypes.c_double,
                               ctypes.c_int]
+    lib.hdo_apply_cells.argtypes = [ctypes.c_void_p, _dp, _dp,
+                                    ctypes.c_double,
+                                    ctypes.POINTER(ctypes.c_longlong),
+                                    ctypes.c_int]
     lib.hdo_interpolate.argtypes = [ctypes.c_void_p, ctypes.c_double, _dp]
     lib.hdo_norm_and_error.argtypes = [ctypes.c_void_p, _dp, ctypes.c_double,
                                        _dp]
@@ -141,6 +145,17 @@ class Problem:
     def apply(self, src, t=0.0, fcl=False, dst=None):
         dst = self.vector() if dst is None else dst
         rc = self.lib.hdo_apply(self.h, _ptr(src), _ptr(dst), t, int(fcl))
+        assert rc == 0
+        return dst
+
+    def apply_cells(self, src, cell_ids, t=0.0, dst=None):
+        """ECL apply on the given cells only; returns dst (other cells
+        untouched)."""
+        dst = self.vector() if dst is None else dst
+        ids = np.ascontiguousarray(cell_ids, dtype=np.int64)
+        rc = self.lib.hdo_apply_cells(
+            self.h, _ptr(src), _ptr(dst), t,
+            ids.ctypes.data_as(ctypes.POINTER(ctypes.c_longlong)), len(ids))
         assert rc == 0
         return dst
 

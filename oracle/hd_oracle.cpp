@@ -691,14 +691,28 @@ inline double face_value(double um, double up, double an, double skew,
 
 // ECL: one macro cell of local_apply_advect_and_inverse_mass_matrix
 // (advection_operation.h:221-566).
+void ecl_cells(const hdo_problem &P, Work &w, const double *src, double *dst,
+               double t, const long long *cells, const bool *valid);
+
 void ecl_macro(const hdo_problem &P, Work &w, const double *src, double *dst,
                double t, long long m)
 {
-  const int D = P.D, nd = P.nd, nq = P.nq;
   long long cells[L];
   bool valid[L];
   int idx[L][6];
   macro_cells(P, m, cells, valid, idx);
+  ecl_cells(P, w, src, dst, t, cells, valid);
+}
+
+// ECL on the L cells given (lanes with valid == false are computed but not
+// written)
+void ecl_cells(const hdo_problem &P, Work &w, const double *src, double *dst,
+               double t, const long long *cells, const bool *valid)
+{
+  const int D = P.D, nd = P.nd, nq = P.nq;
+  int idx[L][6];
+  for (int l = 0; l < L; ++l)
+    P.cell_index(cells[l], idx[l]);
 
   // 1) read and interpolate to the quadrature points
   const size_t ndofs = (size_t)P.dofs_cell;
@@ -1143,6 +1157,35 @@ int hdo_apply(hdo_problem *p, const double *src, double *dst, double t,
     apply_fcl(*p, src, dst, t);
   else
     apply_ecl(*p, src, dst, t);
+  return 0;
+}
+
+int hdo_apply_cells(hdo_problem *p, const double *src, double *dst,
+                    double t, const long long *cell_ids, int n)
+{
+  if (!p || !src || !dst || !cell_ids || n < 0)
+    return -1;
+  for (int i = 0; i < n; ++i)
+    if (cell_ids[i] < 0 || cell_ids[i] >= p->n_cells)
+      return -1;
+  const int nm = (n + L - 1) / L;
+#pragma omp parallel
+  {
+    Work w(*p);
+#pragma omp for schedule(static)
+    for (int m = 0; m < nm; ++m)
+      {
+        long long cells[L];
+        bool valid[L];
+        for (int l = 0; l < L; ++l)
+          {
+            const int i = m * L + l;
+            valid[l] = i < n;
+            cells[l] = cell_ids[valid[l] ? i : n - 1];
+          }
+        ecl_cells(*p, w, src, dst, t, cells, valid);
+      }
+  }
   return 0;
 }
 

@@ -66,6 +66,10 @@ long long hdo_n_cells(const hdo_problem *p);
 int hdo_apply(hdo_problem *p, const double *src, double *dst, double t,
               int use_fcl);
 
+/* ECL apply on the listed cells only (dst written for those cells). */
+int hdo_apply_cells(hdo_problem *p, const double *src, double *dst, double t,
+                    const long long *cell_ids, int n);
+
 /* Interpolate the problem's function (func_id) at time t into dst. */
 int hdo_interpolate(hdo_problem *p, double t, double *dst);
 
