@@ -330,7 +330,10 @@ def test_full_size_spot_check_against_oracle(c2, precision):
         out = torch.empty_like(u32)
         op.apply(out, u32, precision=H.FP32)
         host_u = u32.double().cpu().numpy()
-        tol = 2e-6
+        # FP32 arithmetic: each output sums ~30 products of size up to
+        # |a|/h * max|A| * |u| (~8 * 20 here) into O(1) results, so float
+        # rounding (6e-8) is amplified by that cancellation
+        tol = 1e-5
         del u
     else:
         out = torch.empty_like(u)
