@@ -96,6 +96,21 @@ def load(path=None):
                                          ctypes.c_ulonglong]
     lib.hdo_add_noise.argtypes = [ctypes.c_void_p, _dp, ctypes.c_ulonglong,
                                   ctypes.c_double]
+    lib.hdo_set_vlasov_poisson.argtypes = [ctypes.c_void_p, ctypes.c_int]
+    lib.hdo_x_n_dofs.restype = ctypes.c_longlong
+    lib.hdo_x_n_dofs.argtypes = [ctypes.c_void_p]
+    for f in ("hdo_vp_density", "hdo_vp_rhs", "hdo_poisson_apply",
+              "hdo_vp_diagnostics"):
+        getattr(lib, f).argtypes = [ctypes.c_void_p, _dp, _dp]
+    lib.hdo_poisson_solve.argtypes = [ctypes.c_void_p, _dp, _dp,
+                                      ctypes.c_double, ctypes.c_int]
+    lib.hdo_vp_set_potential.argtypes = [ctypes.c_void_p, _dp]
+    lib.hdo_vp_efield.argtypes = [ctypes.c_void_p, _dp]
+    lib.hdo_vp_energy.argtypes = [ctypes.c_void_p, _dp]
+    lib.hdo_run_vlasov_poisson.argtypes = [
+        ctypes.c_void_p, ctypes.c_char_p, ctypes.c_double, ctypes.c_double,
+        ctypes.c_double, ctypes.c_double, ctypes.c_double, ctypes.c_double,
+        ctypes.c_int, _dp, ctypes.c_int, _dp, ctypes.POINTER(ctypes.c_int)]
     lib.hdo_num_threads.restype = ctypes.c_int
     if path is None:
         _LIB = lib
@@ -191,6 +206,79 @@ class Problem:
                                        max_rows, _ptr(dt))
         assert n >= 0
         return rows[:n].copy(), float(dt[0])
+
+
+class VlasovPoisson(Problem):
+    """The Vlasov-Poisson problem of examples/vlasov_poisson on the oracle:
+    x-velocity v, v-velocity grad phi from the periodic SIPG Poisson solve.
+    `transport` is the case's direction used only for the CFL time step."""
+
+    def __init__(self, dim, degree, n_cells, lo, hi, transport, eps=0.01,
+                 kx=0.5, n_perturbed=1, **kw):
+        super().__init__(dim, dim, degree, n_cells, lo, hi, transport,
+                         func=FUNC_LANDAU, func_params=[eps, kx, n_perturbed],
+                         **kw)
+        assert self.lib.hdo_set_vlasov_poisson(self.h, 1) == 0
+        self.n_x_dofs = self.lib.hdo_x_n_dofs(self.h)
+
+    def xvector(self):
+        return np.zeros(self.n_x_dofs)
+
+    def density(self, f):
+        rho = self.xvector()
+        self.lib.hdo_vp_density(self.h, _ptr(f), _ptr(rho))
+        return rho
+
+    def rhs(self, rho):
+        rho = rho.copy()
+        b = self.xvector()
+        self.lib.hdo_vp_rhs(self.h, _ptr(rho), _ptr(b))
+        return b
+
+    def poisson_apply(self, x):
+        y = self.xvector()
+        self.lib.hdo_poisson_apply(self.h, _ptr(x), _ptr(y))
+        return y
+
+    def poisson_solve(self, b, phi=None, rel_tol=1e-12, max_it=100000):
+        phi = self.xvector() if phi is None else phi
+        it = self.lib.hdo_poisson_solve(self.h, _ptr(b), _ptr(phi), rel_tol,
+                                        max_it)
+        return phi, it
+
+    def set_potential(self, phi):
+        assert self.lib.hdo_vp_set_potential(self.h, _ptr(phi)) == 0
+
+    def efield(self):
+        nqx = self.n_points ** self.dim_x
+        ncx = int(np.prod(self.n_cells[:self.dim_x]))
+        out = np.zeros(ncx * nqx * self.dim_x)
+        self.lib.hdo_vp_efield(self.h, _ptr(out))
+        return out
+
+    def energy(self):
+        out = np.zeros(self.dim_x)
+        self.lib.hdo_vp_energy(self.h, _ptr(out))
+        return out
+
+    def diagnostics(self, f):
+        out = np.zeros(6)
+        self.lib.hdo_vp_diagnostics(self.h, _ptr(f), _ptr(out))
+        return out
+
+    def run(self, rk="rk45", time_step=0.1, start_time=0.0, final_time=0.5,
+            cfl=0.05, diag_tick=0.1, poisson_tol=1e-12,
+            max_steps=100000000, max_rows=10000):
+        width = 1 + self.dim_x + 6
+        rows = np.zeros((max_rows, width))
+        dt = np.zeros(1)
+        it = ctypes.c_int(0)
+        n = self.lib.hdo_run_vlasov_poisson(
+            self.h, rk.encode(), time_step, start_time, final_time, cfl,
+            diag_tick, poisson_tol, max_steps, _ptr(rows), max_rows,
+            _ptr(dt), ctypes.byref(it))
+        assert n >= 0
+        return rows[:n].copy(), float(dt[0]), it.value
 
 
 def tables(degree, n_points=None, collocation=False):

@@ -316,12 +316,14 @@ double eval_function(int id, const std::vector<double> &params, int dim_x,
         }
       case HDO_FUNC_LANDAU:
         {
-          // params: [eps, kx]; x-perturbation over all x directions, then a
-          // Maxwellian in v
+          // params: [eps, kx, np]: 1 + eps prod_{d < np} cos(kx x_d) times
+          // a Maxwellian in v (np = 1 is examples/vlasov_poisson/cases/
+          // hyperrectangle.h:44-58)
           const double eps = params.size() > 0 ? params[0] : 0.01;
           const double kx = params.size() > 1 ? params[1] : 0.5;
+          const int np = params.size() > 2 ? (int)params[2] : dim_x;
           double px = 1.0;
-          for (int d = 0; d < dim_x; ++d)
+          for (int d = 0; d < np; ++d)
             px *= std::cos(kx * p[d]);
           double v2 = 0.0;
           for (int d = dim_x; d < D; ++d)
@@ -486,6 +488,13 @@ struct hdo_problem
   int func_id = HDO_FUNC_ZERO;
   std::vector<double> func_params;
 
+  // Vlasov-Poisson: x-velocity = v at the v quadrature point, v-velocity =
+  // grad phi at the x quadrature point (examples/vlasov_poisson/include/
+  // velocity_field_view.h:111-175)
+  bool vp = false;
+  std::vector<double> efield; // n_x_cells x nq^dim_x x dim_x
+  std::vector<int> qidx;      // nq^D x D multi-indices of quadrature points
+
   Tables1D t;       // tables of the operator quadrature
   Tables1D tg;      // Gauss(k+1) tables (FCL inverse mass, quad no 1)
   std::vector<double> jxw;     // nq^D
@@ -643,7 +652,32 @@ void scatter_add_face_nodes(const hdo_problem &P, double *dst,
 // skew-symmetric part (advection_operation.h:318-403 / 713-787):
 //   data = -skew (a . grad u) JxW + sum_d D_d^T [ (1-skew) u a_d / h_d JxW ]
 // buf holds u at the quadrature points; tmp/tmp2 are scratch.
-void cell_term(const hdo_problem &P, Work &w)
+// Transport velocity of direction d at quadrature point qi of the cell idx
+// (constant, or the phase-space field of the Vlasov-Poisson problem; for a
+// face point qi[d] is not used).
+inline double velocity(const hdo_problem &P, int d, const int *idx,
+                       const int *qi)
+{
+  if (!P.vp)
+    return P.a[d];
+  if (d < P.dim_x)
+    {
+      const int e = P.dim_x + d;
+      return P.lo[e] + (idx[e] + P.t.xq[qi[e]]) * P.h[e];
+    }
+  const int e = d - P.dim_x;
+  long long xc = 0;
+  int qx = 0;
+  for (int j = P.dim_x - 1; j >= 0; --j)
+    {
+      xc = xc * P.n[j] + idx[j];
+      qx = qx * P.nq + qi[j];
+    }
+  const int nqx = ipow(P.nq, P.dim_x);
+  return P.efield[((size_t)xc * nqx + qx) * P.dim_x + e];
+}
+
+void cell_term(const hdo_problem &P, Work &w, const int (*idx)[6] = nullptr)
 {
   const int D = P.D, nq = P.nq;
   const size_t nvol = (size_t)ipow(nq, D);
@@ -651,6 +685,30 @@ void cell_term(const hdo_problem &P, Work &w)
   for (int j = 0; j < D; ++j)
     ext[j] = nq;
   std::fill(w.data.begin(), w.data.begin() + nvol * L, 0.0);
+  if (P.vp)
+    {
+      // per-point velocity: -skew (a . grad u) JxW + D_d^T[(1-skew) u a_d
+      // / h_d JxW] with a = a(x, v)
+      for (int d = 0; d < D; ++d)
+        {
+          if (P.skew != 0.0)
+            contract(D, ext, d, nq, P.t.D.data(), nq, 1, w.buf.data(),
+                     w.tmp2.data(), false);
+          for (size_t q = 0; q < nvol; ++q)
+            for (int l = 0; l < L; ++l)
+              {
+                const double vel =
+                  velocity(P, d, idx[l], &P.qidx[q * D]) / P.h[d] * P.jxw[q];
+                if (P.skew != 0.0)
+                  w.data[q * L + l] += -P.skew * w.tmp2[q * L + l] * vel;
+                w.tmp[q * L + l] = (1.0 - P.skew) * w.buf[q * L + l] * vel;
+              }
+          if (P.skew != 1.0)
+            contract(D, ext, d, nq, P.t.D.data(), 1, nq, w.tmp.data(),
+                     w.data.data(), true);
+        }
+      return;
+    }
   if (P.skew != 0.0)
     for (int d = 0; d < D; ++d)
       {
@@ -730,7 +788,7 @@ void ecl_cells(const hdo_problem &P, Work &w, const double *src, double *dst,
   std::copy(uq, uq + nvol * L, w.buf.begin());
 
   // 2) cell integral
-  cell_term(P, w);
+  cell_term(P, w, idx);
 
   // 3) faces
   const int nfq = ipow(nq, D - 1);
@@ -791,11 +849,32 @@ void ecl_cells(const hdo_problem &P, Work &w, const double *src, double *dst,
               up[q * L + l] = -w.fm[q * L + l] + 2.0 * g;
             }
 
-      const double an = P.a[d] * nrm;
-      for (int q = 0; q < nfq; ++q)
-        for (int l = 0; l < L; ++l)
-          w.fd[q * L + l] = face_value(w.fm[q * L + l], up[q * L + l], an,
-                                       P.skew, P.jxwf[d][q]);
+      if (!P.vp)
+        {
+          const double an = P.a[d] * nrm;
+          for (int q = 0; q < nfq; ++q)
+            for (int l = 0; l < L; ++l)
+              w.fd[q * L + l] = face_value(w.fm[q * L + l], up[q * L + l],
+                                           an, P.skew, P.jxwf[d][q]);
+        }
+      else
+        for (int q = 0; q < nfq; ++q)
+          {
+            int qi[6] = {0, 0, 0, 0, 0, 0};
+            int rem = q;
+            for (int j = 0; j < D; ++j)
+              if (j != d)
+                {
+                  qi[j] = rem % nq;
+                  rem /= nq;
+                }
+            for (int l = 0; l < L; ++l)
+              {
+                const double an = velocity(P, d, idx[l], qi) * nrm;
+                w.fd[q * L + l] = face_value(w.fm[q * L + l], up[q * L + l],
+                                             an, P.skew, P.jxwf[d][q]);
+              }
+          }
       // lift: data(.., p, ..) += F[side][p] * fd  (A(r, 0) = F[side][r])
       fext[d] = 1;
       contract(D, fext, d, nq, P.t.F[side].data(), 1, 0, w.fd.data(),
@@ -1053,6 +1132,411 @@ static void apply_fcl(const hdo_problem &P, const double *src,
 }
 
 // ---------------------------------------------------------------------------
+// Vlasov-Poisson pieces (examples/vlasov_poisson)
+// ---------------------------------------------------------------------------
+namespace
+{
+struct XSpace
+{
+  int dx, nd, nq;
+  long long n_cells, n_dofs;
+  int dofs_cell;
+  int n[3];
+  double h[3];
+  // exact 1D matrices of the nodal (Gauss-Lobatto) basis on [0,1]
+  std::vector<double> M, K;      // nd x nd: int l_i l_j, int l_i' l_j'
+  std::vector<double> d0, d1;    // l_i'(0), l_i'(1)
+  std::vector<double> S, G, w;   // nq x nd values / derivatives at Gauss
+                                 // points of the operator rule, weights
+};
+
+XSpace make_xspace(const hdo_problem &P)
+{
+  XSpace X;
+  X.dx = P.dim_x;
+  X.nd = P.nd;
+  X.nq = P.nq;
+  X.n_cells = 1;
+  for (int d = 0; d < X.dx; ++d)
+    {
+      X.n[d] = P.n[d];
+      X.h[d] = P.h[d];
+      X.n_cells *= P.n[d];
+    }
+  X.dofs_cell = ipow(X.nd, X.dx);
+  X.n_dofs = X.n_cells * X.dofs_cell;
+  const auto &xd = P.t.xd;
+  // exact mass / stiffness with Gauss(k+1)
+  const auto &g = P.tg;
+  X.M.assign(X.nd * X.nd, 0.0);
+  X.K.assign(X.nd * X.nd, 0.0);
+  for (int q = 0; q < g.nq; ++q)
+    for (int i = 0; i < X.nd; ++i)
+      for (int j = 0; j < X.nd; ++j)
+        {
+          const long double li = lagrange_value(xd, i, g.xq[q]);
+          const long double lj = lagrange_value(xd, j, g.xq[q]);
+          const long double di = lagrange_derivative(xd, i, g.xq[q]);
+          const long double dj = lagrange_derivative(xd, j, g.xq[q]);
+          X.M[i * X.nd + j] += (double)(g.wq[q] * li * lj);
+          X.K[i * X.nd + j] += (double)(g.wq[q] * di * dj);
+        }
+  X.d0.resize(X.nd);
+  X.d1.resize(X.nd);
+  for (int i = 0; i < X.nd; ++i)
+    {
+      X.d0[i] = (double)lagrange_derivative(xd, i, 0.0L);
+      X.d1[i] = (double)lagrange_derivative(xd, i, 1.0L);
+    }
+  X.S.assign(X.nq * X.nd, 0.0);
+  X.G.assign(X.nq * X.nd, 0.0);
+  X.w = P.t.wq;
+  for (int q = 0; q < X.nq; ++q)
+    for (int i = 0; i < X.nd; ++i)
+      {
+        X.S[q * X.nd + i] = (double)lagrange_value(xd, i, P.t.xq[q]);
+        X.G[q * X.nd + i] = (double)lagrange_derivative(xd, i, P.t.xq[q]);
+      }
+  return X;
+}
+
+// y = (A_1 (x) ... (x) A_dx) x on one cell, A_j (m x nd) row-major
+void tensor_apply(int dx, int nd, const std::vector<const double *> &A,
+                  const std::vector<int> &m, const double *x, double *y)
+{
+  std::vector<double> cur(x, x + ipow(nd, dx)), nxt;
+  std::vector<int> ext(dx, nd);
+  for (int d = 0; d < dx; ++d)
+    {
+      int pre = 1, post = 1;
+      for (int j = 0; j < d; ++j)
+        pre *= ext[j];
+      for (int j = d + 1; j < dx; ++j)
+        post *= ext[j];
+      nxt.assign((size_t)pre * m[d] * post, 0.0);
+      for (int po = 0; po < post; ++po)
+        for (int r = 0; r < m[d]; ++r)
+          for (int c = 0; c < ext[d]; ++c)
+            {
+              const double a = A[d][r * ext[d] + c];
+              for (int pi = 0; pi < pre; ++pi)
+                nxt[pi + pre * (r + m[d] * po)] +=
+                  a * cur[pi + pre * (c + ext[d] * po)];
+            }
+      ext[d] = m[d];
+      cur.swap(nxt);
+    }
+  std::copy(cur.begin(), cur.end(), y);
+}
+
+void x_cell_index(const XSpace &X, long long c, int *idx)
+{
+  for (int d = 0; d < X.dx; ++d)
+    {
+      idx[d] = (int)(c % X.n[d]);
+      c /= X.n[d];
+    }
+}
+long long x_cell_id(const XSpace &X, const int *idx)
+{
+  long long c = 0;
+  for (int d = X.dx - 1; d >= 0; --d)
+    c = c * X.n[d] + idx[d];
+  return c;
+}
+
+// Periodic SIPG Laplacian (examples/vlasov_poisson/include/poisson.h:100-254):
+//   a(u, v) = sum_K (grad u, grad v) + sum_F ( sigma [u][v] - {d_n u}[v]
+//             - [u]{d_n v} ),  sigma = k(k+1) (1/h- + 1/h+)
+// with every integral exact (the reference's Gauss(k+1) rule is exact for
+// these integrands), applied to src minus its mean (vmult, :100-123).
+void sipg_apply(const XSpace &X, int degree, const double *src_in, double *dst)
+{
+  const long long nx = X.n_dofs;
+  const int nd = X.nd, dc = X.dofs_cell, dx = X.dx;
+  double mean = 0.0;
+  for (long long i = 0; i < nx; ++i)
+    mean += src_in[i];
+  mean /= (double)nx;
+  std::vector<double> src(src_in, src_in + nx);
+  for (auto &v : src)
+    v -= mean;
+  std::fill(dst, dst + nx, 0.0);
+  const double kk = std::max(degree, 1) * (degree + 1.0);
+  // scaled 1D matrices per direction
+  std::vector<std::vector<double>> Mh(dx), Kh(dx);
+  for (int d = 0; d < dx; ++d)
+    {
+      Mh[d] = X.M;
+      Kh[d] = X.K;
+      for (auto &v : Mh[d])
+        v *= X.h[d];
+      for (auto &v : Kh[d])
+        v /= X.h[d];
+    }
+  std::vector<double> tmp(dc);
+  for (long long c = 0; c < X.n_cells; ++c)
+    {
+      // cell term: sum_d (x)_{j != d} h_j Mhat (x) Khat / h_d
+      for (int d = 0; d < dx; ++d)
+        {
+          std::vector<const double *> A(dx);
+          std::vector<int> m(dx, nd);
+          for (int j = 0; j < dx; ++j)
+            A[j] = j == d ? Kh[j].data() : Mh[j].data();
+          tensor_apply(dx, nd, A, m, src.data() + c * dc, tmp.data());
+          for (int i = 0; i < dc; ++i)
+            dst[c * dc + i] += tmp[i];
+        }
+    }
+  // faces: between cell L and its right neighbour R in direction d
+  const int nf = ipow(nd, dx - 1);
+  for (int d = 0; d < dx; ++d)
+    {
+      const double h = X.h[d];
+      const double sigma = kk * (1.0 / h + 1.0 / h);
+      const int sd = ipow(nd, d);
+      for (long long c = 0; c < X.n_cells; ++c)
+        {
+          int idx[3];
+          x_cell_index(X, c, idx);
+          int jr[3] = {idx[0], idx[1], idx[2]};
+          jr[d] = (idx[d] + 1) % X.n[d];
+          const long long r = x_cell_id(X, jr);
+          // 1D face quantities for every tangential node, then the
+          // tangential mass
+          std::vector<double> fl(nf * nd, 0.0), fr(nf * nd, 0.0);
+          for (int f = 0; f < nf; ++f)
+            {
+              int rem = f, base = 0, pw = 1;
+              for (int j = 0; j < dx; ++j)
+                {
+                  if (j != d)
+                    {
+                      base += (rem % nd) * pw;
+                      rem /= nd;
+                    }
+                  pw *= nd;
+                }
+              const double *uL = src.data() + c * dc + base;
+              const double *uR = src.data() + r * dc + base;
+              const double um = uL[(nd - 1) * sd], upl = uR[0];
+              double dm = 0.0, dp = 0.0;
+              for (int i = 0; i < nd; ++i)
+                {
+                  dm += X.d1[i] * uL[i * sd] / h;
+                  dp += X.d0[i] * uR[i * sd] / h;
+                }
+              const double jump = um - upl, avgd = 0.5 * (dm + dp);
+              const double val = sigma * jump - avgd;
+              for (int i = 0; i < nd; ++i)
+                {
+                  const double eL = (i == nd - 1) ? 1.0 : 0.0;
+                  const double eR = (i == 0) ? 1.0 : 0.0;
+                  fl[f * nd + i] = val * eL - 0.5 * jump * X.d1[i] / h;
+                  fr[f * nd + i] = -val * eR - 0.5 * jump * X.d0[i] / h;
+                }
+            }
+          // tangential mass on the face arrays (face index f over j != d,
+          // normal index i), scatter into the cells
+          for (int side = 0; side < 2; ++side)
+            {
+              const std::vector<double> &fa = side ? fr : fl;
+              std::vector<double> out(nf * nd, 0.0);
+              for (int i = 0; i < nd; ++i)
+                {
+                  std::vector<double> col(nf), res(nf);
+                  for (int f = 0; f < nf; ++f)
+                    col[f] = fa[f * nd + i];
+                  if (dx == 1)
+                    res = col;
+                  else
+                    {
+                      std::vector<const double *> A;
+                      std::vector<int> m;
+                      for (int j = 0; j < dx; ++j)
+                        if (j != d)
+                          {
+                            A.push_back(Mh[j].data());
+                            m.push_back(nd);
+                          }
+                      tensor_apply(dx - 1, nd, A, m, col.data(), res.data());
+                    }
+                  for (int f = 0; f < nf; ++f)
+                    out[f * nd + i] = res[f];
+                }
+              const long long cell = side ? r : c;
+              for (int f = 0; f < nf; ++f)
+                {
+                  int rem = f, base = 0, pw = 1;
+                  for (int j = 0; j < dx; ++j)
+                    {
+                      if (j != d)
+                        {
+                          base += (rem % nd) * pw;
+                          rem /= nd;
+                        }
+                      pw *= nd;
+                    }
+                  for (int i = 0; i < nd; ++i)
+                    dst[cell * dc + base + i * sd] += out[f * nd + i];
+                }
+            }
+        }
+    }
+}
+
+// rho(x node) = sum over v cells and v nodes of f JxW_v at the Gauss-Lobatto
+// points (numerics/vector_tools.h:230-315)
+void vp_density(const hdo_problem &P, const XSpace &X, const double *f,
+                double *rho)
+{
+  std::fill(rho, rho + X.n_dofs, 0.0);
+  const int dv = P.dim_v, nd = P.nd;
+  const int dcx = X.dofs_cell, dcv = ipow(nd, dv);
+  std::vector<double> wv(dcv);
+  for (int q = 0; q < dcv; ++q)
+    {
+      double v = 1.0;
+      int rem = q;
+      for (int j = 0; j < dv; ++j)
+        {
+          v *= P.h[P.dim_x + j] * P.t.wd[rem % nd];
+          rem /= nd;
+        }
+      wv[q] = v;
+    }
+  for (long long c = 0; c < P.n_cells; ++c)
+    {
+      const long long xc = c % X.n_cells;
+      for (int qv = 0; qv < dcv; ++qv)
+        for (int qx = 0; qx < dcx; ++qx)
+          rho[xc * dcx + qx] += f[c * P.dofs_cell + qx + dcx * qv] * wv[qv];
+    }
+}
+
+// b = -M (rho - mean), then b -= mean (vp/application.h:531-568)
+void vp_rhs(const XSpace &X, double *rho, double *b)
+{
+  const long long nx = X.n_dofs;
+  double mean = 0.0;
+  for (long long i = 0; i < nx; ++i)
+    mean += rho[i];
+  mean /= (double)nx;
+  for (long long i = 0; i < nx; ++i)
+    rho[i] -= mean;
+  std::vector<std::vector<double>> Mh(X.dx);
+  std::vector<const double *> A(X.dx);
+  std::vector<int> m(X.dx, X.nd);
+  for (int d = 0; d < X.dx; ++d)
+    {
+      Mh[d] = X.M;
+      for (auto &v : Mh[d])
+        v *= X.h[d];
+      A[d] = Mh[d].data();
+    }
+  for (long long c = 0; c < X.n_cells; ++c)
+    tensor_apply(X.dx, X.nd, A, m, rho + c * X.dofs_cell, b + c * X.dofs_cell);
+  for (long long i = 0; i < nx; ++i)
+    b[i] = -b[i];
+  mean = 0.0;
+  for (long long i = 0; i < nx; ++i)
+    mean += b[i];
+  mean /= (double)nx;
+  for (long long i = 0; i < nx; ++i)
+    b[i] -= mean;
+}
+
+// conjugate gradients on the SIPG operator, warm-started from phi, until
+// the relative residual ||b - A phi|| <= rel_tol ||b||; returns the
+// iteration count
+int poisson_cg(const XSpace &X, int degree, const double *b, double *phi,
+               double rel_tol, int max_it)
+{
+  const long long n = X.n_dofs;
+  std::vector<double> r(n), p(n), Ap(n);
+  sipg_apply(X, degree, phi, Ap.data());
+  double rr = 0.0, bb = 0.0;
+  for (long long i = 0; i < n; ++i)
+    {
+      r[i] = b[i] - Ap[i];
+      p[i] = r[i];
+      rr += r[i] * r[i];
+      bb += b[i] * b[i];
+    }
+  const double stop = rel_tol * std::sqrt(bb);
+  int it = 0;
+  while (std::sqrt(rr) > stop && it < max_it)
+    {
+      sipg_apply(X, degree, p.data(), Ap.data());
+      double pAp = 0.0;
+      for (long long i = 0; i < n; ++i)
+        pAp += p[i] * Ap[i];
+      if (!(pAp > 0.0))
+        break; // breakdown: the residual left lies in the null space
+      const double alpha = rr / pAp;
+      double rr2 = 0.0;
+      for (long long i = 0; i < n; ++i)
+        {
+          phi[i] += alpha * p[i];
+          r[i] -= alpha * Ap[i];
+          rr2 += r[i] * r[i];
+        }
+      const double beta = rr2 / rr;
+      rr = rr2;
+      for (long long i = 0; i < n; ++i)
+        p[i] = r[i] + beta * p[i];
+      ++it;
+    }
+  return it;
+}
+
+// grad phi at the x quadrature points (derivative_container.h:158-251)
+void vp_set_field(hdo_problem &P, const XSpace &X, const double *phi)
+{
+  const int dx = X.dx, nd = X.nd, nq = X.nq;
+  const int nqx = ipow(nq, dx);
+  P.efield.assign((size_t)X.n_cells * nqx * dx, 0.0);
+  std::vector<double> g(nqx);
+  for (long long c = 0; c < X.n_cells; ++c)
+    for (int e = 0; e < dx; ++e)
+      {
+        std::vector<const double *> A(dx);
+        std::vector<int> m(dx, nq);
+        for (int j = 0; j < dx; ++j)
+          A[j] = j == e ? X.G.data() : X.S.data();
+        tensor_apply(dx, nd, A, m, phi + c * X.dofs_cell, g.data());
+        for (int q = 0; q < nqx; ++q)
+          P.efield[((size_t)c * nqx + q) * dx + e] = g[q] / X.h[e];
+      }
+}
+
+// sum_K sum_q (d_e phi)^2 JxW per component (diagnostics.h:95-141)
+void vp_energy(const hdo_problem &P, const XSpace &X, double *out)
+{
+  const int dx = X.dx, nq = X.nq, nqx = ipow(nq, dx);
+  for (int e = 0; e < dx; ++e)
+    out[e] = 0.0;
+  for (long long c = 0; c < X.n_cells; ++c)
+    for (int q = 0; q < nqx; ++q)
+      {
+        double jxw = 1.0;
+        int rem = q;
+        for (int j = 0; j < dx; ++j)
+          {
+            jxw *= X.h[j] * X.w[rem % nq];
+            rem /= nq;
+          }
+        for (int e = 0; e < dx; ++e)
+          {
+            const double g = P.efield[((size_t)c * nqx + q) * dx + e];
+            out[e] += g * g * jxw;
+          }
+      }
+}
+} // namespace
+
+// ---------------------------------------------------------------------------
 // C interface
 // ---------------------------------------------------------------------------
 extern "C" {
@@ -1091,7 +1575,7 @@ hdo_problem *hdo_create(int dim_x, int dim_v, int degree, int n_points,
   P->func_params.assign(func_params ? func_params : P->a,
                         (func_params ? func_params : P->a) + P->D);
   if (func_id == HDO_FUNC_LANDAU && func_params)
-    P->func_params.assign(func_params, func_params + 2);
+    P->func_params.assign(func_params, func_params + 3);
   P->dofs_cell = ipow(P->nd, P->D);
   P->t.build(degree, n_points, P->collocation);
   P->tg.build(degree, degree + 1, false);
@@ -1124,6 +1608,16 @@ hdo_problem *hdo_create(int dim_x, int dim_v, int degree, int n_points,
               rem /= nq;
             }
           P->jxwf[d][q] = v;
+        }
+    }
+  P->qidx.assign((size_t)ipow(nq, D) * D, 0);
+  for (int q = 0; q < ipow(nq, D); ++q)
+    {
+      int rem = q;
+      for (int j = 0; j < D; ++j)
+        {
+          P->qidx[(size_t)q * D + j] = rem % nq;
+          rem /= nq;
         }
     }
   const Tables1D &tg = P->collocation ? P->t : P->tg;
@@ -1425,6 +1919,201 @@ int hdo_add_noise(const hdo_problem *p, double *dst, unsigned long long seed,
   for (long long i = 0; i < n; ++i)
     dst[i] += amplitude * hdo_splitmix_uniform(seed, (unsigned long long)i);
   return 0;
+}
+
+int hdo_set_vlasov_poisson(hdo_problem *p, int enable)
+{
+  if (!p || (enable && p->dim_x != p->dim_v))
+    return -1; // the reference's v-integration assumes dim_x == dim_v
+  p->vp = enable != 0;
+  if (p->vp)
+    {
+      XSpace X = make_xspace(*p);
+      p->efield.assign((size_t)X.n_cells * ipow(p->nq, p->dim_x) * p->dim_x,
+                       0.0);
+    }
+  return 0;
+}
+
+long long hdo_x_n_dofs(const hdo_problem *p)
+{
+  long long n = 1;
+  for (int d = 0; d < p->dim_x; ++d)
+    n *= p->n[d];
+  return n * ipow(p->nd, p->dim_x);
+}
+
+int hdo_vp_density(hdo_problem *p, const double *f, double *rho)
+{
+  if (!p || !f || !rho)
+    return -1;
+  vp_density(*p, make_xspace(*p), f, rho);
+  return 0;
+}
+
+int hdo_vp_rhs(hdo_problem *p, double *rho, double *b)
+{
+  if (!p || !rho || !b)
+    return -1;
+  vp_rhs(make_xspace(*p), rho, b);
+  return 0;
+}
+
+int hdo_poisson_apply(hdo_problem *p, const double *src, double *dst)
+{
+  if (!p || !src || !dst)
+    return -1;
+  sipg_apply(make_xspace(*p), p->degree, src, dst);
+  return 0;
+}
+
+int hdo_poisson_solve(hdo_problem *p, const double *b, double *phi,
+                      double rel_tol, int max_it)
+{
+  if (!p || !b || !phi)
+    return -1;
+  return poisson_cg(make_xspace(*p), p->degree, b, phi, rel_tol, max_it);
+}
+
+int hdo_vp_set_potential(hdo_problem *p, const double *phi)
+{
+  if (!p || !phi || !p->vp)
+    return -1;
+  vp_set_field(*p, make_xspace(*p), phi);
+  return 0;
+}
+
+int hdo_vp_efield(const hdo_problem *p, double *out)
+{
+  if (!p || !out || !p->vp)
+    return -1;
+  std::copy(p->efield.begin(), p->efield.end(), out);
+  return 0;
+}
+
+int hdo_vp_energy(hdo_problem *p, double *out)
+{
+  if (!p || !out || !p->vp)
+    return -1;
+  vp_energy(*p, make_xspace(*p), out);
+  return 0;
+}
+
+int hdo_vp_diagnostics(hdo_problem *p, const double *f, double *out)
+{
+  // mass, l2 norm, kinetic energy, momentum (3) at the quadrature points
+  // (examples/vlasov_poisson/include/diagnostics.h:28-87)
+  if (!p || !f || !out)
+    return -1;
+  const hdo_problem &P = *p;
+  const int D = P.D, nd = P.nd, nq = P.nq, dv = P.dim_v;
+  const long long nm = n_macro(P);
+  double acc[6] = {0, 0, 0, 0, 0, 0};
+  for (long long m = 0; m < nm; ++m)
+    {
+      Work w(P);
+      long long cells[L];
+      bool valid[L];
+      int idx[L][6];
+      macro_cells(P, m, cells, valid, idx);
+      for (long long i = 0; i < P.dofs_cell; ++i)
+        for (int l = 0; l < L; ++l)
+          w.u[i * L + l] = f[cells[l] * P.dofs_cell + i];
+      int ext[6];
+      for (int j = 0; j < D; ++j)
+        ext[j] = nd;
+      double *uq = w.u.data();
+      if (!P.collocation)
+        uq = contract_dirs(P, ext, -1, nq, P.t.S.data(), nd, 1, w.u.data(),
+                           w.uq.data(), false);
+      const int nvol = ipow(nq, D);
+      for (int q = 0; q < nvol; ++q)
+        for (int l = 0; l < L; ++l)
+          {
+            if (!valid[l])
+              continue;
+            const double fv = uq[q * L + l], jxw = P.jxw[q];
+            double v2 = 0.0, vd[3] = {0, 0, 0};
+            for (int e = 0; e < dv; ++e)
+              {
+                const int j = P.dim_x + e;
+                vd[e] = P.lo[j] +
+                        (idx[l][j] + P.t.xq[P.qidx[(size_t)q * D + j]]) *
+                          P.h[j];
+                v2 += vd[e] * vd[e];
+              }
+            acc[0] += fv * jxw;
+            acc[1] += fv * fv * jxw;
+            acc[2] += v2 * fv * jxw;
+            for (int e = 0; e < dv; ++e)
+              acc[3 + e] += vd[e] * fv * jxw;
+          }
+    }
+  acc[1] = std::sqrt(acc[1]);
+  std::copy(acc, acc + 6, out);
+  return 0;
+}
+
+int hdo_run_vlasov_poisson(hdo_problem *p, const char *rk_type,
+                           double time_step, double start_time,
+                           double final_time, double cfl, double diag_tick,
+                           double poisson_tol, int max_steps, double *rows,
+                           int max_rows, double *dt_used, int *cg_iterations)
+{
+  // vp/application.h:470-660: interpolate f0, dt from the CFL rule with the
+  // case's transport direction (p->a), LSRK with the RHS rho -> b -> CG ->
+  // grad phi -> advection, diagnostics rows (t, energy (dim_x), mass, l2,
+  // kinetic, momentum (3))
+  std::vector<double> a, b;
+  if (!p || !rk_coefficients(rk_type, a, b) || hdo_set_vlasov_poisson(p, 1))
+    return -1;
+  XSpace X = make_xspace(*p);
+  const size_t n = (size_t)hdo_n_dofs(p);
+  std::vector<double> sol(n), ki(n), ti(n);
+  std::vector<double> rho(X.n_dofs), rhs(X.n_dofs), phi(X.n_dofs, 0.0);
+  hdo_interpolate(p, start_time, sol.data());
+  const double crit = hdo_critical_time_step(p);
+  double dt = std::min(time_step, cfl * crit / std::pow((double)p->degree, 1.5));
+  dt = (final_time - start_time) / std::ceil((final_time - start_time) / dt);
+  if (dt_used)
+    *dt_used = dt;
+  long long total_it = 0;
+  const int width = 1 + p->dim_x + 6;
+  int n_rows = 0;
+  auto diagnostics = [&](double cur_time) {
+    if (cur_time != start_time &&
+        (int)((cur_time + 0.00000000001 - start_time) / diag_tick) ==
+          (int)((cur_time + 0.00000000001 - start_time - dt) / diag_tick))
+      return;
+    if (n_rows < max_rows)
+      {
+        double *row = rows + (size_t)n_rows * width;
+        row[0] = cur_time;
+        vp_energy(*p, X, row + 1); // field of the last RHS evaluation
+        hdo_vp_diagnostics(p, sol.data(), row + 1 + p->dim_x);
+      }
+    ++n_rows;
+  };
+  Op op = [&](const double *src, double *dst, double time) {
+    vp_density(*p, X, src, rho.data());
+    vp_rhs(X, rho.data(), rhs.data());
+    total_it += poisson_cg(X, p->degree, rhs.data(), phi.data(), poisson_tol,
+                           100000);
+    vp_set_field(*p, X, phi.data());
+    apply_ecl(*p, src, dst, time);
+  };
+  diagnostics(start_time);
+  int step = 1;
+  for (double time = start_time + dt;
+       time <= final_time * 1.0000000000001 && step <= max_steps;
+       time += dt, ++step)
+    {
+      lsrk_step(a, b, n, sol.data(), ki.data(), ti.data(), time - dt, dt, op);
+      diagnostics(time);
+    }
+  if (cg_iterations)
+    *cg_iterations = (int)std::min<long long>(total_it, 2147483647LL);
+  return std::min(n_rows, max_rows);
 }
 
 int hdo_num_threads(void)

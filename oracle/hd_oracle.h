@@ -34,7 +34,7 @@ enum hdo_function
   HDO_FUNC_CONSTANT_ONE = 2,
   HDO_FUNC_SIN_COS = 3,        /* sin(pi x0) prod_{d>0} cos(pi x_d)                      */
   HDO_FUNC_PRODUCT_SINES = 4,  /* prod_d sin(2 pi (x_d - a_d t))                          */
-  HDO_FUNC_LANDAU = 5          /* (1 + eps cos(kx x0) cos(kx x1)) exp(-|v|^2/2)/(2pi)^{dv/2} */
+  HDO_FUNC_LANDAU = 5          /* (1 + eps prod_{d<np} cos(kx x_d)) exp(-|v|^2/2)/(2pi)^{dv/2}, params eps, kx, np */
 };
 
 /* Boundary treatment of non-periodic directions (inhomogeneous Dirichlet uses
@@ -113,6 +113,34 @@ double hdo_splitmix_uniform(unsigned long long seed, unsigned long long i);
 /* dst += amplitude * U(-1,1) noise keyed by the global DoF index. */
 int hdo_add_noise(const hdo_problem *p, double *dst, unsigned long long seed,
                   double amplitude);
+
+/* ---- Vlasov-Poisson (examples/vlasov_poisson), requires dim_x == dim_v ----
+ * The x-space vectors (rho, b, phi) hold (k+1)^dim_x nodal values per x cell,
+ * x cells lexicographic. */
+int hdo_set_vlasov_poisson(hdo_problem *p, int enable);
+long long hdo_x_n_dofs(const hdo_problem *p);
+/* rho = integral of f over v (Gauss-Lobatto v rule on the nodal values) */
+int hdo_vp_density(hdo_problem *p, const double *f, double *rho);
+/* rho -= mean; b = -M rho; b -= mean */
+int hdo_vp_rhs(hdo_problem *p, double *rho, double *b);
+/* dst = A (src - mean(src)), A the periodic SIPG Laplacian */
+int hdo_poisson_apply(hdo_problem *p, const double *src, double *dst);
+/* CG on A phi = b from the given phi; returns iterations */
+int hdo_poisson_solve(hdo_problem *p, const double *b, double *phi,
+                      double rel_tol, int max_it);
+/* store grad phi at the x quadrature points as the v-velocity */
+int hdo_vp_set_potential(hdo_problem *p, const double *phi);
+int hdo_vp_efield(const hdo_problem *p, double *out);
+/* sum (d_e phi)^2 JxW per x component, from the stored field */
+int hdo_vp_energy(hdo_problem *p, double *out);
+/* mass, l2 norm, kinetic energy, momentum[3] of f */
+int hdo_vp_diagnostics(hdo_problem *p, const double *f, double *out);
+/* the Vlasov-Poisson example end to end; rows of 1 + dim_x + 6 values */
+int hdo_run_vlasov_poisson(hdo_problem *p, const char *rk_type,
+                           double time_step, double start_time,
+                           double final_time, double cfl, double diag_tick,
+                           double poisson_tol, int max_steps, double *rows,
+                           int max_rows, double *dt_used, int *cg_iterations);
 
 /* Number of OpenMP threads the oracle uses. */
 int hdo_num_threads(void);

@@ -105,3 +105,56 @@ def test_vector_tools_01_golden():
     for (n0, e0), (n1, e1) in zip(ours, vals):
         assert float("%g" % n0) == pytest.approx(n1, rel=1e-5, abs=1e-12)
         assert abs(e0 - e1) <= max(1e-15, 6e-6 * abs(e1))
+
+
+def _vp_golden():
+    rows = []
+    for l in open(os.path.join(GOLDEN, "vlasov_poisson",
+                               "vp_2D_2D_k3.hyperrectangle_01.out")):
+        if not l.startswith("#"):
+            rows.append([float(x) for x in l.split()])
+    return np.array(rows)
+
+
+def test_vlasov_poisson_golden():
+    """examples/vlasov_poisson/tests/vp_2D_2D_k3.hyperrectangle_01: Landau
+    damping on [0,4pi]^2 x [-6,6]^2, 4^4 cells, k=3, CFL 0.05, T=0.5; rows
+    (t, field energy x2, mass, ||f||, kinetic energy, momentum x3).  The
+    reference prints t with 3 decimals (application.h:632-642)."""
+    P = O.VlasovPoisson(2, 3, [4, 4, 4, 4], [0, 0, -6, -6],
+                        [4 * np.pi, 4 * np.pi, 6, 6], [1, 1, 6, 6],
+                        eps=0.01, kx=0.5, n_perturbed=1)
+    rows, dt, it = P.run(final_time=0.5, cfl=0.05, diag_tick=0.1,
+                         poisson_tol=1e-12)
+    gold = _vp_golden()
+    rows[:, 0] = [float("%.3f" % t) for t in rows[:, 0]]
+    assert rows.shape == gold.shape
+    assert AC.numdiff_ok(rows, gold)
+    # the damped field energy and the invariants to ~1e-9
+    assert np.allclose(rows[:, 1], gold[:, 1], rtol=1e-8, atol=1e-12)
+    assert np.allclose(rows[:, 3:6], gold[:, 3:6], rtol=1e-10, atol=0)
+
+
+def test_vlasov_poisson_pieces():
+    """rho, Poisson and grad phi against the closed form for the Landau
+    density: -lap phi = -(rho - mean) with rho = 1 + eps cos(k x0) gives
+    phi = -eps cos(k x0) / k^2 and d phi/dx0 = eps sin(k x0) / k."""
+    P = O.VlasovPoisson(2, 3, [8, 8, 4, 4], [0, 0, -6, -6],
+                        [4 * np.pi, 4 * np.pi, 6, 6], [1, 1, 6, 6])
+    f = P.interpolate()
+    rho = P.density(f)
+    b = P.rhs(rho)
+    assert abs(b.sum()) < 1e-12
+    phi, it = P.poisson_solve(b, rel_tol=1e-12)
+    assert np.linalg.norm(b - P.poisson_apply(phi)) <= 1e-11 * np.linalg.norm(b)
+    P.set_potential(phi)
+    E = P.efield().reshape(64, 16, 2)
+    t = O.tables(3)
+    h = 4 * np.pi / 8
+    x0 = np.array([(c % 8 + t["qpoints"][q % 4]) * h for c in range(64)
+                   for q in range(16)]).reshape(64, 16)
+    # the v-integral of the interpolated Maxwellian is not exactly 1; scale
+    mass_v = rho.mean()
+    exact = 0.01 * mass_v * np.sin(0.5 * x0) / 0.5
+    assert np.max(np.abs(E[:, :, 0] - exact)) < 2e-3 * np.max(np.abs(exact))
+    assert np.max(np.abs(E[:, :, 1])) < 1e-12
