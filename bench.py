@@ -104,11 +104,15 @@ def main():
     ap.add_argument("--only", choices=["all", "fp64", "fp32", "stage"],
                     default="all", help="restrict the timed kernels "
                     "(profiling runs)")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="also run a plain device copy of one FP64 vector "
+                    "(8.6 GB read + 8.6 GB written), the reference point for "
+                    "scaling the FETCH_SIZE / WRITE_SIZE counters")
     args = ap.parse_args()
     if args.gpus != 1:
-        print("bench.py measures one GPU: this build has no multi-GPU path "
-              "(the operator would shard by x/v phase-space blocks with face "
-              "halos; see DESIGN.md). Run with --gpus 1.", file=sys.stderr)
+        print("bench.py measures one GPU: there is no multi-GPU path (the "
+              "operator would shard into x/v phase-space blocks exchanging "
+              "inflow face halos). Run with --gpus 1.", file=sys.stderr)
         sys.exit(2)
 
     import torch
@@ -128,6 +132,12 @@ def main():
     H.VectorTools.interpolate(mf, u, H.FUNC_PRODUCT_SINES, [0.0] * 6)
     H.VectorTools.add_noise(mf, u, SEED_NOISE, 1e-3)
 
+    if args.copy_calibration:
+        cp = torch.empty_like(u)
+        for _ in range(3):
+            cp.copy_(u)
+        torch.cuda.synchronize()
+        del cp
     if args.only in ("all", "fp64"):
         dst = torch.empty_like(u)
         t = timed(lambda: op.apply(dst, u, 0.0, H.FP64, stream), args.steps,

@@ -21,7 +21,8 @@ import of the binding fails loudly.
 import ctypes
 import os
 
-__all__ = ["MatrixFree", "AdvectionOperation", "LowStorageRungeKuttaIntegrator",
+__all__ = ["MatrixFree", "AdvectionOperation", "VlasovPoissonOperation",
+           "vp_moments", "LowStorageRungeKuttaIntegrator",
            "VectorTools", "HyperdealError", "FP64", "FP32", "library_path",
            "load_library", "compute_critical_time_step"]
 
@@ -30,7 +31,7 @@ FP64, FP32 = 0, 1
 ALGO_AUTO, ALGO_QUADRATURE, ALGO_COLLAPSED = 0, 1, 2
 BC_HOMOGENEOUS, BC_INHOMOGENEOUS = 0, 1
 FUNC_ZERO, FUNC_HYPERRECTANGLE, FUNC_CONSTANT_ONE, FUNC_SIN_COS, \
-    FUNC_PRODUCT_SINES = range(5)
+    FUNC_PRODUCT_SINES, FUNC_LANDAU = range(6)
 
 # every symbol include/hyperdeal_amd.h declares
 EXPORTED = [
@@ -43,7 +44,10 @@ EXPORTED = [
     "hd_lsrk_step", "hd_interpolate", "hd_add_noise", "hd_norm_and_error",
     "hd_vector_convert", "hd_critical_time_step", "hd_device_malloc",
     "hd_device_free", "hd_memcpy_htod", "hd_memcpy_dtoh",
-    "hd_device_synchronize",
+    "hd_device_synchronize", "hd_vlasov_poisson_create",
+    "hd_vp_update_field", "hd_vp_set_auto_update",
+    "hd_vp_set_poisson_tolerance", "hd_vp_solver_stats", "hd_vp_x_n_dofs",
+    "hd_vp_potential", "hd_vp_field_energy", "hd_vp_moments",
 ]
 
 
@@ -111,6 +115,20 @@ def load_library(path=None):
     lib.hd_critical_time_step.restype = ctypes.c_double
     lib.hd_critical_time_step.argtypes = [_vp, _dp]
     lib.hd_device_synchronize.argtypes = []
+    lib.hd_vlasov_poisson_create.argtypes = [_vp, ctypes.c_double,
+                                             ctypes.c_double,
+                                             ctypes.POINTER(_vp)]
+    lib.hd_vp_update_field.argtypes = [_vp, _vp, _vp]
+    lib.hd_vp_set_auto_update.argtypes = [_vp, ctypes.c_int]
+    lib.hd_vp_set_poisson_tolerance.argtypes = [_vp, ctypes.c_double,
+                                                ctypes.c_int]
+    lib.hd_vp_solver_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong),
+                                       ctypes.POINTER(ctypes.c_longlong), _dp]
+    lib.hd_vp_x_n_dofs.restype = ctypes.c_longlong
+    lib.hd_vp_x_n_dofs.argtypes = [_vp]
+    lib.hd_vp_potential.argtypes = [_vp, _dp, ctypes.c_int]
+    lib.hd_vp_field_energy.argtypes = [_vp, _dp]
+    lib.hd_vp_moments.argtypes = [_vp, _vp, _dp, _vp]
     if path is None:
         _lib = lib
     return lib
@@ -263,6 +281,83 @@ class AdvectionOperation:
                                        float(time), float(a_dt), float(b_dt),
                                        _stream(stream)),
                "hd_lsrk_stage")
+
+
+class VlasovPoissonOperation(AdvectionOperation):
+    """The Vlasov-Poisson right-hand side (examples/vlasov_poisson/include/
+    application.h:516-602): advection with velocity (v, grad phi), phi from
+    the periodic SIPG Poisson problem for rho = int f dv, solved with FP64
+    CG from the previous phi.  Every apply/stage recomputes the field from
+    its input unless auto update is switched off."""
+
+    def __init__(self, matrix_free, skew_factor=0.0, poisson_tol=1e-10):
+        self.mf = matrix_free
+        self._lib = matrix_free._lib
+        self.velocity = None
+        h = _vp()
+        _check(self._lib.hd_vlasov_poisson_create(matrix_free.handle,
+                                                  float(skew_factor),
+                                                  float(poisson_tol),
+                                                  ctypes.byref(h)),
+               "hd_vlasov_poisson_create")
+        self.handle = h
+
+    def update_field(self, f, stream=None):
+        import torch
+        _check(self._lib.hd_vp_update_field(
+            self.handle, _ptr(f, torch.float64, self.mf.n_dofs()),
+            _stream(stream)), "hd_vp_update_field")
+
+    def set_auto_update(self, on):
+        _check(self._lib.hd_vp_set_auto_update(self.handle, int(on)),
+               "hd_vp_set_auto_update")
+
+    def set_poisson_tolerance(self, rel_tol, max_it=100000):
+        _check(self._lib.hd_vp_set_poisson_tolerance(self.handle,
+                                                     float(rel_tol), max_it),
+               "hd_vp_set_poisson_tolerance")
+
+    def solver_stats(self):
+        it, n = ctypes.c_longlong(), ctypes.c_longlong()
+        res = ctypes.c_double()
+        _check(self._lib.hd_vp_solver_stats(self.handle, ctypes.byref(it),
+                                            ctypes.byref(n),
+                                            ctypes.byref(res)),
+               "hd_vp_solver_stats")
+        return it.value, n.value, res.value
+
+    def x_n_dofs(self):
+        return self._lib.hd_vp_x_n_dofs(self.handle)
+
+    def potential(self):
+        import numpy as np
+        out = np.zeros(self.x_n_dofs())
+        _check(self._lib.hd_vp_potential(
+            self.handle, out.ctypes.data_as(_dp), 1), "hd_vp_potential")
+        return out
+
+    def set_potential(self, phi):
+        import numpy as np
+        phi = np.ascontiguousarray(phi, dtype=np.float64)
+        _check(self._lib.hd_vp_potential(
+            self.handle, phi.ctypes.data_as(_dp), 0), "hd_vp_potential")
+
+    def field_energy(self):
+        out = (ctypes.c_double * 3)()
+        _check(self._lib.hd_vp_field_energy(self.handle, out),
+               "hd_vp_field_energy")
+        return [out[i] for i in range(self.mf.dim_x)]
+
+
+def vp_moments(mf, f, stream=None):
+    """mass, ||f||_L2, kinetic energy, momentum (3) of f
+    (vlasov_poisson/include/diagnostics.h:28-87)."""
+    import torch
+    out = (ctypes.c_double * 6)()
+    _check(mf._lib.hd_vp_moments(mf.handle, _ptr(f, torch.float64,
+                                                   mf.n_dofs()), out,
+                                 _stream(stream)), "hd_vp_moments")
+    return [out[i] for i in range(6)]
 
 
 class LowStorageRungeKuttaIntegrator:

@@ -61,6 +61,18 @@ struct hd_advection
   float *coef32 = nullptr;
   hd::InflowDev inflow{};
   hd::QuadDev quad{};
+  // Vlasov-Poisson state (velocity (v, grad phi) from the Poisson solve)
+  bool vp = false;
+  bool vp_auto = true;        // stages recompute the field from their input
+  double poisson_tol = 1e-10; // CG relative residual
+  int poisson_max_it = 100000;
+  hd::XDev xdev{};
+  hd::VpDev vpdev{};
+  long long n_x_cells = 0, n_v_cells = 0, n_x_dofs = 0;
+  int density_chunks = 1;
+  double *efield = nullptr, *rho = nullptr, *phi = nullptr, *work = nullptr,
+         *partial = nullptr;
+  long long *stats = nullptr; // [iterations, solves, last relative residual]
 };
 
 namespace
@@ -84,11 +96,32 @@ int choose_algorithm(const hd_advection *op, int precision)
     }
 }
 
+int vp_update(hd_advection *op, const double *f, hipStream_t stream)
+{
+  hipError_t e = hd::vp_density(f, nullptr, op->vpdev, op->n_x_cells,
+                                op->n_v_cells, op->partial,
+                                op->density_chunks, op->rho, stream);
+  if (e == hipSuccess)
+    e = hd::vp_poisson(op->xdev, op->mf->nd, op->rho, op->phi, op->efield,
+                       op->work, op->poisson_tol, op->poisson_max_it,
+                       op->stats, stream);
+  return e == hipSuccess ? HD_OK : hip_fail(e, "Vlasov-Poisson field update");
+}
+
 int run_operator(hd_advection *op, int mode, int precision, const void *src,
                  void *dst, const hd::StageDev &stage, double time,
                  hipStream_t stream)
 {
   hd_matrix_free *mf = op->mf;
+  if (op->vp && op->vp_auto)
+    {
+      if (precision != HD_FP64)
+        return fail(HD_ERR_UNSUPPORTED,
+                    "the Vlasov-Poisson operator runs in FP64");
+      const int rc = vp_update(op, static_cast<const double *>(src), stream);
+      if (rc != HD_OK)
+        return rc;
+    }
   const int algo = choose_algorithm(op, precision);
   if (algo < 0)
     return fail(HD_ERR_UNSUPPORTED,
@@ -116,7 +149,8 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
     e = hd::quadrature_apply(mf->D, mf->nd, mode,
                              static_cast<const double *>(src),
                              static_cast<double *>(dst), op->quad, mf->mesh,
-                             stage, time, mf->dim_x, mf->collocation, stream);
+                             stage, time, mf->dim_x, mf->collocation,
+                             op->vp ? op->efield : nullptr, stream);
   if (e != hipSuccess)
     return hip_fail(e, "kernel launch");
   return ok();
@@ -354,11 +388,222 @@ int hd_advection_destroy(hd_advection *op)
 {
   if (!op)
     return ok();
-  if (op->coef64)
-    (void)hipFree(op->coef64);
-  if (op->coef32)
-    (void)hipFree(op->coef32);
+  for (void *p : {(void *)op->coef64, (void *)op->coef32, (void *)op->efield,
+                  (void *)op->rho, (void *)op->phi, (void *)op->work,
+                  (void *)op->partial, (void *)op->stats})
+    if (p)
+      (void)hipFree(p);
   delete op;
+  return ok();
+}
+
+int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
+                             double poisson_tol, hd_advection **out)
+{
+  if (!mf || !out)
+    return fail(HD_ERR_INVALID, "hd_vlasov_poisson_create: null argument");
+  *out = nullptr;
+  if (mf->dim_x != mf->dim_v)
+    return fail(HD_ERR_UNSUPPORTED,
+                "Vlasov-Poisson needs dim_x == dim_v (the reference's "
+                "v-integration assumes it)");
+  for (int d = 0; d < mf->D; ++d)
+    if (!mf->periodic[d])
+      return fail(HD_ERR_UNSUPPORTED, "Vlasov-Poisson needs a periodic box");
+  if (mf->nq != mf->nd || !hd::quadrature_supported(mf->D, mf->nd))
+    return fail(HD_ERR_UNSUPPORTED,
+                "Vlasov-Poisson runs the quadrature kernel: n_points must "
+                "be degree + 1");
+  long long ncx = 1;
+  for (int d = 0; d < mf->dim_x; ++d)
+    ncx *= mf->mesh.n[d];
+  if (!hd::vp_poisson_supported(mf->dim_x, mf->nd, ncx))
+    return fail(HD_ERR_UNSUPPORTED, "x mesh too large for the Poisson solver");
+  // the operator: constant part unused, velocity from the field
+  std::vector<double> zero(mf->D, 0.0);
+  int rc = hd_advection_create(mf, zero.data(), skew_factor,
+                               HD_BC_HOMOGENEOUS, HD_FUNC_ZERO, nullptr, out);
+  if (rc != HD_OK)
+    return rc;
+  hd_advection *op = *out;
+  op->vp = true;
+  op->collapsed_ok = false;
+  op->poisson_tol = poisson_tol > 0 ? poisson_tol : 1e-10;
+  op->quad.vp = 1;
+  op->quad.n_x_cells = ncx;
+  op->n_x_cells = ncx;
+  op->n_v_cells = mf->mesh.n_cells / ncx;
+  const int nd = mf->nd, nq = mf->nq;
+  const auto &t = mf->tables;
+  op->n_x_dofs = ncx * hd::ipow(nd, mf->dim_x);
+  // x-space matrices (exact: the Gauss rule has k+1 points)
+  auto &x = op->xdev;
+  x.dim = mf->dim_x;
+  x.n_cells = ncx;
+  x.kk = std::max(mf->degree, 1) * (mf->degree + 1.0);
+  for (int d = 0; d < mf->dim_x; ++d)
+    {
+      x.n[d] = mf->mesh.n[d];
+      x.h[d] = mf->mesh.h[d];
+    }
+  for (int i = 0; i < nd; ++i)
+    for (int j = 0; j < nd; ++j)
+      {
+        double m = 0.0, k = 0.0;
+        for (int q = 0; q < nq; ++q)
+          {
+            m += t.qweights[q] * t.S[q * nd + i] * t.S[q * nd + j];
+            k += t.qweights[q] * t.G[q * nd + i] * t.G[q * nd + j];
+          }
+        x.M[i * nd + j] = m;
+        x.K[i * nd + j] = k;
+      }
+  for (int i = 0; i < nd; ++i)
+    {
+      x.d0[i] = t.dnode0[i];
+      x.d1[i] = t.dnode1[i];
+    }
+  std::copy(t.S.begin(), t.S.end(), x.S);
+  std::copy(t.G.begin(), t.G.end(), x.G);
+  // v weights of the nodal points (Gauss-Lobatto rule of the nodes)
+  auto &v = op->vpdev;
+  v.dofs_x = hd::ipow(nd, mf->dim_x);
+  v.dofs_v = hd::ipow(nd, mf->dim_v);
+  for (int q = 0; q < v.dofs_v; ++q)
+    {
+      double w = 1.0;
+      int rem = q;
+      for (int j = 0; j < mf->dim_v; ++j)
+        {
+          w *= mf->mesh.h[mf->dim_x + j] * t.node_weights[rem % nd];
+          rem /= nd;
+        }
+      v.wv[q] = w;
+    }
+  op->density_chunks =
+    (int)std::min<long long>(op->n_v_cells, std::max<long long>(1, 4096 / ncx));
+  const size_t nxd = (size_t)op->n_x_dofs;
+  const size_t nef = (size_t)ncx * hd::ipow(nq, mf->dim_x) * mf->dim_x;
+  hipError_t e = hipMalloc(&op->efield, nef * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&op->rho, nxd * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&op->phi, nxd * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&op->work, 3 * nxd * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&op->partial,
+                  (size_t)ncx * op->density_chunks * v.dofs_x * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&op->stats, 4 * sizeof(long long));
+  if (e == hipSuccess)
+    e = hipMemset(op->phi, 0, nxd * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMemset(op->efield, 0, nef * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMemset(op->stats, 0, 4 * sizeof(long long));
+  if (e != hipSuccess)
+    {
+      hd_advection_destroy(op);
+      *out = nullptr;
+      return hip_fail(e, "hd_vlasov_poisson_create");
+    }
+  return ok();
+}
+
+int hd_vp_update_field(hd_advection *op, const double *f, void *stream)
+{
+  if (!op || !f || !op->vp)
+    return fail(HD_ERR_INVALID, "hd_vp_update_field: not a Vlasov-Poisson "
+                                "operator or null argument");
+  return vp_update(op, f, as_stream(stream));
+}
+
+int hd_vp_set_auto_update(hd_advection *op, int enable)
+{
+  if (!op || !op->vp)
+    return fail(HD_ERR_INVALID, "not a Vlasov-Poisson operator");
+  op->vp_auto = enable != 0;
+  return ok();
+}
+
+int hd_vp_set_poisson_tolerance(hd_advection *op, double rel_tol, int max_it)
+{
+  if (!op || !op->vp || !(rel_tol > 0) || max_it < 1)
+    return fail(HD_ERR_INVALID, "invalid Poisson tolerance");
+  op->poisson_tol = rel_tol;
+  op->poisson_max_it = max_it;
+  return ok();
+}
+
+int hd_vp_solver_stats(hd_advection *op, long long *iterations,
+                       long long *solves, double *last_rel_residual)
+{
+  if (!op || !op->vp)
+    return fail(HD_ERR_INVALID, "not a Vlasov-Poisson operator");
+  long long s[3] = {0, 0, 0};
+  const hipError_t e =
+    hipMemcpy(s, op->stats, 3 * sizeof(long long), hipMemcpyDeviceToHost);
+  if (e != hipSuccess)
+    return hip_fail(e, "hd_vp_solver_stats");
+  if (iterations)
+    *iterations = s[0];
+  if (solves)
+    *solves = s[1];
+  if (last_rel_residual)
+    std::memcpy(last_rel_residual, &s[2], sizeof(double));
+  return ok();
+}
+
+long long hd_vp_x_n_dofs(const hd_advection *op)
+{
+  return op && op->vp ? op->n_x_dofs : -1;
+}
+
+int hd_vp_potential(hd_advection *op, double *phi, int device_to_host)
+{
+  if (!op || !op->vp || !phi)
+    return fail(HD_ERR_INVALID, "hd_vp_potential: invalid argument");
+  const hipError_t e =
+    hipMemcpy(device_to_host ? (void *)phi : (void *)op->phi,
+              device_to_host ? (void *)op->phi : (void *)phi,
+              op->n_x_dofs * sizeof(double),
+              device_to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice);
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_vp_potential");
+}
+
+int hd_vp_field_energy(hd_advection *op, double *out)
+{
+  // sum_K sum_q (d_e phi)^2 JxW (vlasov_poisson/include/diagnostics.h:
+  // 95-141) from the stored field; synchronous, host reduction
+  if (!op || !op->vp || !out)
+    return fail(HD_ERR_INVALID, "hd_vp_field_energy: invalid argument");
+  hd_matrix_free *mf = op->mf;
+  const int dx = mf->dim_x, nq = mf->nq, nqx = hd::ipow(nq, dx);
+  std::vector<double> ef((size_t)op->n_x_cells * nqx * dx);
+  const hipError_t e = hipMemcpy(ef.data(), op->efield,
+                                 ef.size() * sizeof(double),
+                                 hipMemcpyDeviceToHost);
+  if (e != hipSuccess)
+    return hip_fail(e, "hd_vp_field_energy");
+  for (int k = 0; k < dx; ++k)
+    out[k] = 0.0;
+  for (long long c = 0; c < op->n_x_cells; ++c)
+    for (int q = 0; q < nqx; ++q)
+      {
+        double jxw = 1.0;
+        int rem = q;
+        for (int j = 0; j < dx; ++j)
+          {
+            jxw *= mf->mesh.h[j] * mf->tables.qweights[rem % nq];
+            rem /= nq;
+          }
+        for (int k = 0; k < dx; ++k)
+          {
+            const double g = ef[((size_t)c * nqx + q) * dx + k];
+            out[k] += g * g * jxw;
+          }
+      }
   return ok();
 }
 
@@ -467,7 +712,8 @@ int hd_interpolate(hd_matrix_free *mf, int function, const double *params,
     return fail(HD_ERR_INVALID, "hd_interpolate: null argument");
   hd::FuncDev f{};
   f.id = function;
-  for (int d = 0; d < mf->D; ++d)
+  const int n_params = function == HD_FUNC_LANDAU ? 3 : mf->D;
+  for (int d = 0; d < n_params; ++d)
     f.prm[d] = params ? params[d] : 0.0;
   std::copy(mf->tables.nodes.begin(), mf->tables.nodes.end(), f.nodes);
   f.nd = mf->nd;
@@ -554,6 +800,45 @@ int hd_norm_and_error(hd_matrix_free *mf, const double *src, int function,
     }
   out[0] = std::sqrt(s0);
   out[1] = std::sqrt(s1);
+  return ok();
+}
+
+int hd_vp_moments(hd_matrix_free *mf, const double *f, double *out,
+                  void *stream)
+{
+  // mass, ||f||, kinetic energy, momentum (3) at the quadrature points
+  // (vlasov_poisson/include/diagnostics.h:28-87); synchronous
+  if (!mf || !f || !out)
+    return fail(HD_ERR_INVALID, "hd_vp_moments: null argument");
+  const long long nc = mf->mesh.n_cells;
+  double *partial = nullptr;
+  hipError_t e = hipMalloc(&partial, 6 * nc * sizeof(double));
+  if (e != hipSuccess)
+    return hip_fail(e, "hd_vp_moments");
+  hd::NormDev nt{};
+  std::copy(mf->tables.S.begin(), mf->tables.S.end(), nt.S);
+  std::copy(mf->tables.qpoints.begin(), mf->tables.qpoints.end(), nt.qp);
+  std::copy(mf->tables.qweights.begin(), mf->tables.qweights.end(), nt.w);
+  nt.nd = mf->nd;
+  nt.nq = mf->nq;
+  const int vol = hd::ipow(std::max(mf->nd, mf->nq), mf->D);
+  e = hd::vp_moments(f, mf->mesh, nt, mf->dim_x, vol, partial,
+                     as_stream(stream));
+  std::vector<double> host(6 * nc);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(host.data(), partial, 6 * nc * sizeof(double),
+                       hipMemcpyDeviceToHost, as_stream(stream));
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(as_stream(stream));
+  (void)hipFree(partial);
+  if (e != hipSuccess)
+    return hip_fail(e, "hd_vp_moments");
+  for (int k = 0; k < 6; ++k)
+    out[k] = 0.0;
+  for (long long cc = 0; cc < nc; ++cc)
+    for (int k = 0; k < 6; ++k)
+      out[k] += host[6 * cc + k];
+  out[1] = std::sqrt(out[1]);
   return ok();
 }
 

@@ -14,13 +14,23 @@
 // host by hd::collapse_direction) and src_nb,d the upwind neighbour across d.
 // The outflow neighbour never enters the upwind flux.
 //
-// Work decomposition (one cell of N^D values per block for 3+3, several for
-// smaller D): a thread owns an N x N block of points in a pair of directions
-// (layout p owns directions 2p, 2p+1), so the 1D contractions of both run in
-// registers; the cell is staged in LDS (coalesced global loads, padded by one
-// value per N^2 to keep the strided layout reads bank-conflict free) and the
-// per-layout contributions are summed through the same LDS buffer.
+// Work decomposition.  A block of 256 threads works on CPB cells at a time
+// (one cell of N^D values for 3+3, k=3).  A thread owns an N x N block of
+// points in a pair of directions (layout p owns directions 2p, 2p+1), so the
+// 1D contractions of both run in registers.  The cell sits in LDS (U, padded
+// by one value per N^2 so the three strided layouts read without bank
+// conflicts); each layout adds its contribution into a second LDS buffer R,
+// which is stored with coalesced 16-byte accesses.
+//
+// Blocks are persistent (two per CU) and walk an XCD-local range of cells:
+// while a block computes one cell, the next cell's values and all its inflow
+// face values are already in flight into registers, so HBM and L2 latency
+// overlap with the arithmetic.  The logical cells of one XCD are handed out
+// round-robin to its blocks, so the blocks of an XCD work on neighbouring
+// cells at the same time and face reads hit its L2 or the MALL.
 #include "hd_device.h"
+
+#include <type_traits>
 
 namespace hd
 {
@@ -28,27 +38,36 @@ namespace collapsed
 {
 constexpr int kBlock = 256;
 
-template <int D, int N>
+// LDS per block: two padded buffers of CPB cells, at most this many bytes so
+// that two blocks fit on a CU (160 KB)
+constexpr int kLdsPerBlock = 76 * 1024;
+
+template <int D, int N, typename T>
 struct Shape
 {
   static constexpr int NN = N * N;
   static constexpr int CELL = ipow(N, D);
-  static constexpr int TPC = ipow(N, D - 2);   // threads per cell
-  static constexpr int CPB = kBlock / TPC;      // cells per block
-  static constexpr int THREADS = CPB * TPC;
+  static constexpr int TPC = ipow(N, D - 2);    // threads per cell
   static constexpr int CELLP = CELL + CELL / NN; // padded cell in LDS
-  static constexpr int P = (D + 1) / 2;          // layouts
+  static constexpr int CPB_THREADS = kBlock / TPC;
+  static constexpr int CPB_LDS =
+    kLdsPerBlock / (2 * CELLP * (int)sizeof(T)) > 0
+      ? kLdsPerBlock / (2 * CELLP * (int)sizeof(T))
+      : 1;
+  static constexpr int CPB = CPB_THREADS < CPB_LDS ? CPB_THREADS : CPB_LDS;
+  static constexpr int THREADS = CPB * TPC;       // active threads
+  static constexpr int P = (D + 1) / 2;           // layouts
   static_assert(TPC <= kBlock, "cell too large for one block");
 };
 
-__device__ constexpr int lay_a(int p) { return 2 * p; }
+__host__ __device__ constexpr int lay_a(int p) { return 2 * p; }
 template <int D>
-__device__ constexpr int lay_b(int p)
+__host__ __device__ constexpr int lay_b(int p)
 {
   return 2 * p + 1 < D ? 2 * p + 1 : (2 * p == 0 ? 1 : 0);
 }
 template <int D>
-__device__ constexpr bool lay_b_active(int p)
+__host__ __device__ constexpr bool lay_b_active(int p)
 {
   return 2 * p + 1 < D;
 }
@@ -90,56 +109,116 @@ __device__ inline int layout_base(int tau, int a, int b)
   return base;
 }
 
-// Contributions of directions (a [, b]) for one thread's N x N block.
-//   u: the block's values, r = i_a + N i_b; out: accumulated contributions
+// Registers a thread carries from one cell to the next: its share of the
+// cell's values (coalesced chunks) and the inflow face values of its lines.
 template <int D, int N, typename T>
-__device__ inline void block_contrib(int p, int base, long long cell,
-                                     const int *coord,
-                                     const T *__restrict__ src,
-                                     const T *__restrict__ coef,
-                                     const MeshDev &mesh,
-                                     const InflowDev &inflow, const T *u,
-                                     T *out)
+struct Prefetch
 {
-  constexpr int NN = N * N;
+  using S = Shape<D, N, T>;
+  static constexpr int VEC = chunk_width<T, S::CELL>();
+  static constexpr int CHUNKS = S::CPB * S::CELL / VEC;
+  static constexpr int CPT = (CHUNKS + kBlock - 1) / kBlock;
+  Chunk<T, VEC> c[CPT];
+  T face[S::P][2][N];
+};
+
+template <int D, int N, typename T>
+__device__ inline void issue_chunks(Prefetch<D, N, T> &pf, const T *src,
+                                    long long cell0, int ncell)
+{
+  using S = Shape<D, N, T>;
+  using PF = Prefetch<D, N, T>;
+  constexpr int CELL = S::CELL, VEC = PF::VEC;
+  const Chunk<T, VEC> *g =
+    reinterpret_cast<const Chunk<T, VEC> *>(src + cell0 * CELL);
+  const int nchunks = ncell * CELL / VEC;
+#pragma unroll
+  for (int k = 0; k < PF::CPT; ++k)
+    {
+      const int c = threadIdx.x + k * kBlock;
+      if (c < nchunks)
+        pf.c[k] = g[c];
+    }
+}
+
+// cell coordinates (cell ids fit in 32 bits; checked on the host)
+template <int D>
+__device__ inline void cell_coords(long long cell, const MeshDev &mesh,
+                                   int *coord)
+{
+  unsigned rem = (unsigned)cell;
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    {
+      const unsigned q = rem / (unsigned)mesh.n[d];
+      coord[d] = (int)(rem - q * (unsigned)mesh.n[d]);
+      rem = q;
+    }
+}
+
+// the inflow face values of layout p's lines of `cell`
+template <int D, int N, typename T>
+__device__ inline void issue_faces(Prefetch<D, N, T> &pf, int p,
+                                   const T *src, long long cell,
+                                   const int *coord, int base,
+                                   const MeshDev &mesh,
+                                   const InflowDev &inflow)
+{
   constexpr int CELL = ipow(N, D);
-  const int a = lay_a(p), b = lay_b<D>(p);
-  const int sa = ipow(N, a), sb = ipow(N, b);
 #pragma unroll
   for (int pass = 0; pass < 2; ++pass)
     {
       if (pass == 1 && !lay_b_active<D>(p))
         break;
+      const int a = lay_a(p), b = lay_b<D>(p);
       const int d = pass == 0 ? a : b;
       const int side = inflow.side[d];
       if (side == 0)
         continue;
-      const T *A = coef + d * (NN + N);
-      const T *bv = A + NN;
-      // inflow face of the upwind neighbour: its node p_d = N-1 (from the
-      // left) or 0 (from the right), other indices equal
+      const int sa = ipow(N, a), sb = ipow(N, b);
       const long long st = mesh.stride[d];
       const int nd = mesh.n[d];
       const long long nb =
         side > 0 ? (coord[d] == 0 ? cell + (nd - 1) * st : cell - st)
                  : (coord[d] == nd - 1 ? cell - (nd - 1) * st : cell + st);
       const int fnode = side > 0 ? N - 1 : 0;
-      const T *nsrc = src + nb * CELL + base + fnode * (pass == 0 ? sa : sb);
+      const T *ns = src + nb * CELL + base + fnode * (pass == 0 ? sa : sb);
+#pragma unroll
+      for (int l = 0; l < N; ++l)
+        pf.face[p][pass][l] = ns[l * (pass == 0 ? sb : sa)];
+    }
+}
+
+// Contributions of directions (a [, b]) for one thread's N x N block.
+//   u: the block's values, r = i_a + N i_b; out: contributions
+template <int D, int N, typename T>
+__device__ inline void block_contrib(int p, const T *__restrict__ coef,
+                                     const InflowDev &inflow, const T *u,
+                                     const T (*face)[N], T *out)
+{
+  constexpr int NN = N * N;
+  const int a = lay_a(p), b = lay_b<D>(p);
+#pragma unroll
+  for (int pass = 0; pass < 2; ++pass)
+    {
+      if (pass == 1 && !lay_b_active<D>(p))
+        break;
+      const int d = pass == 0 ? a : b;
+      if (inflow.side[d] == 0)
+        continue;
+      const T *A = coef + d * (NN + N);
+      const T *bv = A + NN;
 #pragma unroll
       for (int l = 0; l < N; ++l) // line index: i_b (pass 0) or i_a (pass 1)
         {
-          const T face = nsrc[l * (pass == 0 ? sb : sa)];
-          T line[N];
-#pragma unroll
-          for (int j = 0; j < N; ++j)
-            line[j] = pass == 0 ? u[j + N * l] : u[l + N * j];
+          const T f = face[pass][l];
 #pragma unroll
           for (int i = 0; i < N; ++i)
             {
-              T acc = bv[i] * face;
+              T acc = bv[i] * f;
 #pragma unroll
               for (int j = 0; j < N; ++j)
-                acc += A[i * N + j] * line[j];
+                acc += A[i * N + j] * (pass == 0 ? u[j + N * l] : u[l + N * j]);
               if (pass == 0)
                 out[i + N * l] += acc;
               else
@@ -151,137 +230,183 @@ __device__ inline void block_contrib(int p, int base, long long cell,
 
 // MODE 0: dst = apply(src).  MODE 1: LSRK stage with K = apply(src).
 template <int D, int N, typename T, int MODE>
-__global__ __launch_bounds__(kBlock) void apply_kernel(
+__global__ __launch_bounds__(kBlock, 2) void apply_kernel(
   const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ coef,
-  MeshDev mesh, InflowDev inflow, StageDev stage, long long n_logical)
+  MeshDev mesh, InflowDev inflow, StageDev stage, long long n_logical,
+  int slots)
 {
-  using S = Shape<D, N>;
+  using S = Shape<D, N, T>;
+  using PF = Prefetch<D, N, T>;
   constexpr int NN = S::NN, CELL = S::CELL, TPC = S::TPC, CPB = S::CPB;
-  constexpr int CELLP = S::CELLP, P = S::P;
-  __shared__ T lds[CPB * CELLP];
+  constexpr int CELLP = S::CELLP, P = S::P, VEC = PF::VEC;
+  __shared__ T U[CPB * CELLP];
+  __shared__ T R[CPB * CELLP];
 
-  const long long lb = xcd_chunked_block(n_logical);
-  if (lb >= n_logical)
-    return; // whole block idle: uniform exit before any barrier
+  // this block's share of its XCD's contiguous range of logical cells
+  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const long long per = (n_logical + 7) / 8;
+  const long long lend = min(n_logical, (xcd + 1) * per);
+  long long lb = xcd * per + slot;
+  if (lb >= lend)
+    return; // uniform for the block, before any barrier
+
   const int tid = threadIdx.x;
-  const long long cell0 = lb * CPB;
-  const int ncell = (int)min((long long)CPB, mesh.n_cells - cell0);
-
-  // 1) coalesced load of the block's cells into LDS (16-byte chunks)
-  constexpr int VEC = chunk_width<T, CELL>();
-  using V = Chunk<T, VEC>;
-  {
-    const V *g = reinterpret_cast<const V *>(src + cell0 * CELL);
-    const int nchunks = ncell * CELL / VEC;
-    for (int c = tid; c < nchunks; c += S::THREADS)
-      {
-        const V v = g[c];
-        const int i = c * VEC; // block-local value index
-        const int lc = i / CELL, li = i - lc * CELL;
-        T *dstl = lds + lc * CELLP + padded<N>(li);
-#pragma unroll
-        for (int e = 0; e < VEC; ++e)
-          dstl[e] = v.v[e];
-      }
-  }
-  __syncthreads();
-
   const int lc = tid / TPC, tau = tid - lc * TPC;
-  const bool active = tid < S::THREADS && lc < ncell;
-  const long long cell = CPB == 1 ? cell0 : cell0 + lc;
-  T *cl = lds + lc * CELLP;
-  // cell coordinates (cell ids fit in 32 bits; checked on the host)
-  int coord[D];
-  {
-    unsigned rem = (unsigned)cell;
-#pragma unroll
-    for (int d = 0; d < D; ++d)
-      {
-        const unsigned q = rem / (unsigned)mesh.n[d];
-        coord[d] = (int)(rem - q * (unsigned)mesh.n[d]);
-        rem = q;
-      }
-  }
-
-  // 2) every layout's values into registers
-  T u[P][NN];
+  const int lbase = lc * CELLP;
   int base[P];
 #pragma unroll
   for (int p = 0; p < P; ++p)
-    {
-      const int a = lay_a(p), b = lay_b<D>(p);
-      base[p] = layout_base<D, N>(tau, a, b);
-      const int sa = ipow(N, a), sb = ipow(N, b);
-#pragma unroll
-      for (int r = 0; r < NN; ++r)
-        u[p][r] = active ? cl[padded<N>(base[p] + (r % N) * sa + (r / N) * sb)]
-                         : T(0);
-    }
-  __syncthreads();
+    base[p] = layout_base<D, N>(tau, lay_a(p), lay_b<D>(p));
 
-  // 3) contributions per layout, summed in LDS
-#pragma unroll
-  for (int p = 0; p < P; ++p)
-    {
-      T out[NN];
-#pragma unroll
-      for (int r = 0; r < NN; ++r)
-        out[r] = T(0);
-      if (active)
-        block_contrib<D, N, T>(p, base[p], cell, coord, src, coef, mesh,
-                               inflow, u[p], out);
-      const int a = lay_a(p), b = lay_b<D>(p);
-      const int sa = ipow(N, a), sb = ipow(N, b);
-      if (active)
-#pragma unroll
-        for (int r = 0; r < NN; ++r)
-          {
-            T *q = cl + padded<N>(base[p] + (r % N) * sa + (r / N) * sb);
-            if (p == 0)
-              *q = out[r];
-            else
-              *q += out[r];
-          }
-      __syncthreads();
-    }
-
-  // 4) coalesced epilogue
+  PF pf;
   {
-    const int nchunks = ncell * CELL / VEC;
-    for (int c = tid; c < nchunks; c += S::THREADS)
+    const long long c0 = lb * CPB;
+    const int nc = (int)min((long long)CPB, mesh.n_cells - c0);
+    issue_chunks<D, N, T>(pf, src, c0, nc);
+    if (tid < S::THREADS && lc < nc)
       {
-        const int i = c * VEC;
-        const int lcc = i / CELL, li = i - lcc * CELL;
-        const T *k = lds + lcc * CELLP + padded<N>(li);
-        const long long g = cell0 * CELL + i;
-        if (MODE == 0)
-          {
-            V v;
+        const long long cell = CPB == 1 ? c0 : c0 + lc;
+        int coord[D];
+        cell_coords<D>(cell, mesh, coord);
 #pragma unroll
-            for (int e = 0; e < VEC; ++e)
-              v.v[e] = k[e];
-            reinterpret_cast<V *>(dst)[g / VEC] = v;
-          }
-        else
-          {
-            double s[VEC];
-#pragma unroll
-            for (int e = 0; e < VEC; ++e)
-              s[e] = stage.sol[g + e];
-#pragma unroll
-            for (int e = 0; e < VEC; ++e)
-              stage.sol[g + e] = s[e] + stage.b_dt * (double)k[e];
-            if (stage.ti_out)
-              {
-                V v;
-#pragma unroll
-                for (int e = 0; e < VEC; ++e)
-                  v.v[e] = (T)(s[e] + stage.a_dt * (double)k[e]);
-                reinterpret_cast<V *>(stage.ti_out)[g / VEC] = v;
-              }
-          }
+        for (int p = 0; p < P; ++p)
+          issue_faces<D, N, T>(pf, p, src, cell, coord, base[p], mesh,
+                               inflow);
       }
   }
+
+  for (; lb < lend; lb += slots)
+    {
+      const long long cell0 = lb * CPB;
+      const int ncell = (int)min((long long)CPB, mesh.n_cells - cell0);
+      const bool active = tid < S::THREADS && lc < ncell;
+      const int nchunks = ncell * CELL / VEC;
+      // 1) this cell's values into LDS, keep its faces
+#pragma unroll
+      for (int k = 0; k < PF::CPT; ++k)
+        {
+          const int c = tid + k * kBlock;
+          if (c < nchunks)
+            {
+              const int i = c * VEC;
+              const int l = i / CELL, li = i - l * CELL;
+              T *dl = U + l * CELLP + padded<N>(li);
+#pragma unroll
+              for (int e = 0; e < VEC; ++e)
+                dl[e] = pf.c[k].v[e];
+            }
+        }
+      __syncthreads();
+
+      // 2) the next cell's values go out now and land during the arithmetic;
+      // its face values follow layout by layout, as soon as the current
+      // layout has consumed its own
+      const long long ln = lb + slots;
+      const bool next = ln < lend;
+      long long ncell0 = 0, nxt_cell = 0;
+      bool nxt_active = false;
+      int ncoord[D];
+      if (next)
+        {
+          ncell0 = ln * CPB;
+          const int nc = (int)min((long long)CPB, mesh.n_cells - ncell0);
+          issue_chunks<D, N, T>(pf, src, ncell0, nc);
+          nxt_active = tid < S::THREADS && lc < nc;
+          nxt_cell = CPB == 1 ? ncell0 : ncell0 + lc;
+          if (nxt_active)
+            cell_coords<D>(nxt_cell, mesh, ncoord);
+        }
+
+      // 3) per layout: read the block from U, add its directions into R
+#pragma unroll
+      for (int p = 0; p < P; ++p)
+        {
+          const int sa = ipow(N, lay_a(p)), sb = ipow(N, lay_b<D>(p));
+          if (active)
+            {
+              T u[NN], out[NN];
+#pragma unroll
+              for (int r = 0; r < NN; ++r)
+                {
+                  u[r] = U[lbase +
+                           padded<N>(base[p] + (r % N) * sa + (r / N) * sb)];
+                  out[r] = T(0);
+                }
+              block_contrib<D, N, T>(p, coef, inflow, u, pf.face[p], out);
+#pragma unroll
+              for (int r = 0; r < NN; ++r)
+                {
+                  T *q = R + lbase +
+                         padded<N>(base[p] + (r % N) * sa + (r / N) * sb);
+                  if (p == 0)
+                    *q = out[r];
+                  else
+                    *q += out[r];
+                }
+            }
+          if (nxt_active)
+            issue_faces<D, N, T>(pf, p, src, nxt_cell, ncoord, base[p], mesh,
+                                 inflow);
+          __syncthreads();
+        }
+
+      // 4) coalesced epilogue from R.  U is rewritten at the top of the next
+      // iteration only after every thread passed the layout barriers, and R
+      // only in the next iteration's first layout, after the barrier that
+      // follows the U write.
+#pragma unroll
+      for (int k = 0; k < PF::CPT; ++k)
+        {
+          const int c = tid + k * kBlock;
+          if (c >= nchunks)
+            continue;
+          const int i = c * VEC;
+          const int l = i / CELL, li = i - l * CELL;
+          const T *kv = R + l * CELLP + padded<N>(li);
+          const long long g = cell0 * CELL + i;
+          if (MODE == 0)
+            {
+              Chunk<T, VEC> v;
+#pragma unroll
+              for (int e = 0; e < VEC; ++e)
+                v.v[e] = kv[e];
+              reinterpret_cast<Chunk<T, VEC> *>(dst)[g / VEC] = v;
+            }
+          else
+            {
+              double s[VEC];
+#pragma unroll
+              for (int e = 0; e < VEC; ++e)
+                s[e] = stage.sol[g + e];
+#pragma unroll
+              for (int e = 0; e < VEC; ++e)
+                stage.sol[g + e] = s[e] + stage.b_dt * (double)kv[e];
+              if (stage.ti_out)
+                {
+                  Chunk<T, VEC> v;
+#pragma unroll
+                  for (int e = 0; e < VEC; ++e)
+                    v.v[e] = (T)(s[e] + stage.a_dt * (double)kv[e]);
+                  reinterpret_cast<Chunk<T, VEC> *>(stage.ti_out)[g / VEC] = v;
+                }
+            }
+        }
+    }
+}
+
+int g_slots_per_xcd = 0;
+
+int slots_per_xcd()
+{
+  if (g_slots_per_xcd == 0)
+    {
+      int dev = 0, cus = 256;
+      if (hipGetDevice(&dev) == hipSuccess)
+        (void)hipDeviceGetAttribute(&cus,
+                                    hipDeviceAttributeMultiprocessorCount, dev);
+      g_slots_per_xcd = 2 * ((cus + 7) / 8); // two blocks per CU
+    }
+  return g_slots_per_xcd;
 }
 
 template <int D, int N, typename T>
@@ -289,17 +414,20 @@ hipError_t launch(int mode, const T *src, T *dst, const T *coef,
                   const MeshDev &mesh, const InflowDev &inflow,
                   const StageDev &stage, hipStream_t stream)
 {
-  using S = Shape<D, N>;
+  using S = Shape<D, N, T>;
   const long long n_logical = (mesh.n_cells + S::CPB - 1) / S::CPB;
-  const long long grid = 8 * ((n_logical + 7) / 8);
+  // no more blocks per XCD than the XCD has logical cells
+  const long long per = (n_logical + 7) / 8;
+  const int used = (int)min((long long)slots_per_xcd(), per);
+  const unsigned grid = 8u * (unsigned)used;
   if (mode == 0)
-    hipLaunchKernelGGL((apply_kernel<D, N, T, 0>), dim3((unsigned)grid),
-                       dim3(kBlock), 0, stream, src, dst, coef, mesh, inflow,
-                       stage, n_logical);
+    hipLaunchKernelGGL((apply_kernel<D, N, T, 0>), dim3(grid), dim3(kBlock),
+                       0, stream, src, dst, coef, mesh, inflow, stage,
+                       n_logical, used);
   else
-    hipLaunchKernelGGL((apply_kernel<D, N, T, 1>), dim3((unsigned)grid),
-                       dim3(kBlock), 0, stream, src, dst, coef, mesh, inflow,
-                       stage, n_logical);
+    hipLaunchKernelGGL((apply_kernel<D, N, T, 1>), dim3(grid), dim3(kBlock),
+                       0, stream, src, dst, coef, mesh, inflow, stage,
+                       n_logical, used);
   return hipGetLastError();
 }
 

@@ -77,6 +77,11 @@ struct QuadDev
   int bc_inhomogeneous; // Dirichlet: 1 -> u+ = -u- + 2 g, 0 -> u+ = -u-
   int bc_function;
   double bc_params[kMaxDim + 2];
+  // Vlasov-Poisson velocity: x-velocity = v at the v quadrature point,
+  // v-velocity = grad phi at the x quadrature point (field array
+  // [x cell][x quadrature point][dim_x])
+  int vp;
+  long long n_x_cells;
 };
 
 // an analytic function and the 1D nodes it is interpolated at
@@ -97,6 +102,29 @@ struct NormDev
   int nd, nq;
 };
 
+// Vlasov-Poisson: the v quadrature weights of the nodal v points
+struct VpDev
+{
+  double wv[64]; // JxW_v at the Gauss-Lobatto v nodes ((k+1)^dim_v <= 64)
+  int dofs_x, dofs_v;
+};
+
+// the x-space (Poisson) problem: exact 1D matrices of the nodal basis
+struct XDev
+{
+  int dim;
+  int n[3];
+  double h[3];
+  long long n_cells;
+  double kk;      // penalty factor k (k+1)
+  double M[64];   // int l_i l_j on [0,1]          (N x N)
+  double K[64];   // int l_i' l_j' on [0,1]
+  double d0[8];   // l_i'(0)
+  double d1[8];   // l_i'(1)
+  double S[64];   // l_j at the quadrature points  (N x N)
+  double G[64];   // l_j' at the quadrature points
+};
+
 // host launchers (one translation unit per kernel family)
 bool collapsed_supported(int D, int N);
 hipError_t collapsed_apply_f64(int D, int N, int mode, const double *src,
@@ -112,7 +140,7 @@ hipError_t quadrature_apply(int D, int N, int mode, const double *src,
                             double *dst, const QuadDev &q,
                             const MeshDev &mesh, const StageDev &stage,
                             double time, int dim_x, int collocation,
-                            hipStream_t stream);
+                            const double *efield, hipStream_t stream);
 hipError_t launch_interpolate(double *dst, const MeshDev &mesh,
                               const FuncDev &f, double t, long long n_dofs,
                               int dofs_cell, hipStream_t stream);
@@ -120,6 +148,16 @@ hipError_t launch_noise(double *dst, uint64_t seed, double amplitude,
                         long long n, hipStream_t stream);
 hipError_t launch_convert_f32(const double *src, float *dst, long long n,
                               hipStream_t stream);
+hipError_t vp_density(const double *f, const float *f32, const VpDev &vp,
+                      long long n_x_cells, long long n_v_cells,
+                      double *partial, int n_chunks, double *rho,
+                      hipStream_t stream);
+bool vp_poisson_supported(int dim_x, int N, long long n_x_cells);
+hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
+                      double *efield, double *work, double rel_tol,
+                      int max_it, long long *stats, hipStream_t stream);
+hipError_t vp_moments(const double *f, const MeshDev &mesh, const NormDev &nt,
+                      int dim_x, int vol, double *partial, hipStream_t stream);
 hipError_t launch_norm(const double *src, const MeshDev &mesh,
                        const NormDev &nt, const FuncDev &f, double t, int vol,
                        double *partial, hipStream_t stream);

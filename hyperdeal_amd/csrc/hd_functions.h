@@ -37,6 +37,20 @@ __host__ __device__ inline double eval_function(int id, const double *prm, int d
             r *= sin(2.0 * pi * (x[d] - t * prm[d]));
           return r;
         }
+      case 5: // Landau: (1 + eps prod_{d < np} cos(k x_d)) Maxwellian(v),
+              // params eps, k, np (vlasov_poisson/cases/hyperrectangle.h:
+              // 44-58 is np = 1)
+        {
+          double px = 1.0;
+          const int np = (int)prm[2];
+          for (int d = 0; d < np; ++d)
+            px *= cos(prm[1] * x[d]);
+          double v2 = 0.0;
+          for (int d = dim_x; d < D; ++d)
+            v2 += x[d] * x[d];
+          return (1.0 + prm[0] * px) * exp(-0.5 * v2) /
+                 pow(2.0 * pi, 0.5 * (D - dim_x));
+        }
       default:
         return 0.0;
     }

@@ -79,6 +79,33 @@ __device__ inline void contract_lines(double *buf, int cellp, int d,
     }
 }
 
+// transport velocity of direction d at the in-cell point pt of `cell`
+// (constant, or the Vlasov-Poisson field; a face point's digit d is unused)
+template <int D, int N>
+__device__ inline double point_velocity(const QuadDev &q, const MeshDev &m,
+                                        const double *efield, int dim_x,
+                                        int d, long long cell, int pt)
+{
+  if (!q.vp)
+    return q.a[d];
+  if (d < dim_x)
+    {
+      const int e = dim_x + d;
+      int pe = 1;
+      for (int j = 0; j < e; ++j)
+        pe *= N;
+      const int qe = (pt / pe) % N;
+      const int ce = (int)((cell / m.stride[e]) % m.n[e]);
+      return m.lo[e] + (ce + q.qp[qe]) * m.h[e];
+    }
+  const int e = d - dim_x;
+  int nqx = 1;
+  for (int j = 0; j < dim_x; ++j)
+    nqx *= N;
+  const long long xc = cell % q.n_x_cells;
+  return efield[(xc * nqx + pt % nqx) * dim_x + e];
+}
+
 template <int D, int N>
 __device__ inline double jxw_point(const QuadDev &q, const MeshDev &m, int i)
 {
@@ -96,7 +123,8 @@ __device__ inline double jxw_point(const QuadDev &q, const MeshDev &m, int i)
 template <int D, int N, int MODE>
 __global__ __launch_bounds__(kBlock) void apply_kernel(
   const double *__restrict__ src, double *__restrict__ dst, QuadDev q,
-  MeshDev mesh, StageDev stage, double time, int dim_x, int collocation)
+  MeshDev mesh, StageDev stage, double time, int dim_x, int collocation,
+  const double *__restrict__ efield)
 {
   using Sh = Shape<D, N>;
   constexpr int CELL = Sh::CELL, FACE = Sh::FACE, CPB = Sh::CPB;
@@ -136,7 +164,6 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   for (int d = 0; d < D; ++d)
     {
       const int sd = ipow(N, d);
-      const double c = q.a[d] / mesh.h[d];
       for (int l = tid; l < ncell * FACE; l += kBlock)
         {
           const int lc = l / FACE;
@@ -148,9 +175,14 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
           for (int j = 0; j < N; ++j)
             {
               uq[j] = u[j * sd];
-              jw[j] = jxw_point<D, N>(q, mesh, base + j * sd);
+              // velocity / h folded into the quadrature weight
+              jw[j] = jxw_point<D, N>(q, mesh, base + j * sd) *
+                      point_velocity<D, N>(q, mesh, efield, dim_x, d,
+                                           cell0 + lc, base + j * sd) /
+                      mesh.h[d];
               acc[j] = 0.0;
             }
+          const double c = 1.0;
           if (q.skew != 0.0)
 #pragma unroll
             for (int i = 0; i < N; ++i)
@@ -183,7 +215,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
     {
       const int d = f / 2, side = f % 2;
       const int sd = ipow(N, d);
-      const double an = q.a[d] * (side ? 1.0 : -1.0);
+      const double nrm = side ? 1.0 : -1.0;
       // u- (trace) and the neighbour's face nodes
       for (int l = tid; l < ncell * FACE; l += kBlock)
         {
@@ -302,6 +334,8 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
                 }
               up = -um + 2.0 * g;
             }
+          const double an =
+            point_velocity<D, N>(q, mesh, efield, dim_x, d, cell, base) * nrm;
           const double flux =
             0.5 * ((um + up) * an + fabs(an) * (um - up));
           const double fd = -(flux - q.skew * um * an) * jf;
@@ -349,7 +383,7 @@ template <int D, int N>
 hipError_t launch(int mode, const double *src, double *dst,
                   const QuadDev &q, const MeshDev &mesh,
                   const StageDev &stage, double time, int dim_x,
-                  int collocation, hipStream_t stream)
+                  int collocation, const double *efield, hipStream_t stream)
 {
   using Sh = Shape<D, N>;
   const size_t lds = sizeof(double) *
@@ -365,11 +399,11 @@ hipError_t launch(int mode, const double *src, double *dst,
   if (mode == 0)
     hipLaunchKernelGGL((apply_kernel<D, N, 0>), dim3((unsigned)grid),
                        dim3(kBlock), lds, stream, src, dst, q, mesh, stage,
-                       time, dim_x, collocation);
+                       time, dim_x, collocation, efield);
   else
     hipLaunchKernelGGL((apply_kernel<D, N, 1>), dim3((unsigned)grid),
                        dim3(kBlock), lds, stream, src, dst, q, mesh, stage,
-                       time, dim_x, collocation);
+                       time, dim_x, collocation, efield);
   return hipGetLastError();
 }
 } // namespace quad
@@ -387,12 +421,12 @@ hipError_t quadrature_apply(int D, int N, int mode, const double *src,
                             double *dst, const QuadDev &q,
                             const MeshDev &mesh, const StageDev &stage,
                             double time, int dim_x, int collocation,
-                            hipStream_t stream)
+                            const double *efield, hipStream_t stream)
 {
 #define HD_CASE(DD, NN_)                                                     \
   if (D == DD && N == NN_)                                                   \
     return quad::launch<DD, NN_>(mode, src, dst, q, mesh, stage, time, dim_x, \
-                                 collocation, stream);
+                                 collocation, efield, stream);
   HD_CASE(2, 2) HD_CASE(2, 3) HD_CASE(2, 4) HD_CASE(2, 5) HD_CASE(2, 6)
   HD_CASE(3, 2) HD_CASE(3, 3) HD_CASE(3, 4) HD_CASE(3, 5) HD_CASE(3, 6)
   HD_CASE(4, 2) HD_CASE(4, 3) HD_CASE(4, 4) HD_CASE(4, 5) HD_CASE(4, 6)

@@ -179,8 +179,7 @@ bool Tables1D::build(int degree, int n_points, bool coll)
   nd = degree + 1;
   nq = n_points;
   collocation = coll;
-  std::vector<double> wn;
-  gauss_lobatto(nd, nodes, wn);
+  gauss_lobatto(nd, nodes, node_weights);
   if (coll)
     gauss_lobatto(nq, qpoints, qweights);
   else
@@ -194,6 +193,34 @@ bool Tables1D::build(int degree, int n_points, bool coll)
       S[q * nd + i] = coll ? (q == i ? 1.0 : 0.0)
                            : (double)lagrange(nodes, bwn, i, qpoints[q]);
   Dq = diff_matrix(qpoints);
+  // derivatives of the nodal basis: at the quadrature points and the ends
+  {
+    const auto dnodal = [&](int i, ld x) {
+      ld sum = 0.0L;
+      for (int m = 0; m < nd; ++m)
+        {
+          if (m == i)
+            continue;
+          ld prod = 1.0L / ((ld)nodes[i] - nodes[m]);
+          for (int l = 0; l < nd; ++l)
+            if (l != i && l != m)
+              prod *= (x - nodes[l]) / ((ld)nodes[i] - nodes[l]);
+          sum += prod;
+        }
+      return sum;
+    };
+    G.assign(nq * nd, 0.0);
+    for (int q = 0; q < nq; ++q)
+      for (int i = 0; i < nd; ++i)
+        G[q * nd + i] = (double)dnodal(i, qpoints[q]);
+    dnode0.assign(nd, 0.0);
+    dnode1.assign(nd, 0.0);
+    for (int i = 0; i < nd; ++i)
+      {
+        dnode0[i] = (double)dnodal(i, 0.0L);
+        dnode1[i] = (double)dnodal(i, 1.0L);
+      }
+  }
   F.assign(2 * nq, 0.0);
   for (int s = 0; s < 2; ++s)
     for (int p = 0; p < nq; ++p)

@@ -23,6 +23,10 @@ struct Tables1D
                             //           Lagrange basis at quadrature points
   std::vector<double> F;    // 2 x nq    quadrature-point basis at 0 and 1
   std::vector<double> Sinv; // nd x nq   (S^T W S)^{-1} S^T W
+  std::vector<double> node_weights; // Gauss-Lobatto weights of the nodes
+  std::vector<double> dnode0, dnode1; // nodal basis derivatives at 0 and 1
+  std::vector<double> G;    // nq x nd   nodal basis derivatives at the
+                            //           quadrature points
 
   // returns false for unsupported (degree, n_points, collocation)
   bool build(int degree, int n_points, bool coll);

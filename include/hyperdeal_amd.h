@@ -67,7 +67,10 @@ typedef enum hd_function
                                  examples/advection/cases/hyperrectangle.h:33-55 */
   HD_FUNC_CONSTANT_ONE = 2,
   HD_FUNC_SIN_COS = 3,        /* sin(pi x0) prod cos(pi xd), tests_functions.h */
-  HD_FUNC_PRODUCT_SINES = 4   /* prod sin(2pi(xd - ad t)) */
+  HD_FUNC_PRODUCT_SINES = 4,  /* prod sin(2pi(xd - ad t)) */
+  HD_FUNC_LANDAU = 5          /* (1 + eps prod_{d<np} cos(k x_d)) exp(-|v|^2/2)
+                                 / (2pi)^{dim_v/2}, params (eps, k, np),
+                                 vlasov_poisson/cases/hyperrectangle.h:44-58 */
 } hd_function;
 
 /* A Cartesian phase-space box: dim_x + dim_v directions (x first). */
@@ -155,6 +158,37 @@ int hd_lsrk_coefficients(const char *type, double *a, double *b);
 int hd_lsrk_step(hd_advection *op, const char *type, int precision,
                  double *sol, void *ti_a, void *ti_b, double time, double dt,
                  void *stream);
+
+/*
+ * Vlasov-Poisson operator (examples/vlasov_poisson/include/application.h:
+ * 516-602): the advection apply with the phase-space velocity (v, grad phi)
+ * (vlasov_poisson/include/velocity_field_view.h:28-209), where phi solves
+ * the periodic SIPG Poisson problem -lap phi = -(rho - mean rho)
+ * (poisson.h:52-312) for rho = int f dv (numerics/vector_tools.h:230-315).
+ * Needs dim_x == dim_v, a periodic box, n_points = degree + 1; FP64.  With
+ * auto update on (the default) every apply / stage first recomputes rho,
+ * phi (FP64 conjugate gradients from the previous phi to poisson_tol
+ * relative residual) and grad phi from its input, as the reference's RK
+ * right-hand side does.
+ */
+int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
+                             double poisson_tol, hd_advection **out);
+/* rho, phi and grad phi from f (FP64 phase-space vector) */
+int hd_vp_update_field(hd_advection *op, const double *f, void *stream);
+int hd_vp_set_auto_update(hd_advection *op, int enable);
+int hd_vp_set_poisson_tolerance(hd_advection *op, double rel_tol, int max_it);
+/* accumulated CG iterations and solves, last relative residual (sync) */
+int hd_vp_solver_stats(hd_advection *op, long long *iterations,
+                       long long *solves, double *last_rel_residual);
+/* x-space DoFs of phi / rho: (k+1)^dim_x per x cell */
+long long hd_vp_x_n_dofs(const hd_advection *op);
+/* copy phi device -> host (device_to_host != 0) or host -> device */
+int hd_vp_potential(hd_advection *op, double *phi, int device_to_host);
+/* sum (d_e phi)^2 JxW per x component (sync) */
+int hd_vp_field_energy(hd_advection *op, double *out);
+/* mass, ||f||_L2, kinetic energy, momentum[3] of f (sync) */
+int hd_vp_moments(hd_matrix_free *mf, const double *f, double *out,
+                  void *stream);
 
 /* dst (FP64) = function at the Gauss-Lobatto nodes (numerics/
  * vector_tools.h:81-135); params: D values or NULL. */
