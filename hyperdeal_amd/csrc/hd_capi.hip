@@ -134,6 +134,11 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
   hipError_t e;
   if (algo == HD_ALGO_COLLAPSED)
     {
+      // the cell and face loads are 16-byte vector loads
+      if (reinterpret_cast<uintptr_t>(src) % 16 != 0)
+        return fail(HD_ERR_INVALID,
+                    "the collapsed kernel needs a 16-byte aligned source "
+                    "vector");
       if (precision == HD_FP64)
         e = hd::collapsed_apply_f64(mf->D, mf->nd, mode,
                                     static_cast<const double *>(src),

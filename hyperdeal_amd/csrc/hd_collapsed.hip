@@ -15,22 +15,27 @@
 // The outflow neighbour never enters the upwind flux.
 //
 // Work decomposition.  A block of 256 threads works on CPB cells at a time
-// (one cell of N^D values for 3+3, k=3).  A thread owns an N x N block of
-// points in a pair of directions (layout p owns directions 2p, 2p+1), so the
-// 1D contractions of both run in registers.  The cell sits in LDS (U, padded
-// by one value per N^2 so the three strided layouts read without bank
-// conflicts); each layout adds its contribution into a second LDS buffer R,
-// which is stored with coalesced 16-byte accesses.
+// (one cell of N^D values for 3+3, k=3).  A thread owns an N x N tile of
+// points in a pair of directions (a "layout"), so the 1D contractions of both
+// directions run in registers; the D directions are covered by (D+1)/2
+// layouts.  The cell sits in LDS (U) in a layout whose strides are chosen so
+// that every layout reads it without bank conflicts; the layouts
+// but the last add their contributions into a second LDS buffer R, and the
+// last layout adds R to its own contribution and stores to HBM directly (its
+// threads own points that are contiguous in memory).
 //
-// Blocks are persistent (two per CU) and walk an XCD-local range of cells:
-// while a block computes one cell, the next cell's values and all its inflow
-// face values are already in flight into registers, so HBM and L2 latency
-// overlap with the arithmetic.  The logical cells of one XCD are handed out
-// round-robin to its blocks, so the blocks of an XCD work on neighbouring
-// cells at the same time and face reads hit its L2 or the MALL.
+// Latency.  Blocks are persistent (two per CU in FP64, four in FP32) and walk
+// an XCD-local range of cells.  While a block computes one cell, the next
+// cell's values and inflow faces are already in flight into registers: the
+// loads are unconditional (the last iteration re-reads its own cell) and the
+// barriers order LDS only, so the compiler waits for each load with a counted
+// vmcnt instead of draining the queue.  The logical cells of one XCD are
+// handed out round-robin to its blocks, so the blocks of an XCD work on
+// neighbouring cells at the same time and face reads hit its L2 or the MALL.
 #include "hd_device.h"
 
 #include <type_traits>
+#include <utility>
 
 namespace hd
 {
@@ -38,211 +43,433 @@ namespace collapsed
 {
 constexpr int kBlock = 256;
 
-// LDS per block: two padded buffers of CPB cells, at most this many bytes so
-// that two blocks fit on a CU (160 KB)
-constexpr int kLdsPerBlock = 76 * 1024;
+// Orders the block's LDS accesses.  __syncthreads() would also wait for every
+// outstanding global load, i.e. for the prefetch of the next cell.
+__device__ inline void lds_barrier()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
+template <int I>
+using ic = std::integral_constant<int, I>;
+
+template <typename F, int... I>
+__device__ inline void static_for_impl(F &&f, std::integer_sequence<int, I...>)
+{
+  (f(ic<I>{}), ...);
+}
+
+// f(ic<0>), ..., f(ic<M-1>)
+template <int M, typename F>
+__device__ inline void static_for(F &&f)
+{
+  static_for_impl(f, std::make_integer_sequence<int, M>{});
+}
+
+// Layouts and the LDS layout of a cell.  Layout p owns directions a(p) and
+// b(p) (b only as a tile index when !b_active(p)).  Element (i_0..i_{D-1})
+// of a cell sits at LDS offset sum_d i_d W(d).
+//
+// 3+3, N=4: layouts (0,4), (1,5), (2,3), unpadded (W = 4^d).  gfx950's LDS
+// (64 banks of 4 bytes) checks a ds_read/write_b32 for bank conflicts in
+// groups of 16 lanes and a b64 / b128 access in groups of 8 lanes (measured);
+// under these rules every single-element tile access of the three layouts,
+// the 16-byte row accesses of the (0,4) layout along direction 0 and the
+// 16-byte chunk writes are conflict-free.  The paired forms
+// (ds_read2/write2_b64) check 16 lanes per address slot and would conflict,
+// so single-element tile accesses are volatile, which keeps the compiler from
+// pairing them.  The (0,4) layout's inflow faces of direction 4 are
+// contiguous in memory, and the (2,3) layout's points are contiguous runs of
+// 16 values, so its stores coalesce.
+// Other shapes: layouts (0,1), (2,3), ..., W = N^d + N^(d-2) for d >= 2.
+template <int D, int N>
+struct Plan
+{
+  static constexpr bool kTuned = (D == 6 && N == 4);
+  static constexpr int P = (D + 1) / 2;
+  __host__ __device__ static constexpr int a(int p)
+  {
+    return kTuned ? p : 2 * p;
+  }
+  __host__ __device__ static constexpr int b(int p)
+  {
+    return kTuned ? (p == 0 ? 4 : p == 1 ? 5 : 3)
+                  : (2 * p + 1 < D ? 2 * p + 1 : (p == 0 ? 1 : 0));
+  }
+  __host__ __device__ static constexpr bool b_active(int p)
+  {
+    return kTuned || 2 * p + 1 < D;
+  }
+  __host__ __device__ static constexpr int W(int d)
+  {
+    return ipow(N, d) + (!kTuned && d >= 2 ? ipow(N, d - 2) : 0);
+  }
+  __host__ __device__ static constexpr int span()
+  {
+    int s = 0;
+    for (int d = 0; d < D; ++d)
+      s += (N - 1) * W(d);
+    return s + 1;
+  }
+  // padded cell size, a multiple of N so that cells keep row alignment
+  static constexpr int CELLP = (span() + N - 1) / N * N;
+  // every W(d), d >= 1, is a multiple of N: rows along direction 0 start at
+  // multiples of N
+  static constexpr bool kAlignedRows = kTuned;
+  // a block walks whole rows of cells along direction 0 in upwind order, so
+  // the direction-0 inflow face of the next cell is the current cell's
+  // outflow face, already in LDS (one cell per block)
+  static constexpr bool kRowWalk = kTuned;
+};
 
 template <int D, int N, typename T>
 struct Shape
 {
+  using PL = Plan<D, N>;
+  static constexpr int P = PL::P;
   static constexpr int NN = N * N;
   static constexpr int CELL = ipow(N, D);
-  static constexpr int TPC = ipow(N, D - 2);    // threads per cell
-  static constexpr int CELLP = CELL + CELL / NN; // padded cell in LDS
+  static constexpr int TPC = ipow(N, D - 2);   // threads per cell
+  static constexpr int CELLP = PL::CELLP;
+  // LDS budget per block: that of the most resident blocks per CU
+  static constexpr int kLds = 160 * 1024 / (sizeof(T) == 8 ? 2 : 4) - 2048;
+  static constexpr int NBUF = P > 1 ? 2 : 1;
   static constexpr int CPB_THREADS = kBlock / TPC;
   static constexpr int CPB_LDS =
-    kLdsPerBlock / (2 * CELLP * (int)sizeof(T)) > 0
-      ? kLdsPerBlock / (2 * CELLP * (int)sizeof(T))
+    kLds / (NBUF * CELLP * (int)sizeof(T)) > 0
+      ? kLds / (NBUF * CELLP * (int)sizeof(T))
       : 1;
   static constexpr int CPB = CPB_THREADS < CPB_LDS ? CPB_THREADS : CPB_LDS;
-  static constexpr int THREADS = CPB * TPC;       // active threads
-  static constexpr int P = (D + 1) / 2;           // layouts
+  static constexpr int THREADS = CPB * TPC; // threads that own points
+  static constexpr bool kAllOwn = THREADS == kBlock;
+  // 16-byte chunks of the coalesced cell loads
+  static constexpr int VEC =
+    CELL % (16 / (int)sizeof(T)) == 0 ? 16 / (int)sizeof(T) : 1;
+  static constexpr int CHUNKS = CPB * CELL / VEC;
+  static constexpr int CPT = (CHUNKS + kBlock - 1) / kBlock;
+  // a chunk is one aligned run in LDS
+  static constexpr bool kChunkRuns = PL::kAlignedRows && N % VEC == 0;
+  // rows along direction 0 as 16-byte LDS / global vectors
+  static constexpr int RV = 16 / (int)sizeof(T);
+  static constexpr bool kRowVec = PL::kAlignedRows && N % RV == 0;
+  static constexpr bool kFaceVec = N % RV == 0;
   static_assert(TPC <= kBlock, "cell too large for one block");
 };
 
-__host__ __device__ constexpr int lay_a(int p) { return 2 * p; }
-template <int D>
-__host__ __device__ constexpr int lay_b(int p)
+template <typename T, int V>
+struct alignas(V * sizeof(T)) Vec
 {
-  return 2 * p + 1 < D ? 2 * p + 1 : (2 * p == 0 ? 1 : 0);
-}
-template <int D>
-__host__ __device__ constexpr bool lay_b_active(int p)
-{
-  return 2 * p + 1 < D;
-}
-
-// 16-byte (or smaller, when a cell is not a multiple of 16 bytes) chunk
-template <typename T, int VEC>
-struct alignas(VEC * sizeof(T)) Chunk
-{
-  T v[VEC];
+  T v[V];
 };
 
-template <typename T, int CELL>
-__host__ __device__ constexpr int chunk_width()
-{
-  return (CELL % (16 / (int)sizeof(T)) == 0) ? 16 / (int)sizeof(T) : 1;
-}
-
-template <int N>
-__device__ constexpr int padded(int i)
-{
-  return i + i / (N * N);
-}
-
-// index of the thread's (i_a, i_b) = (0, 0) point for layout (a, b)
+// LDS offset of the in-cell lexicographic index i
 template <int D, int N>
-__device__ inline int layout_base(int tau, int a, int b)
+__device__ inline int lds_offset(int i)
 {
-  int base = 0, pw = 1;
+  int off = 0;
 #pragma unroll
-  for (int j = 0; j < D; ++j)
+  for (int d = 0; d < D; ++d)
     {
-      if (j != a && j != b)
+      off += (i % N) * Plan<D, N>::W(d);
+      i /= N;
+    }
+  return off;
+}
+
+// global (in-cell) and LDS offsets of the (0, 0) point of the tile that
+// thread tau owns in the layout with tile directions (a, b)
+template <int D, int N>
+__device__ inline void tile_base(int tau, int a, int b, int &g, int &l)
+{
+  g = 0;
+  l = 0;
+  int pw = 1;
+#pragma unroll
+  for (int d = 0; d < D; ++d)
+    {
+      if (d != a && d != b)
         {
-          base += (tau % N) * pw;
+          const int dig = tau % N;
           tau /= N;
+          g += dig * pw;
+          l += dig * Plan<D, N>::W(d);
         }
       pw *= N;
     }
-  return base;
 }
 
-// Registers a thread carries from one cell to the next: its share of the
-// cell's values (coalesced chunks) and the inflow face values of its lines.
-template <int D, int N, typename T>
-struct Prefetch
+// tile element r = i_a + N i_b of layout p: read from an LDS buffer
+template <int D, int N, typename T, int p>
+__device__ inline void tile_read(const T *buf, int base, T *u)
 {
   using S = Shape<D, N, T>;
-  static constexpr int VEC = chunk_width<T, S::CELL>();
-  static constexpr int CHUNKS = S::CPB * S::CELL / VEC;
-  static constexpr int CPT = (CHUNKS + kBlock - 1) / kBlock;
-  Chunk<T, VEC> c[CPT];
-  T face[S::P][2][N];
-};
-
-template <int D, int N, typename T>
-__device__ inline void issue_chunks(Prefetch<D, N, T> &pf, const T *src,
-                                    long long cell0, int ncell)
-{
-  using S = Shape<D, N, T>;
-  using PF = Prefetch<D, N, T>;
-  constexpr int CELL = S::CELL, VEC = PF::VEC;
-  const Chunk<T, VEC> *g =
-    reinterpret_cast<const Chunk<T, VEC> *>(src + cell0 * CELL);
-  const int nchunks = ncell * CELL / VEC;
-#pragma unroll
-  for (int k = 0; k < PF::CPT; ++k)
+  using PL = Plan<D, N>;
+  constexpr int a = PL::a(p), b = PL::b(p);
+  if constexpr (a == 0 && S::kRowVec)
     {
-      const int c = threadIdx.x + k * kBlock;
-      if (c < nchunks)
-        pf.c[k] = g[c];
+      constexpr int RV = S::RV;
+#pragma unroll
+      for (int ib = 0; ib < N; ++ib)
+#pragma unroll
+        for (int c = 0; c < N; c += RV)
+          {
+            const Vec<T, RV> v = *reinterpret_cast<const Vec<T, RV> *>(
+              buf + base + ib * PL::W(b) + c);
+#pragma unroll
+            for (int e = 0; e < RV; ++e)
+              u[c + e + N * ib] = v.v[e];
+          }
+    }
+  else
+    {
+      const volatile __attribute__((address_space(3))) T *vb =
+        (const volatile __attribute__((address_space(3))) T *)buf;
+#pragma unroll
+      for (int r = 0; r < N * N; ++r)
+        u[r] = vb[base + (r % N) * PL::W(a) + (r / N) * PL::W(b)];
     }
 }
 
-// cell coordinates (cell ids fit in 32 bits; checked on the host)
+template <int D, int N, typename T, int p>
+__device__ inline void tile_write(T *buf, int base, const T *u)
+{
+  using S = Shape<D, N, T>;
+  using PL = Plan<D, N>;
+  constexpr int a = PL::a(p), b = PL::b(p);
+  if constexpr (a == 0 && S::kRowVec)
+    {
+      constexpr int RV = S::RV;
+#pragma unroll
+      for (int ib = 0; ib < N; ++ib)
+#pragma unroll
+        for (int c = 0; c < N; c += RV)
+          {
+            Vec<T, RV> v;
+#pragma unroll
+            for (int e = 0; e < RV; ++e)
+              v.v[e] = u[c + e + N * ib];
+            *reinterpret_cast<Vec<T, RV> *>(buf + base + ib * PL::W(b) + c) =
+              v;
+          }
+    }
+  else
+    {
+      volatile __attribute__((address_space(3))) T *vb =
+        (volatile __attribute__((address_space(3))) T *)buf;
+#pragma unroll
+      for (int r = 0; r < N * N; ++r)
+        vb[base + (r % N) * PL::W(a) + (r / N) * PL::W(b)] = u[r];
+    }
+}
+
+// Contributions of layout p's directions to its tile: out += A u + b face.
+//   u: tile values, r = i_a + N i_b; face[0][l]: inflow value of the line
+//   along a with i_b = l; face[1][l]: of the line along b with i_a = l
+// read-only data in the constant address space: uniform loads from it are
+// scalar loads through the scalar cache
+template <typename T>
+using cptr = const __attribute__((address_space(4))) T *;
+
+template <int D, int N, typename T, int p>
+__device__ inline void tile_contrib(cptr<T> coef, const T *u,
+                                    const T (*face)[N], T *out)
+{
+  using PL = Plan<D, N>;
+  constexpr int NN = N * N;
+  constexpr int a = PL::a(p), b = PL::b(p);
+  // The coefficients of all directions do not fit in SGPRs: hide the
+  // pointer from loop-invariant code motion so that each direction's scalar
+  // loads stay next to their use instead of being spilled across the loop.
+  asm volatile("" : "+s"(coef));
+  {
+    cptr<T> A = coef + a * (NN + N);
+    cptr<T> bv = A + NN;
+#pragma unroll
+    for (int l = 0; l < N; ++l)
+#pragma unroll
+      for (int i = 0; i < N; ++i)
+        {
+          T acc = bv[i] * face[0][l];
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            acc += A[i * N + j] * u[j + N * l];
+          out[i + N * l] += acc;
+        }
+  }
+  if constexpr (PL::b_active(p))
+    {
+      asm volatile("" : "+s"(coef));
+      cptr<T> A = coef + b * (NN + N);
+      cptr<T> bv = A + NN;
+#pragma unroll
+      for (int l = 0; l < N; ++l)
+#pragma unroll
+        for (int i = 0; i < N; ++i)
+          {
+            T acc = bv[i] * face[1][l];
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+              acc += A[i * N + j] * u[l + N * j];
+            out[l + N * i] += acc;
+          }
+    }
+}
+
+// the inflow face values of layout p's lines of one cell
+//   nb[d]: the inflow neighbour cell across d; fnode[d]: its face node
+//   from_lds: the direction-a face comes from the previous cell in LDS
+//   (U, at this thread's tile base lbase) instead of from src
+template <int D, int N, typename T, int p>
+__device__ inline void face_load(const T *__restrict__ src, int gbase,
+                                 const long long *nb, const int *fnode,
+                                 T (*face)[N], bool from_lds = false,
+                                 const T *U = nullptr, int lbase = 0)
+{
+  using S = Shape<D, N, T>;
+  using PL = Plan<D, N>;
+  constexpr int CELL = S::CELL;
+  constexpr int a = PL::a(p), b = PL::b(p);
+  constexpr int sa = ipow(N, a), sb = ipow(N, b);
+  if (from_lds)
+    {
+      const volatile __attribute__((address_space(3))) T *vu =
+        (const volatile __attribute__((address_space(3))) T *)U;
+#pragma unroll
+      for (int l = 0; l < N; ++l)
+        face[0][l] = vu[lbase + fnode[a] * PL::W(a) + l * PL::W(b)];
+    }
+  else
+    {
+      const T *s0 = src + nb[a] * CELL + gbase + fnode[a] * sa;
+#pragma unroll
+      for (int l = 0; l < N; ++l)
+        face[0][l] = s0[l * sb];
+    }
+  if constexpr (PL::b_active(p))
+    {
+      const T *s1 = src + nb[b] * CELL + gbase + fnode[b] * sb;
+      if constexpr (a == 0 && S::kFaceVec)
+        {
+          // the N values of the b face along direction 0 are contiguous and
+          // start at a multiple of N elements
+          constexpr int RV = S::RV;
+#pragma unroll
+          for (int c = 0; c < N; c += RV)
+            {
+              const Vec<T, RV> v =
+                *reinterpret_cast<const Vec<T, RV> *>(s1 + c);
+#pragma unroll
+              for (int e = 0; e < RV; ++e)
+                face[1][c + e] = v.v[e];
+            }
+        }
+      else
+        {
+#pragma unroll
+          for (int l = 0; l < N; ++l)
+            face[1][l] = s1[l * sa];
+        }
+    }
+}
+
+// this block's cells (from c0, ncell of them) as 16-byte chunks, chunk
+// tid + k kBlock into ch[k]; indices past the last chunk re-read it
+template <int D, int N, typename T, typename Chunks>
+__device__ __forceinline__ void load_chunks(Chunks &ch,
+                                            const T *__restrict__ src,
+                                            long long c0, int ncell)
+{
+  using S = Shape<D, N, T>;
+  constexpr int VEC = S::VEC;
+  const Vec<T, VEC> *g =
+    reinterpret_cast<const Vec<T, VEC> *>(src + c0 * S::CELL);
+  const int last = ncell * S::CELL / VEC - 1;
+#pragma unroll
+  for (int k = 0; k < S::CPT; ++k)
+    {
+      const Vec<T, VEC> v = g[min((int)threadIdx.x + k * kBlock, last)];
+#pragma unroll
+      for (int e = 0; e < VEC; ++e)
+        ch[k * VEC + e] = v.v[e];
+    }
+}
+
+template <int D, int N, typename T, typename Chunks>
+__device__ __forceinline__ void store_chunks(const Chunks &ch, T *U)
+{
+  using S = Shape<D, N, T>;
+  constexpr int VEC = S::VEC, CELL = S::CELL, CELLP = S::CELLP;
+#pragma unroll
+  for (int k = 0; k < S::CPT; ++k)
+    {
+      const int c = min((int)threadIdx.x + k * kBlock, S::CHUNKS - 1);
+      const int i = c * VEC, l = i / CELL, li = i - l * CELL;
+      if constexpr (S::kChunkRuns)
+        {
+          Vec<T, VEC> v;
+#pragma unroll
+          for (int e = 0; e < VEC; ++e)
+            v.v[e] = ch[k * VEC + e];
+          *reinterpret_cast<Vec<T, VEC> *>(U + l * CELLP +
+                                           lds_offset<D, N>(li)) = v;
+        }
+      else
+#pragma unroll
+        for (int e = 0; e < VEC; ++e)
+          U[l * CELLP + lds_offset<D, N>(li + e)] = ch[k * VEC + e];
+    }
+}
+
+// upwind neighbours of `cell` (cell ids fit in 32 bits; checked on the host)
 template <int D>
-__device__ inline void cell_coords(long long cell, const MeshDev &mesh,
-                                   int *coord)
+__device__ inline void inflow_cells(long long cell, const MeshDev &mesh,
+                                    const InflowDev &inflow, long long *nb)
 {
   unsigned rem = (unsigned)cell;
 #pragma unroll
   for (int d = 0; d < D; ++d)
     {
-      const unsigned q = rem / (unsigned)mesh.n[d];
-      coord[d] = (int)(rem - q * (unsigned)mesh.n[d]);
+      const unsigned n = (unsigned)mesh.n[d];
+      const unsigned q = rem / n;
+      const unsigned c = rem - q * n;
       rem = q;
-    }
-}
-
-// the inflow face values of layout p's lines of `cell`
-template <int D, int N, typename T>
-__device__ inline void issue_faces(Prefetch<D, N, T> &pf, int p,
-                                   const T *src, long long cell,
-                                   const int *coord, int base,
-                                   const MeshDev &mesh,
-                                   const InflowDev &inflow)
-{
-  constexpr int CELL = ipow(N, D);
-#pragma unroll
-  for (int pass = 0; pass < 2; ++pass)
-    {
-      if (pass == 1 && !lay_b_active<D>(p))
-        break;
-      const int a = lay_a(p), b = lay_b<D>(p);
-      const int d = pass == 0 ? a : b;
-      const int side = inflow.side[d];
-      if (side == 0)
-        continue;
-      const int sa = ipow(N, a), sb = ipow(N, b);
       const long long st = mesh.stride[d];
-      const int nd = mesh.n[d];
-      const long long nb =
-        side > 0 ? (coord[d] == 0 ? cell + (nd - 1) * st : cell - st)
-                 : (coord[d] == nd - 1 ? cell - (nd - 1) * st : cell + st);
-      const int fnode = side > 0 ? N - 1 : 0;
-      const T *ns = src + nb * CELL + base + fnode * (pass == 0 ? sa : sb);
-#pragma unroll
-      for (int l = 0; l < N; ++l)
-        pf.face[p][pass][l] = ns[l * (pass == 0 ? sb : sa)];
+      const long long left = c == 0 ? cell + (long long)(n - 1) * st : cell - st;
+      const long long right =
+        c == n - 1 ? cell - (long long)(n - 1) * st : cell + st;
+      // a direction without transport has zero coefficients: any cell will
+      // do, take the left one
+      nb[d] = inflow.side[d] < 0 ? right : left;
     }
 }
 
-// Contributions of directions (a [, b]) for one thread's N x N block.
-//   u: the block's values, r = i_a + N i_b; out: contributions
-template <int D, int N, typename T>
-__device__ inline void block_contrib(int p, const T *__restrict__ coef,
-                                     const InflowDev &inflow, const T *u,
-                                     const T (*face)[N], T *out)
-{
-  constexpr int NN = N * N;
-  const int a = lay_a(p), b = lay_b<D>(p);
-#pragma unroll
-  for (int pass = 0; pass < 2; ++pass)
-    {
-      if (pass == 1 && !lay_b_active<D>(p))
-        break;
-      const int d = pass == 0 ? a : b;
-      if (inflow.side[d] == 0)
-        continue;
-      const T *A = coef + d * (NN + N);
-      const T *bv = A + NN;
-#pragma unroll
-      for (int l = 0; l < N; ++l) // line index: i_b (pass 0) or i_a (pass 1)
-        {
-          const T f = face[pass][l];
-#pragma unroll
-          for (int i = 0; i < N; ++i)
-            {
-              T acc = bv[i] * f;
-#pragma unroll
-              for (int j = 0; j < N; ++j)
-                acc += A[i * N + j] * (pass == 0 ? u[j + N * l] : u[l + N * j]);
-              if (pass == 0)
-                out[i + N * l] += acc;
-              else
-                out[l + N * i] += acc;
-            }
-        }
-    }
-}
+// MODE 0: dst = apply(src).  MODE 1: LSRK stage with K = apply(src),
+// sol += b_dt K, ti_out = sol_old + a_dt K.  MODE 2: the stage without
+// ti_out (last stage).
+// resident blocks per CU: LDS allows two (FP64) or four (FP32) of the 3+3
+// cell; the FP32 stage keeps the FP64 solution values in registers and runs
+// three to stay clear of spills
+template <typename T, int MODE>
+constexpr int kBlocksPerCu = sizeof(T) == 8 ? 2 : (MODE == 0 ? 4 : 3);
 
-// MODE 0: dst = apply(src).  MODE 1: LSRK stage with K = apply(src).
 template <int D, int N, typename T, int MODE>
-__global__ __launch_bounds__(kBlock, 2) void apply_kernel(
+__global__ __launch_bounds__(kBlock, (kBlocksPerCu<T, MODE>)) void apply_kernel(
   const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ coef,
   MeshDev mesh, InflowDev inflow, StageDev stage, long long n_logical,
   int slots)
 {
   using S = Shape<D, N, T>;
-  using PF = Prefetch<D, N, T>;
-  constexpr int NN = S::NN, CELL = S::CELL, TPC = S::TPC, CPB = S::CPB;
-  constexpr int CELLP = S::CELLP, P = S::P, VEC = PF::VEC;
-  __shared__ T U[CPB * CELLP];
-  __shared__ T R[CPB * CELLP];
+  using PL = Plan<D, N>;
+  constexpr int P = S::P, NN = S::NN, CELL = S::CELL, TPC = S::TPC;
+  constexpr int CPB = S::CPB, CELLP = S::CELLP, VEC = S::VEC, CPT = S::CPT;
+  __shared__ __align__(16) T U[CPB * CELLP];
+  __shared__ __align__(16) T R[P > 1 ? CPB * CELLP : 1];
+  const cptr<T> ccoef = (cptr<T>)coef;
 
-  // this block's share of its XCD's contiguous range of logical cells
+  // this block's share of its XCD's contiguous range of logical units: rows
+  // of cells along direction 0 (row walk) or groups of CPB cells
   const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
   const long long per = (n_logical + 7) / 8;
   const long long lend = min(n_logical, (xcd + 1) * per);
@@ -251,162 +478,165 @@ __global__ __launch_bounds__(kBlock, 2) void apply_kernel(
     return; // uniform for the block, before any barrier
 
   const int tid = threadIdx.x;
-  const int lc = tid / TPC, tau = tid - lc * TPC;
-  const int lbase = lc * CELLP;
-  int base[P];
-#pragma unroll
-  for (int p = 0; p < P; ++p)
-    base[p] = layout_base<D, N>(tau, lay_a(p), lay_b<D>(p));
+  // threads past CPB * TPC shadow the last cell: they load and compute but
+  // write nothing
+  const int lc = min(tid / TPC, CPB - 1), tau = tid % TPC;
+  const bool owner = tid < S::THREADS;
 
-  PF pf;
-  {
-    const long long c0 = lb * CPB;
-    const int nc = (int)min((long long)CPB, mesh.n_cells - c0);
-    issue_chunks<D, N, T>(pf, src, c0, nc);
-    if (tid < S::THREADS && lc < nc)
-      {
-        const long long cell = CPB == 1 ? c0 : c0 + lc;
-        int coord[D];
-        cell_coords<D>(cell, mesh, coord);
+  int gbase[P], lbase[P];
+  static_for<P>([&](auto pc) {
+    constexpr int p = decltype(pc)::value;
+    tile_base<D, N>(tau, PL::a(p), PL::b(p), gbase[p], lbase[p]);
+    lbase[p] += lc * CELLP;
+  });
+  int fnode[D];
 #pragma unroll
-        for (int p = 0; p < P; ++p)
-          issue_faces<D, N, T>(pf, p, src, cell, coord, base[p], mesh,
-                               inflow);
-      }
+  for (int d = 0; d < D; ++d)
+    fnode[d] = inflow.side[d] < 0 ? 0 : N - 1;
+
+  T ch[CPT * VEC];
+  T fc[P][2][N];
+
+  // first cell of logical unit lb at row position k
+  constexpr bool RW = PL::kRowWalk;
+  static_assert(!RW || CPB == 1, "the row walk handles one cell at a time");
+  const int rowlen = RW ? mesh.n[0] : 1;
+  const bool fwd = inflow.side[0] >= 0;
+  auto first_cell = [&](long long unit, int k) -> long long {
+    return RW ? unit * rowlen + (fwd ? k : rowlen - 1 - k) : unit * CPB;
+  };
+  int k = 0;
+  long long c0 = first_cell(lb, 0);
+  int ncell = (int)min((long long)CPB, mesh.n_cells - c0);
+  {
+    long long nb[D];
+    inflow_cells<D>(min(c0 + lc, mesh.n_cells - 1), mesh, inflow, nb);
+    load_chunks<D, N, T>(ch, src, c0, ncell);
+    static_for<P>([&](auto pc) {
+      constexpr int p = decltype(pc)::value;
+      face_load<D, N, T, p>(src, gbase[p], nb, fnode, fc[p]);
+    });
   }
 
-  for (; lb < lend; lb += slots)
+  for (;;)
     {
-      const long long cell0 = lb * CPB;
-      const int ncell = (int)min((long long)CPB, mesh.n_cells - cell0);
-      const bool active = tid < S::THREADS && lc < ncell;
-      const int nchunks = ncell * CELL / VEC;
-      // 1) this cell's values into LDS, keep its faces
-#pragma unroll
-      for (int k = 0; k < PF::CPT; ++k)
-        {
-          const int c = tid + k * kBlock;
-          if (c < nchunks)
-            {
-              const int i = c * VEC;
-              const int l = i / CELL, li = i - l * CELL;
-              T *dl = U + l * CELLP + padded<N>(li);
-#pragma unroll
-              for (int e = 0; e < VEC; ++e)
-                dl[e] = pf.c[k].v[e];
-            }
-        }
-      __syncthreads();
+      // 1) this cell's values into U
+      store_chunks<D, N, T>(ch, U);
+      lds_barrier();
 
-      // 2) the next cell's values go out now and land during the arithmetic;
-      // its face values follow layout by layout, as soon as the current
-      // layout has consumed its own
-      const long long ln = lb + slots;
-      const bool next = ln < lend;
-      long long ncell0 = 0, nxt_cell = 0;
-      bool nxt_active = false;
-      int ncoord[D];
-      if (next)
-        {
-          ncell0 = ln * CPB;
-          const int nc = (int)min((long long)CPB, mesh.n_cells - ncell0);
-          issue_chunks<D, N, T>(pf, src, ncell0, nc);
-          nxt_active = tid < S::THREADS && lc < nc;
-          nxt_cell = CPB == 1 ? ncell0 : ncell0 + lc;
-          if (nxt_active)
-            cell_coords<D>(nxt_cell, mesh, ncoord);
-        }
+      // 2) the next cell's values go out now and land during the arithmetic
+      // (the last iteration re-reads its own cell); its faces follow layout
+      // by layout, as soon as the current layout has consumed its own
+      const bool in_row = RW && k + 1 < rowlen;
+      const long long ln = in_row ? lb : lb + slots;
+      const int nk = in_row ? k + 1 : 0;
+      const bool more = ln < lend;
+      const long long nc0 = more ? first_cell(ln, nk) : c0;
+      const int nncell = (int)min((long long)CPB, mesh.n_cells - nc0);
+      load_chunks<D, N, T>(ch, src, nc0, nncell);
+      long long nnb[D];
+      inflow_cells<D>(min(nc0 + lc, mesh.n_cells - 1), mesh, inflow, nnb);
 
-      // 3) per layout: read the block from U, add its directions into R
+      // 3) layouts 0 .. P-2 add their contributions into R
+      T ul[NN];
+      if constexpr (P > 1)
+        tile_read<D, N, T, P - 1>(U, lbase[P - 1], ul);
+      static_for<P - 1>([&](auto pc) {
+        constexpr int p = decltype(pc)::value;
+        T u[NN], out[NN];
+        tile_read<D, N, T, p>(U, lbase[p], u);
+        if constexpr (p == 0)
+          {
 #pragma unroll
-      for (int p = 0; p < P; ++p)
-        {
-          const int sa = ipow(N, lay_a(p)), sb = ipow(N, lay_b<D>(p));
-          if (active)
-            {
-              T u[NN], out[NN];
-#pragma unroll
-              for (int r = 0; r < NN; ++r)
-                {
-                  u[r] = U[lbase +
-                           padded<N>(base[p] + (r % N) * sa + (r / N) * sb)];
-                  out[r] = T(0);
-                }
-              block_contrib<D, N, T>(p, coef, inflow, u, pf.face[p], out);
-#pragma unroll
-              for (int r = 0; r < NN; ++r)
-                {
-                  T *q = R + lbase +
-                         padded<N>(base[p] + (r % N) * sa + (r / N) * sb);
-                  if (p == 0)
-                    *q = out[r];
-                  else
-                    *q += out[r];
-                }
-            }
-          if (nxt_active)
-            issue_faces<D, N, T>(pf, p, src, nxt_cell, ncoord, base[p], mesh,
-                                 inflow);
-          __syncthreads();
-        }
+            for (int r = 0; r < NN; ++r)
+              out[r] = T(0);
+          }
+        else
+          tile_read<D, N, T, p>(R, lbase[p], out);
+        tile_contrib<D, N, T, p>(ccoef, u, fc[p], out);
+        if (S::kAllOwn || owner)
+          tile_write<D, N, T, p>(R, lbase[p], out);
+        if constexpr (RW && p == 0 && PL::a(0) == 0)
+          face_load<D, N, T, p>(src, gbase[p], nnb, fnode, fc[p], in_row, U,
+                                lbase[p]);
+        else
+          face_load<D, N, T, p>(src, gbase[p], nnb, fnode, fc[p]);
+        lds_barrier();
+      });
 
-      // 4) coalesced epilogue from R.  U is rewritten at the top of the next
-      // iteration only after every thread passed the layout barriers, and R
-      // only in the next iteration's first layout, after the barrier that
-      // follows the U write.
+      // 4) the last layout: R + its own contribution, straight to HBM
+      {
+        constexpr int p = P - 1;
+        constexpr int a = PL::a(p), b = PL::b(p);
+        const bool store =
+          S::kAllOwn && CPB == 1 ? true : (owner && lc < ncell);
+        const long long g0 = (min(c0 + lc, mesh.n_cells - 1)) * CELL + gbase[p];
+        // the stage's solution values go out before the arithmetic
+        double sv[MODE == 0 ? 1 : NN];
+        if constexpr (MODE != 0)
+          {
 #pragma unroll
-      for (int k = 0; k < PF::CPT; ++k)
-        {
-          const int c = tid + k * kBlock;
-          if (c >= nchunks)
-            continue;
-          const int i = c * VEC;
-          const int l = i / CELL, li = i - l * CELL;
-          const T *kv = R + l * CELLP + padded<N>(li);
-          const long long g = cell0 * CELL + i;
-          if (MODE == 0)
-            {
-              Chunk<T, VEC> v;
+            for (int r = 0; r < NN; ++r)
+              sv[r] = stage.sol[g0 + (r % N) * ipow(N, a) +
+                                (r / N) * ipow(N, b)];
+          }
+        T out[NN];
+        if constexpr (P > 1)
+          tile_read<D, N, T, p>(R, lbase[p], out);
+        else
+          {
+            tile_read<D, N, T, p>(U, lbase[p], ul);
 #pragma unroll
-              for (int e = 0; e < VEC; ++e)
-                v.v[e] = kv[e];
-              reinterpret_cast<Chunk<T, VEC> *>(dst)[g / VEC] = v;
-            }
-          else
-            {
-              double s[VEC];
+            for (int r = 0; r < NN; ++r)
+              out[r] = T(0);
+          }
+        tile_contrib<D, N, T, p>(ccoef, ul, fc[p], out);
+        face_load<D, N, T, p>(src, gbase[p], nnb, fnode, fc[p]);
+        if (store)
+          {
 #pragma unroll
-              for (int e = 0; e < VEC; ++e)
-                s[e] = stage.sol[g + e];
-#pragma unroll
-              for (int e = 0; e < VEC; ++e)
-                stage.sol[g + e] = s[e] + stage.b_dt * (double)kv[e];
-              if (stage.ti_out)
-                {
-                  Chunk<T, VEC> v;
-#pragma unroll
-                  for (int e = 0; e < VEC; ++e)
-                    v.v[e] = (T)(s[e] + stage.a_dt * (double)kv[e]);
-                  reinterpret_cast<Chunk<T, VEC> *>(stage.ti_out)[g / VEC] = v;
-                }
-            }
-        }
+            for (int r = 0; r < NN; ++r)
+              {
+                const long long g =
+                  g0 + (r % N) * ipow(N, a) + (r / N) * ipow(N, b);
+                if constexpr (MODE == 0)
+                  dst[g] = out[r];
+                else
+                  {
+                    stage.sol[g] = sv[r] + stage.b_dt * (double)out[r];
+                    if constexpr (MODE == 1)
+                      static_cast<T *>(stage.ti_out)[g] =
+                        (T)(sv[r] + stage.a_dt * (double)out[r]);
+                  }
+              }
+          }
+      }
+      if (!more)
+        break;
+      lb = ln;
+      k = nk;
+      c0 = nc0;
+      ncell = nncell;
+      // one layout: its only U read was this iteration's, before the
+      // chunks of the next cell overwrite U
+      if constexpr (P == 1)
+        lds_barrier();
     }
 }
 
-int g_slots_per_xcd = 0;
+int g_cus = 0;
 
-int slots_per_xcd()
+int cus_per_xcd()
 {
-  if (g_slots_per_xcd == 0)
+  if (g_cus == 0)
     {
       int dev = 0, cus = 256;
       if (hipGetDevice(&dev) == hipSuccess)
         (void)hipDeviceGetAttribute(&cus,
                                     hipDeviceAttributeMultiprocessorCount, dev);
-      g_slots_per_xcd = 2 * ((cus + 7) / 8); // two blocks per CU
+      g_cus = (cus + 7) / 8;
     }
-  return g_slots_per_xcd;
+  return g_cus;
 }
 
 template <int D, int N, typename T>
@@ -415,19 +645,24 @@ hipError_t launch(int mode, const T *src, T *dst, const T *coef,
                   const StageDev &stage, hipStream_t stream)
 {
   using S = Shape<D, N, T>;
-  const long long n_logical = (mesh.n_cells + S::CPB - 1) / S::CPB;
+  const long long n_logical =
+    Plan<D, N>::kRowWalk ? mesh.n_cells / mesh.n[0]
+                         : (mesh.n_cells + S::CPB - 1) / S::CPB;
   // no more blocks per XCD than the XCD has logical cells
   const long long per = (n_logical + 7) / 8;
-  const int used = (int)min((long long)slots_per_xcd(), per);
-  const unsigned grid = 8u * (unsigned)used;
-  if (mode == 0)
-    hipLaunchKernelGGL((apply_kernel<D, N, T, 0>), dim3(grid), dim3(kBlock),
-                       0, stream, src, dst, coef, mesh, inflow, stage,
-                       n_logical, used);
+  auto go = [&](auto kernel, int bpc) {
+    const int used = (int)min((long long)(bpc * cus_per_xcd()), per);
+    hipLaunchKernelGGL(kernel, dim3(8u * (unsigned)used), dim3(kBlock), 0,
+                       stream, src, dst, coef, mesh, inflow, stage, n_logical,
+                       used);
+  };
+  const int m = mode == 0 ? 0 : (stage.ti_out ? 1 : 2);
+  if (m == 0)
+    go(apply_kernel<D, N, T, 0>, kBlocksPerCu<T, 0>);
+  else if (m == 1)
+    go(apply_kernel<D, N, T, 1>, kBlocksPerCu<T, 1>);
   else
-    hipLaunchKernelGGL((apply_kernel<D, N, T, 1>), dim3(grid), dim3(kBlock),
-                       0, stream, src, dst, coef, mesh, inflow, stage,
-                       n_logical, used);
+    go(apply_kernel<D, N, T, 2>, kBlocksPerCu<T, 2>);
   return hipGetLastError();
 }
 

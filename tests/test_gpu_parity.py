@@ -76,6 +76,35 @@ APPLY_CASES = [
 ]
 
 
+@pytest.mark.parametrize("a0", [0.7, -0.55, 0.0])
+@pytest.mark.parametrize("n0", [1, 3])
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_collapsed_3d3d_row_walk(a0, n0, precision):
+    """3+3, k=3: each block walks rows of cells along x0 in upwind order and
+    takes the x0 inflow face from the previous cell in LDS; rows of one and
+    three cells, transport to +x0, -x0 and none, against the oracle."""
+    print("case: row walk a0=%g n0=%d %s" % (a0, n0, precision), flush=True)
+    a = [a0, 0.3, -0.8, 0.45, -0.2, 0.9]
+    n_cells = [n0, 2, 3, 2, 2, 2]
+    P, mf, op, a = _pair((3, 3), 3, n_cells, seed=11, a=a)
+    op.set_algorithm(H.ALGO_COLLAPSED)
+    r = _rng(12)
+    u = P.interpolate() * (1.0 + 0.1 * r.standard_normal(P.n_dofs))
+    ref = P.apply(u)
+    if precision == "fp64":
+        out = torch.empty(P.n_dofs, dtype=torch.float64, device="cuda")
+        op.apply(out, torch.from_numpy(u).cuda())
+        tol = 1e-12
+    else:
+        u32 = torch.from_numpy(u).cuda().float()
+        ref = P.apply(u32.double().cpu().numpy())
+        out = torch.empty_like(u32)
+        op.apply(out, u32, precision=H.FP32)
+        tol = 2e-6
+    torch.cuda.synchronize()
+    assert _rel_err(out.double().cpu().numpy(), ref) <= tol
+
+
 @pytest.mark.parametrize("algo", ["quadrature", "collapsed"])
 @pytest.mark.parametrize("dims,degree,n_cells", APPLY_CASES,
                          ids=[f"{d[0]}+{d[1]}_k{k}" for d, k, _ in
