@@ -24,14 +24,17 @@
 // last layout adds R to its own contribution and stores to HBM directly (its
 // threads own points that are contiguous in memory).
 //
-// Latency.  Blocks are persistent (two per CU in FP64, four in FP32) and walk
-// an XCD-local range of cells.  While a block computes one cell, the next
-// cell's values and inflow faces are already in flight into registers: the
-// loads are unconditional (the last iteration re-reads its own cell) and the
-// barriers order LDS only, so the compiler waits for each load with a counted
-// vmcnt instead of draining the queue.  The logical cells of one XCD are
-// handed out round-robin to its blocks, so the blocks of an XCD work on
-// neighbouring cells at the same time and face reads hit its L2 or the MALL.
+// Latency.  Two (FP64) or four (FP32) blocks are resident per CU and each
+// walks several cells: for 3+3, k=3 one row of cells along x0 in upwind
+// order (the next cell's x0 inflow face is then the current cell's outflow
+// face, already in LDS), one block per row so that the dispatcher hands the
+// rows of an XCD out in order; for other shapes persistent blocks over an
+// XCD-local range.  While a block computes one cell, the next cell's values
+// and inflow faces are already in flight into registers: the loads are
+// unconditional (the last iteration re-reads its own cell) and the barriers
+// order LDS only, so the compiler waits for each load with a counted vmcnt
+// instead of draining the queue.  The blocks of an XCD work on neighbouring
+// cells at the same time, so face reads hit its L2 or the MALL.
 #include "hd_device.h"
 
 #include <type_traits>
@@ -118,13 +121,17 @@ struct Plan
   // every W(d), d >= 1, is a multiple of N: rows along direction 0 start at
   // multiples of N
   static constexpr bool kAlignedRows = kTuned;
-  // a block walks whole rows of cells along direction 0 in upwind order, so
-  // the direction-0 inflow face of the next cell is the current cell's
-  // outflow face, already in LDS (one cell per block)
+  // Row walk: a block owns G cells that are neighbours along direction 1
+  // (one per 256 threads) and walks their rows along direction 0 in upwind
+  // order.  The direction-0 inflow face of the next cell is then the current
+  // cell's outflow face, and the direction-1 inflow face of all but one of
+  // the G cells is the outflow face of a cell of the same block: both are
+  // read from LDS instead of HBM / L2.
   static constexpr bool kRowWalk = kTuned;
 };
 
-template <int D, int N, typename T>
+// G: cells per block along direction 1 (row walk only)
+template <int D, int N, typename T, int G = 1>
 struct Shape
 {
   using PL = Plan<D, N>;
@@ -133,6 +140,8 @@ struct Shape
   static constexpr int CELL = ipow(N, D);
   static constexpr int TPC = ipow(N, D - 2);   // threads per cell
   static constexpr int CELLP = PL::CELLP;
+  static constexpr bool kGroup = PL::kRowWalk;
+  static_assert(kGroup || G == 1, "cell groups need the row walk");
   // LDS budget per block: that of the most resident blocks per CU
   static constexpr int kLds = 160 * 1024 / (sizeof(T) == 8 ? 2 : 4) - 2048;
   static constexpr int NBUF = P > 1 ? 2 : 1;
@@ -141,14 +150,18 @@ struct Shape
     kLds / (NBUF * CELLP * (int)sizeof(T)) > 0
       ? kLds / (NBUF * CELLP * (int)sizeof(T))
       : 1;
-  static constexpr int CPB = CPB_THREADS < CPB_LDS ? CPB_THREADS : CPB_LDS;
+  static constexpr int CPB =
+    kGroup ? G : (CPB_THREADS < CPB_LDS ? CPB_THREADS : CPB_LDS);
+  static constexpr int BLOCK = kGroup ? G * TPC : kBlock;
   static constexpr int THREADS = CPB * TPC; // threads that own points
-  static constexpr bool kAllOwn = THREADS == kBlock;
-  // 16-byte chunks of the coalesced cell loads
+  static constexpr bool kAllOwn = THREADS == BLOCK;
+  // 16-byte chunks of the coalesced cell loads: per cell (groups) or over
+  // the block's consecutive cells
   static constexpr int VEC =
     CELL % (16 / (int)sizeof(T)) == 0 ? 16 / (int)sizeof(T) : 1;
   static constexpr int CHUNKS = CPB * CELL / VEC;
-  static constexpr int CPT = (CHUNKS + kBlock - 1) / kBlock;
+  static constexpr int CPT =
+    kGroup ? CELL / VEC / TPC : (CHUNKS + kBlock - 1) / kBlock;
   // a chunk is one aligned run in LDS
   static constexpr bool kChunkRuns = PL::kAlignedRows && N % VEC == 0;
   // rows along direction 0 as 16-byte LDS / global vectors
@@ -156,6 +169,7 @@ struct Shape
   static constexpr bool kRowVec = PL::kAlignedRows && N % RV == 0;
   static constexpr bool kFaceVec = N % RV == 0;
   static_assert(TPC <= kBlock, "cell too large for one block");
+  static_assert(!kGroup || CELL % (VEC * TPC) == 0, "chunks per cell");
 };
 
 template <typename T, int V>
@@ -319,28 +333,19 @@ __device__ inline void tile_contrib(cptr<T> coef, const T *u,
 
 // the inflow face values of layout p's lines of one cell
 //   nb[d]: the inflow neighbour cell across d; fnode[d]: its face node
-//   from_lds: the direction-a face comes from the previous cell in LDS
-//   (U, at this thread's tile base lbase) instead of from src
+//   skip_a: the direction-a face will be taken from LDS (a wave-uniform
+//   condition): no loads for it
 template <int D, int N, typename T, int p>
 __device__ inline void face_load(const T *__restrict__ src, int gbase,
                                  const long long *nb, const int *fnode,
-                                 T (*face)[N], bool from_lds = false,
-                                 const T *U = nullptr, int lbase = 0)
+                                 T (*face)[N], bool skip_a = false)
 {
   using S = Shape<D, N, T>;
   using PL = Plan<D, N>;
   constexpr int CELL = S::CELL;
   constexpr int a = PL::a(p), b = PL::b(p);
   constexpr int sa = ipow(N, a), sb = ipow(N, b);
-  if (from_lds)
-    {
-      const volatile __attribute__((address_space(3))) T *vu =
-        (const volatile __attribute__((address_space(3))) T *)U;
-#pragma unroll
-      for (int l = 0; l < N; ++l)
-        face[0][l] = vu[lbase + fnode[a] * PL::W(a) + l * PL::W(b)];
-    }
-  else
+  if (!skip_a)
     {
       const T *s0 = src + nb[a] * CELL + gbase + fnode[a] * sa;
 #pragma unroll
@@ -376,36 +381,66 @@ __device__ inline void face_load(const T *__restrict__ src, int gbase,
 
 // this block's cells (from c0, ncell of them) as 16-byte chunks, chunk
 // tid + k kBlock into ch[k]; indices past the last chunk re-read it
-template <int D, int N, typename T, typename Chunks>
+//   groups: the chunks of cell `mycell` (the thread's own), tau + k TPC
+template <int D, int N, typename T, int G, typename Chunks>
 __device__ __forceinline__ void load_chunks(Chunks &ch,
                                             const T *__restrict__ src,
-                                            long long c0, int ncell)
+                                            long long c0, int ncell,
+                                            long long mycell, int tau)
 {
-  using S = Shape<D, N, T>;
+  using S = Shape<D, N, T, G>;
   constexpr int VEC = S::VEC;
-  const Vec<T, VEC> *g =
-    reinterpret_cast<const Vec<T, VEC> *>(src + c0 * S::CELL);
-  const int last = ncell * S::CELL / VEC - 1;
-#pragma unroll
-  for (int k = 0; k < S::CPT; ++k)
+  if constexpr (S::kGroup)
     {
-      const Vec<T, VEC> v = g[min((int)threadIdx.x + k * kBlock, last)];
+      const Vec<T, VEC> *g =
+        reinterpret_cast<const Vec<T, VEC> *>(src + mycell * S::CELL);
 #pragma unroll
-      for (int e = 0; e < VEC; ++e)
-        ch[k * VEC + e] = v.v[e];
+      for (int k = 0; k < S::CPT; ++k)
+        {
+          const Vec<T, VEC> v = g[tau + k * S::TPC];
+#pragma unroll
+          for (int e = 0; e < VEC; ++e)
+            ch[k * VEC + e] = v.v[e];
+        }
+    }
+  else
+    {
+      const Vec<T, VEC> *g =
+        reinterpret_cast<const Vec<T, VEC> *>(src + c0 * S::CELL);
+      const int last = ncell * S::CELL / VEC - 1;
+#pragma unroll
+      for (int k = 0; k < S::CPT; ++k)
+        {
+          const Vec<T, VEC> v = g[min((int)threadIdx.x + k * kBlock, last)];
+#pragma unroll
+          for (int e = 0; e < VEC; ++e)
+            ch[k * VEC + e] = v.v[e];
+        }
     }
 }
 
-template <int D, int N, typename T, typename Chunks>
-__device__ __forceinline__ void store_chunks(const Chunks &ch, T *U)
+template <int D, int N, typename T, int G, typename Chunks>
+__device__ __forceinline__ void store_chunks(const Chunks &ch, T *U, int lc,
+                                             int tau)
 {
-  using S = Shape<D, N, T>;
+  using S = Shape<D, N, T, G>;
   constexpr int VEC = S::VEC, CELL = S::CELL, CELLP = S::CELLP;
 #pragma unroll
   for (int k = 0; k < S::CPT; ++k)
     {
-      const int c = min((int)threadIdx.x + k * kBlock, S::CHUNKS - 1);
-      const int i = c * VEC, l = i / CELL, li = i - l * CELL;
+      int l, li;
+      if constexpr (S::kGroup)
+        {
+          l = lc;
+          li = (tau + k * S::TPC) * VEC;
+        }
+      else
+        {
+          const int c = min((int)threadIdx.x + k * kBlock, S::CHUNKS - 1);
+          const int i = c * VEC;
+          l = i / CELL;
+          li = i - l * CELL;
+        }
       if constexpr (S::kChunkRuns)
         {
           Vec<T, VEC> v;
@@ -448,28 +483,34 @@ __device__ inline void inflow_cells(long long cell, const MeshDev &mesh,
 // MODE 0: dst = apply(src).  MODE 1: LSRK stage with K = apply(src),
 // sol += b_dt K, ti_out = sol_old + a_dt K.  MODE 2: the stage without
 // ti_out (last stage).
-// resident blocks per CU: LDS allows two (FP64) or four (FP32) of the 3+3
-// cell; the FP32 stage keeps the FP64 solution values in registers and runs
-// three to stay clear of spills
-template <typename T, int MODE>
-constexpr int kBlocksPerCu = sizeof(T) == 8 ? 2 : (MODE == 0 ? 4 : 3);
+// Resident blocks per CU: LDS allows two (FP64) or four (FP32) 3+3 cells
+// with their R buffers; the FP32 stage keeps the FP64 solution values in
+// registers and runs three to stay clear of spills.  A block of G cells
+// counts G times.
+template <typename T, int MODE, int G>
+constexpr int kBlocksPerCu =
+  (sizeof(T) == 8 ? 2 : (MODE == 0 ? 4 : 3)) / G > 0
+    ? (sizeof(T) == 8 ? 2 : (MODE == 0 ? 4 : 3)) / G
+    : 1;
 
-template <int D, int N, typename T, int MODE>
-__global__ __launch_bounds__(kBlock, (kBlocksPerCu<T, MODE>)) void apply_kernel(
+template <int D, int N, typename T, int MODE, int G>
+__global__ __launch_bounds__((Shape<D, N, T, G>::BLOCK),
+                             (kBlocksPerCu<T, MODE, G>)) void apply_kernel(
   const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ coef,
   MeshDev mesh, InflowDev inflow, StageDev stage, long long n_logical,
   int slots)
 {
-  using S = Shape<D, N, T>;
+  using S = Shape<D, N, T, G>;
   using PL = Plan<D, N>;
   constexpr int P = S::P, NN = S::NN, CELL = S::CELL, TPC = S::TPC;
   constexpr int CPB = S::CPB, CELLP = S::CELLP, VEC = S::VEC, CPT = S::CPT;
+  constexpr bool RW = PL::kRowWalk;
   __shared__ __align__(16) T U[CPB * CELLP];
   __shared__ __align__(16) T R[P > 1 ? CPB * CELLP : 1];
   const cptr<T> ccoef = (cptr<T>)coef;
 
-  // this block's share of its XCD's contiguous range of logical units: rows
-  // of cells along direction 0 (row walk) or groups of CPB cells
+  // this block's share of its XCD's contiguous range of logical units: G
+  // rows of cells along direction 0 (row walk) or groups of CPB cells
   const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
   const long long per = (n_logical + 7) / 8;
   const long long lend = min(n_logical, (xcd + 1) * per);
@@ -480,7 +521,12 @@ __global__ __launch_bounds__(kBlock, (kBlocksPerCu<T, MODE>)) void apply_kernel(
   const int tid = threadIdx.x;
   // threads past CPB * TPC shadow the last cell: they load and compute but
   // write nothing
-  const int lc = min(tid / TPC, CPB - 1), tau = tid % TPC;
+  int lc = min(tid / TPC, CPB - 1);
+  // TPC is a multiple of the wave size: lc is uniform over a wave, and
+  // making that visible keeps the cell arithmetic on the scalar unit
+  if constexpr (TPC % 64 == 0)
+    lc = __builtin_amdgcn_readfirstlane(lc);
+  const int tau = tid % TPC;
   const bool owner = tid < S::THREADS;
 
   int gbase[P], lbase[P];
@@ -497,45 +543,64 @@ __global__ __launch_bounds__(kBlock, (kBlocksPerCu<T, MODE>)) void apply_kernel(
   T ch[CPT * VEC];
   T fc[P][2][N];
 
-  // first cell of logical unit lb at row position k
-  constexpr bool RW = PL::kRowWalk;
-  static_assert(!RW || CPB == 1, "the row walk handles one cell at a time");
+  // Cells.  Row walk: unit u holds the rows u G .. u G + G - 1 (neighbours
+  // along direction 1); at row position k the block works on their cells
+  // c0 + l n_0, l < G, with c0 = u G n_0 + x0(k).  Otherwise unit u is the
+  // cells u CPB .. u CPB + CPB - 1.
   const int rowlen = RW ? mesh.n[0] : 1;
   const bool fwd = inflow.side[0] >= 0;
   auto first_cell = [&](long long unit, int k) -> long long {
-    return RW ? unit * rowlen + (fwd ? k : rowlen - 1 - k) : unit * CPB;
+    return RW ? unit * CPB * rowlen + (fwd ? k : rowlen - 1 - k)
+              : unit * CPB;
   };
+  auto my_cell = [&](long long first) -> long long {
+    return RW ? first + (long long)lc * rowlen
+              : min(first + lc, mesh.n_cells - 1);
+  };
+  // direction-1 inflow neighbour inside the group: its face is read from U
+  const int nb1 = inflow.side[1] < 0 ? lc + 1 : lc - 1;
+  const bool d1_lds = RW && CPB > 1 && nb1 >= 0 && nb1 < CPB;
+  constexpr int P1 = RW ? 1 : -1; // the layout with a(p) == 1
+  static_assert(!RW || PL::a(1) == 1, "row walk layouts");
+
+  // row walk: the x0 inflow face of the current cell, captured from the
+  // previous cell in U (valid when prev_in_row)
+  T f0[N];
+  bool prev_in_row = false;
+
   int k = 0;
   long long c0 = first_cell(lb, 0);
   int ncell = (int)min((long long)CPB, mesh.n_cells - c0);
   {
     long long nb[D];
-    inflow_cells<D>(min(c0 + lc, mesh.n_cells - 1), mesh, inflow, nb);
-    load_chunks<D, N, T>(ch, src, c0, ncell);
+    const long long mc = my_cell(c0);
+    inflow_cells<D>(mc, mesh, inflow, nb);
+    load_chunks<D, N, T, G>(ch, src, c0, ncell, mc, tau);
     static_for<P>([&](auto pc) {
       constexpr int p = decltype(pc)::value;
-      face_load<D, N, T, p>(src, gbase[p], nb, fnode, fc[p]);
+      face_load<D, N, T, p>(src, gbase[p], nb, fnode, fc[p], p == P1 && d1_lds);
     });
   }
 
   for (;;)
     {
-      // 1) this cell's values into U
-      store_chunks<D, N, T>(ch, U);
+      // 1) this step's cells into U
+      store_chunks<D, N, T, G>(ch, U, lc, tau);
       lds_barrier();
 
-      // 2) the next cell's values go out now and land during the arithmetic
-      // (the last iteration re-reads its own cell); its faces follow layout
-      // by layout, as soon as the current layout has consumed its own
+      // 2) the next step's values go out now and land during the arithmetic
+      // (the last iteration re-reads its own cells); their faces follow
+      // layout by layout, as soon as the current layout has consumed its own
       const bool in_row = RW && k + 1 < rowlen;
       const long long ln = in_row ? lb : lb + slots;
       const int nk = in_row ? k + 1 : 0;
       const bool more = ln < lend;
       const long long nc0 = more ? first_cell(ln, nk) : c0;
       const int nncell = (int)min((long long)CPB, mesh.n_cells - nc0);
-      load_chunks<D, N, T>(ch, src, nc0, nncell);
+      const long long nmc = my_cell(nc0);
+      load_chunks<D, N, T, G>(ch, src, nc0, nncell, nmc, tau);
       long long nnb[D];
-      inflow_cells<D>(min(nc0 + lc, mesh.n_cells - 1), mesh, inflow, nnb);
+      inflow_cells<D>(nmc, mesh, inflow, nnb);
 
       // 3) layouts 0 .. P-2 add their contributions into R
       T ul[NN];
@@ -545,6 +610,27 @@ __global__ __launch_bounds__(kBlock, (kBlocksPerCu<T, MODE>)) void apply_kernel(
         constexpr int p = decltype(pc)::value;
         T u[NN], out[NN];
         tile_read<D, N, T, p>(U, lbase[p], u);
+        if constexpr (RW && p == 0)
+          {
+            if (prev_in_row)
+#pragma unroll
+              for (int l = 0; l < N; ++l)
+                fc[p][0][l] = f0[l];
+          }
+        if constexpr (p == P1)
+          {
+            // the x1 inflow face of a cell whose x1 neighbour is in the block
+            if (d1_lds)
+              {
+                const volatile __attribute__((address_space(3))) T *vu =
+                  (const volatile __attribute__((address_space(3))) T *)U;
+                const int base = lbase[p] + (nb1 - lc) * CELLP +
+                                 fnode[1] * PL::W(1);
+#pragma unroll
+                for (int l = 0; l < N; ++l)
+                  fc[p][0][l] = vu[base + l * PL::W(PL::b(p))];
+              }
+          }
         if constexpr (p == 0)
           {
 #pragma unroll
@@ -556,21 +642,33 @@ __global__ __launch_bounds__(kBlock, (kBlocksPerCu<T, MODE>)) void apply_kernel(
         tile_contrib<D, N, T, p>(ccoef, u, fc[p], out);
         if (S::kAllOwn || owner)
           tile_write<D, N, T, p>(R, lbase[p], out);
-        if constexpr (RW && p == 0 && PL::a(0) == 0)
-          face_load<D, N, T, p>(src, gbase[p], nnb, fnode, fc[p], in_row, U,
-                                lbase[p]);
-        else
-          face_load<D, N, T, p>(src, gbase[p], nnb, fnode, fc[p]);
+        face_load<D, N, T, p>(src, gbase[p], nnb, fnode, fc[p],
+                              (RW && p == 0 && in_row) || (p == P1 && d1_lds));
+        if constexpr (RW && p == 0)
+          {
+            // the next cell's x0 inflow face is this cell's outflow face
+            if (in_row)
+              {
+                const volatile __attribute__((address_space(3))) T *vu =
+                  (const volatile __attribute__((address_space(3))) T *)U;
+#pragma unroll
+                for (int l = 0; l < N; ++l)
+                  f0[l] = vu[lbase[p] + fnode[0] * PL::W(0) +
+                             l * PL::W(PL::b(p))];
+              }
+          }
         lds_barrier();
       });
 
       // 4) the last layout: R + its own contribution, straight to HBM
+      // (non-temporal: the output is not read again by this kernel and
+      // should not push face lines out of L2)
       {
         constexpr int p = P - 1;
         constexpr int a = PL::a(p), b = PL::b(p);
         const bool store =
-          S::kAllOwn && CPB == 1 ? true : (owner && lc < ncell);
-        const long long g0 = (min(c0 + lc, mesh.n_cells - 1)) * CELL + gbase[p];
+          S::kAllOwn && (RW || CPB == 1) ? true : (owner && lc < ncell);
+        const long long g0 = my_cell(c0) * CELL + gbase[p];
         // the stage's solution values go out before the arithmetic
         double sv[MODE == 0 ? 1 : NN];
         if constexpr (MODE != 0)
@@ -600,13 +698,15 @@ __global__ __launch_bounds__(kBlock, (kBlocksPerCu<T, MODE>)) void apply_kernel(
                 const long long g =
                   g0 + (r % N) * ipow(N, a) + (r / N) * ipow(N, b);
                 if constexpr (MODE == 0)
-                  dst[g] = out[r];
+                  __builtin_nontemporal_store(out[r], &dst[g]);
                 else
                   {
-                    stage.sol[g] = sv[r] + stage.b_dt * (double)out[r];
+                    __builtin_nontemporal_store(
+                      sv[r] + stage.b_dt * (double)out[r], &stage.sol[g]);
                     if constexpr (MODE == 1)
-                      static_cast<T *>(stage.ti_out)[g] =
-                        (T)(sv[r] + stage.a_dt * (double)out[r]);
+                      __builtin_nontemporal_store(
+                        (T)(sv[r] + stage.a_dt * (double)out[r]),
+                        &static_cast<T *>(stage.ti_out)[g]);
                   }
               }
           }
@@ -617,6 +717,7 @@ __global__ __launch_bounds__(kBlock, (kBlocksPerCu<T, MODE>)) void apply_kernel(
       k = nk;
       c0 = nc0;
       ncell = nncell;
+      prev_in_row = in_row;
       // one layout: its only U read was this iteration's, before the
       // chunks of the next cell overwrite U
       if constexpr (P == 1)
@@ -639,31 +740,71 @@ int cus_per_xcd()
   return g_cus;
 }
 
-template <int D, int N, typename T>
-hipError_t launch(int mode, const T *src, T *dst, const T *coef,
-                  const MeshDev &mesh, const InflowDev &inflow,
-                  const StageDev &stage, hipStream_t stream)
+template <int D, int N, typename T, int G>
+hipError_t launch_g(int mode, const T *src, T *dst, const T *coef,
+                    const MeshDev &mesh, const InflowDev &inflow,
+                    const StageDev &stage, hipStream_t stream)
 {
-  using S = Shape<D, N, T>;
+  using S = Shape<D, N, T, G>;
+  constexpr bool RW = Plan<D, N>::kRowWalk;
   const long long n_logical =
-    Plan<D, N>::kRowWalk ? mesh.n_cells / mesh.n[0]
-                         : (mesh.n_cells + S::CPB - 1) / S::CPB;
-  // no more blocks per XCD than the XCD has logical cells
+    RW ? mesh.n_cells / ((long long)mesh.n[0] * G)
+       : (mesh.n_cells + S::CPB - 1) / S::CPB;
   const long long per = (n_logical + 7) / 8;
   auto go = [&](auto kernel, int bpc) {
-    const int used = (int)min((long long)(bpc * cus_per_xcd()), per);
-    hipLaunchKernelGGL(kernel, dim3(8u * (unsigned)used), dim3(kBlock), 0,
+    // Row walk: one block per unit, so the dispatcher hands the rows of an
+    // XCD out in order and the blocks resident on it work on neighbouring
+    // rows (their x2 / x3 inflow faces are lines another block has just
+    // loaded into the XCD's L2).  Otherwise persistent blocks, no more per
+    // XCD than it has units.
+    const int used =
+      RW ? (int)per : (int)min((long long)(bpc * cus_per_xcd()), per);
+    hipLaunchKernelGGL(kernel, dim3(8u * (unsigned)used), dim3(S::BLOCK), 0,
                        stream, src, dst, coef, mesh, inflow, stage, n_logical,
                        used);
   };
   const int m = mode == 0 ? 0 : (stage.ti_out ? 1 : 2);
   if (m == 0)
-    go(apply_kernel<D, N, T, 0>, kBlocksPerCu<T, 0>);
+    go(apply_kernel<D, N, T, 0, G>, kBlocksPerCu<T, 0, G>);
   else if (m == 1)
-    go(apply_kernel<D, N, T, 1>, kBlocksPerCu<T, 1>);
+    go(apply_kernel<D, N, T, 1, G>, kBlocksPerCu<T, 1, G>);
   else
-    go(apply_kernel<D, N, T, 2>, kBlocksPerCu<T, 2>);
+    go(apply_kernel<D, N, T, 2, G>, kBlocksPerCu<T, 2, G>);
   return hipGetLastError();
+}
+
+// Cells per block along direction 1 for the row walk.  Groups of two (FP64)
+// or four (FP32) read the x1 inflow face of all but one cell from LDS, but on
+// MI355X at 3+3, k=3, 8^6 cells they measured no faster (FP64 251 vs 252
+// GDoF/s) or slower (FP32, four cells of 1024 threads: 419 vs 485), so one
+// cell per block is the default; the group size stays a tuning parameter.
+template <typename T>
+int preferred_group(int mode)
+{
+  (void)mode;
+  return 1;
+}
+
+template <int D, int N, typename T>
+hipError_t launch(int mode, const T *src, T *dst, const T *coef,
+                  const MeshDev &mesh, const InflowDev &inflow,
+                  const StageDev &stage, hipStream_t stream)
+{
+  if constexpr (Plan<D, N>::kRowWalk)
+    {
+      int g = preferred_group<T>(mode);
+      while (g > 1 && mesh.n[1] % g != 0)
+        g /= 2;
+      if constexpr (sizeof(T) == 4)
+        if (g == 4)
+          return launch_g<D, N, T, 4>(mode, src, dst, coef, mesh, inflow,
+                                      stage, stream);
+      if (g == 2)
+        return launch_g<D, N, T, 2>(mode, src, dst, coef, mesh, inflow, stage,
+                                    stream);
+    }
+  return launch_g<D, N, T, 1>(mode, src, dst, coef, mesh, inflow, stage,
+                              stream);
 }
 
 template <typename T>

@@ -76,16 +76,22 @@ APPLY_CASES = [
 ]
 
 
-@pytest.mark.parametrize("a0", [0.7, -0.55, 0.0])
+@pytest.mark.parametrize("a01", [(0.7, 0.3), (-0.55, -0.3), (0.0, 0.3),
+                                 (0.7, -0.3), (-0.55, 0.0)])
 @pytest.mark.parametrize("n0", [1, 3])
 @pytest.mark.parametrize("precision", ["fp64", "fp32"])
-def test_collapsed_3d3d_row_walk(a0, n0, precision):
-    """3+3, k=3: each block walks rows of cells along x0 in upwind order and
-    takes the x0 inflow face from the previous cell in LDS; rows of one and
-    three cells, transport to +x0, -x0 and none, against the oracle."""
-    print("case: row walk a0=%g n0=%d %s" % (a0, n0, precision), flush=True)
-    a = [a0, 0.3, -0.8, 0.45, -0.2, 0.9]
-    n_cells = [n0, 2, 3, 2, 2, 2]
+def test_collapsed_3d3d_row_walk(a01, n0, precision):
+    """3+3, k=3: a block owns cells that are neighbours along x1 and walks
+    their rows along x0 in upwind order; x0 inflow faces come from the
+    previous cell, x1 inflow faces of all but one cell from the block's
+    neighbour cell, both in LDS.  Rows of one and three cells, four cells
+    along x1 (groups of two or four), transport to +, - and none in x0 and
+    x1, against the oracle."""
+    a0, a1 = a01
+    print("case: row walk a0=%g a1=%g n0=%d %s" % (a0, a1, n0, precision),
+          flush=True)
+    a = [a0, a1, -0.8, 0.45, -0.2, 0.9]
+    n_cells = [n0, 4, 3, 2, 1, 2]
     P, mf, op, a = _pair((3, 3), 3, n_cells, seed=11, a=a)
     op.set_algorithm(H.ALGO_COLLAPSED)
     r = _rng(12)
@@ -197,6 +203,35 @@ def test_lsrk_step_matches_oracle(rk, algo):
         integ.perform_time_step(op, sol, ta, tb, step * dt, dt)
     torch.cuda.synchronize()
     assert _rel_err(sol.cpu().numpy(), sol_ref) <= 1e-12
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_lsrk_3d3d_row_walk(precision):
+    """Fused LSRK stages of the 3+3, k=3 row-walk kernel (groups of cells
+    along x1, stage with and without the next stage input) against the
+    oracle's rk45 steps.  FP32-mixed: the stage input is float, the solution
+    and the update are double; the bar is that of float rounding carried by
+    three steps."""
+    print("case: lsrk 3+3 row walk", precision, flush=True)
+    a = [0.6, -0.45, 0.8, -0.3, 0.55, -0.9]
+    P, mf, op, a = _pair((3, 3), 3, [3, 4, 2, 2, 1, 2], seed=16, a=a)
+    op.set_algorithm(H.ALGO_COLLAPSED)
+    u = P.interpolate() * (1.0 + 0.1 * _rng(7).standard_normal(P.n_dofs))
+    sol_ref, ki, ti = u.copy(), P.vector(), P.vector()
+    dt = 0.1 * H.compute_critical_time_step(mf, a) / 3 ** 1.5
+    for step in range(3):
+        P.lsrk_step("rk45", sol_ref, ki, ti, step * dt, dt)
+    prec, dtype, tol = ((H.FP64, torch.float64, 1e-12) if precision == "fp64"
+                        else (H.FP32, torch.float32, 1e-6))
+    sol = torch.from_numpy(u.copy()).cuda()
+    ta = torch.empty(P.n_dofs, dtype=dtype, device="cuda")
+    tb = torch.empty_like(ta)
+    integ = H.LowStorageRungeKuttaIntegrator("rk45")
+    for step in range(3):
+        integ.perform_time_step(op, sol, ta, tb, step * dt, dt,
+                                precision=prec)
+    torch.cuda.synchronize()
+    assert _rel_err(sol.cpu().numpy(), sol_ref) <= tol
 
 
 def test_lsrk_fp32_mixed_tracks_fp64():
