@@ -30,6 +30,23 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0
 SEED_VELOCITY = 20261202
 SEED_NOISE = 20261203
+# HBM bytes per FP64 apply launch, measured with rocprofv3 --pmc FETCH_SIZE /
+# WRITE_SIZE and scaled by the same counters on a plain copy (the command and
+# figures are in this file)
+TRAFFIC_FILE = os.path.join(ROOT, "profiles",
+                            "collapsed_apply_3d3d_k3_8p6_traffic.txt")
+
+
+def measured_traffic():
+    """bytes per launch of the FP64 apply from the committed counter summary,
+    or None"""
+    try:
+        for line in open(TRAFFIC_FILE):
+            if line.startswith("traffic_bytes_per_launch:"):
+                return float(line.split(":", 1)[1])
+    except OSError:
+        return None
+    return None
 
 
 def splitmix_uniform(seed, i):
@@ -105,9 +122,10 @@ def main():
                     default="all", help="restrict the timed kernels "
                     "(profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
-                    help="also run a plain device copy of one FP64 vector "
-                    "(8.6 GB read + 8.6 GB written), the reference point for "
-                    "scaling the FETCH_SIZE / WRITE_SIZE counters")
+                    help="run the plain device copy of one FP64 vector (8.6 GB "
+                    "read + 8.6 GB written) before the timed kernels too, the "
+                    "reference point for scaling the FETCH_SIZE / WRITE_SIZE "
+                    "counters")
     args = ap.parse_args()
     if args.gpus != 1:
         print("bench.py measures one GPU: there is no multi-GPU path (the "
@@ -132,12 +150,13 @@ def main():
     H.VectorTools.interpolate(mf, u, H.FUNC_PRODUCT_SINES, [0.0] * 6)
     H.VectorTools.add_noise(mf, u, SEED_NOISE, 1e-3)
 
-    if args.copy_calibration:
-        cp = torch.empty_like(u)
-        for _ in range(3):
-            cp.copy_(u)
-        torch.cuda.synchronize()
-        del cp
+    # a plain device copy of the vector: what HBM delivers to a streaming
+    # kernel on this box (read + write bytes / time)
+    cp = torch.empty_like(u)
+    t = timed(lambda: cp.copy_(u), 3 if args.copy_calibration else 10, 2,
+              stream)
+    results["copy"] = t
+    del cp
     if args.only in ("all", "fp64"):
         dst = torch.empty_like(u)
         t = timed(lambda: op.apply(dst, u, 0.0, H.FP64, stream), args.steps,
@@ -187,10 +206,16 @@ def main():
         line["value"] = round(gdofs(t), 3)
         line["ms_per_step"] = round(t * 1e3, 4)
         achieved = 16.0 * n / t / 1e9
+        traffic = measured_traffic()
         line["roofline"] = {"bound": "hbm", "achieved": round(achieved, 1),
                             "peak": HBM_PEAK_GBS, "unit": "GB/s",
                             "frac": round(achieved / HBM_PEAK_GBS, 4),
-                            "traffic": None}
+                            "traffic": traffic}
+    if "copy" in results:
+        line["device_copy"] = {
+            "GBs": round(16.0 * n / results["copy"] / 1e9, 1),
+            "note": "torch copy_ of one FP64 vector (8.6 GB read + 8.6 GB "
+                    "written), same stream and events"}
     if "fp32" in results:
         t = results["fp32"]
         line["fp32_mixed"] = {
