@@ -112,13 +112,69 @@ def cpu_baseline(velocity):
                       % (P.n_dofs, n_apply, arch, cores)}
 
 
+def batch_workload(H, torch, stream, steps):
+    """64 independent 2+2 advection instances (instance i: seed 20261204 + i
+    draws cells per direction from {4, 6, 8, 12, 16}, degree from
+    {2, 3, 4, 5} and the velocity from U(-1, 1)^4, SplitMix64), FP32-mixed
+    rk45 steps: one launch per stage for all instances, captured in a
+    hipGraph, after the autotuner picked the work split per degree; against
+    64 sequential single-instance steps."""
+    insts, n_total = [], 0
+    for i in range(64):
+        seed = 20261204 + i
+        nc = [4, 6, 8, 12, 16][int((splitmix_uniform(seed, 0) + 1) / 2 * 5) % 5]
+        k = [2, 3, 4, 5][int((splitmix_uniform(seed, 1) + 1) / 2 * 4) % 4]
+        a = [splitmix_uniform(seed, 2 + d) for d in range(4)]
+        mf = H.MatrixFree(2, 2, k, [nc] * 4, [0.0] * 4, [1.0] * 4)
+        op = H.AdvectionOperation(mf, a)
+        sol = torch.empty(mf.n_dofs(), dtype=torch.float64, device="cuda")
+        H.VectorTools.interpolate(mf, sol, H.FUNC_PRODUCT_SINES, [0.0] * 4)
+        ta = torch.empty(mf.n_dofs(), dtype=torch.float32, device="cuda")
+        tb = torch.empty_like(ta)
+        insts.append((mf, op, sol, ta, tb))
+        n_total += mf.n_dofs()
+    batch = H.Batch([x[1] for x in insts])
+    sols = [x[2] for x in insts]
+    tas = [x[3] for x in insts]
+    tbs = [x[4] for x in insts]
+    for t, s_ in zip(tas, sols):
+        t.copy_(s_)
+    batch.autotune(tas, tbs, precision=H.FP32)
+    dt = 1e-4
+    graph = batch.graph("rk45", sols, tas, tbs, dt, precision=H.FP32)
+    t_graph = timed(lambda: graph.launch(stream), steps, 1, stream)
+    t_eager = timed(lambda: batch.lsrk_step("rk45", sols, tas, tbs, dt,
+                                            precision=H.FP32, stream=stream),
+                    steps, 1, stream)
+    integ = H.LowStorageRungeKuttaIntegrator("rk45")
+
+    def sequential():
+        for mf, op, sol, ta, tb in insts:
+            integ.perform_time_step(op, sol, ta, tb, 0.0, dt, precision=H.FP32,
+                                    stream=stream)
+    t_seq = timed(sequential, steps, 1, stream)
+    stages = integ.n_stages()
+    tuned = {k: batch.cells_per_task(k, precision=H.FP32) for k in (2, 3, 4, 5)}
+    del graph
+    return {"instances": 64, "dof": n_total, "precision": "fp32-mixed",
+            "rk": "rk45", "unit": "GDoF/s (stage applies)",
+            "value": round(stages * n_total / t_graph / 1e9, 2),
+            "ms_per_step": round(t_graph * 1e3, 3),
+            "eager_batch_GDoFs": round(stages * n_total / t_eager / 1e9, 2),
+            "sequential_GDoFs": round(stages * n_total / t_seq / 1e9, 2),
+            "sequential_ms_per_step": round(t_seq * 1e3, 3),
+            "speedup_vs_sequential": round(t_seq / t_graph, 3),
+            "cells_per_task": tuned}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
-    ap.add_argument("--only", choices=["all", "fp64", "fp32", "stage"],
+    ap.add_argument("--only", choices=["all", "fp64", "fp32", "stage",
+                                       "batch"],
                     default="all", help="restrict the timed kernels "
                     "(profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -182,6 +238,12 @@ def main():
             results["stage_fp64" if prec == H.FP64 else "stage_fp32"] = t
             del sol, ta, tb
     torch.cuda.synchronize()
+    batch_result = None
+    if args.only in ("all", "batch"):
+        del u
+        torch.cuda.empty_cache()
+        batch_result = batch_workload(H, torch, stream, max(3, args.steps // 5))
+        torch.cuda.empty_cache()
 
     def gdofs(t):
         return n / t / 1e9
@@ -233,6 +295,8 @@ def main():
                          "ms_per_stage": round(t * 1e3, 4),
                          "bytes_per_dof": bpd,
                          "achieved_GBs": round(bpd * n / t / 1e9, 1)}
+    if batch_result is not None:
+        line["batch_2d2d"] = batch_result
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(velocity)
     print(json.dumps(line), flush=True)

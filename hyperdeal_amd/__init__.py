@@ -48,7 +48,12 @@ EXPORTED = [
     "hd_vp_update_field", "hd_vp_set_auto_update",
     "hd_vp_set_poisson_tolerance", "hd_vp_solver_stats", "hd_vp_x_n_dofs",
     "hd_vp_potential", "hd_vp_field_energy", "hd_vp_moments",
+    "hd_batch_create", "hd_batch_destroy", "hd_batch_size", "hd_batch_apply",
+    "hd_batch_lsrk_step", "hd_batch_autotune", "hd_batch_cells_per_task",
+    "hd_batch_set_cells_per_task", "hd_batch_graph_create",
+    "hd_graph_launch", "hd_graph_destroy",
 ]
+BATCH_MAX = 64
 
 
 class HyperdealError(RuntimeError):
@@ -129,6 +134,22 @@ def load_library(path=None):
     lib.hd_vp_potential.argtypes = [_vp, _dp, ctypes.c_int]
     lib.hd_vp_field_energy.argtypes = [_vp, _dp]
     lib.hd_vp_moments.argtypes = [_vp, _vp, _dp, _vp]
+    _pa = ctypes.POINTER(_vp)
+    lib.hd_batch_create.argtypes = [_pa, ctypes.c_int, ctypes.POINTER(_vp)]
+    lib.hd_batch_destroy.argtypes = [_vp]
+    lib.hd_batch_size.argtypes = [_vp]
+    lib.hd_batch_apply.argtypes = [_vp, _pa, _pa, ctypes.c_int, _vp]
+    lib.hd_batch_lsrk_step.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int,
+                                       _pa, _pa, _pa, ctypes.c_double, _vp]
+    lib.hd_batch_autotune.argtypes = [_vp, ctypes.c_int, _pa, _pa, _vp]
+    lib.hd_batch_cells_per_task.argtypes = [_vp, ctypes.c_int, ctypes.c_int]
+    lib.hd_batch_set_cells_per_task.argtypes = [_vp, ctypes.c_int,
+                                                ctypes.c_int, ctypes.c_int]
+    lib.hd_batch_graph_create.argtypes = [_vp, ctypes.c_char_p, ctypes.c_int,
+                                          _pa, _pa, _pa, ctypes.c_double,
+                                          ctypes.POINTER(_vp)]
+    lib.hd_graph_launch.argtypes = [_vp, _vp]
+    lib.hd_graph_destroy.argtypes = [_vp]
     if path is None:
         _lib = lib
     return lib
@@ -358,6 +379,102 @@ def vp_moments(mf, f, stream=None):
                                                    mf.n_dofs()), out,
                                  _stream(stream)), "hd_vp_moments")
     return [out[i] for i in range(6)]
+
+
+class Batch:
+    """Independent advection operators (2+2, collapsed, any degree <= 5)
+    applied / time-stepped together, one launch per apply or stage (no
+    counterpart in the reference, which steps one operator per rank)."""
+
+    def __init__(self, ops):
+        ops = list(ops)
+        if not 1 <= len(ops) <= BATCH_MAX:
+            raise HyperdealError("a batch holds 1 to %d operators" % BATCH_MAX)
+        self.ops = ops
+        self._lib = ops[0]._lib
+        arr = (_vp * len(ops))(*[o.handle for o in ops])
+        h = _vp()
+        _check(self._lib.hd_batch_create(arr, len(ops), ctypes.byref(h)),
+               "hd_batch_create")
+        self.handle = h
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._lib.hd_batch_destroy(self.handle)
+            self.handle = None
+
+    def __len__(self):
+        return len(self.ops)
+
+    def _arr(self, tensors, dtype):
+        if len(tensors) != len(self.ops):
+            raise HyperdealError("one vector per operator")
+        return (_vp * len(self.ops))(*[
+            _ptr(t, dtype, o.mf.n_dofs()) for t, o in zip(tensors, self.ops)])
+
+    def apply(self, dst, src, precision=FP64, stream=None):
+        dt = _torch_dtype(precision)
+        _check(self._lib.hd_batch_apply(self.handle, self._arr(src, dt),
+                                        self._arr(dst, dt), precision,
+                                        _stream(stream)), "hd_batch_apply")
+        return dst
+
+    def lsrk_step(self, rk_type, sol, ti_a, ti_b, dt, precision=FP64,
+                  stream=None):
+        import torch
+        t = _torch_dtype(precision)
+        _check(self._lib.hd_batch_lsrk_step(
+            self.handle, rk_type.encode(), precision,
+            self._arr(sol, torch.float64), self._arr(ti_a, t),
+            self._arr(ti_b, t), float(dt), _stream(stream)),
+            "hd_batch_lsrk_step")
+
+    def autotune(self, src, dst, precision=FP64, stream=None):
+        dt = _torch_dtype(precision)
+        _check(self._lib.hd_batch_autotune(self.handle, precision,
+                                           self._arr(src, dt),
+                                           self._arr(dst, dt),
+                                           _stream(stream)),
+               "hd_batch_autotune")
+
+    def cells_per_task(self, degree, precision=FP64):
+        return _check(self._lib.hd_batch_cells_per_task(self.handle, degree,
+                                                        precision),
+                      "hd_batch_cells_per_task")
+
+    def set_cells_per_task(self, degree, cells, precision=FP64):
+        _check(self._lib.hd_batch_set_cells_per_task(self.handle, degree,
+                                                     precision, cells),
+               "hd_batch_set_cells_per_task")
+
+    def graph(self, rk_type, sol, ti_a, ti_b, dt, precision=FP64):
+        """A hipGraph of one LSRK step of every operator (fixed vectors and
+        dt); Graph.launch() replays it."""
+        import torch
+        t = _torch_dtype(precision)
+        h = _vp()
+        _check(self._lib.hd_batch_graph_create(
+            self.handle, rk_type.encode(), precision,
+            self._arr(sol, torch.float64), self._arr(ti_a, t),
+            self._arr(ti_b, t), float(dt), ctypes.byref(h)),
+            "hd_batch_graph_create")
+        return Graph(self, h)
+
+
+class Graph:
+    def __init__(self, batch, handle):
+        self.batch = batch          # the batch outlives its graphs
+        self._lib = batch._lib
+        self.handle = handle
+
+    def launch(self, stream=None):
+        _check(self._lib.hd_graph_launch(self.handle, _stream(stream)),
+               "hd_graph_launch")
+
+    def __del__(self):
+        if getattr(self, "handle", None):
+            self._lib.hd_graph_destroy(self.handle)
+            self.handle = None
 
 
 class LowStorageRungeKuttaIntegrator:

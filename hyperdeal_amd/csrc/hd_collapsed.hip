@@ -37,6 +37,7 @@
 // cells at the same time, so face reads hit its L2 or the MALL.
 #include "hd_device.h"
 
+#include <algorithm>
 #include <type_traits>
 #include <utility>
 
@@ -493,30 +494,27 @@ constexpr int kBlocksPerCu =
     ? (sizeof(T) == 8 ? 2 : (MODE == 0 ? 4 : 3)) / G
     : 1;
 
+// One block's work on one operator: the logical units lb, lb + slots, ...
+// below lend, with the cells in U and the partial sums in R (LDS buffers of
+// CPB * CELLP values each).  Logical units: G rows of cells along direction
+// 0 (row walk) or groups of CPB cells.
 template <int D, int N, typename T, int MODE, int G>
-__global__ __launch_bounds__((Shape<D, N, T, G>::BLOCK),
-                             (kBlocksPerCu<T, MODE, G>)) void apply_kernel(
-  const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ coef,
-  MeshDev mesh, InflowDev inflow, StageDev stage, long long n_logical,
-  int slots)
+__device__ __forceinline__ void
+apply_block(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ src,
+            T *__restrict__ dst, const T *__restrict__ coef,
+            const MeshDev &mesh, const InflowDev &inflow,
+            const StageDev &stage, long long lb, long long lend, int slots)
 {
   using S = Shape<D, N, T, G>;
   using PL = Plan<D, N>;
   constexpr int P = S::P, NN = S::NN, CELL = S::CELL, TPC = S::TPC;
   constexpr int CPB = S::CPB, CELLP = S::CELLP, VEC = S::VEC, CPT = S::CPT;
   constexpr bool RW = PL::kRowWalk;
-  __shared__ __align__(16) T U[CPB * CELLP];
-  __shared__ __align__(16) T R[P > 1 ? CPB * CELLP : 1];
+  // Large tiles (N >= 5): the last layout reads its tile from U after the
+  // other layouts instead of before them, which keeps one tile fewer in
+  // registers and costs one more barrier per step.
+  constexpr bool kLateLast = P > 1 && N >= 5;
   const cptr<T> ccoef = (cptr<T>)coef;
-
-  // this block's share of its XCD's contiguous range of logical units: G
-  // rows of cells along direction 0 (row walk) or groups of CPB cells
-  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-  const long long per = (n_logical + 7) / 8;
-  const long long lend = min(n_logical, (xcd + 1) * per);
-  long long lb = xcd * per + slot;
-  if (lb >= lend)
-    return; // uniform for the block, before any barrier
 
   const int tid = threadIdx.x;
   // threads past CPB * TPC shadow the last cell: they load and compute but
@@ -604,7 +602,7 @@ __global__ __launch_bounds__((Shape<D, N, T, G>::BLOCK),
 
       // 3) layouts 0 .. P-2 add their contributions into R
       T ul[NN];
-      if constexpr (P > 1)
+      if constexpr (P > 1 && !kLateLast)
         tile_read<D, N, T, P - 1>(U, lbase[P - 1], ul);
       static_for<P - 1>([&](auto pc) {
         constexpr int p = decltype(pc)::value;
@@ -669,9 +667,11 @@ __global__ __launch_bounds__((Shape<D, N, T, G>::BLOCK),
         const bool store =
           S::kAllOwn && (RW || CPB == 1) ? true : (owner && lc < ncell);
         const long long g0 = my_cell(c0) * CELL + gbase[p];
-        // the stage's solution values go out before the arithmetic
+        // the stage's solution values go out before the arithmetic (small
+        // tiles; large ones load them at the store)
+        constexpr bool kEarlySol = MODE != 0 && NN <= 16;
         double sv[MODE == 0 ? 1 : NN];
-        if constexpr (MODE != 0)
+        if constexpr (kEarlySol)
           {
 #pragma unroll
             for (int r = 0; r < NN; ++r)
@@ -679,6 +679,8 @@ __global__ __launch_bounds__((Shape<D, N, T, G>::BLOCK),
                                 (r / N) * ipow(N, b)];
           }
         T out[NN];
+        if constexpr (kLateLast)
+          tile_read<D, N, T, p>(U, lbase[p], ul);
         if constexpr (P > 1)
           tile_read<D, N, T, p>(R, lbase[p], out);
         else
@@ -692,6 +694,13 @@ __global__ __launch_bounds__((Shape<D, N, T, G>::BLOCK),
         face_load<D, N, T, p>(src, gbase[p], nnb, fnode, fc[p]);
         if (store)
           {
+            if constexpr (MODE != 0 && !kEarlySol)
+              {
+#pragma unroll
+                for (int r = 0; r < NN; ++r)
+                  sv[r] = stage.sol[g0 + (r % N) * ipow(N, a) +
+                                    (r / N) * ipow(N, b)];
+              }
 #pragma unroll
             for (int r = 0; r < NN; ++r)
               {
@@ -718,11 +727,32 @@ __global__ __launch_bounds__((Shape<D, N, T, G>::BLOCK),
       c0 = nc0;
       ncell = nncell;
       prev_in_row = in_row;
-      // one layout: its only U read was this iteration's, before the
-      // chunks of the next cell overwrite U
-      if constexpr (P == 1)
+      // one layout, or the last layout's late tile read: this iteration's
+      // U reads precede the next step's chunks
+      if constexpr (P == 1 || kLateLast)
         lds_barrier();
     }
+}
+
+template <int D, int N, typename T, int MODE, int G>
+__global__ __launch_bounds__((Shape<D, N, T, G>::BLOCK),
+                             (kBlocksPerCu<T, MODE, G>)) void apply_kernel(
+  const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ coef,
+  MeshDev mesh, InflowDev inflow, StageDev stage, long long n_logical,
+  int slots)
+{
+  using S = Shape<D, N, T, G>;
+  __shared__ __align__(16) T U[S::CPB * S::CELLP];
+  __shared__ __align__(16) T R[S::P > 1 ? S::CPB * S::CELLP : 1];
+  // this block's share of its XCD's contiguous range of logical units
+  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const long long per = (n_logical + 7) / 8;
+  const long long lend = min(n_logical, (xcd + 1) * per);
+  const long long lb = xcd * per + slot;
+  if (lb >= lend)
+    return; // uniform for the block, before any barrier
+  apply_block<D, N, T, MODE, G>(U, R, src, dst, coef, mesh, inflow, stage, lb,
+                                lend, slots);
 }
 
 int g_cus = 0;
@@ -827,6 +857,107 @@ hipError_t dispatch(int D, int N, int mode, const T *src, T *dst,
   return hipErrorInvalidValue;
 }
 
+// ---------------------------------------------------------------------------
+// Batches: independent operators of one dimension D (any supported N) in one
+// persistent launch.  A task is a range of logical units of one operator; a
+// block loops over tasks, switching on the operator's N into the same block
+// code as the single-operator kernel, with one LDS buffer sized for the
+// largest cell.
+template <int D, typename T, int N>
+constexpr int lds_elems()
+{
+  using S = Shape<D, N, T>;
+  return S::NBUF * S::CPB * S::CELLP;
+}
+
+template <int D, typename T>
+constexpr int batch_lds_elems()
+{
+  static_assert(D == 4, "batches of 2+2 operators");
+  int m = lds_elems<D, T, 2>();
+  m = m > lds_elems<D, T, 3>() ? m : lds_elems<D, T, 3>();
+  m = m > lds_elems<D, T, 4>() ? m : lds_elems<D, T, 4>();
+  m = m > lds_elems<D, T, 5>() ? m : lds_elems<D, T, 5>();
+  m = m > lds_elems<D, T, 6>() ? m : lds_elems<D, T, 6>();
+  return m;
+}
+
+// The batch kernel's registers are those of its largest cell (6 x 6 tiles
+// at N = 6): two blocks per CU leave it 256 VGPRs.
+constexpr int kBatchBlocksPerCu = 2;
+
+template <int D, typename T, int MODE>
+__global__ __launch_bounds__(kBlock, kBatchBlocksPerCu) void
+  batch_kernel(const BatchOp *__restrict__ ops,
+               const BatchTask *__restrict__ tasks, int n_tasks,
+               BatchPtrs ptrs)
+{
+  __shared__ __align__(16) T lds[batch_lds_elems<D, T>()];
+  for (int t = blockIdx.x; t < n_tasks; t += gridDim.x)
+    {
+      const BatchTask task = tasks[t];
+      const BatchOp &op = ops[task.op];
+      const StageDev stage{ptrs.a_dt, ptrs.b_dt, ptrs.sol[task.op],
+                           ptrs.ti[task.op]};
+      const T *src = static_cast<const T *>(ptrs.src[task.op]);
+      T *dst = static_cast<T *>(ptrs.dst[task.op]);
+      const T *coef = static_cast<const T *>(op.coef);
+      switch (op.N)
+        {
+#define HD_BCASE(NN_)                                                        \
+  case NN_:                                                                  \
+    apply_block<D, NN_, T, MODE, 1>(                                         \
+      lds, lds + Shape<D, NN_, T>::CPB * Shape<D, NN_, T>::CELLP, src, dst,  \
+      coef, op.mesh, op.inflow, stage, task.lb, task.le, 1);                 \
+    break;
+          HD_BCASE(2) HD_BCASE(3) HD_BCASE(4) HD_BCASE(5) HD_BCASE(6)
+#undef HD_BCASE
+          default:
+            break;
+        }
+      // the next task may lay out U and R differently: every read of this
+      // task's buffers precedes its first write
+      lds_barrier();
+    }
+}
+
+template <typename T>
+__global__ void batch_convert_kernel(BatchPtrs ptrs, BatchSizes sizes)
+{
+  const long long total = sizes.off[sizes.n];
+  int op = 0;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       i < total; i += (long long)gridDim.x * blockDim.x)
+    {
+      while (i >= sizes.off[op + 1])
+        ++op;
+      const long long j = i - sizes.off[op];
+      static_cast<T *>(ptrs.dst[op])[j] = (T)ptrs.sol[op][j];
+    }
+}
+
+template <typename T>
+hipError_t batch_launch(int D, int mode, const BatchOp *ops,
+                        const BatchTask *tasks, int n_tasks,
+                        const BatchPtrs &ptrs, hipStream_t stream)
+{
+  if (D != 4)
+    return hipErrorInvalidValue;
+  auto go = [&](auto kernel, int bpc) {
+    const int grid = (int)std::min<long long>(n_tasks, (long long)bpc * 8 *
+                                                         cus_per_xcd());
+    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, stream, ops, tasks,
+                       n_tasks, ptrs);
+  };
+  if (mode == 0)
+    go(batch_kernel<4, T, 0>, kBatchBlocksPerCu);
+  else if (mode == 1)
+    go(batch_kernel<4, T, 1>, kBatchBlocksPerCu);
+  else
+    go(batch_kernel<4, T, 2>, kBatchBlocksPerCu);
+  return hipGetLastError();
+}
+
 } // namespace collapsed
 
 bool collapsed_supported(int D, int N)
@@ -853,5 +984,67 @@ hipError_t collapsed_apply_f32(int D, int N, int mode, const float *src,
 {
   return collapsed::dispatch<float>(D, N, mode, src, dst, coef, mesh, inflow,
                                     stage, stream);
+}
+bool collapsed_batch_supported(int D, int N)
+{
+  return D == 4 && N >= 2 && N <= 6;
+}
+
+int collapsed_batch_cells_per_block(int D, int N, int precision)
+{
+  if (!collapsed_batch_supported(D, N))
+    return 0;
+  const bool f64 = precision == 0;
+  switch (N)
+    {
+      case 2:
+        return f64 ? collapsed::Shape<4, 2, double>::CPB
+                   : collapsed::Shape<4, 2, float>::CPB;
+      case 3:
+        return f64 ? collapsed::Shape<4, 3, double>::CPB
+                   : collapsed::Shape<4, 3, float>::CPB;
+      case 4:
+        return f64 ? collapsed::Shape<4, 4, double>::CPB
+                   : collapsed::Shape<4, 4, float>::CPB;
+      case 5:
+        return f64 ? collapsed::Shape<4, 5, double>::CPB
+                   : collapsed::Shape<4, 5, float>::CPB;
+      default:
+        return f64 ? collapsed::Shape<4, 6, double>::CPB
+                   : collapsed::Shape<4, 6, float>::CPB;
+    }
+}
+
+hipError_t collapsed_batch(int D, int precision, int mode, const BatchOp *ops,
+                           const BatchTask *tasks, int n_tasks,
+                           const BatchPtrs &ptrs, hipStream_t stream)
+{
+  if (n_tasks <= 0)
+    return hipSuccess;
+  return precision == 0
+           ? collapsed::batch_launch<double>(D, mode, ops, tasks, n_tasks,
+                                             ptrs, stream)
+           : collapsed::batch_launch<float>(D, mode, ops, tasks, n_tasks, ptrs,
+                                            stream);
+}
+
+hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,
+                         const BatchPtrs &ptrs, hipStream_t stream)
+{
+  BatchSizes sizes{};
+  sizes.n = n_ops;
+  for (int i = 0; i < n_ops; ++i)
+    sizes.off[i + 1] = sizes.off[i] + n_dofs[i];
+  const long long total = sizes.off[n_ops];
+  if (total == 0)
+    return hipSuccess;
+  const int grid = (int)std::min<long long>((total + 255) / 256, 8192);
+  if (precision == 0)
+    hipLaunchKernelGGL(collapsed::batch_convert_kernel<double>, dim3(grid),
+                       dim3(256), 0, stream, ptrs, sizes);
+  else
+    hipLaunchKernelGGL(collapsed::batch_convert_kernel<float>, dim3(grid),
+                       dim3(256), 0, stream, ptrs, sizes);
+  return hipGetLastError();
 }
 } // namespace hd

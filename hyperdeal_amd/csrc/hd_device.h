@@ -125,6 +125,46 @@ struct XDev
   double G[64];   // l_j' at the quadrature points
 };
 
+// Batched collapsed apply: several independent operators of one dimension
+// D in one launch.
+constexpr int kBatchMax = 64;
+
+// per operator, constant over calls (device table)
+struct BatchOp
+{
+  const void *coef; // FP64 or FP32 collapsed coefficients
+  MeshDev mesh;
+  InflowDev inflow;
+  int N;
+  long long n_dofs;
+};
+
+// a block's work item: logical units [lb, le) of operator op
+struct BatchTask
+{
+  int op;
+  int pad;
+  long long lb, le;
+};
+
+// per call: the vectors of every operator (kernel argument) and the stage
+// coefficients
+struct BatchPtrs
+{
+  const void *src[kBatchMax];
+  void *dst[kBatchMax];
+  double *sol[kBatchMax];
+  void *ti[kBatchMax];
+  double a_dt, b_dt;
+};
+
+// prefix sums of the operators' DoF counts (batched elementwise kernels)
+struct BatchSizes
+{
+  long long off[kBatchMax + 1];
+  int n;
+};
+
 // host launchers (one translation unit per kernel family)
 bool collapsed_supported(int D, int N);
 hipError_t collapsed_apply_f64(int D, int N, int mode, const double *src,
@@ -135,6 +175,17 @@ hipError_t collapsed_apply_f32(int D, int N, int mode, const float *src,
                                float *dst, const float *coef,
                                const MeshDev &mesh, const InflowDev &inflow,
                                const StageDev &stage, hipStream_t stream);
+// batches: dimensions with a batch kernel, the cells one block handles at a
+// time (a task's units are multiples of it), the batched apply / stage
+// (mode as above; ptrs.ti[i] == nullptr selects the last-stage form for
+// every operator) and the batched FP64 -> precision copy of sol into dst
+bool collapsed_batch_supported(int D, int N);
+int collapsed_batch_cells_per_block(int D, int N, int precision);
+hipError_t collapsed_batch(int D, int precision, int mode, const BatchOp *ops,
+                           const BatchTask *tasks, int n_tasks,
+                           const BatchPtrs &ptrs, hipStream_t stream);
+hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,
+                         const BatchPtrs &ptrs, hipStream_t stream);
 bool quadrature_supported(int D, int N);
 hipError_t quadrature_apply(int D, int N, int mode, const double *src,
                             double *dst, const QuadDev &q,

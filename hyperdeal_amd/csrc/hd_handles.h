@@ -1,0 +1,76 @@
+// hd_handles.h -- the opaque handles of the C ABI and the error helpers
+// shared by the translation units that implement it (host code only).
+#pragma once
+
+#include "../../include/hyperdeal_amd.h"
+#include "hd_device.h"
+#include "hd_tables.h"
+
+#include <string>
+
+namespace hd_abi
+{
+// message of the last failure on this thread (hd_last_error)
+extern thread_local std::string g_last_error;
+
+inline int fail(int code, const std::string &msg)
+{
+  g_last_error = msg;
+  return code;
+}
+
+inline int hip_fail(hipError_t e, const char *where)
+{
+  return fail(HD_ERR_HIP, std::string(where) + ": " + hipGetErrorString(e));
+}
+
+inline int ok()
+{
+  return HD_OK;
+}
+
+inline hipStream_t as_stream(void *s)
+{
+  return reinterpret_cast<hipStream_t>(s);
+}
+} // namespace hd_abi
+
+struct hd_matrix_free
+{
+  int dim_x = 0, dim_v = 0, D = 0, degree = 0, nd = 0, nq = 0;
+  int collocation = 0;
+  bool periodic[6] = {true, true, true, true, true, true};
+  hd::MeshDev mesh{};
+  hd::Tables1D tables;
+  long long n_dofs = 0;
+  int dofs_cell = 0;
+  double *partial = nullptr; // norm partial sums (2 per cell)
+};
+
+struct hd_advection
+{
+  hd_matrix_free *mf = nullptr;
+  double a[6] = {0, 0, 0, 0, 0, 0};
+  double skew = 0.0;
+  int bc_kind = HD_BC_INHOMOGENEOUS;
+  int bc_function = HD_FUNC_ZERO;
+  double bc_params[8] = {0};
+  int algorithm = HD_ALGO_AUTO;
+  bool collapsed_ok = false;
+  double *coef64 = nullptr;
+  float *coef32 = nullptr;
+  hd::InflowDev inflow{};
+  hd::QuadDev quad{};
+  // Vlasov-Poisson state (velocity (v, grad phi) from the Poisson solve)
+  bool vp = false;
+  bool vp_auto = true;        // stages recompute the field from their input
+  double poisson_tol = 1e-10; // CG relative residual
+  int poisson_max_it = 100000;
+  hd::XDev xdev{};
+  hd::VpDev vpdev{};
+  long long n_x_cells = 0, n_v_cells = 0, n_x_dofs = 0;
+  int density_chunks = 1;
+  double *efield = nullptr, *rho = nullptr, *phi = nullptr, *work = nullptr,
+         *partial = nullptr;
+  long long *stats = nullptr; // [iterations, solves, last relative residual]
+};

@@ -160,6 +160,49 @@ int hd_lsrk_step(hd_advection *op, const char *type, int precision,
                  void *stream);
 
 /*
+ * Batches of independent advection operators.  The reference has no
+ * counterpart (it runs one operator per MPI rank); a batch replaces many
+ * calls of AdvectionOperation::apply (advection_operation.h:134-209) and
+ * LowStorageRungeKuttaIntegrator::perform_time_step (time_integrators.
+ * templates.h:91-184) on small meshes by one launch per apply / stage.  The
+ * operators must use the collapsed kernel on 2+2 periodic meshes (degree
+ * 1..5, meshes and degrees may differ); at most HD_BATCH_MAX of them.  Array
+ * arguments hold one pointer per operator, in the order of hd_batch_create.
+ * Graphs must be destroyed before their batch.
+ */
+#define HD_BATCH_MAX 64
+typedef struct hd_batch hd_batch;
+typedef struct hd_graph hd_graph;
+
+int hd_batch_create(hd_advection *const *ops, int n_ops, hd_batch **out);
+int hd_batch_destroy(hd_batch *batch);
+int hd_batch_size(const hd_batch *batch);
+/* dst[i] = apply_i(src[i]) for every operator, one launch */
+int hd_batch_apply(hd_batch *batch, const void *const *src, void *const *dst,
+                   int precision, void *stream);
+/* one LSRK step of every operator: sol[i] FP64, ti_a[i] != ti_b[i] stage
+ * registers of `precision`; one copy launch and one launch per stage */
+int hd_batch_lsrk_step(hd_batch *batch, const char *type, int precision,
+                       double *const *sol, void *const *ti_a,
+                       void *const *ti_b, double dt, void *stream);
+/* Autotuner: per degree, times the batched apply of that degree's operators
+ * on src -> dst for each candidate work split (cells per task) and keeps the
+ * fastest.  Synchronous on `stream`. */
+int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
+                      void *const *dst, void *stream);
+/* cells per task used for a degree (tuned or set), or an error */
+int hd_batch_cells_per_task(const hd_batch *batch, int degree, int precision);
+int hd_batch_set_cells_per_task(hd_batch *batch, int degree, int precision,
+                                int cells);
+/* a hipGraph of hd_batch_lsrk_step with these vectors and dt: each launch
+ * advances every operator by dt */
+int hd_batch_graph_create(hd_batch *batch, const char *type, int precision,
+                          double *const *sol, void *const *ti_a,
+                          void *const *ti_b, double dt, hd_graph **out);
+int hd_graph_launch(hd_graph *graph, void *stream);
+int hd_graph_destroy(hd_graph *graph);
+
+/*
  * Vlasov-Poisson operator (examples/vlasov_poisson/include/application.h:
  * 516-602): the advection apply with the phase-space velocity (v, grad phi)
  * (vlasov_poisson/include/velocity_field_view.h:28-209), where phi solves
