@@ -51,7 +51,12 @@ EXPORTED = [
     "hd_batch_create", "hd_batch_destroy", "hd_batch_size", "hd_batch_apply",
     "hd_batch_lsrk_step", "hd_batch_autotune", "hd_batch_cells_per_task",
     "hd_batch_set_cells_per_task", "hd_batch_graph_create",
-    "hd_graph_launch", "hd_graph_destroy",
+    "hd_graph_launch", "hd_graph_destroy", "hd_vp_solver_start",
+    "hd_vp_learned_guess_enable", "hd_vp_learned_guess_disable",
+    "hd_vp_learned_guess_stats", "hd_vp_learned_guess_net",
+    "hd_cnn_n_params", "hd_cnn_create", "hd_cnn_destroy", "hd_cnn_params",
+    "hd_cnn_forward", "hd_cnn_loss_grad", "hd_cnn_train_step",
+    "hd_cnn_last_loss",
 ]
 BATCH_MAX = 64
 
@@ -129,6 +134,27 @@ def load_library(path=None):
                                                 ctypes.c_int]
     lib.hd_vp_solver_stats.argtypes = [_vp, ctypes.POINTER(ctypes.c_longlong),
                                        ctypes.POINTER(ctypes.c_longlong), _dp]
+    _llp = ctypes.POINTER(ctypes.c_longlong)
+    lib.hd_vp_solver_start.argtypes = [_vp, _dp, _dp, _dp, _llp]
+    lib.hd_vp_learned_guess_enable.argtypes = [
+        _vp, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+        ctypes.c_int, ctypes.c_float, ctypes.c_int]
+    lib.hd_vp_learned_guess_disable.argtypes = [_vp]
+    lib.hd_vp_learned_guess_stats.argtypes = [_vp, _llp, _llp, _llp, _dp]
+    lib.hd_vp_learned_guess_net.restype = _vp
+    lib.hd_vp_learned_guess_net.argtypes = [_vp]
+    _fp = ctypes.POINTER(ctypes.c_float)
+    lib.hd_cnn_n_params.argtypes = []
+    lib.hd_cnn_create.argtypes = [ctypes.c_int, ctypes.c_int,
+                                  ctypes.c_ulonglong, ctypes.POINTER(_vp)]
+    lib.hd_cnn_destroy.argtypes = [_vp]
+    lib.hd_cnn_params.argtypes = [_vp, _fp, ctypes.c_int]
+    lib.hd_cnn_forward.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+    lib.hd_cnn_loss_grad.argtypes = [_vp, _vp, _vp, ctypes.c_int, _dp, _fp,
+                                     _vp]
+    lib.hd_cnn_train_step.argtypes = [_vp, _vp, _vp, ctypes.c_int,
+                                      ctypes.c_float, _vp]
+    lib.hd_cnn_last_loss.argtypes = [_vp, _dp]
     lib.hd_vp_x_n_dofs.restype = ctypes.c_longlong
     lib.hd_vp_x_n_dofs.argtypes = [_vp]
     lib.hd_vp_potential.argtypes = [_vp, _dp, ctypes.c_int]
@@ -347,6 +373,49 @@ class VlasovPoissonOperation(AdvectionOperation):
                "hd_vp_solver_stats")
         return it.value, n.value, res.value
 
+    def solver_start(self):
+        """(relative residual of the last solve's start, that of the previous
+        phi alone, weights (previous phi, guess), solves from a guess)."""
+        r0, rp = ctypes.c_double(), ctypes.c_double()
+        w = (ctypes.c_double * 2)()
+        g = ctypes.c_longlong()
+        _check(self._lib.hd_vp_solver_start(self.handle, ctypes.byref(r0),
+                                            ctypes.byref(rp), w,
+                                            ctypes.byref(g)),
+               "hd_vp_solver_start")
+        return r0.value, rp.value, (w[0], w[1]), g.value
+
+    def enable_learned_guess(self, seed=20261205, capacity=64, train_every=1,
+                             train_batch=8, train_steps=1,
+                             learning_rate=1e-3, min_pairs=4):
+        """Start the potential solves from the CNN guess combined with the
+        previous phi; the network trains on this operator's own solves."""
+        _check(self._lib.hd_vp_learned_guess_enable(
+            self.handle, int(seed), int(capacity), int(train_every),
+            int(train_batch), int(train_steps), float(learning_rate),
+            int(min_pairs)), "hd_vp_learned_guess_enable")
+
+    def disable_learned_guess(self):
+        _check(self._lib.hd_vp_learned_guess_disable(self.handle),
+               "hd_vp_learned_guess_disable")
+
+    def learned_guess_stats(self):
+        """(Adam steps, solves that used the guess, pairs stored, last
+        training loss)."""
+        st, gs, pr = (ctypes.c_longlong() for _ in range(3))
+        loss = ctypes.c_double()
+        _check(self._lib.hd_vp_learned_guess_stats(
+            self.handle, ctypes.byref(st), ctypes.byref(gs), ctypes.byref(pr),
+            ctypes.byref(loss)), "hd_vp_learned_guess_stats")
+        return st.value, gs.value, pr.value, loss.value
+
+    def learned_guess_net(self):
+        """The network of the learned guess (borrowed from this operator)."""
+        h = self._lib.hd_vp_learned_guess_net(self.handle)
+        if not h:
+            raise HyperdealError("no learned guess on this operator")
+        return Cnn(_handle=h, _owner=self)
+
     def x_n_dofs(self):
         return self._lib.hd_vp_x_n_dofs(self.handle)
 
@@ -368,6 +437,93 @@ class VlasovPoissonOperation(AdvectionOperation):
         _check(self._lib.hd_vp_field_energy(self.handle, out),
                "hd_vp_field_energy")
         return [out[i] for i in range(self.mf.dim_x)]
+
+
+class Cnn:
+    """The Poisson-guess network on its own: four 3x3 periodic convolutions
+    1 -> 32 -> 32 -> 32 -> 1, leaky ReLU 1/8, BF16 weights, FP32 accumulation,
+    Adam on FP32 master parameters.  Images are float32 CUDA tensors of shape
+    (batch, height, width), height and width multiples of 16."""
+
+    def __init__(self, height=None, width=None, seed=0, lib=None,
+                 _handle=None, _owner=None):
+        self._lib = lib or _lib or load_library()
+        self._owner = _owner
+        if _handle is not None:
+            self.handle = _vp(_handle)
+            self._owned = False
+            return
+        h = _vp()
+        _check(self._lib.hd_cnn_create(int(height), int(width), int(seed),
+                                       ctypes.byref(h)), "hd_cnn_create")
+        self.handle = h
+        self._owned = True
+
+    @staticmethod
+    def n_params(lib=None):
+        return (lib or _lib or load_library()).hd_cnn_n_params()
+
+    def params(self):
+        import numpy as np
+        out = np.zeros(self._lib.hd_cnn_n_params(), dtype=np.float32)
+        _check(self._lib.hd_cnn_params(
+            self.handle, out.ctypes.data_as(ctypes.POINTER(ctypes.c_float)),
+            1), "hd_cnn_params")
+        return out
+
+    def set_params(self, p):
+        import numpy as np
+        p = np.ascontiguousarray(p, dtype=np.float32)
+        if p.size != self._lib.hd_cnn_n_params():
+            raise HyperdealError("wrong number of parameters")
+        _check(self._lib.hd_cnn_params(
+            self.handle, p.ctypes.data_as(ctypes.POINTER(ctypes.c_float)), 0),
+            "hd_cnn_params")
+
+    @staticmethod
+    def _images(x):
+        import torch
+        if x.dtype != torch.float32 or not x.is_cuda or x.dim() != 3 or \
+                not x.is_contiguous():
+            raise HyperdealError("images must be contiguous float32 CUDA "
+                                 "tensors (batch, height, width)")
+        return ctypes.c_void_p(x.data_ptr())
+
+    def forward(self, x, out=None, stream=None):
+        import torch
+        if out is None:
+            out = torch.empty_like(x)
+        _check(self._lib.hd_cnn_forward(self.handle, self._images(x),
+                                        self._images(out), x.shape[0],
+                                        _stream(stream)), "hd_cnn_forward")
+        return out
+
+    def loss_grad(self, x, target, stream=None):
+        import numpy as np
+        g = np.zeros(self._lib.hd_cnn_n_params(), dtype=np.float32)
+        loss = ctypes.c_double()
+        _check(self._lib.hd_cnn_loss_grad(
+            self.handle, self._images(x), self._images(target), x.shape[0],
+            ctypes.byref(loss), g.ctypes.data_as(
+                ctypes.POINTER(ctypes.c_float)), _stream(stream)),
+            "hd_cnn_loss_grad")
+        return loss.value, g
+
+    def train_step(self, x, target, learning_rate=1e-3, stream=None):
+        _check(self._lib.hd_cnn_train_step(
+            self.handle, self._images(x), self._images(target), x.shape[0],
+            float(learning_rate), _stream(stream)), "hd_cnn_train_step")
+
+    def last_loss(self):
+        loss = ctypes.c_double()
+        _check(self._lib.hd_cnn_last_loss(self.handle, ctypes.byref(loss)),
+               "hd_cnn_last_loss")
+        return loss.value
+
+    def __del__(self):
+        if getattr(self, "_owned", False) and self.handle:
+            self._lib.hd_cnn_destroy(self.handle)
+            self.handle = None
 
 
 def vp_moments(mf, f, stream=None):

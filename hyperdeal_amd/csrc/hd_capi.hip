@@ -30,8 +30,9 @@ using hd_abi::as_stream;
 int choose_algorithm(const hd_advection *op, int precision)
 {
   const hd_matrix_free *mf = op->mf;
-  const bool quad_ok = precision == HD_FP64 && mf->nq == mf->nd &&
-                       hd::quadrature_supported(mf->D, mf->nd);
+  (void)precision; // both kernels have FP64 and FP32-mixed forms
+  const bool quad_ok =
+    mf->nq == mf->nd && hd::quadrature_supported(mf->D, mf->nd);
   const bool coll_ok = op->collapsed_ok;
   switch (op->algorithm)
     {
@@ -44,15 +45,23 @@ int choose_algorithm(const hd_advection *op, int precision)
     }
 }
 
-int vp_update(hd_advection *op, const double *f, hipStream_t stream)
+// rho from f (FP64, or the FP32 stage input of the mixed form), the
+// potential solve (with the learned guess when it is on), grad phi
+int vp_update(hd_advection *op, const double *f, const float *f32,
+              hipStream_t stream)
 {
-  hipError_t e = hd::vp_density(f, nullptr, op->vpdev, op->n_x_cells,
+  hipError_t e = hd::vp_density(f, f32, op->vpdev, op->n_x_cells,
                                 op->n_v_cells, op->partial,
                                 op->density_chunks, op->rho, stream);
+  const double *guess = nullptr;
+  if (e == hipSuccess && op->learned)
+    e = hd_abi::learned_before_solve(*op->learned, op->rho, &guess, stream);
   if (e == hipSuccess)
-    e = hd::vp_poisson(op->xdev, op->mf->nd, op->rho, op->phi, op->efield,
-                       op->work, op->poisson_tol, op->poisson_max_it,
-                       op->stats, stream);
+    e = hd::vp_poisson(op->xdev, op->mf->nd, op->rho, op->phi, guess,
+                       op->efield, op->work, op->poisson_tol,
+                       op->poisson_max_it, op->stats, stream);
+  if (e == hipSuccess && op->learned)
+    e = hd_abi::learned_after_solve(*op->learned, op->phi, stream);
   return e == hipSuccess ? HD_OK : hip_fail(e, "Vlasov-Poisson field update");
 }
 
@@ -63,10 +72,10 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
   hd_matrix_free *mf = op->mf;
   if (op->vp && op->vp_auto)
     {
-      if (precision != HD_FP64)
-        return fail(HD_ERR_UNSUPPORTED,
-                    "the Vlasov-Poisson operator runs in FP64");
-      const int rc = vp_update(op, static_cast<const double *>(src), stream);
+      const int rc =
+        precision == HD_FP64
+          ? vp_update(op, static_cast<const double *>(src), nullptr, stream)
+          : vp_update(op, nullptr, static_cast<const float *>(src), stream);
       if (rc != HD_OK)
         return rc;
     }
@@ -98,12 +107,19 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
                                     static_cast<float *>(dst), op->coef32,
                                     mf->mesh, op->inflow, stage, stream);
     }
-  else
+  else if (precision == HD_FP64)
     e = hd::quadrature_apply(mf->D, mf->nd, mode,
                              static_cast<const double *>(src),
                              static_cast<double *>(dst), op->quad, mf->mesh,
                              stage, time, mf->dim_x, mf->collocation,
                              op->vp ? op->efield : nullptr, stream);
+  else
+    e = hd::quadrature_apply_f32(mf->D, mf->nd, mode,
+                                 static_cast<const float *>(src),
+                                 static_cast<float *>(dst), op->quad,
+                                 mf->mesh, stage, time, mf->dim_x,
+                                 mf->collocation,
+                                 op->vp ? op->efield : nullptr, stream);
   if (e != hipSuccess)
     return hip_fail(e, "kernel launch");
   return ok();
@@ -346,6 +362,7 @@ int hd_advection_destroy(hd_advection *op)
                   (void *)op->partial, (void *)op->stats})
     if (p)
       (void)hipFree(p);
+  hd_abi::learned_release(op->learned);
   delete op;
   return ok();
 }
@@ -448,13 +465,13 @@ int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
     e = hipMalloc(&op->partial,
                   (size_t)ncx * op->density_chunks * v.dofs_x * sizeof(double));
   if (e == hipSuccess)
-    e = hipMalloc(&op->stats, 4 * sizeof(long long));
+    e = hipMalloc(&op->stats, 8 * sizeof(long long));
   if (e == hipSuccess)
     e = hipMemset(op->phi, 0, nxd * sizeof(double));
   if (e == hipSuccess)
     e = hipMemset(op->efield, 0, nef * sizeof(double));
   if (e == hipSuccess)
-    e = hipMemset(op->stats, 0, 4 * sizeof(long long));
+    e = hipMemset(op->stats, 0, 8 * sizeof(long long));
   if (e != hipSuccess)
     {
       hd_advection_destroy(op);
@@ -469,7 +486,7 @@ int hd_vp_update_field(hd_advection *op, const double *f, void *stream)
   if (!op || !f || !op->vp)
     return fail(HD_ERR_INVALID, "hd_vp_update_field: not a Vlasov-Poisson "
                                 "operator or null argument");
-  return vp_update(op, f, as_stream(stream));
+  return vp_update(op, f, nullptr, as_stream(stream));
 }
 
 int hd_vp_set_auto_update(hd_advection *op, int enable)
@@ -505,6 +522,28 @@ int hd_vp_solver_stats(hd_advection *op, long long *iterations,
     *solves = s[1];
   if (last_rel_residual)
     std::memcpy(last_rel_residual, &s[2], sizeof(double));
+  return ok();
+}
+
+int hd_vp_solver_start(hd_advection *op, double *start_rel_residual,
+                       double *previous_rel_residual, double *weights,
+                       long long *guessed_solves)
+{
+  if (!op || !op->vp)
+    return fail(HD_ERR_INVALID, "not a Vlasov-Poisson operator");
+  long long s[8];
+  const hipError_t e =
+    hipMemcpy(s, op->stats, 8 * sizeof(long long), hipMemcpyDeviceToHost);
+  if (e != hipSuccess)
+    return hip_fail(e, "hd_vp_solver_start");
+  if (start_rel_residual)
+    std::memcpy(start_rel_residual, &s[3], sizeof(double));
+  if (previous_rel_residual)
+    std::memcpy(previous_rel_residual, &s[4], sizeof(double));
+  if (weights)
+    std::memcpy(weights, &s[6], 2 * sizeof(double));
+  if (guessed_solves)
+    *guessed_solves = s[5];
   return ok();
 }
 

@@ -1,0 +1,666 @@
+// hd_cnn.hip -- a small convolutional network that maps the x-space charge
+// density of the Vlasov-Poisson problem to an initial guess for the
+// potential (no counterpart in the reference, which starts CG from the
+// previous potential, vlasov_poisson/include/application.h:575-576).
+//
+// Four 3x3 convolutions with periodic padding on an H x W image of nodal
+// values, channels 1 -> 32 -> 32 -> 32 -> 1, leaky ReLU (slope 1/8) after
+// the first three.  Weights are read in BF16, two channels per 32-bit word,
+// through scalar loads (they are uniform over a workgroup), and every
+// product is accumulated in FP32; biases, activations and the Adam state are
+// FP32.  Convolutions run as direct stencils over 16 x 16 pixel tiles staged
+// in LDS; the weight gradient is one workgroup per (output, input) channel
+// pair reducing over the pixels in a fixed order, so every result is
+// deterministic.
+#include "hd_cnn.h"
+#include "hd_device.h"
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <vector>
+
+namespace hd
+{
+namespace cnn
+{
+constexpr int kHalo = kTile + 2;
+constexpr float kSlope = 0.125f;
+
+template <class T>
+using cptr = const __attribute__((address_space(4))) T *;
+
+// FP32 -> BF16, round to nearest even (NaN stays NaN)
+__host__ __device__ inline uint16_t to_bf16(float x)
+{
+  uint32_t u;
+  memcpy(&u, &x, 4);
+  if ((u & 0x7fffffffu) > 0x7f800000u)
+    return (uint16_t)((u >> 16) | 0x40u);
+  u += 0x7fffu + ((u >> 16) & 1u);
+  return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bits_to_float(uint32_t u)
+{
+  float x;
+  memcpy(&x, &u, 4);
+  return x;
+}
+
+__device__ inline int wrap(int i, int n)
+{
+  return i < 0 ? i + n : (i >= n ? i - n : i);
+}
+
+// out[b][o] = act(bias[o] + sum_{i,k} w[i][k][o] in[b][i] shifted by k),
+// w packed as [CIN][9][COUT rounded up to even] BF16.  Grid
+// (W / 16, H / 16, batch * COUT / OT), one output pixel and OT channels per
+// thread.
+template <int CIN, int COUT, bool ACT>
+__global__ __launch_bounds__(kTile *kTile) void conv_kernel(
+  const float *__restrict__ in, const uint32_t *__restrict__ w,
+  const float *__restrict__ bias, float *__restrict__ out, int H, int W)
+{
+  constexpr int OT = COUT < 8 ? COUT : 8;
+  constexpr int NG = COUT / OT;
+  constexpr int WPR = (COUT + (COUT & 1)) / 2; // words per (i, k)
+  constexpr int IT = CIN < 8 ? CIN : 8;
+  static_assert(COUT % OT == 0 && CIN % IT == 0, "channel tiling");
+  __shared__ float tile[IT][kHalo][kHalo];
+  const cptr<uint32_t> wc = (cptr<uint32_t>)w;
+  const int tx = threadIdx.x % kTile, ty = threadIdx.x / kTile;
+  const int b = blockIdx.z / NG, og = blockIdx.z % NG;
+  const int x0 = blockIdx.x * kTile, y0 = blockIdx.y * kTile;
+  const long long plane = (long long)H * W;
+  float acc[OT];
+#pragma unroll
+  for (int o = 0; o < OT; ++o)
+    acc[o] = 0.0f;
+  for (int i0 = 0; i0 < CIN; i0 += IT)
+    {
+      for (int e = threadIdx.x; e < IT * kHalo * kHalo; e += kTile * kTile)
+        {
+          const int i = e / (kHalo * kHalo), r = e % (kHalo * kHalo);
+          const int yy = wrap(y0 + r / kHalo - 1, H);
+          const int xx = wrap(x0 + r % kHalo - 1, W);
+          tile[i][r / kHalo][r % kHalo] =
+            in[((long long)b * CIN + i0 + i) * plane + (long long)yy * W + xx];
+        }
+      __syncthreads();
+#pragma unroll
+      for (int i = 0; i < IT; ++i)
+#pragma unroll
+        for (int k = 0; k < 9; ++k)
+          {
+            const float v = tile[i][ty + k / 3][tx + k % 3];
+            const int base = ((i0 + i) * 9 + k) * WPR + og * (OT / 2);
+            if constexpr (OT == 1)
+              acc[0] += bits_to_float(wc[base] << 16) * v;
+            else
+#pragma unroll
+              for (int j = 0; j < OT / 2; ++j)
+                {
+                  const uint32_t u = wc[base + j];
+                  acc[2 * j] += bits_to_float(u << 16) * v;
+                  acc[2 * j + 1] += bits_to_float(u & 0xffff0000u) * v;
+                }
+          }
+      __syncthreads();
+    }
+  const int x = x0 + tx, y = y0 + ty;
+#pragma unroll
+  for (int o = 0; o < OT; ++o)
+    {
+      const int oc = og * OT + o;
+      float v = acc[o] + (bias ? ((cptr<float>)bias)[oc] : 0.0f);
+      if (ACT)
+        v = v > 0.0f ? v : kSlope * v;
+      out[((long long)b * COUT + oc) * plane + (long long)y * W + x] = v;
+    }
+}
+
+// g *= leaky-ReLU'(pre-activation); a is the post-activation output, which
+// has the sign of the pre-activation
+__global__ void act_grad_kernel(float *__restrict__ g,
+                                const float *__restrict__ a, long long n)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    g[i] *= a[i] > 0.0f ? 1.0f : kSlope;
+}
+
+// Weight and bias gradients of one layer, block (o, i): four pixels of a
+// row per thread and step, then a fixed-order block reduction; blocks with
+// i == 0 also write the bias gradient of o.
+template <int CIN, int COUT>
+__global__ __launch_bounds__(256) void wgrad_kernel(
+  const float *__restrict__ in, const float *__restrict__ g,
+  float *__restrict__ gw, float *__restrict__ gb, int B, int H, int W)
+{
+  __shared__ float red[10][256];
+  const int o = blockIdx.x / CIN, i = blockIdx.x % CIN;
+  const long long plane = (long long)H * W;
+  const int W4 = W / 4;
+  const long long groups = (long long)B * H * W4;
+  float s[10];
+#pragma unroll
+  for (int k = 0; k < 10; ++k)
+    s[k] = 0.0f;
+  for (long long e = threadIdx.x; e < groups; e += blockDim.x)
+    {
+      const int b = (int)(e / ((long long)H * W4));
+      const int r = (int)(e % ((long long)H * W4));
+      const int y = r / W4, x = (r % W4) * 4;
+      const float4 gv = *reinterpret_cast<const float4 *>(
+        g + ((long long)b * COUT + o) * plane + (long long)y * W + x);
+      const float *ib = in + ((long long)b * CIN + i) * plane;
+#pragma unroll
+      for (int dy = 0; dy < 3; ++dy)
+        {
+          const float *row = ib + (long long)wrap(y + dy - 1, H) * W;
+          const float4 mid = *reinterpret_cast<const float4 *>(row + x);
+          const float v[6] = {row[wrap(x - 1, W)], mid.x, mid.y, mid.z,
+                              mid.w, row[wrap(x + 4, W)]};
+#pragma unroll
+          for (int dx = 0; dx < 3; ++dx)
+            s[dy * 3 + dx] += gv.x * v[dx] + gv.y * v[dx + 1] +
+                              gv.z * v[dx + 2] + gv.w * v[dx + 3];
+        }
+      s[9] += (gv.x + gv.y) + (gv.z + gv.w);
+    }
+#pragma unroll
+  for (int k = 0; k < 10; ++k)
+    red[k][threadIdx.x] = s[k];
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1)
+    {
+      if ((int)threadIdx.x < st)
+#pragma unroll
+        for (int k = 0; k < 10; ++k)
+          red[k][threadIdx.x] += red[k][threadIdx.x + st];
+      __syncthreads();
+    }
+  if (threadIdx.x < 9)
+    gw[((long long)o * CIN + i) * 9 + threadIdx.x] = red[threadIdx.x][0];
+  if (threadIdx.x == 9 && i == 0)
+    gb[o] = red[9][0];
+}
+
+// loss = mean (out - target)^2 (FP64 sum), g = d loss / d out; one block
+__global__ __launch_bounds__(1024) void mse_kernel(
+  const float *__restrict__ out, const float *__restrict__ target,
+  float *__restrict__ g, long long n, double *__restrict__ loss)
+{
+  __shared__ double red[1024];
+  double s = 0.0;
+  const float sc = (float)(2.0 / (double)n);
+  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+    {
+      const float d = out[i] - target[i];
+      g[i] = sc * d;
+      s += (double)d * d;
+    }
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1)
+    {
+      if ((int)threadIdx.x < st)
+        red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+  if (threadIdx.x == 0)
+    *loss = red[0] / (double)n;
+}
+
+// Adam on the FP32 master parameters (bias-corrected step sizes c1, c2)
+__global__ void adam_kernel(float *__restrict__ p, float *__restrict__ m,
+                            float *__restrict__ v,
+                            const float *__restrict__ grad, int n, float lr,
+                            float b1, float b2, float c1, float c2, float eps)
+{
+  for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n;
+       i += gridDim.x * blockDim.x)
+    {
+      const float gr = grad[i];
+      const float mi = b1 * m[i] + (1.0f - b1) * gr;
+      const float vi = b2 * v[i] + (1.0f - b2) * gr * gr;
+      m[i] = mi;
+      v[i] = vi;
+      p[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+    }
+}
+
+// master weights w[Co][Ci][9] of one layer -> the two packed BF16 copies
+__global__ void pack_kernel(const float *__restrict__ w, int Ci, int Co,
+                            uint32_t *__restrict__ fwd,
+                            uint32_t *__restrict__ bwd)
+{
+  const int wpo = (Co + 1) / 2, wpi = (Ci + 1) / 2;
+  const int n_f = Ci * 9 * wpo, n_b = Co * 9 * wpi;
+  for (int q = blockIdx.x * blockDim.x + threadIdx.x; q < n_f + n_b;
+       q += gridDim.x * blockDim.x)
+    {
+      uint32_t lo = 0, hi = 0;
+      if (q < n_f)
+        {
+          const int pr = q % wpo, k = (q / wpo) % 9, i = q / (wpo * 9);
+          const int o0 = 2 * pr, o1 = 2 * pr + 1;
+          lo = to_bf16(w[(o0 * Ci + i) * 9 + k]);
+          if (o1 < Co)
+            hi = to_bf16(w[(o1 * Ci + i) * 9 + k]);
+          fwd[q] = lo | (hi << 16);
+        }
+      else
+        {
+          const int qb = q - n_f;
+          const int pr = qb % wpi, k = (qb / wpi) % 9, o = qb / (wpi * 9);
+          const int i0 = 2 * pr, i1 = 2 * pr + 1;
+          lo = to_bf16(w[(o * Ci + i0) * 9 + 8 - k]);
+          if (i1 < Ci)
+            hi = to_bf16(w[(o * Ci + i1) * 9 + 8 - k]);
+          bwd[qb] = lo | (hi << 16);
+        }
+    }
+}
+
+// ---- x-space nodal values <-> images ----
+__device__ inline long long pixel_of(long long dof, int n0, int N)
+{
+  const int NN = N * N;
+  const long long c = dof / NN;
+  const int i = (int)(dof - c * NN);
+  const long long cx = c % n0, cy = c / n0;
+  const int ix = i % N, iy = i / N;
+  return (cy * N + iy) * ((long long)n0 * N) + cx * N + ix;
+}
+
+__device__ inline double block_sum(double v, double *red)
+{
+  red[threadIdx.x] = v;
+  __syncthreads();
+  for (int st = blockDim.x / 2; st > 0; st >>= 1)
+    {
+      if ((int)threadIdx.x < st)
+        red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+  const double r = red[0];
+  __syncthreads();
+  return r;
+}
+
+__global__ __launch_bounds__(1024) void dofs_to_image_kernel(
+  const double *__restrict__ u, long long n, int n0, int N,
+  float *__restrict__ image, double *__restrict__ scale)
+{
+  __shared__ double red[1024];
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+    s += u[i];
+  const double mean = block_sum(s, red) / (double)n;
+  s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+    s += (u[i] - mean) * (u[i] - mean);
+  const double rms = sqrt(block_sum(s, red) / (double)n);
+  const double inv = rms > 0.0 ? 1.0 / rms : 0.0;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+    image[pixel_of(i, n0, N)] = (float)((u[i] - mean) * inv);
+  if (threadIdx.x == 0)
+    *scale = rms;
+}
+
+__global__ void image_to_dofs_kernel(const float *__restrict__ image,
+                                     const double *__restrict__ scale,
+                                     long long n, int n0, int N,
+                                     double *__restrict__ u)
+{
+  const double sc = *scale;
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    u[i] = sc * (double)image[pixel_of(i, n0, N)];
+}
+
+__global__ __launch_bounds__(1024) void target_image_kernel(
+  const double *__restrict__ phi, const double *__restrict__ scale,
+  long long n, int n0, int N, float *__restrict__ image)
+{
+  __shared__ double red[1024];
+  double s = 0.0;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+    s += phi[i];
+  const double mean = block_sum(s, red) / (double)n;
+  const double sc = *scale;
+  const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
+  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+    image[pixel_of(i, n0, N)] = (float)((phi[i] - mean) * inv);
+}
+
+// dst[j] = src[idx_j] for n images of `plane` floats; idx_j drawn on the
+// device from SplitMix64 (seed, counter0 + j) uniformly in [0, count)
+__global__ void gather_kernel(const float *__restrict__ src,
+                              const float *__restrict__ src2, uint64_t seed,
+                              uint64_t counter0, int count, int n,
+                              long long plane, float *__restrict__ dst,
+                              float *__restrict__ dst2)
+{
+  const long long total = (long long)n * plane;
+  for (long long e = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       e < total; e += (long long)gridDim.x * blockDim.x)
+    {
+      const long long j = e / plane, p = e - j * plane;
+      const double u = 0.5 * (splitmix_uniform(seed, counter0 + j) + 1.0);
+      const int idx = min((int)(u * count), count - 1);
+      dst[e] = src[(long long)idx * plane + p];
+      dst2[e] = src2[(long long)idx * plane + p];
+    }
+}
+} // namespace cnn
+
+// ---------------------------------------------------------------------------
+// CnnNet
+
+int CnnNet::in_channels(int l) { return l == 0 ? 1 : cnn::kChannels; }
+int CnnNet::out_channels(int l)
+{
+  return l == cnn::kLayers - 1 ? 1 : cnn::kChannels;
+}
+
+int CnnNet::param_offset_w(int l)
+{
+  int o = 0;
+  for (int j = 0; j < l; ++j)
+    o += out_channels(j) * in_channels(j) * 9 + out_channels(j);
+  return o;
+}
+
+int CnnNet::param_offset_b(int l)
+{
+  return param_offset_w(l) + out_channels(l) * in_channels(l) * 9;
+}
+
+int CnnNet::n_params() { return param_offset_w(cnn::kLayers); }
+
+namespace
+{
+int fwd_words(int l)
+{
+  return CnnNet::in_channels(l) * 9 * ((CnnNet::out_channels(l) + 1) / 2);
+}
+int bwd_words(int l)
+{
+  return CnnNet::out_channels(l) * 9 * ((CnnNet::in_channels(l) + 1) / 2);
+}
+} // namespace
+
+int CnnNet::pack_offset_fwd(int l)
+{
+  int o = 0;
+  for (int j = 0; j < l; ++j)
+    o += fwd_words(j) + bwd_words(j);
+  return o;
+}
+
+int CnnNet::pack_offset_bwd(int l) { return pack_offset_fwd(l) + fwd_words(l); }
+
+int CnnNet::n_packed() { return pack_offset_fwd(cnn::kLayers); }
+
+bool CnnNet::shape_ok(int h, int w)
+{
+  return h >= cnn::kTile && w >= cnn::kTile && h % cnn::kTile == 0 &&
+         w % cnn::kTile == 0;
+}
+
+hipError_t CnnNet::init(int h, int w, uint64_t seed)
+{
+  H = h;
+  W = w;
+  const int n = n_params();
+  hipError_t e = hipMalloc(&p, n * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMalloc(&m, n * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMalloc(&v, n * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMalloc(&grad, n * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMalloc(&pk, n_packed() * sizeof(uint32_t));
+  if (e == hipSuccess)
+    e = hipMalloc(&loss_dev, sizeof(double));
+  if (e != hipSuccess)
+    return e;
+  // uniform weights in +-sqrt(6 / fan_in) (the last layer scaled by 1/10)
+  // from SplitMix64 counters 0, 1, ... in canonical order; biases zero
+  std::vector<float> hp(n, 0.0f);
+  uint64_t ctr = 0;
+  for (int l = 0; l < cnn::kLayers; ++l)
+    {
+      const double lim = std::sqrt(6.0 / (in_channels(l) * 9)) *
+                         (l == cnn::kLayers - 1 ? 0.1 : 1.0);
+      const int ow = param_offset_w(l);
+      for (int k = 0; k < out_channels(l) * in_channels(l) * 9; ++k)
+        hp[ow + k] = (float)(lim * splitmix_uniform(seed, ctr++));
+    }
+  e = hipMemcpy(p, hp.data(), n * sizeof(float), hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemset(m, 0, n * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMemset(v, 0, n * sizeof(float));
+  if (e == hipSuccess)
+    e = hipMemset(grad, 0, n * sizeof(float));
+  if (e == hipSuccess)
+    e = pack(nullptr);
+  if (e == hipSuccess)
+    e = hipDeviceSynchronize();
+  steps = 0;
+  return e;
+}
+
+void CnnNet::release()
+{
+  (void)hipFree(p);
+  (void)hipFree(m);
+  (void)hipFree(v);
+  (void)hipFree(grad);
+  (void)hipFree(pk);
+  (void)hipFree(loss_dev);
+  (void)hipFree(act);
+  p = m = v = grad = act = nullptr;
+  pk = nullptr;
+  loss_dev = nullptr;
+  act_batch = 0;
+}
+
+hipError_t CnnNet::pack(hipStream_t s)
+{
+  for (int l = 0; l < cnn::kLayers; ++l)
+    {
+      const int n = fwd_words(l) + bwd_words(l);
+      hipLaunchKernelGGL(cnn::pack_kernel, dim3((n + 255) / 256), dim3(256), 0,
+                         s, p + param_offset_w(l), in_channels(l),
+                         out_channels(l), pk + pack_offset_fwd(l),
+                         pk + pack_offset_bwd(l));
+    }
+  return hipGetLastError();
+}
+
+hipError_t CnnNet::reserve(int batch)
+{
+  if (batch <= act_batch)
+    return hipSuccess;
+  // the buffers may still be in use by queued kernels
+  hipError_t e = hipDeviceSynchronize();
+  if (e != hipSuccess)
+    return e;
+  (void)hipFree(act);
+  act = nullptr;
+  act_batch = 0;
+  // a1, a2, a3 (C channels), out (1), g0, g1 (C channels)
+  const size_t per =
+    (size_t)H * W * (3 * cnn::kChannels + 1 + 2 * cnn::kChannels);
+  e = hipMalloc(&act, per * batch * sizeof(float));
+  if (e == hipSuccess)
+    act_batch = batch;
+  return e;
+}
+
+namespace
+{
+struct Bufs
+{
+  float *a1, *a2, *a3, *out, *g0, *g1;
+};
+
+// the layout of the activation buffer for batch size nb (<= act_batch)
+Bufs buffers(float *act, int nb, int H, int W)
+{
+  const size_t cp = (size_t)nb * cnn::kChannels * H * W;
+  Bufs b;
+  b.a1 = act;
+  b.a2 = b.a1 + cp;
+  b.a3 = b.a2 + cp;
+  b.out = b.a3 + cp;
+  b.g0 = b.out + (size_t)nb * H * W;
+  b.g1 = b.g0 + cp;
+  return b;
+}
+
+template <int CIN, int COUT, bool ACT>
+void conv(const float *in, const uint32_t *w, const float *bias, float *out,
+          int batch, int H, int W, hipStream_t s)
+{
+  constexpr int OT = COUT < 8 ? COUT : 8;
+  hipLaunchKernelGGL((cnn::conv_kernel<CIN, COUT, ACT>),
+                     dim3(W / cnn::kTile, H / cnn::kTile,
+                          batch * (COUT / OT)),
+                     dim3(cnn::kTile * cnn::kTile), 0, s, in, w, bias, out, H,
+                     W);
+}
+
+void act_grad(float *g, const float *a, long long n, hipStream_t s)
+{
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 8192);
+  hipLaunchKernelGGL(cnn::act_grad_kernel, dim3(blocks), dim3(256), 0, s, g, a,
+                     n);
+}
+} // namespace
+
+hipError_t CnnNet::forward(const float *in, float *out, int batch,
+                           hipStream_t s)
+{
+  hipError_t e = reserve(batch);
+  if (e != hipSuccess)
+    return e;
+  const Bufs b = buffers(act, batch, H, W);
+  conv<1, 32, true>(in, pk + pack_offset_fwd(0), p + param_offset_b(0), b.a1,
+                    batch, H, W, s);
+  conv<32, 32, true>(b.a1, pk + pack_offset_fwd(1), p + param_offset_b(1),
+                     b.a2, batch, H, W, s);
+  conv<32, 32, true>(b.a2, pk + pack_offset_fwd(2), p + param_offset_b(2),
+                     b.a3, batch, H, W, s);
+  conv<32, 1, false>(b.a3, pk + pack_offset_fwd(3), p + param_offset_b(3),
+                     out, batch, H, W, s);
+  return hipGetLastError();
+}
+
+hipError_t CnnNet::loss_and_grad(const float *in, const float *target,
+                                 int batch, hipStream_t s)
+{
+  hipError_t e = reserve(batch);
+  if (e != hipSuccess)
+    return e;
+  const Bufs b = buffers(act, batch, H, W);
+  e = forward(in, b.out, batch, s);
+  if (e != hipSuccess)
+    return e;
+  const long long plane = (long long)H * W;
+  const long long nC = (long long)batch * cnn::kChannels * plane;
+  const int C = cnn::kChannels;
+  hipLaunchKernelGGL(cnn::mse_kernel, dim3(1), dim3(1024), 0, s, b.out, target,
+                     b.g0, batch * plane, loss_dev);
+  // layer 3 (C -> 1, linear)
+  hipLaunchKernelGGL((cnn::wgrad_kernel<32, 1>), dim3(C), dim3(256), 0, s,
+                     b.a3, b.g0, grad + param_offset_w(3),
+                     grad + param_offset_b(3), batch, H, W);
+  conv<1, 32, false>(b.g0, pk + pack_offset_bwd(3), nullptr, b.g1, batch, H, W,
+                     s);
+  act_grad(b.g1, b.a3, nC, s);
+  // layer 2
+  hipLaunchKernelGGL((cnn::wgrad_kernel<32, 32>), dim3(C * C), dim3(256), 0, s,
+                     b.a2, b.g1, grad + param_offset_w(2),
+                     grad + param_offset_b(2), batch, H, W);
+  conv<32, 32, false>(b.g1, pk + pack_offset_bwd(2), nullptr, b.g0, batch, H,
+                      W, s);
+  act_grad(b.g0, b.a2, nC, s);
+  // layer 1
+  hipLaunchKernelGGL((cnn::wgrad_kernel<32, 32>), dim3(C * C), dim3(256), 0, s,
+                     b.a1, b.g0, grad + param_offset_w(1),
+                     grad + param_offset_b(1), batch, H, W);
+  conv<32, 32, false>(b.g0, pk + pack_offset_bwd(1), nullptr, b.g1, batch, H,
+                      W, s);
+  act_grad(b.g1, b.a1, nC, s);
+  // layer 0 (1 -> C)
+  hipLaunchKernelGGL((cnn::wgrad_kernel<1, 32>), dim3(C), dim3(256), 0, s, in,
+                     b.g1, grad + param_offset_w(0), grad + param_offset_b(0),
+                     batch, H, W);
+  return hipGetLastError();
+}
+
+hipError_t CnnNet::adam(float lr, hipStream_t s)
+{
+  ++steps;
+  const double b1 = 0.9, b2 = 0.999;
+  const float c1 = (float)(1.0 - std::pow(b1, steps));
+  const float c2 = (float)(1.0 - std::pow(b2, steps));
+  const int n = n_params();
+  hipLaunchKernelGGL(cnn::adam_kernel, dim3((n + 255) / 256), dim3(256), 0, s,
+                     p, m, v, grad, n, lr, (float)b1, (float)b2, c1, c2, 1e-8f);
+  hipError_t e = hipGetLastError();
+  return e == hipSuccess ? pack(s) : e;
+}
+
+// ---------------------------------------------------------------------------
+// image helpers
+
+hipError_t cnn_dofs_to_image(const double *u, int n0, int n1, int N,
+                             float *image, double *scale, hipStream_t s)
+{
+  const long long n = (long long)n0 * n1 * N * N;
+  hipLaunchKernelGGL(cnn::dofs_to_image_kernel, dim3(1), dim3(1024), 0, s, u,
+                     n, n0, N, image, scale);
+  return hipGetLastError();
+}
+
+hipError_t cnn_image_to_dofs(const float *image, const double *scale, int n0,
+                             int n1, int N, double *u, hipStream_t s)
+{
+  const long long n = (long long)n0 * n1 * N * N;
+  const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4096);
+  hipLaunchKernelGGL(cnn::image_to_dofs_kernel, dim3(blocks), dim3(256), 0, s,
+                     image, scale, n, n0, N, u);
+  return hipGetLastError();
+}
+
+hipError_t cnn_target_image(const double *phi, const double *scale, int n0,
+                            int n1, int N, float *image, hipStream_t s)
+{
+  const long long n = (long long)n0 * n1 * N * N;
+  hipLaunchKernelGGL(cnn::target_image_kernel, dim3(1), dim3(1024), 0, s, phi,
+                     scale, n, n0, N, image);
+  return hipGetLastError();
+}
+
+hipError_t cnn_sample(const float *src, const float *src2, uint64_t seed,
+                      uint64_t counter0, int count, int n, long long plane,
+                      float *dst, float *dst2, hipStream_t s)
+{
+  const long long total = (long long)n * plane;
+  const unsigned blocks =
+    (unsigned)std::min<long long>((total + 255) / 256, 4096);
+  hipLaunchKernelGGL(cnn::gather_kernel, dim3(blocks), dim3(256), 0, s, src,
+                     src2, seed, counter0, count, n, plane, dst, dst2);
+  return hipGetLastError();
+}
+
+uint16_t cnn_to_bf16(float x) { return cnn::to_bf16(x); }
+} // namespace hd

@@ -192,6 +192,12 @@ hipError_t quadrature_apply(int D, int N, int mode, const double *src,
                             const MeshDev &mesh, const StageDev &stage,
                             double time, int dim_x, int collocation,
                             const double *efield, hipStream_t stream);
+// the FP32-mixed form: FP32 vectors and contractions, FP64 LSRK update
+hipError_t quadrature_apply_f32(int D, int N, int mode, const float *src,
+                                float *dst, const QuadDev &q,
+                                const MeshDev &mesh, const StageDev &stage,
+                                double time, int dim_x, int collocation,
+                                const double *efield, hipStream_t stream);
 hipError_t launch_interpolate(double *dst, const MeshDev &mesh,
                               const FuncDev &f, double t, long long n_dofs,
                               int dofs_cell, hipStream_t stream);
@@ -204,9 +210,14 @@ hipError_t vp_density(const double *f, const float *f32, const VpDev &vp,
                       double *partial, int n_chunks, double *rho,
                       hipStream_t stream);
 bool vp_poisson_supported(int dim_x, int N, long long n_x_cells);
+// stats: [0] iterations (summed), [1] solves, [2] final relative residual,
+// [3] relative residual of the start, [4] of the previous potential,
+// [5] solves started from a guess, [6], [7] the weights of the previous
+// potential and of the guess in the start (doubles in slots 2-4, 6, 7)
 hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
-                      double *efield, double *work, double rel_tol,
-                      int max_it, long long *stats, hipStream_t stream);
+                      const double *guess, double *efield, double *work,
+                      double rel_tol, int max_it, long long *stats,
+                      hipStream_t stream);
 hipError_t vp_moments(const double *f, const MeshDev &mesh, const NormDev &nt,
                       int dim_x, int vol, double *partial, hipStream_t stream);
 hipError_t launch_norm(const double *src, const MeshDev &mesh,

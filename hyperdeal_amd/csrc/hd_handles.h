@@ -3,6 +3,7 @@
 #pragma once
 
 #include "../../include/hyperdeal_amd.h"
+#include "hd_cnn.h"
 #include "hd_device.h"
 #include "hd_tables.h"
 
@@ -47,6 +48,30 @@ struct hd_matrix_free
   double *partial = nullptr; // norm partial sums (2 per cell)
 };
 
+// a standalone network (hd_cnn_*), also the one inside a learned guess
+struct hd_cnn
+{
+  hd::CnnNet net;
+};
+
+// the learned initial guess of a Vlasov-Poisson operator's potential solve:
+// the network, a ring of (density, potential) training images taken from
+// the operator's own converged solves, and the training schedule
+struct hd_learned_guess
+{
+  hd_cnn cnn;
+  int n0 = 0, n1 = 0, N = 0; // x cells per direction, nodes per direction
+  long long plane = 0;       // pixels per image
+  int capacity = 0, count = 0, head = 0;
+  float *ring_in = nullptr, *ring_tgt = nullptr;
+  float *img_out = nullptr, *batch_in = nullptr, *batch_tgt = nullptr;
+  double *scale = nullptr, *guess = nullptr;
+  uint64_t seed = 0, draws = 0;
+  long long solves = 0, guessed = 0;
+  int train_every = 1, train_batch = 8, train_steps = 1, min_pairs = 4;
+  float lr = 1e-3f;
+};
+
 struct hd_advection
 {
   hd_matrix_free *mf = nullptr;
@@ -72,5 +97,18 @@ struct hd_advection
   int density_chunks = 1;
   double *efield = nullptr, *rho = nullptr, *phi = nullptr, *work = nullptr,
          *partial = nullptr;
-  long long *stats = nullptr; // [iterations, solves, last relative residual]
+  long long *stats = nullptr; // solver statistics, see hd::vp_poisson
+  hd_learned_guess *learned = nullptr;
 };
+
+namespace hd_abi
+{
+// the learned guess around one potential solve: before it (density image,
+// network inference, the guess in x-space DoFs; returns the guess or null),
+// and after it (training pair, scheduled training steps)
+hipError_t learned_before_solve(hd_learned_guess &lg, const double *rho,
+                                const double **guess, hipStream_t s);
+hipError_t learned_after_solve(hd_learned_guess &lg, const double *phi,
+                               hipStream_t s);
+void learned_release(hd_learned_guess *lg);
+} // namespace hd_abi

@@ -1,0 +1,350 @@
+// hd_learned.hip -- the C ABI of the convolutional network (hd_cnn_*) and the
+// learned initial guess of the Vlasov-Poisson potential solve
+// (hd_vp_learned_guess_*).  Neither has a counterpart in the reference,
+// whose solve starts from the previous potential
+// (vlasov_poisson/include/application.h:575-576); that start stays one of
+// the two directions the solver combines, and CG to the relative tolerance
+// decides the result, so a poor guess costs one extra matrix-vector product
+// and never the accuracy.
+//
+// Per solve, with the guess on:
+//   image  <- (rho - mean) / rms(rho - mean)    (a ring slot: the training
+//                                               input of this solve)
+//   guess  <- rms * net(image)                  once the net has trained
+//   CG from the best combination of phi_prev and guess
+//   target <- (phi - mean phi) / rms            (the ring slot's target)
+//   every train_every solves: train_steps Adam steps on batches drawn from
+//   the ring on the device
+// Nothing here synchronises with the host.
+#include "hd_handles.h"
+
+#include <algorithm>
+#include <cstdint>
+#include <cstring>
+#include <new>
+
+using hd_abi::fail;
+using hd_abi::hip_fail;
+using hd_abi::ok;
+using hd_abi::as_stream;
+
+namespace hd_abi
+{
+hipError_t learned_before_solve(hd_learned_guess &lg, const double *rho,
+                                const double **guess, hipStream_t s)
+{
+  *guess = nullptr;
+  float *slot = lg.ring_in + (long long)lg.head * lg.plane;
+  hipError_t e = hd::cnn_dofs_to_image(rho, lg.n0, lg.n1, lg.N, slot,
+                                       lg.scale, s);
+  if (e != hipSuccess || lg.cnn.net.steps == 0)
+    return e;
+  e = lg.cnn.net.forward(slot, lg.img_out, 1, s);
+  if (e == hipSuccess)
+    e = hd::cnn_image_to_dofs(lg.img_out, lg.scale, lg.n0, lg.n1, lg.N,
+                              lg.guess, s);
+  if (e == hipSuccess)
+    {
+      *guess = lg.guess;
+      ++lg.guessed;
+    }
+  return e;
+}
+
+hipError_t learned_after_solve(hd_learned_guess &lg, const double *phi,
+                               hipStream_t s)
+{
+  hipError_t e =
+    hd::cnn_target_image(phi, lg.scale, lg.n0, lg.n1, lg.N,
+                         lg.ring_tgt + (long long)lg.head * lg.plane, s);
+  if (e != hipSuccess)
+    return e;
+  lg.head = (lg.head + 1) % lg.capacity;
+  lg.count = std::min(lg.count + 1, lg.capacity);
+  ++lg.solves;
+  if (lg.train_every <= 0 || lg.count < lg.min_pairs ||
+      lg.solves % lg.train_every != 0)
+    return hipSuccess;
+  for (int t = 0; t < lg.train_steps && e == hipSuccess; ++t)
+    {
+      e = hd::cnn_sample(lg.ring_in, lg.ring_tgt, lg.seed, lg.draws, lg.count,
+                         lg.train_batch, lg.plane, lg.batch_in, lg.batch_tgt,
+                         s);
+      lg.draws += (uint64_t)lg.train_batch;
+      if (e == hipSuccess)
+        e = lg.cnn.net.loss_and_grad(lg.batch_in, lg.batch_tgt,
+                                     lg.train_batch, s);
+      if (e == hipSuccess)
+        e = lg.cnn.net.adam(lg.lr, s);
+    }
+  return e;
+}
+
+void learned_release(hd_learned_guess *lg)
+{
+  if (!lg)
+    return;
+  lg->cnn.net.release();
+  for (void *p : {(void *)lg->ring_in, (void *)lg->ring_tgt,
+                  (void *)lg->img_out, (void *)lg->batch_in,
+                  (void *)lg->batch_tgt, (void *)lg->scale, (void *)lg->guess})
+    (void)hipFree(p);
+  delete lg;
+}
+} // namespace hd_abi
+
+namespace
+{
+bool aligned16(const void *p)
+{
+  return reinterpret_cast<uintptr_t>(p) % 16 == 0;
+}
+
+int check_images(const hd_cnn *net, const void *a, const void *b, int batch,
+                 const char *where)
+{
+  if (!net || !a || !b)
+    return fail(HD_ERR_INVALID, std::string(where) + ": null argument");
+  if (batch < 1)
+    return fail(HD_ERR_INVALID, std::string(where) + ": batch must be >= 1");
+  if (!aligned16(a) || !aligned16(b))
+    return fail(HD_ERR_INVALID,
+                std::string(where) + ": images must be 16-byte aligned");
+  return HD_OK;
+}
+} // namespace
+
+extern "C" {
+
+int hd_cnn_n_params(void) { return hd::CnnNet::n_params(); }
+
+int hd_cnn_create(int height, int width, unsigned long long seed,
+                  hd_cnn **out)
+{
+  if (!out)
+    return fail(HD_ERR_INVALID, "hd_cnn_create: null argument");
+  *out = nullptr;
+  if (!hd::CnnNet::shape_ok(height, width))
+    return fail(HD_ERR_INVALID,
+                "hd_cnn_create: height and width must be positive multiples "
+                "of 16");
+  auto *net = new (std::nothrow) hd_cnn();
+  if (!net)
+    return fail(HD_ERR_NOMEM, "out of host memory");
+  const hipError_t e = net->net.init(height, width, seed);
+  if (e != hipSuccess)
+    {
+      net->net.release();
+      delete net;
+      return hip_fail(e, "hd_cnn_create");
+    }
+  *out = net;
+  return ok();
+}
+
+int hd_cnn_destroy(hd_cnn *net)
+{
+  if (!net)
+    return ok();
+  net->net.release();
+  delete net;
+  return ok();
+}
+
+int hd_cnn_params(hd_cnn *net, float *host, int device_to_host)
+{
+  if (!net || !host)
+    return fail(HD_ERR_INVALID, "hd_cnn_params: null argument");
+  const size_t bytes = hd::CnnNet::n_params() * sizeof(float);
+  hd::CnnNet &n = net->net;
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    {
+      if (device_to_host)
+        e = hipMemcpy(host, n.p, bytes, hipMemcpyDeviceToHost);
+      else
+        {
+          // new parameters start a new optimisation
+          e = hipMemcpy(n.p, host, bytes, hipMemcpyHostToDevice);
+          if (e == hipSuccess)
+            e = hipMemset(n.m, 0, bytes);
+          if (e == hipSuccess)
+            e = hipMemset(n.v, 0, bytes);
+          if (e == hipSuccess)
+            e = n.pack(nullptr);
+          if (e == hipSuccess)
+            e = hipDeviceSynchronize();
+          n.steps = 0;
+        }
+    }
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_cnn_params");
+}
+
+int hd_cnn_forward(hd_cnn *net, const float *in, float *out, int batch,
+                   void *stream)
+{
+  const int rc = check_images(net, in, out, batch, "hd_cnn_forward");
+  if (rc != HD_OK)
+    return rc;
+  const hipError_t e = net->net.forward(in, out, batch, as_stream(stream));
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_cnn_forward");
+}
+
+int hd_cnn_loss_grad(hd_cnn *net, const float *in, const float *target,
+                     int batch, double *loss, float *grad, void *stream)
+{
+  const int rc = check_images(net, in, target, batch, "hd_cnn_loss_grad");
+  if (rc != HD_OK)
+    return rc;
+  hipStream_t s = as_stream(stream);
+  hipError_t e = net->net.loss_and_grad(in, target, batch, s);
+  if (e == hipSuccess)
+    e = hipStreamSynchronize(s);
+  if (e == hipSuccess && loss)
+    e = hipMemcpy(loss, net->net.loss_dev, sizeof(double),
+                  hipMemcpyDeviceToHost);
+  if (e == hipSuccess && grad)
+    e = hipMemcpy(grad, net->net.grad,
+                  hd::CnnNet::n_params() * sizeof(float),
+                  hipMemcpyDeviceToHost);
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_cnn_loss_grad");
+}
+
+int hd_cnn_train_step(hd_cnn *net, const float *in, const float *target,
+                      int batch, float learning_rate, void *stream)
+{
+  const int rc = check_images(net, in, target, batch, "hd_cnn_train_step");
+  if (rc != HD_OK)
+    return rc;
+  if (!(learning_rate > 0.0f))
+    return fail(HD_ERR_INVALID, "hd_cnn_train_step: learning rate <= 0");
+  hipStream_t s = as_stream(stream);
+  hipError_t e = net->net.loss_and_grad(in, target, batch, s);
+  if (e == hipSuccess)
+    e = net->net.adam(learning_rate, s);
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_cnn_train_step");
+}
+
+int hd_cnn_last_loss(hd_cnn *net, double *loss)
+{
+  if (!net || !loss)
+    return fail(HD_ERR_INVALID, "hd_cnn_last_loss: null argument");
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    e = hipMemcpy(loss, net->net.loss_dev, sizeof(double),
+                  hipMemcpyDeviceToHost);
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_cnn_last_loss");
+}
+
+int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
+                               int capacity, int train_every, int train_batch,
+                               int train_steps, float learning_rate,
+                               int min_pairs)
+{
+  if (!op || !op->vp)
+    return fail(HD_ERR_INVALID, "hd_vp_learned_guess_enable: not a "
+                                "Vlasov-Poisson operator");
+  const hd_matrix_free *mf = op->mf;
+  if (mf->dim_x != 2)
+    return fail(HD_ERR_UNSUPPORTED,
+                "the learned guess works on 2D x-spaces (an image of nodes)");
+  const int n0 = mf->mesh.n[0], n1 = mf->mesh.n[1], N = mf->nd;
+  if (!hd::CnnNet::shape_ok(n1 * N, n0 * N))
+    return fail(HD_ERR_UNSUPPORTED,
+                "the learned guess needs (cells x nodes) per x direction to "
+                "be a multiple of 16");
+  if (capacity < 1 || train_batch < 1 || train_steps < 0 || train_every < 0 ||
+      min_pairs < 1 || !(learning_rate > 0.0f))
+    return fail(HD_ERR_INVALID, "hd_vp_learned_guess_enable: invalid "
+                                "training parameters");
+  hd_abi::learned_release(op->learned);
+  op->learned = nullptr;
+  auto *lg = new (std::nothrow) hd_learned_guess();
+  if (!lg)
+    return fail(HD_ERR_NOMEM, "out of host memory");
+  lg->n0 = n0;
+  lg->n1 = n1;
+  lg->N = N;
+  lg->plane = (long long)n0 * n1 * N * N;
+  lg->capacity = capacity;
+  lg->seed = seed;
+  lg->train_every = train_every;
+  lg->train_batch = train_batch;
+  lg->train_steps = train_steps;
+  lg->lr = learning_rate;
+  lg->min_pairs = min_pairs;
+  const size_t img = (size_t)lg->plane * sizeof(float);
+  hipError_t e = lg->cnn.net.init(n1 * N, n0 * N, seed);
+  if (e == hipSuccess)
+    e = hipMalloc(&lg->ring_in, img * capacity);
+  if (e == hipSuccess)
+    e = hipMalloc(&lg->ring_tgt, img * capacity);
+  if (e == hipSuccess)
+    e = hipMalloc(&lg->img_out, img);
+  if (e == hipSuccess)
+    e = hipMalloc(&lg->batch_in, img * train_batch);
+  if (e == hipSuccess)
+    e = hipMalloc(&lg->batch_tgt, img * train_batch);
+  if (e == hipSuccess)
+    e = hipMalloc(&lg->scale, sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&lg->guess, (size_t)op->n_x_dofs * sizeof(double));
+  // the activation buffers for training and inference, up front
+  if (e == hipSuccess)
+    e = lg->cnn.net.reserve(train_batch);
+  if (e != hipSuccess)
+    {
+      hd_abi::learned_release(lg);
+      return hip_fail(e, "hd_vp_learned_guess_enable");
+    }
+  op->learned = lg;
+  return ok();
+}
+
+int hd_vp_learned_guess_disable(hd_advection *op)
+{
+  if (!op || !op->vp)
+    return fail(HD_ERR_INVALID, "hd_vp_learned_guess_disable: not a "
+                                "Vlasov-Poisson operator");
+  const hipError_t e = hipDeviceSynchronize();
+  hd_abi::learned_release(op->learned);
+  op->learned = nullptr;
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_vp_learned_guess_disable");
+}
+
+hd_cnn *hd_vp_learned_guess_net(hd_advection *op)
+{
+  return op && op->learned ? &op->learned->cnn : nullptr;
+}
+
+int hd_vp_learned_guess_stats(hd_advection *op, long long *train_steps,
+                              long long *guessed_solves, long long *pairs,
+                              double *last_loss)
+{
+  if (!op || !op->learned)
+    return fail(HD_ERR_INVALID, "hd_vp_learned_guess_stats: no learned "
+                                "guess on this operator");
+  const hd_learned_guess &lg = *op->learned;
+  if (train_steps)
+    *train_steps = lg.cnn.net.steps;
+  if (guessed_solves)
+    *guessed_solves = lg.guessed;
+  if (pairs)
+    *pairs = lg.solves;
+  if (last_loss)
+    {
+      *last_loss = 0.0;
+      if (lg.cnn.net.steps > 0)
+        {
+          hipError_t e = hipDeviceSynchronize();
+          if (e == hipSuccess)
+            e = hipMemcpy(last_loss, lg.cnn.net.loss_dev, sizeof(double),
+                          hipMemcpyDeviceToHost);
+          if (e != hipSuccess)
+            return hip_fail(e, "hd_vp_learned_guess_stats");
+        }
+    }
+  return ok();
+}
+
+} // extern "C"

@@ -52,25 +52,24 @@ __device__ inline int line_base(int f, int d)
 
 // in-place 1D contraction of all lines along d of the CPB cells in buf:
 // line <- M line, M(r, c) = M[r * sr + c * sc]
-template <int D, int N>
-__device__ inline void contract_lines(double *buf, int cellp, int d,
-                                      const double *M, int sr, int sc,
-                                      int ncell)
+template <int D, int N, class T>
+__device__ inline void contract_lines(T *buf, int cellp, int d, const T *M,
+                                      int sr, int sc, int ncell)
 {
   constexpr int FACE = ipow(N, D - 1);
   const int sd = ipow(N, d);
   for (int l = threadIdx.x; l < ncell * FACE; l += kBlock)
     {
       const int lc = l / FACE;
-      double *p = buf + lc * cellp + line_base<D, N>(l - lc * FACE, d);
-      double in[N];
+      T *p = buf + lc * cellp + line_base<D, N>(l - lc * FACE, d);
+      T in[N];
 #pragma unroll
       for (int j = 0; j < N; ++j)
         in[j] = p[j * sd];
 #pragma unroll
       for (int i = 0; i < N; ++i)
         {
-          double acc = 0.0;
+          T acc = 0;
 #pragma unroll
           for (int j = 0; j < N; ++j)
             acc += M[i * sr + j * sc] * in[j];
@@ -78,6 +77,14 @@ __device__ inline void contract_lines(double *buf, int cellp, int d,
         }
     }
 }
+
+// the 1D tables in the arithmetic type: FP64 straight from the kernel
+// arguments, FP32 converted once per block into LDS
+template <class T>
+struct QTab
+{
+  const T *S, *Dq, *F, *Sinv;
+};
 
 // transport velocity of direction d at the in-cell point pt of `cell`
 // (constant, or the Vlasov-Poisson field; a face point's digit d is unused)
@@ -120,20 +127,38 @@ __device__ inline double jxw_point(const QuadDev &q, const MeshDev &m, int i)
 }
 
 // MODE 0: dst = apply(src); MODE 1: LSRK stage epilogue
-template <int D, int N, int MODE>
+// T: vector, LDS and contraction type (double, or float for the FP32-mixed
+// form: the LSRK update of the FP64 solution stays in FP64)
+template <int D, int N, int MODE, class T>
 __global__ __launch_bounds__(kBlock) void apply_kernel(
-  const double *__restrict__ src, double *__restrict__ dst, QuadDev q,
-  MeshDev mesh, StageDev stage, double time, int dim_x, int collocation,
+  const T *__restrict__ src, T *__restrict__ dst, QuadDev q, MeshDev mesh,
+  StageDev stage, double time, int dim_x, int collocation,
   const double *__restrict__ efield)
 {
   using Sh = Shape<D, N>;
   constexpr int CELL = Sh::CELL, FACE = Sh::FACE, CPB = Sh::CPB;
   constexpr int CELLP = CELL + 1;
-  extern __shared__ double smem[];
-  double *U = smem;                 // CPB x CELLP
-  double *R = U + CPB * CELLP;      // CPB x CELLP
-  double *Fm = R + CPB * CELLP;     // CPB x FACE  (u- then flux)
-  double *Fp = Fm + CPB * FACE;     // CPB x FACE  (u+)
+  extern __shared__ __align__(16) unsigned char smem_raw[];
+  T *U = reinterpret_cast<T *>(smem_raw); // CPB x CELLP
+  T *R = U + CPB * CELLP;                 // CPB x CELLP
+  T *Fm = R + CPB * CELLP;                // CPB x FACE  (u- then flux)
+  T *Fp = Fm + CPB * FACE;                // CPB x FACE  (u+)
+  QTab<T> tb;
+  if constexpr (sizeof(T) == sizeof(double))
+    tb = {q.S, q.Dq, q.F, q.Sinv};
+  else
+    {
+      __shared__ T tabs[64 + 64 + 16 + 64];
+      for (int i = threadIdx.x; i < 64; i += kBlock)
+        {
+          tabs[i] = (T)q.S[i];
+          tabs[64 + i] = (T)q.Dq[i];
+          tabs[144 + i] = (T)q.Sinv[i];
+          if (i < 16)
+            tabs[128 + i] = (T)q.F[i];
+        }
+      tb = {tabs, tabs + 64, tabs + 128, tabs + 144};
+    }
 
   const long long cell0 = (long long)blockIdx.x * CPB;
   const int ncell = (int)min((long long)CPB, mesh.n_cells - cell0);
@@ -150,7 +175,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   if (!collocation)
     for (int d = 0; d < D; ++d)
       {
-        contract_lines<D, N>(U, CELLP, d, q.S, N, 1, ncell);
+        contract_lines<D, N, T>(U, CELLP, d, tb.S, N, 1, ncell);
         __syncthreads();
       }
 
@@ -158,7 +183,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   for (int i = tid; i < ncell * CELL; i += kBlock)
     {
       const int lc = i / CELL;
-      R[lc * CELLP + (i - lc * CELL)] = 0.0;
+      R[lc * CELLP + (i - lc * CELL)] = 0;
     }
   __syncthreads();
   for (int d = 0; d < D; ++d)
@@ -168,39 +193,39 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
         {
           const int lc = l / FACE;
           const int base = line_base<D, N>(l - lc * FACE, d);
-          const double *u = U + lc * CELLP + base;
-          double *r = R + lc * CELLP + base;
-          double uq[N], jw[N], acc[N];
+          const T *u = U + lc * CELLP + base;
+          T *r = R + lc * CELLP + base;
+          T uq[N], jw[N], acc[N];
+          const T skew = (T)q.skew;
 #pragma unroll
           for (int j = 0; j < N; ++j)
             {
               uq[j] = u[j * sd];
               // velocity / h folded into the quadrature weight
-              jw[j] = jxw_point<D, N>(q, mesh, base + j * sd) *
-                      point_velocity<D, N>(q, mesh, efield, dim_x, d,
-                                           cell0 + lc, base + j * sd) /
-                      mesh.h[d];
-              acc[j] = 0.0;
+              jw[j] = (T)(jxw_point<D, N>(q, mesh, base + j * sd) *
+                          point_velocity<D, N>(q, mesh, efield, dim_x, d,
+                                               cell0 + lc, base + j * sd) /
+                          mesh.h[d]);
+              acc[j] = 0;
             }
-          const double c = 1.0;
           if (q.skew != 0.0)
 #pragma unroll
             for (int i = 0; i < N; ++i)
               {
-                double g = 0.0;
+                T g = 0;
 #pragma unroll
                 for (int j = 0; j < N; ++j)
-                  g += q.Dq[i * N + j] * uq[j];
-                acc[i] += -q.skew * (g * c) * jw[i];
+                  g += tb.Dq[i * N + j] * uq[j];
+                acc[i] += -skew * g * jw[i];
               }
           if (q.skew != 1.0)
 #pragma unroll
             for (int i = 0; i < N; ++i)
               {
-                double s = 0.0;
+                T s = 0;
 #pragma unroll
                 for (int j = 0; j < N; ++j)
-                  s += q.Dq[j * N + i] * ((1.0 - q.skew) * uq[j] * c * jw[j]);
+                  s += tb.Dq[j * N + i] * ((1 - skew) * uq[j] * jw[j]);
                 acc[i] += s;
               }
 #pragma unroll
@@ -221,18 +246,18 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
         {
           const int lc = l / FACE, fi = l - lc * FACE;
           const int base = line_base<D, N>(fi, d);
-          const double *u = U + lc * CELLP + base;
-          double um = 0.0;
+          const T *u = U + lc * CELLP + base;
+          T um = 0;
 #pragma unroll
           for (int p = 0; p < N; ++p)
-            um += q.F[side * N + p] * u[p * sd];
+            um += tb.F[side * N + p] * u[p * sd];
           Fm[lc * FACE + fi] = um;
           // neighbour
           const long long cell = cell0 + lc;
           const long long s = mesh.stride[d];
           const int c = (int)((cell / s) % mesh.n[d]);
           const int cn = c + (side ? 1 : -1);
-          double up = 0.0;
+          T up = 0;
           if (cn >= 0 && cn < mesh.n[d])
             up = src[(cell + (side ? s : -s)) * CELL + base +
                      (side ? 0 : (N - 1)) * sd];
@@ -270,18 +295,18 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
                       }
                     pw *= N;
                   }
-                double *p = Fp + lc * FACE + base;
-                double in[N];
+                T *p = Fp + lc * FACE + base;
+                T in[N];
 #pragma unroll
                 for (int t = 0; t < N; ++t)
                   in[t] = p[t * sj];
 #pragma unroll
                 for (int i = 0; i < N; ++i)
                   {
-                    double acc = 0.0;
+                    T acc = 0;
 #pragma unroll
                     for (int t = 0; t < N; ++t)
-                      acc += q.S[i * N + t] * in[t];
+                      acc += tb.S[i * N + t] * in[t];
                     p[i * sj] = acc;
                   }
               }
@@ -338,11 +363,11 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
             point_velocity<D, N>(q, mesh, efield, dim_x, d, cell, base) * nrm;
           const double flux =
             0.5 * ((um + up) * an + fabs(an) * (um - up));
-          const double fd = -(flux - q.skew * um * an) * jf;
-          double *r = R + lc * CELLP + base;
+          const T fd = (T)(-(flux - q.skew * um * an) * jf);
+          T *r = R + lc * CELLP + base;
 #pragma unroll
           for (int p = 0; p < N; ++p)
-            r[p * sd] += q.F[side * N + p] * fd;
+            r[p * sd] += tb.F[side * N + p] * fd;
         }
       __syncthreads();
     }
@@ -351,13 +376,13 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   for (int i = tid; i < ncell * CELL; i += kBlock)
     {
       const int lc = i / CELL, li = i - lc * CELL;
-      R[lc * CELLP + li] /= jxw_point<D, N>(q, mesh, li);
+      R[lc * CELLP + li] /= (T)jxw_point<D, N>(q, mesh, li);
     }
   __syncthreads();
   if (!collocation)
     for (int d = D - 1; d >= 0; --d)
       {
-        contract_lines<D, N>(R, CELLP, d, q.Sinv, N, 1, ncell);
+        contract_lines<D, N, T>(R, CELLP, d, tb.Sinv, N, 1, ncell);
         __syncthreads();
       }
 
@@ -365,43 +390,43 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   for (int i = tid; i < ncell * CELL; i += kBlock)
     {
       const int lc = i / CELL;
-      const double k = R[lc * CELLP + (i - lc * CELL)];
+      const T k = R[lc * CELLP + (i - lc * CELL)];
       const long long g = cell0 * CELL + i;
       if (MODE == 0)
         dst[g] = k;
       else
         {
           const double s = stage.sol[g];
-          stage.sol[g] = s + stage.b_dt * k;
+          stage.sol[g] = s + stage.b_dt * (double)k;
           if (stage.ti_out)
-            static_cast<double *>(stage.ti_out)[g] = s + stage.a_dt * k;
+            static_cast<T *>(stage.ti_out)[g] = (T)(s + stage.a_dt * (double)k);
         }
     }
 }
 
-template <int D, int N>
-hipError_t launch(int mode, const double *src, double *dst,
-                  const QuadDev &q, const MeshDev &mesh,
-                  const StageDev &stage, double time, int dim_x,
-                  int collocation, const double *efield, hipStream_t stream)
+template <int D, int N, class T>
+hipError_t launch(int mode, const T *src, T *dst, const QuadDev &q,
+                  const MeshDev &mesh, const StageDev &stage, double time,
+                  int dim_x, int collocation, const double *efield,
+                  hipStream_t stream)
 {
   using Sh = Shape<D, N>;
-  const size_t lds = sizeof(double) *
+  const size_t lds = sizeof(T) *
                      (2 * Sh::CPB * (Sh::CELL + 1) + 2 * Sh::CPB * Sh::FACE);
   const long long grid = (mesh.n_cells + Sh::CPB - 1) / Sh::CPB;
   // gfx950 has 160 KB of LDS per CU; opt in above the 64 KB default
   hipError_t e = hipFuncSetAttribute(
-    reinterpret_cast<const void *>(mode == 0 ? &apply_kernel<D, N, 0>
-                                             : &apply_kernel<D, N, 1>),
+    reinterpret_cast<const void *>(mode == 0 ? &apply_kernel<D, N, 0, T>
+                                             : &apply_kernel<D, N, 1, T>),
     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess)
     return e;
   if (mode == 0)
-    hipLaunchKernelGGL((apply_kernel<D, N, 0>), dim3((unsigned)grid),
+    hipLaunchKernelGGL((apply_kernel<D, N, 0, T>), dim3((unsigned)grid),
                        dim3(kBlock), lds, stream, src, dst, q, mesh, stage,
                        time, dim_x, collocation, efield);
   else
-    hipLaunchKernelGGL((apply_kernel<D, N, 1>), dim3((unsigned)grid),
+    hipLaunchKernelGGL((apply_kernel<D, N, 1, T>), dim3((unsigned)grid),
                        dim3(kBlock), lds, stream, src, dst, q, mesh, stage,
                        time, dim_x, collocation, efield);
   return hipGetLastError();
@@ -417,16 +442,18 @@ bool quadrature_supported(int D, int N)
   return 2 * (cell + 1) * 8 + 2 * (cell / N) * 8 <= 160 * 1024;
 }
 
-hipError_t quadrature_apply(int D, int N, int mode, const double *src,
-                            double *dst, const QuadDev &q,
-                            const MeshDev &mesh, const StageDev &stage,
-                            double time, int dim_x, int collocation,
-                            const double *efield, hipStream_t stream)
+namespace
+{
+template <class T>
+hipError_t dispatch(int D, int N, int mode, const T *src, T *dst,
+                    const QuadDev &q, const MeshDev &mesh,
+                    const StageDev &stage, double time, int dim_x,
+                    int collocation, const double *efield, hipStream_t stream)
 {
 #define HD_CASE(DD, NN_)                                                     \
   if (D == DD && N == NN_)                                                   \
-    return quad::launch<DD, NN_>(mode, src, dst, q, mesh, stage, time, dim_x, \
-                                 collocation, efield, stream);
+    return quad::launch<DD, NN_, T>(mode, src, dst, q, mesh, stage, time,    \
+                                    dim_x, collocation, efield, stream);
   HD_CASE(2, 2) HD_CASE(2, 3) HD_CASE(2, 4) HD_CASE(2, 5) HD_CASE(2, 6)
   HD_CASE(3, 2) HD_CASE(3, 3) HD_CASE(3, 4) HD_CASE(3, 5) HD_CASE(3, 6)
   HD_CASE(4, 2) HD_CASE(4, 3) HD_CASE(4, 4) HD_CASE(4, 5) HD_CASE(4, 6)
@@ -434,5 +461,26 @@ hipError_t quadrature_apply(int D, int N, int mode, const double *src,
   HD_CASE(6, 2) HD_CASE(6, 3) HD_CASE(6, 4)
 #undef HD_CASE
   return hipErrorInvalidValue;
+}
+} // namespace
+
+hipError_t quadrature_apply(int D, int N, int mode, const double *src,
+                            double *dst, const QuadDev &q,
+                            const MeshDev &mesh, const StageDev &stage,
+                            double time, int dim_x, int collocation,
+                            const double *efield, hipStream_t stream)
+{
+  return dispatch<double>(D, N, mode, src, dst, q, mesh, stage, time, dim_x,
+                          collocation, efield, stream);
+}
+
+hipError_t quadrature_apply_f32(int D, int N, int mode, const float *src,
+                                float *dst, const QuadDev &q,
+                                const MeshDev &mesh, const StageDev &stage,
+                                double time, int dim_x, int collocation,
+                                const double *efield, hipStream_t stream)
+{
+  return dispatch<float>(D, N, mode, src, dst, q, mesh, stage, time, dim_x,
+                         collocation, efield, stream);
 }
 } // namespace hd

@@ -260,8 +260,9 @@ __device__ inline void sipg_cell(const XDev &xd, const double *src,
 template <int DX, int N>
 __global__ __launch_bounds__(kSolverThreads) void poisson_kernel(
   XDev xd, const double *__restrict__ rho_in, double *__restrict__ phi,
-  double *__restrict__ efield, double *__restrict__ work, double rel_tol,
-  int max_it, long long *__restrict__ stats)
+  const double *__restrict__ guess, double *__restrict__ efield,
+  double *__restrict__ work, double rel_tol, int max_it,
+  long long *__restrict__ stats)
 {
   constexpr int DOFS = ipow(N, DX);
   extern __shared__ double sm[];
@@ -332,16 +333,65 @@ __global__ __launch_bounds__(kSolverThreads) void poisson_kernel(
   };
   // r = b - A phi
   apply_A(phi, Ap);
+  double alpha0 = 1.0, beta0 = 0.0, rr_warm = -1.0;
+  if (guess)
+    {
+      // start from the minimiser of the energy 1/2 x.A x - b.x over
+      // x = alpha phi + beta g (phi the previous potential, g the guess):
+      // [phi.A phi, phi.A g; phi.A g, g.A g] (alpha, beta) = (phi.b, g.b)
+      apply_A(guess, dir); // dir = A g
+      double d[6] = {0, 0, 0, 0, 0, 0};
+      for (long long c = threadIdx.x; c < nc; c += T)
+        for (int i = 0; i < DOFS; ++i)
+          {
+            const long long g = c * DOFS + i;
+            const double b = r[g], x = phi[g], y = guess[g];
+            d[0] += x * Ap[g];
+            d[1] += x * dir[g];
+            d[2] += y * dir[g];
+            d[3] += x * b;
+            d[4] += y * b;
+            d[5] += (b - Ap[g]) * (b - Ap[g]);
+          }
+      for (int j = 0; j < 6; ++j)
+        d[j] = block_sum(d[j], red);
+      rr_warm = d[5];
+      const double det = d[0] * d[2] - d[1] * d[1];
+      if (d[0] > 0.0 && d[2] > 0.0 && det > 1e-12 * d[0] * d[2])
+        {
+          alpha0 = (d[3] * d[2] - d[4] * d[1]) / det;
+          beta0 = (d[0] * d[4] - d[1] * d[3]) / det;
+        }
+      else if (!(d[0] > 0.0) && d[2] > 0.0)
+        {
+          alpha0 = 0.0;
+          beta0 = d[4] / d[2];
+        }
+      if (!isfinite(alpha0) || !isfinite(beta0))
+        {
+          alpha0 = 1.0;
+          beta0 = 0.0;
+        }
+    }
   double rr = 0.0;
   for (long long c = threadIdx.x; c < nc; c += T)
     for (int i = 0; i < DOFS; ++i)
       {
         const long long g = c * DOFS + i;
+        if (guess)
+          {
+            phi[g] = alpha0 * phi[g] + beta0 * guess[g];
+            Ap[g] = alpha0 * Ap[g] + beta0 * dir[g];
+          }
         r[g] -= Ap[g];
-        dir[g] = r[g];
         rr += r[g] * r[g];
       }
   rr = block_sum(rr, red);
+  for (long long c = threadIdx.x; c < nc; c += T)
+    for (int i = 0; i < DOFS; ++i)
+      dir[c * DOFS + i] = r[c * DOFS + i];
+  __syncthreads();
+  const double rr_start = rr;
   const double stop = rel_tol * sqrt(bb);
   int it = 0;
   while (sqrt(rr) > stop && it < max_it)
@@ -394,9 +444,20 @@ __global__ __launch_bounds__(kSolverThreads) void poisson_kernel(
     }
   if (threadIdx.x == 0)
     {
+      double *ds = reinterpret_cast<double *>(stats);
+      const double nb = sqrt(bb) + 1e-300;
       stats[0] += it; // iterations, accumulated over solves
       stats[1] += 1;  // solves
-      reinterpret_cast<double *>(stats)[2] = sqrt(rr) / (sqrt(bb) + 1e-300);
+      ds[2] = sqrt(rr) / nb;
+      // relative residual of the start, and of the previous potential
+      ds[3] = sqrt(rr_start) / nb;
+      ds[4] = (guess ? sqrt(rr_warm) : sqrt(rr_start)) / nb;
+      if (guess)
+        {
+          stats[5] += 1; // solves that started from a guess
+          ds[6] = alpha0;
+          ds[7] = beta0;
+        }
     }
 }
 
@@ -516,8 +577,9 @@ bool vp_poisson_supported(int dim_x, int N, long long n_x_cells)
 }
 
 hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
-                      double *efield, double *work, double rel_tol,
-                      int max_it, long long *stats, hipStream_t stream)
+                      const double *guess, double *efield, double *work,
+                      double rel_tol, int max_it, long long *stats,
+                      hipStream_t stream)
 {
   const long long dofs = xd.n_cells * ipow(N, xd.dim);
   const size_t lds = (size_t)(dofs + vlasov::kSolverThreads) * sizeof(double);
@@ -531,7 +593,7 @@ hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
         return e;                                                              \
       hipLaunchKernelGGL((vlasov::poisson_kernel<DX, NN_>), dim3(1),           \
                          dim3(vlasov::kSolverThreads), lds, stream, xd, rho,   \
-                         phi, efield, work, rel_tol, max_it, stats);           \
+                         phi, guess, efield, work, rel_tol, max_it, stats);    \
       return hipGetLastError();                                                \
     }
   HD_VP(1, 2) HD_VP(1, 3) HD_VP(1, 4) HD_VP(1, 5) HD_VP(1, 6) HD_VP(1, 7)

@@ -208,11 +208,13 @@ int hd_graph_destroy(hd_graph *graph);
  * (vlasov_poisson/include/velocity_field_view.h:28-209), where phi solves
  * the periodic SIPG Poisson problem -lap phi = -(rho - mean rho)
  * (poisson.h:52-312) for rho = int f dv (numerics/vector_tools.h:230-315).
- * Needs dim_x == dim_v, a periodic box, n_points = degree + 1; FP64.  With
- * auto update on (the default) every apply / stage first recomputes rho,
- * phi (FP64 conjugate gradients from the previous phi to poisson_tol
- * relative residual) and grad phi from its input, as the reference's RK
- * right-hand side does.
+ * Needs dim_x == dim_v, a periodic box, n_points = degree + 1.  The apply
+ * runs in FP64 or in the FP32-mixed form (FP32 vectors and contractions;
+ * the density, the potential solve and the LSRK update of the solution stay
+ * FP64).  With auto update on (the default) every apply / stage first
+ * recomputes rho, phi (FP64 conjugate gradients from the previous phi to
+ * poisson_tol relative residual) and grad phi from its input, as the
+ * reference's RK right-hand side does.
  */
 int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
                              double poisson_tol, hd_advection **out);
@@ -223,6 +225,12 @@ int hd_vp_set_poisson_tolerance(hd_advection *op, double rel_tol, int max_it);
 /* accumulated CG iterations and solves, last relative residual (sync) */
 int hd_vp_solver_stats(hd_advection *op, long long *iterations,
                        long long *solves, double *last_rel_residual);
+/* the start of the last solve (sync): its relative residual, that of the
+ * previous phi alone, the weights (previous phi, guess) of the start, and
+ * the number of solves that started from a learned guess */
+int hd_vp_solver_start(hd_advection *op, double *start_rel_residual,
+                       double *previous_rel_residual, double *weights,
+                       long long *guessed_solves);
 /* x-space DoFs of phi / rho: (k+1)^dim_x per x cell */
 long long hd_vp_x_n_dofs(const hd_advection *op);
 /* copy phi device -> host (device_to_host != 0) or host -> device */
@@ -232,6 +240,57 @@ int hd_vp_field_energy(hd_advection *op, double *out);
 /* mass, ||f||_L2, kinetic energy, momentum[3] of f (sync) */
 int hd_vp_moments(hd_matrix_free *mf, const double *f, double *out,
                   void *stream);
+
+/*
+ * Learned initial guess of the potential solve.  No counterpart in the
+ * reference (its CG starts from the previous phi, vlasov_poisson/include/
+ * application.h:575-576); for callers that solve many Poisson problems along
+ * one trajectory.  A convolutional network (four 3x3 periodic convolutions,
+ * 1 -> 32 -> 32 -> 32 -> 1 channels, leaky ReLU 1/8, BF16 weights, FP32
+ * accumulation and Adam state) maps the normalised density image (x cells x
+ * nodes per direction, a multiple of 16 each; dim_x == 2) to a potential
+ * guess; it trains on the operator's own converged (rho, phi) pairs, kept in
+ * a ring of `capacity`, with train_steps Adam steps on batches of
+ * train_batch every train_every solves once min_pairs are stored
+ * (train_every 0: no training).  CG starts from the energy-minimising
+ * combination of the previous phi and the guess and still runs to
+ * poisson_tol, so the guess changes the iteration count, not the result's
+ * tolerance.
+ */
+int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
+                               int capacity, int train_every, int train_batch,
+                               int train_steps, float learning_rate,
+                               int min_pairs);
+int hd_vp_learned_guess_disable(hd_advection *op);
+/* Adam steps taken, solves that used the guess, pairs stored in total,
+ * last training loss (sync) */
+int hd_vp_learned_guess_stats(hd_advection *op, long long *train_steps,
+                              long long *guessed_solves, long long *pairs,
+                              double *last_loss);
+
+/*
+ * The network on its own (no counterpart in the reference): images are
+ * device arrays of batch x height x width FP32, 16-byte aligned; height and
+ * width multiples of 16.  Parameters in the canonical FP32 layout, per layer
+ * w[out][in][3][3] then b[out] (hd_cnn_n_params() floats); setting them
+ * restarts Adam.  hd_cnn_loss_grad returns loss = mean (net(in) - target)^2
+ * and its gradient (host arrays, either may be NULL; sync).
+ */
+typedef struct hd_cnn hd_cnn;
+int hd_cnn_n_params(void);
+int hd_cnn_create(int height, int width, unsigned long long seed,
+                  hd_cnn **out);
+int hd_cnn_destroy(hd_cnn *net);
+int hd_cnn_params(hd_cnn *net, float *host, int device_to_host);
+int hd_cnn_forward(hd_cnn *net, const float *in, float *out, int batch,
+                   void *stream);
+int hd_cnn_loss_grad(hd_cnn *net, const float *in, const float *target,
+                     int batch, double *loss, float *grad, void *stream);
+int hd_cnn_train_step(hd_cnn *net, const float *in, const float *target,
+                      int batch, float learning_rate, void *stream);
+int hd_cnn_last_loss(hd_cnn *net, double *loss);
+/* the network of an operator's learned guess (borrowed; NULL if off) */
+hd_cnn *hd_vp_learned_guess_net(hd_advection *op);
 
 /* dst (FP64) = function at the Gauss-Lobatto nodes (numerics/
  * vector_tools.h:81-135); params: D values or NULL. */

@@ -1,0 +1,203 @@
+"""The Poisson-guess network's HIP kernels against the CPU oracle
+(oracle/cnn.py), and the learned guess inside the Vlasov-Poisson solve.
+
+Bars (FP32 accumulation in a different order than numpy's):
+  - forward output within 2e-5 of max |out|;
+  - loss within 1e-5 relative; every gradient block within 1e-4 of its
+    largest entry;
+  - Adam steps: the GPU update of the GPU gradient equals the oracle's
+    update of the same gradient to 1e-6 * lr (absolute);
+  - training on fixed pairs lowers the loss;
+  - with the learned guess on, the Vlasov-Poisson solution matches the run
+    without it to the Poisson tolerance's effect (FP64: 1e-8 relative max;
+    FP32-mixed: 1e-6 relative L2), and a solve started from the exact
+    previous answer takes no CG iteration.
+"""
+
+import numpy as np
+import pytest
+
+import oracle as O
+from oracle import cnn as C
+
+torch = pytest.importorskip("torch")
+pytestmark = pytest.mark.gpu
+H = None
+
+
+def setup_module(module):
+    global H
+    if not torch.cuda.is_available():
+        pytest.skip("no GPU")
+    import hyperdeal_amd
+    H = hyperdeal_amd
+
+
+def _images(rng, b, h, w):
+    x = rng.standard_normal((b, h, w)).astype(np.float32)
+    return x, torch.from_numpy(x).cuda()
+
+
+def _trained_like_params(seed):
+    """parameters with non-zero biases and a last layer of full size, so
+    every path of the backward pass carries weight"""
+    p = C.init_params(seed, O.splitmix_uniform)
+    rng = np.random.default_rng(seed)
+    o = 0
+    for l, (ci, co) in enumerate(C.layer_shapes()):
+        if l == C.LAYERS - 1:
+            p[o:o + co * ci * 9] *= 10
+        o += co * ci * 9
+        p[o:o + co] = rng.uniform(-0.2, 0.2, co)
+        o += co
+    return p
+
+
+def test_initial_parameters_match_oracle():
+    net = H.Cnn(16, 16, seed=20261205)
+    ref = C.init_params(20261205, O.splitmix_uniform)
+    assert np.array_equal(net.params(), ref)
+
+
+@pytest.mark.parametrize("shape", [(1, 16, 16), (3, 32, 48)])
+def test_forward_matches_oracle(shape):
+    print("case: cnn forward", shape, flush=True)
+    rng = np.random.default_rng(1)
+    net = H.Cnn(shape[1], shape[2], seed=3)
+    p = _trained_like_params(3)
+    net.set_params(p)
+    x, xd = _images(rng, *shape)
+    out = net.forward(xd)
+    torch.cuda.synchronize()
+    ref = C.forward(p, x)
+    got = out.cpu().numpy()
+    assert np.max(np.abs(got - ref)) <= 2e-5 * np.max(np.abs(ref))
+
+
+def test_loss_and_gradient_match_oracle():
+    print("case: cnn loss and gradient", flush=True)
+    rng = np.random.default_rng(2)
+    net = H.Cnn(32, 16, seed=4)
+    p = _trained_like_params(4)
+    net.set_params(p)
+    x, xd = _images(rng, 2, 32, 16)
+    t, td = _images(rng, 2, 32, 16)
+    loss, g = net.loss_grad(xd, td)
+    rloss, rg = C.loss_and_grad(p, x, t)
+    assert abs(loss - rloss) <= 1e-5 * rloss
+    o = 0
+    for ci, co in C.layer_shapes():
+        for n in (co * ci * 9, co):
+            blk, rblk = g[o:o + n], rg[o:o + n]
+            assert np.max(np.abs(blk - rblk)) <= 1e-4 * np.max(np.abs(rblk)), \
+                (o, n)
+            o += n
+
+
+def test_adam_steps_match_oracle():
+    print("case: cnn adam", flush=True)
+    rng = np.random.default_rng(3)
+    net = H.Cnn(16, 32, seed=5)
+    p = _trained_like_params(5)
+    net.set_params(p)
+    x, xd = _images(rng, 2, 16, 32)
+    t, td = _images(rng, 2, 16, 32)
+    opt = C.Adam(C.n_params())
+    lr = 1e-3
+    for step in range(3):
+        _, g = net.loss_grad(xd, td)
+        net.train_step(xd, td, lr)
+        p = opt.step(p, g, lr)
+        got = net.params()
+        assert np.max(np.abs(got - p)) <= 1e-6 * lr + 1e-7 * np.max(np.abs(p)), \
+            step
+
+
+def test_training_lowers_the_loss():
+    print("case: cnn training", flush=True)
+    rng = np.random.default_rng(4)
+    net = H.Cnn(32, 32, seed=6)
+    # targets: a fixed smoothing of the inputs (a local linear map the
+    # network can represent)
+    x = rng.standard_normal((8, 32, 32)).astype(np.float32)
+    t = sum(np.roll(x, s, axis=a) for a in (1, 2) for s in (-1, 1)) / 4
+    xd, td = torch.from_numpy(x).cuda(), torch.from_numpy(t.astype(
+        np.float32)).cuda()
+    l0, _ = net.loss_grad(xd, td)
+    for _ in range(300):
+        net.train_step(xd, td, 2e-3)
+    l1 = net.last_loss()
+    assert l1 < 0.1 * l0, (l0, l1)
+
+
+def test_rejects_bad_images():
+    net = H.Cnn(16, 16)
+    with pytest.raises(H.HyperdealError):
+        H.Cnn(20, 16)
+    x = torch.zeros(2, 16, 16, device="cuda", dtype=torch.float64)
+    with pytest.raises(H.HyperdealError):
+        net.forward(x)
+    buf = torch.zeros(2 * 256 + 1, device="cuda")
+    with pytest.raises(H.HyperdealError):
+        net.forward(buf[1:].view(2, 16, 16))   # 4-byte offset: misaligned
+
+
+# ---- the learned guess inside the Vlasov-Poisson right-hand side ----
+
+def _landau(n_x=8, n_v=4, k=3):
+    L = 4 * np.pi
+    mf = H.MatrixFree(2, 2, k, [n_x, n_x, n_v, n_v], [0, 0, -6, -6],
+                      [L, L, 6, 6])
+    op = H.VlasovPoissonOperation(mf, poisson_tol=1e-10)
+    f = mf.initialize_dof_vector()
+    # (1 + 0.01 cos(x1/2) cos(x2/2)) Maxwellian: parameters eps, k, np
+    H.VectorTools.interpolate(mf, f, H.FUNC_LANDAU, [0.01, 0.5, 2.0])
+    return mf, op, f
+
+
+def _steps(mf, op, f, n, dt, precision):
+    dtype = torch.float64 if precision == H.FP64 else torch.float32
+    a = torch.empty(f.numel(), dtype=dtype, device="cuda")
+    b = torch.empty_like(a)
+    integ = H.LowStorageRungeKuttaIntegrator("rk45")
+    for s in range(n):
+        integ.perform_time_step(op, f, a, b, s * dt, dt, precision=precision)
+    torch.cuda.synchronize()
+
+
+@pytest.mark.parametrize("precision", ["fp64", "fp32"])
+def test_learned_guess_keeps_the_solution(precision):
+    print("case: vp learned guess", precision, flush=True)
+    prec = H.FP64 if precision == "fp64" else H.FP32
+    mf, op, f = _landau()
+    mf2, op2, f2 = _landau()
+    op2.enable_learned_guess(seed=1, train_every=1, train_batch=4,
+                             train_steps=2, min_pairs=2)
+    _steps(mf, op, f, 6, 0.05, prec)
+    _steps(mf2, op2, f2, 6, 0.05, prec)
+    steps, guessed, pairs, loss = op2.learned_guess_stats()
+    assert pairs == 30 and steps > 0 and guessed > 0
+    assert np.isfinite(loss)
+    if prec == H.FP64:
+        diff = (f - f2).abs().max().item() / f.abs().max().item()
+        assert diff <= 1e-8, diff
+    else:
+        # a field change at the CG tolerance flips FP32 roundings of the
+        # apply: the FP32-mixed bar (relative L2 1e-6) applies
+        diff = (torch.linalg.vector_norm(f - f2) /
+                torch.linalg.vector_norm(f)).item()
+        assert diff <= 1e-6, diff
+    # the guess enters with finite weights and a finite start residual
+    r0, rp, (wa, wb), g = op2.solver_start()
+    assert np.isfinite(wa) and np.isfinite(wb) and g == guessed
+    assert np.isfinite(r0) and np.isfinite(rp)
+
+
+def test_exact_start_needs_no_iteration():
+    print("case: vp exact start", flush=True)
+    mf, op, f = _landau()
+    op.update_field(f)
+    it0, n0, _ = op.solver_stats()
+    op.update_field(f)      # same rho: the previous phi is the answer
+    it1, n1, res = op.solver_stats()
+    assert n1 == n0 + 1 and it1 - it0 <= 1 and res <= 1e-10
