@@ -167,6 +167,107 @@ def batch_workload(H, torch, stream, steps):
             "cells_per_task": tuned}
 
 
+def gl_weights(N):
+    """Gauss-Lobatto quadrature weights of N points on [0, 1]."""
+    import numpy as np
+    from numpy.polynomial import legendre as L
+    c = np.zeros(N)
+    c[-1] = 1.0
+    x = np.concatenate([[-1.0], np.sort(L.legroots(L.legder(c))), [1.0]])
+    return 1.0 / ((N - 1) * N * L.legval(x, c) ** 2)
+
+
+def c3_workload(H, torch, stream, steps):
+    """3+3, k=5, 6^3 x 6^3 cells (46,656 DoF per cell, 2,176,782,336 DoF):
+    the split collapsed apply in FP64 and FP32-mixed, then 20 rk45 steps in
+    each precision from the same f0 (product of sines + 1e-3 noise, the C2
+    seeds), dt = 0.1 * dt_crit / k^1.5.  Reported: apply GDoF/s and their
+    ratio, the relative L2 difference FP32-mixed vs FP64 after 20 steps and
+    the mass drift of each run (both with the Gauss-Lobatto rule on the
+    nodes)."""
+    import numpy as np
+    k, N, nc = 5, 6, [6] * 6
+    vel = [splitmix_uniform(SEED_VELOCITY, i) for i in range(6)]
+    mf = H.MatrixFree(3, 3, k, nc, [0.0] * 6, [1.0] * 6)
+    op = H.AdvectionOperation(mf, vel)
+    n = mf.n_dofs()
+    cell = N ** 6
+    u = torch.empty(n, dtype=torch.float64, device="cuda")
+    H.VectorTools.interpolate(mf, u, H.FUNC_PRODUCT_SINES, [0.0] * 6)
+    H.VectorTools.add_noise(mf, u, SEED_NOISE, 1e-3)
+    dst = torch.empty_like(u)
+    t64 = timed(lambda: op.apply(dst, u, 0.0, H.FP64, stream), steps, 2,
+                stream)
+    del dst
+    u32 = u.float()
+    d32 = torch.empty_like(u32)
+    t32 = timed(lambda: op.apply(d32, u32, 0.0, H.FP32, stream), steps, 2,
+                stream)
+    del u32, d32
+    torch.cuda.empty_cache()
+    w1 = gl_weights(N) / 6.0  # h = 1/6
+    W = w1
+    for _ in range(5):
+        W = np.multiply.outer(W, w1)
+    W = torch.from_numpy(W.reshape(-1)).cuda()   # node index x0 fastest
+
+    def mass(f):
+        return float((f.view(-1, cell) @ W).sum())
+
+    dt = 0.1 * H.compute_critical_time_step(mf, vel) / k ** 1.5
+    integ = H.LowStorageRungeKuttaIntegrator("rk45")
+    runs = {}
+    for name, prec, dtype in (("fp64", H.FP64, torch.float64),
+                              ("fp32_mixed", H.FP32, torch.float32)):
+        sol = u.clone()
+        ta = torch.empty(n, dtype=dtype, device="cuda")
+        tb = torch.empty_like(ta)
+        m0 = mass(sol)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for s in range(20):
+            integ.perform_time_step(op, sol, ta, tb, s * dt, dt,
+                                    precision=prec, stream=stream)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        del ta, tb
+        torch.cuda.empty_cache()
+        runs[name] = (sol, m0, mass(sol), wall)
+    s64, s32 = runs["fp64"][0], runs["fp32_mixed"][0]
+    num = den = 0.0
+    chunk = 4096 * cell
+    for o in range(0, n, chunk):
+        a = s64[o:o + chunk].view(-1, cell)
+        b = s32[o:o + chunk].view(-1, cell)
+        num += float((((b - a) ** 2) @ W).sum())
+        den += float(((a ** 2) @ W).sum())
+    l1 = float((u.abs().view(-1, cell) @ W).sum())
+    out = {
+        "workload": "3D+3D apply, periodic [0,1]^6, 6^3 x 6^3 cells, k=5 "
+                    "(46656 DoF/cell, %d DoF), split collapsed form" % n,
+        "fp64": {"value": round(n / t64 / 1e9, 3), "unit": "GDoF/s",
+                 "ms_per_apply": round(t64 * 1e3, 3),
+                 "achieved_GBs": round(16.0 * n / t64 / 1e9, 1)},
+        "fp32_mixed": {"value": round(n / t32 / 1e9, 3), "unit": "GDoF/s",
+                       "ms_per_apply": round(t32 * 1e3, 3),
+                       "achieved_GBs": round(8.0 * n / t32 / 1e9, 1)},
+        "ratio_fp32_vs_fp64": round(t64 / t32, 3),
+        "lsrk_20_steps": {
+            "dt": dt,
+            "rel_l2_fp32_mixed_vs_fp64": (num / den) ** 0.5,
+            "mass_drift_fp64": runs["fp64"][2] - runs["fp64"][1],
+            "mass_drift_fp32_mixed":
+                runs["fp32_mixed"][2] - runs["fp32_mixed"][1],
+            "mass_scale": l1,
+            "note": "mass drifts are absolute; mass_scale = int |f0|",
+            "s_per_step_fp64": round(runs["fp64"][3] / 20, 4),
+            "s_per_step_fp32_mixed": round(runs["fp32_mixed"][3] / 20, 4)},
+    }
+    del runs, s64, s32, u
+    torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -174,7 +275,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["all", "fp64", "fp32", "stage",
-                                       "batch"],
+                                       "batch", "c3"],
                     default="all", help="restrict the timed kernels "
                     "(profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -238,10 +339,12 @@ def main():
             results["stage_fp64" if prec == H.FP64 else "stage_fp32"] = t
             del sol, ta, tb
     torch.cuda.synchronize()
-    batch_result = None
+    del u
+    torch.cuda.empty_cache()
+    batch_result = c3_result = None
+    if args.only in ("all", "c3"):
+        c3_result = c3_workload(H, torch, stream, max(3, args.steps // 10))
     if args.only in ("all", "batch"):
-        del u
-        torch.cuda.empty_cache()
         batch_result = batch_workload(H, torch, stream, max(3, args.steps // 5))
         torch.cuda.empty_cache()
 
@@ -297,6 +400,8 @@ def main():
                          "achieved_GBs": round(bpd * n / t / 1e9, 1)}
     if batch_result is not None:
         line["batch_2d2d"] = batch_result
+    if c3_result is not None:
+        line["c3_3d3d_k5"] = c3_result
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(velocity)
     print(json.dumps(line), flush=True)

@@ -388,8 +388,9 @@ class VlasovPoissonOperation(AdvectionOperation):
     def enable_learned_guess(self, seed=20261205, capacity=64, train_every=1,
                              train_batch=8, train_steps=1,
                              learning_rate=1e-3, min_pairs=4):
-        """Start the potential solves from the CNN guess combined with the
-        previous phi; the network trains on this operator's own solves."""
+        """Start the potential solves from the previous phi combined with
+        the CNN's correction for its residual; the network trains on this
+        operator's own solves (residual -> correction pairs)."""
         _check(self._lib.hd_vp_learned_guess_enable(
             self.handle, int(seed), int(capacity), int(train_every),
             int(train_batch), int(train_steps), float(learning_rate),

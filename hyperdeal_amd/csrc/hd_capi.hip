@@ -55,13 +55,13 @@ int vp_update(hd_advection *op, const double *f, const float *f32,
                                 op->density_chunks, op->rho, stream);
   const double *guess = nullptr;
   if (e == hipSuccess && op->learned)
-    e = hd_abi::learned_before_solve(*op->learned, op->rho, &guess, stream);
+    e = hd_abi::learned_before_solve(*op, &guess, stream);
   if (e == hipSuccess)
     e = hd::vp_poisson(op->xdev, op->mf->nd, op->rho, op->phi, guess,
                        op->efield, op->work, op->poisson_tol,
-                       op->poisson_max_it, op->stats, stream);
+                       op->poisson_max_it, op->stats, stream, op->ptab);
   if (e == hipSuccess && op->learned)
-    e = hd_abi::learned_after_solve(*op->learned, op->phi, stream);
+    e = hd_abi::learned_after_solve(*op, stream);
   return e == hipSuccess ? HD_OK : hip_fail(e, "Vlasov-Poisson field update");
 }
 
@@ -78,6 +78,14 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
           : vp_update(op, nullptr, static_cast<const float *>(src), stream);
       if (rc != HD_OK)
         return rc;
+    }
+  if (op->vp && op->vtab && op->algorithm != HD_ALGO_QUADRATURE)
+    {
+      const hipError_t e =
+        hd::vp_apply(mf->nd, mode, precision == HD_FP64 ? 0 : 1, src, dst,
+                     op->vtab, mf->mesh, op->n_x_cells, op->efield, stage,
+                     stream);
+      return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
     }
   const int algo = choose_algorithm(op, precision);
   if (algo < 0)
@@ -96,7 +104,36 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
         return fail(HD_ERR_INVALID,
                     "the collapsed kernel needs a 16-byte aligned source "
                     "vector");
-      if (precision == HD_FP64)
+      if (op->split)
+        {
+          // the first part lands in dst (apply), in ti_out (stage), or in a
+          // free vector (last stage)
+          void *part = mode == 0 ? dst : stage.ti_out;
+          if (!part)
+            part = stage.scratch;
+          if (!part)
+            {
+              if (!op->split_scratch)
+                {
+                  e = hipMalloc(&op->split_scratch,
+                                (size_t)mf->n_dofs * sizeof(double));
+                  if (e != hipSuccess)
+                    return hip_fail(e, "split-form scratch vector");
+                }
+              part = op->split_scratch;
+            }
+          if (precision == HD_FP64)
+            e = hd::collapsed_split_f64(
+              mf->D, mf->nd, mode, static_cast<const double *>(src),
+              static_cast<double *>(dst), static_cast<double *>(part),
+              op->coef64, mf->mesh, op->inflow, stage, stream);
+          else
+            e = hd::collapsed_split_f32(
+              mf->D, mf->nd, mode, static_cast<const float *>(src),
+              static_cast<float *>(dst), static_cast<float *>(part),
+              op->coef32, mf->mesh, op->inflow, stage, stream);
+        }
+      else if (precision == HD_FP64)
         e = hd::collapsed_apply_f64(mf->D, mf->nd, mode,
                                     static_cast<const double *>(src),
                                     static_cast<double *>(dst), op->coef64,
@@ -295,7 +332,10 @@ int hd_advection_create(hd_matrix_free *mf, const double *velocity,
   bool all_periodic = true;
   for (int d = 0; d < D; ++d)
     all_periodic = all_periodic && mf->periodic[d];
-  op->collapsed_ok = all_periodic && hd::collapsed_supported(D, nd);
+  op->split = !hd::collapsed_supported(D, nd) &&
+              hd::collapsed_split_supported(D, nd);
+  op->collapsed_ok =
+    all_periodic && (hd::collapsed_supported(D, nd) || op->split);
   if (op->collapsed_ok)
     {
       const int per = nd * nd + nd;
@@ -359,7 +399,8 @@ int hd_advection_destroy(hd_advection *op)
     return ok();
   for (void *p : {(void *)op->coef64, (void *)op->coef32, (void *)op->efield,
                   (void *)op->rho, (void *)op->phi, (void *)op->work,
-                  (void *)op->partial, (void *)op->stats})
+                  (void *)op->partial, (void *)op->stats,
+                  (void *)op->ptab, (void *)op->vtab, op->split_scratch})
     if (p)
       (void)hipFree(p);
   hd_abi::learned_release(op->learned);
@@ -472,6 +513,50 @@ int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
     e = hipMemset(op->efield, 0, nef * sizeof(double));
   if (e == hipSuccess)
     e = hipMemset(op->stats, 0, 8 * sizeof(long long));
+  if (e == hipSuccess && !mf->collocation &&
+      hd::vp_apply_supported(mf->dim_x, mf->dim_v, nd))
+    {
+      // unit-speed 1D operators per direction and sign (the operator for
+      // speed c is c A+ for c >= 0 and c A- for c < 0), then S, S^-1 and
+      // the Gauss points
+      std::vector<double> tab(hd::vp_apply_table_size(nd), 0.0);
+      const int per = 2 * (nd * nd + nd);
+      for (int d = 0; d < 4; ++d)
+        {
+          const auto cp = hd::collapse_direction(t, 1.0, mf->mesh.h[d],
+                                                 skew_factor);
+          const auto cn = hd::collapse_direction(t, -1.0, mf->mesh.h[d],
+                                                 skew_factor);
+          double *o = tab.data() + d * per;
+          for (int i = 0; i < nd * nd; ++i)
+            {
+              o[i] = cp.A[i];
+              o[nd * nd + nd + i] = -cn.A[i];
+            }
+          for (int i = 0; i < nd; ++i)
+            {
+              o[nd * nd + i] = cp.b[i];
+              o[2 * nd * nd + nd + i] = -cn.b[i];
+            }
+        }
+      double *o = tab.data() + 4 * per;
+      std::copy(t.S.begin(), t.S.begin() + nd * nd, o);
+      std::copy(t.Sinv.begin(), t.Sinv.begin() + nd * nd, o + nd * nd);
+      std::copy(t.qpoints.begin(), t.qpoints.begin() + nd, o + 2 * nd * nd);
+      e = hipMalloc(&op->vtab, tab.size() * sizeof(double));
+      if (e == hipSuccess)
+        e = hipMemcpy(op->vtab, tab.data(), tab.size() * sizeof(double),
+                      hipMemcpyHostToDevice);
+    }
+  if (e == hipSuccess && hd::vp_poisson2d_supported(mf->dim_x, nd, ncx))
+    {
+      std::vector<double> tab(8 * nd * nd);
+      hd::vp_poisson2d_table(op->xdev, nd, tab.data());
+      e = hipMalloc(&op->ptab, tab.size() * sizeof(double));
+      if (e == hipSuccess)
+        e = hipMemcpy(op->ptab, tab.data(), tab.size() * sizeof(double),
+                      hipMemcpyHostToDevice);
+    }
   if (e != hipSuccess)
     {
       hd_advection_destroy(op);
@@ -632,9 +717,27 @@ int hd_advection_apply(hd_advection *op, const void *src, void *dst,
                       as_stream(stream));
 }
 
+namespace
+{
+// scratch: a vector the last stage may use (the free LSRK register)
+int lsrk_stage(hd_advection *op, int precision, const void *ti_in,
+               void *ti_out, void *scratch, double *sol, double time,
+               double a_dt, double b_dt, void *stream);
+} // namespace
+
 int hd_lsrk_stage(hd_advection *op, int precision, const void *ti_in,
                   void *ti_out, double *sol, double time, double a_dt,
                   double b_dt, void *stream)
+{
+  return lsrk_stage(op, precision, ti_in, ti_out, nullptr, sol, time, a_dt,
+                    b_dt, stream);
+}
+
+namespace
+{
+int lsrk_stage(hd_advection *op, int precision, const void *ti_in,
+               void *ti_out, void *scratch, double *sol, double time,
+               double a_dt, double b_dt, void *stream)
 {
   if (!op || !ti_in || !sol)
     return fail(HD_ERR_INVALID, "hd_lsrk_stage: null argument");
@@ -647,9 +750,11 @@ int hd_lsrk_stage(hd_advection *op, int precision, const void *ti_in,
   stage.b_dt = b_dt;
   stage.sol = sol;
   stage.ti_out = ti_out;
+  stage.scratch = ti_out ? nullptr : scratch;
   return run_operator(op, 1, precision, ti_in, nullptr, stage, time,
                       as_stream(stream));
 }
+} // namespace
 
 int hd_lsrk_coefficients(const char *type, double *a, double *b)
 {
@@ -687,9 +792,9 @@ int hd_lsrk_step(hd_advection *op, const char *type, int precision,
       if (s > 0)
         sum_b += b[s - 1];
       const bool last = s == n - 1;
-      rc = hd_lsrk_stage(op, precision, in, last ? nullptr : out, sol,
-                         time + c * dt, last ? 0.0 : a[s] * dt, b[s] * dt,
-                         stream);
+      rc = lsrk_stage(op, precision, in, last ? nullptr : out, out, sol,
+                      time + c * dt, last ? 0.0 : a[s] * dt, b[s] * dt,
+                      stream);
       if (rc != HD_OK)
         return rc;
       std::swap(in, out);

@@ -63,16 +63,16 @@ struct CnnNet
 
 // x-space nodal values (2D, cell-major, x0 cells fastest, N x N nodes per
 // cell, node index i0 + N i1) <-> an (n1 N) x (n0 N) image, row = x1
-//   image = (u - mean u) / scale, scale = rms(u - mean u) computed on the
-//   device into *scale (0 when u is constant, then the image is 0)
-hipError_t cnn_dofs_to_image(const double *u, int n0, int n1, int N,
-                             float *image, double *scale, hipStream_t s);
-//   u = scale * image, scale read from the device
+//   v = u - sub (sub may be null), image = (v - mean v) / scale,
+//   scale = rms(v - mean v) computed on the device into *scale (0 when v
+//   is constant, then the image is 0)
+hipError_t cnn_dofs_to_image(const double *u, const double *sub, int n0,
+                             int n1, int N, float *image, double *scale,
+                             hipStream_t s);
+//   u = scale * image, scale read from the device, continuous across cell
+//   faces (nodes on a face or corner get the mean of their 2 or 4 pixels)
 hipError_t cnn_image_to_dofs(const float *image, const double *scale, int n0,
                              int n1, int N, double *u, hipStream_t s);
-//   target image of a (rho, phi) pair: (phi - mean phi) / scale of rho
-hipError_t cnn_target_image(const double *phi, const double *scale, int n0,
-                            int n1, int N, float *image, hipStream_t s);
 //   a training batch: dst[j] = src[i_j], dst2[j] = src2[i_j] for n images of
 //   `plane` floats, i_j uniform in [0, count) from SplitMix64 (seed,
 //   counter0 + j), drawn on the device

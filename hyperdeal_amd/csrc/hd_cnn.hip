@@ -290,50 +290,61 @@ __device__ inline double block_sum(double v, double *red)
   return r;
 }
 
+// v = u - sub (sub may be null): image = (v - mean v) / rms, scale = rms
 __global__ __launch_bounds__(1024) void dofs_to_image_kernel(
-  const double *__restrict__ u, long long n, int n0, int N,
-  float *__restrict__ image, double *__restrict__ scale)
+  const double *__restrict__ u, const double *__restrict__ sub, long long n,
+  int n0, int N, float *__restrict__ image, double *__restrict__ scale)
 {
   __shared__ double red[1024];
+  auto val = [&](long long i) { return sub ? u[i] - sub[i] : u[i]; };
   double s = 0.0;
   for (long long i = threadIdx.x; i < n; i += blockDim.x)
-    s += u[i];
+    s += val(i);
   const double mean = block_sum(s, red) / (double)n;
   s = 0.0;
   for (long long i = threadIdx.x; i < n; i += blockDim.x)
-    s += (u[i] - mean) * (u[i] - mean);
+    s += (val(i) - mean) * (val(i) - mean);
   const double rms = sqrt(block_sum(s, red) / (double)n);
   const double inv = rms > 0.0 ? 1.0 / rms : 0.0;
   for (long long i = threadIdx.x; i < n; i += blockDim.x)
-    image[pixel_of(i, n0, N)] = (float)((u[i] - mean) * inv);
+    image[pixel_of(i, n0, N)] = (float)((val(i) - mean) * inv);
   if (threadIdx.x == 0)
     *scale = rms;
 }
 
+// u = scale * image, continuous across cell faces: the Gauss-Lobatto end
+// nodes of neighbouring cells sit at one point (two pixels, four at a
+// corner), and every such node gets the mean of its pixels -- a guess with
+// jumps at the faces would pay for them in the SIPG penalty term
 __global__ void image_to_dofs_kernel(const float *__restrict__ image,
                                      const double *__restrict__ scale,
-                                     long long n, int n0, int N,
+                                     long long n, int n0, int n1, int N,
                                      double *__restrict__ u)
 {
   const double sc = *scale;
+  const int W = n0 * N, Hh = n1 * N;
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
-    u[i] = sc * (double)image[pixel_of(i, n0, N)];
-}
-
-__global__ __launch_bounds__(1024) void target_image_kernel(
-  const double *__restrict__ phi, const double *__restrict__ scale,
-  long long n, int n0, int N, float *__restrict__ image)
-{
-  __shared__ double red[1024];
-  double s = 0.0;
-  for (long long i = threadIdx.x; i < n; i += blockDim.x)
-    s += phi[i];
-  const double mean = block_sum(s, red) / (double)n;
-  const double sc = *scale;
-  const double inv = sc > 0.0 ? 1.0 / sc : 0.0;
-  for (long long i = threadIdx.x; i < n; i += blockDim.x)
-    image[pixel_of(i, n0, N)] = (float)((phi[i] - mean) * inv);
+    {
+      const long long p = pixel_of(i, n0, N);
+      const int Y = (int)(p / W), X = (int)(p % W);
+      const int ix = X % N, iy = Y % N;
+      int xs[2] = {X, X}, ys[2] = {Y, Y};
+      int nxs = 1, nys = 1;
+      if (ix == 0 && N > 1)
+        xs[nxs++] = X == 0 ? W - 1 : X - 1;
+      else if (ix == N - 1 && N > 1)
+        xs[nxs++] = X == W - 1 ? 0 : X + 1;
+      if (iy == 0 && N > 1)
+        ys[nys++] = Y == 0 ? Hh - 1 : Y - 1;
+      else if (iy == N - 1 && N > 1)
+        ys[nys++] = Y == Hh - 1 ? 0 : Y + 1;
+      double acc = 0.0;
+      for (int a = 0; a < nys; ++a)
+        for (int b = 0; b < nxs; ++b)
+          acc += (double)image[(long long)ys[a] * W + xs[b]];
+      u[i] = sc * acc / (nxs * nys);
+    }
 }
 
 // dst[j] = src[idx_j] for n images of `plane` floats; idx_j drawn on the
@@ -622,12 +633,13 @@ hipError_t CnnNet::adam(float lr, hipStream_t s)
 // ---------------------------------------------------------------------------
 // image helpers
 
-hipError_t cnn_dofs_to_image(const double *u, int n0, int n1, int N,
-                             float *image, double *scale, hipStream_t s)
+hipError_t cnn_dofs_to_image(const double *u, const double *sub, int n0,
+                             int n1, int N, float *image, double *scale,
+                             hipStream_t s)
 {
   const long long n = (long long)n0 * n1 * N * N;
   hipLaunchKernelGGL(cnn::dofs_to_image_kernel, dim3(1), dim3(1024), 0, s, u,
-                     n, n0, N, image, scale);
+                     sub, n, n0, N, image, scale);
   return hipGetLastError();
 }
 
@@ -637,16 +649,7 @@ hipError_t cnn_image_to_dofs(const float *image, const double *scale, int n0,
   const long long n = (long long)n0 * n1 * N * N;
   const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 4096);
   hipLaunchKernelGGL(cnn::image_to_dofs_kernel, dim3(blocks), dim3(256), 0, s,
-                     image, scale, n, n0, N, u);
-  return hipGetLastError();
-}
-
-hipError_t cnn_target_image(const double *phi, const double *scale, int n0,
-                            int n1, int N, float *image, hipStream_t s)
-{
-  const long long n = (long long)n0 * n1 * N * N;
-  hipLaunchKernelGGL(cnn::target_image_kernel, dim3(1), dim3(1024), 0, s, phi,
-                     scale, n, n0, N, image);
+                     image, scale, n, n0, n1, N, u);
   return hipGetLastError();
 }
 

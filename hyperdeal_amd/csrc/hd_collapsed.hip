@@ -463,7 +463,7 @@ template <int D>
 __device__ inline void inflow_cells(long long cell, const MeshDev &mesh,
                                     const InflowDev &inflow, long long *nb)
 {
-  unsigned rem = (unsigned)cell;
+  unsigned rem = (unsigned)(cell / mesh.inner);
 #pragma unroll
   for (int d = 0; d < D; ++d)
     {
@@ -1046,5 +1046,182 @@ hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,
     hipLaunchKernelGGL(collapsed::batch_convert_kernel<float>, dim3(grid),
                        dim3(256), 0, stream, ptrs, sizes);
   return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// Split form for cells larger than LDS (3+3 at k = 5: 6^6 values, 373 KB in
+// FP64).  The collapsed operator is a sum of 1D operators, one per
+// direction, so it splits exactly:
+//   part = sum_{d < 4} (A_d u along d + b_d face_d)
+//          the 4D kernel, unchanged, on the slabs of fixed (i4, ..): a slab
+//          is N^4 contiguous values and its neighbour across d < 4 is the
+//          same slab of the neighbouring cell, so the slabs form a 4D mesh
+//          with strides scaled by the slab count (MeshDev::inner);
+//   K    = part + sum_{d >= 4} (A_d u along d + b_d face_d)
+//          the plane kernel: one thread per (cell, point of the first four
+//          directions) holds the N^{D-4} values of its plane in registers,
+//          adds the part and finishes the apply or the LSRK stage.
+// Traffic: the slab pass reads u and writes part, the plane pass reads u,
+// part and the faces across d >= 4 and writes the result (FP64 apply about
+// 48 bytes per DoF instead of 16).
+namespace collapsed
+{
+template <int D, int N, typename T, int MODE>
+__global__ __launch_bounds__(256) void plane_kernel(
+  const T *__restrict__ src, const T *part, T *dst,
+  const T *__restrict__ coef, MeshDev mesh, InflowDev inflow, StageDev stage)
+{
+  constexpr int R = D - 4, P = ipow(N, 4), PL = ipow(N, R);
+  constexpr long long CELL = (long long)P * PL;
+  constexpr int NN = N * N;
+  const cptr<T> cc = (cptr<T>)coef;
+  const long long n = mesh.n_cells * P;
+  for (long long idx = blockIdx.x * (long long)blockDim.x + threadIdx.x;
+       idx < n; idx += (long long)gridDim.x * blockDim.x)
+    {
+      const long long cell = idx / P;
+      const int pt = (int)(idx - cell * P);
+      const long long g0 = cell * CELL + pt;
+      T u[PL], k[PL];
+#pragma unroll
+      for (int q = 0; q < PL; ++q)
+        {
+          u[q] = src[g0 + (long long)q * P];
+          k[q] = 0;
+        }
+#pragma unroll
+      for (int r = 0; r < R; ++r)
+        {
+          const int d = 4 + r;
+          const int sd = r == 0 ? 1 : N; // plane stride of direction d
+          const int side = inflow.side[d];
+          const long long nb = inflow_neighbour(mesh, cell, d, side);
+          const int fnode = side < 0 ? 0 : N - 1;
+          const T *fsrc = src + nb * CELL + pt;
+          const cptr<T> A = cc + d * (NN + N);
+          const cptr<T> bv = A + NN;
+#pragma unroll
+          for (int l = 0; l < PL / N; ++l)
+            {
+              // base of line l along d in the plane
+              const int base = r == 0 ? l * N : l;
+              const T fv = fsrc[(long long)(base + fnode * sd) * P];
+#pragma unroll
+              for (int i = 0; i < N; ++i)
+                {
+                  T acc = bv[i] * fv;
+#pragma unroll
+                  for (int j = 0; j < N; ++j)
+                    acc += A[i * N + j] * u[base + j * sd];
+                  k[base + i * sd] += acc;
+                }
+            }
+        }
+#pragma unroll
+      for (int q = 0; q < PL; ++q)
+        {
+          const long long g = g0 + (long long)q * P;
+          const T kq = k[q] + part[g];
+          if constexpr (MODE == 0)
+            dst[g] = kq;
+          else
+            {
+              const double so = stage.sol[g];
+              stage.sol[g] = so + stage.b_dt * (double)kq;
+              if constexpr (MODE == 1)
+                static_cast<T *>(stage.ti_out)[g] =
+                  (T)(so + stage.a_dt * (double)kq);
+            }
+        }
+    }
+}
+
+template <int D, int N, typename T>
+hipError_t split_launch(int mode, const T *src, T *dst, T *part,
+                        const T *coef, const MeshDev &mesh,
+                        const InflowDev &inflow, const StageDev &stage,
+                        hipStream_t stream)
+{
+  static_assert(D == 5 || D == 6, "split form: D = 5, 6");
+  constexpr int R = D - 4;
+  // 1) the slab view: a 4D mesh of n_cells * N^R slabs
+  MeshDev m4 = mesh;
+  m4.dim = 4;
+  m4.inner = ipow(N, R);
+  for (int d = 0; d < 4; ++d)
+    m4.stride[d] = mesh.stride[d] * m4.inner;
+  for (int d = 4; d < kMaxDim; ++d)
+    {
+      m4.n[d] = 1;
+      m4.stride[d] = 0;
+    }
+  m4.n_cells = mesh.n_cells * m4.inner;
+  InflowDev in4 = inflow;
+  for (int d = 4; d < kMaxDim; ++d)
+    in4.side[d] = 0;
+  const StageDev none{};
+  hipError_t e =
+    launch<4, N, T>(0, src, part, coef, m4, in4, none, stream);
+  if (e != hipSuccess)
+    return e;
+  // 2) the planes
+  const long long n = mesh.n_cells * ipow(N, 4);
+  const unsigned blocks =
+    (unsigned)std::min<long long>((n + 255) / 256, 256LL * 16);
+  if (mode == 0)
+    hipLaunchKernelGGL((plane_kernel<D, N, T, 0>), dim3(blocks), dim3(256), 0,
+                       stream, src, part, dst, coef, mesh, inflow, stage);
+  else if (stage.ti_out)
+    hipLaunchKernelGGL((plane_kernel<D, N, T, 1>), dim3(blocks), dim3(256), 0,
+                       stream, src, part, dst, coef, mesh, inflow, stage);
+  else
+    hipLaunchKernelGGL((plane_kernel<D, N, T, 2>), dim3(blocks), dim3(256), 0,
+                       stream, src, part, dst, coef, mesh, inflow, stage);
+  return hipGetLastError();
+}
+
+template <typename T>
+hipError_t split_dispatch(int D, int N, int mode, const T *src, T *dst,
+                          T *part, const T *coef, const MeshDev &mesh,
+                          const InflowDev &inflow, const StageDev &stage,
+                          hipStream_t stream)
+{
+  if (D == 6 && N == 5)
+    return split_launch<6, 5, T>(mode, src, dst, part, coef, mesh, inflow,
+                                 stage, stream);
+  if (D == 6 && N == 6)
+    return split_launch<6, 6, T>(mode, src, dst, part, coef, mesh, inflow,
+                                 stage, stream);
+  if (D == 5 && N == 5)
+    return split_launch<5, 5, T>(mode, src, dst, part, coef, mesh, inflow,
+                                 stage, stream);
+  if (D == 5 && N == 6)
+    return split_launch<5, 6, T>(mode, src, dst, part, coef, mesh, inflow,
+                                 stage, stream);
+  return hipErrorInvalidValue;
+}
+} // namespace collapsed
+
+bool collapsed_split_supported(int D, int N)
+{
+  return (D == 5 || D == 6) && (N == 5 || N == 6);
+}
+
+hipError_t collapsed_split_f64(int D, int N, int mode, const double *src,
+                               double *dst, double *part, const double *coef,
+                               const MeshDev &mesh, const InflowDev &inflow,
+                               const StageDev &stage, hipStream_t stream)
+{
+  return collapsed::split_dispatch<double>(D, N, mode, src, dst, part, coef,
+                                           mesh, inflow, stage, stream);
+}
+
+hipError_t collapsed_split_f32(int D, int N, int mode, const float *src,
+                               float *dst, float *part, const float *coef,
+                               const MeshDev &mesh, const InflowDev &inflow,
+                               const StageDev &stage, hipStream_t stream)
+{
+  return collapsed::split_dispatch<float>(D, N, mode, src, dst, part, coef,
+                                          mesh, inflow, stage, stream);
 }
 } // namespace hd

@@ -23,6 +23,10 @@ struct MeshDev
   long long stride[kMaxDim]; // cell-id stride per direction
   long long n_cells;
   double lo[kMaxDim], h[kMaxDim];
+  // cell id = inner * (lexicographic id) + sub-cell index: > 1 only for the
+  // slab view of the split kernels (a D-dimensional cell seen as inner
+  // cells of its first four directions)
+  int inner = 1;
 };
 
 // Which neighbour feeds the inflow face of each direction (constant
@@ -39,6 +43,7 @@ struct StageDev
   double a_dt, b_dt;
   double *sol;   // FP64 solution, updated in place
   void *ti_out;  // next stage input (Number), may be null (last stage)
+  void *scratch; // a free Number vector for the last stage, or null
 };
 
 // blocks are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -186,6 +191,19 @@ hipError_t collapsed_batch(int D, int precision, int mode, const BatchOp *ops,
                            const BatchPtrs &ptrs, hipStream_t stream);
 hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,
                          const BatchPtrs &ptrs, hipStream_t stream);
+// operators too large for one cell in LDS (D >= 5, N beyond the
+// collapsed kernel's range): the first four directions on the slabs of
+// fixed higher indices by the 4D kernel, the others on the planes of fixed
+// lower indices; part: the first part's result (may be dst or ti_out)
+bool collapsed_split_supported(int D, int N);
+hipError_t collapsed_split_f64(int D, int N, int mode, const double *src,
+                               double *dst, double *part, const double *coef,
+                               const MeshDev &mesh, const InflowDev &inflow,
+                               const StageDev &stage, hipStream_t stream);
+hipError_t collapsed_split_f32(int D, int N, int mode, const float *src,
+                               float *dst, float *part, const float *coef,
+                               const MeshDev &mesh, const InflowDev &inflow,
+                               const StageDev &stage, hipStream_t stream);
 bool quadrature_supported(int D, int N);
 hipError_t quadrature_apply(int D, int N, int mode, const double *src,
                             double *dst, const QuadDev &q,
@@ -214,10 +232,27 @@ bool vp_poisson_supported(int dim_x, int N, long long n_x_cells);
 // [3] relative residual of the start, [4] of the previous potential,
 // [5] solves started from a guess, [6], [7] the weights of the previous
 // potential and of the guess in the start (doubles in slots 2-4, 6, 7)
+// tab2d: the 2D tensor-product SIPG table (vp_poisson2d_table, 8 N^2
+// doubles on the device) selecting the fast 2D solver where it applies, or
+// null.  residual_out (2D solver only): write b - A phi there and stop
+// (phi, efield and stats untouched).
 hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
                       const double *guess, double *efield, double *work,
                       double rel_tol, int max_it, long long *stats,
-                      hipStream_t stream);
+                      hipStream_t stream, const double *tab2d = nullptr,
+                      double *residual_out = nullptr);
+bool vp_poisson2d_supported(int dim_x, int N, long long n_x_cells);
+// the 2+2 Vlasov-Poisson apply with the velocity (v, grad phi): a table of
+// vp_apply_table_size(N) doubles (the unit-speed 1D operators of the four
+// directions for both signs, S, S^-1, Gauss points), precision 0 FP64, 1
+// FP32-mixed; mode 0 apply, 1 LSRK stage
+bool vp_apply_supported(int dim_x, int dim_v, int N);
+int vp_apply_table_size(int N);
+hipError_t vp_apply(int N, int mode, int precision, const void *src, void *dst,
+                    const double *tab, const MeshDev &mesh,
+                    long long n_x_cells, const double *efield,
+                    const StageDev &stage, hipStream_t stream);
+void vp_poisson2d_table(const XDev &xd, int N, double *tab);
 hipError_t vp_moments(const double *f, const MeshDev &mesh, const NormDev &nt,
                       int dim_x, int vol, double *partial, hipStream_t stream);
 hipError_t launch_norm(const double *src, const MeshDev &mesh,

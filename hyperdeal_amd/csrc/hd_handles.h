@@ -55,8 +55,9 @@ struct hd_cnn
 };
 
 // the learned initial guess of a Vlasov-Poisson operator's potential solve:
-// the network, a ring of (density, potential) training images taken from
-// the operator's own converged solves, and the training schedule
+// the network, a ring of (right-hand side, solution) training images taken
+// from the operator's own converged solves -- the residual of the previous
+// potential and the correction CG added to it -- and the training schedule
 struct hd_learned_guess
 {
   hd_cnn cnn;
@@ -65,7 +66,8 @@ struct hd_learned_guess
   int capacity = 0, count = 0, head = 0;
   float *ring_in = nullptr, *ring_tgt = nullptr;
   float *img_out = nullptr, *batch_in = nullptr, *batch_tgt = nullptr;
-  double *scale = nullptr, *guess = nullptr;
+  double *scale = nullptr, *scale_t = nullptr, *guess = nullptr;
+  double *phi_prev = nullptr, *resid = nullptr; // x-space DoFs
   uint64_t seed = 0, draws = 0;
   long long solves = 0, guessed = 0;
   int train_every = 1, train_batch = 8, train_steps = 1, min_pairs = 4;
@@ -82,6 +84,8 @@ struct hd_advection
   double bc_params[8] = {0};
   int algorithm = HD_ALGO_AUTO;
   bool collapsed_ok = false;
+  bool split = false;          // the collapsed form runs split (big cells)
+  void *split_scratch = nullptr; // the last stage's part, when none is given
   double *coef64 = nullptr;
   float *coef32 = nullptr;
   hd::InflowDev inflow{};
@@ -98,17 +102,19 @@ struct hd_advection
   double *efield = nullptr, *rho = nullptr, *phi = nullptr, *work = nullptr,
          *partial = nullptr;
   long long *stats = nullptr; // solver statistics, see hd::vp_poisson
+  double *ptab = nullptr;     // 2D SIPG table of the fast solver, or null
+  double *vtab = nullptr;     // tables of the 2+2 phase-space apply, or null
   hd_learned_guess *learned = nullptr;
 };
 
 namespace hd_abi
 {
-// the learned guess around one potential solve: before it (density image,
-// network inference, the guess in x-space DoFs; returns the guess or null),
-// and after it (training pair, scheduled training steps)
-hipError_t learned_before_solve(hd_learned_guess &lg, const double *rho,
-                                const double **guess, hipStream_t s);
-hipError_t learned_after_solve(hd_learned_guess &lg, const double *phi,
-                               hipStream_t s);
+// the learned guess around one potential solve: before it (the residual of
+// the previous potential as an image, network inference, the guess in
+// x-space DoFs; returns the guess or null) and after it (training pair,
+// scheduled training steps); rho must be current
+hipError_t learned_before_solve(hd_advection &op, const double **guess,
+                                hipStream_t s);
+hipError_t learned_after_solve(hd_advection &op, hipStream_t s);
 void learned_release(hd_learned_guess *lg);
 } // namespace hd_abi

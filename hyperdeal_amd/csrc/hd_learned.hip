@@ -7,15 +7,20 @@
 // decides the result, so a poor guess costs one extra matrix-vector product
 // and never the accuracy.
 //
-// Per solve, with the guess on:
-//   image  <- (rho - mean) / rms(rho - mean)    (a ring slot: the training
+// Per solve, with the guess on (phi_prev: the potential before the solve):
+//   r0     <- b - A phi_prev                    (the start's residual)
+//   image  <- (r0 - mean) / rms                 (a ring slot: the training
 //                                               input of this solve)
-//   guess  <- rms * net(image)                  once the net has trained
-//   CG from the best combination of phi_prev and guess
-//   target <- (phi - mean phi) / rms            (the ring slot's target)
+//   g      <- net(image)                        once the net has trained
+//   CG from the energy-minimising combination of phi_prev and g
+//   target <- (phi - phi_prev - mean) / rms     (the ring slot's target: the
+//                                               correction CG found, the
+//                                               solution for right-hand
+//                                               side r0)
 //   every train_every solves: train_steps Adam steps on batches drawn from
 //   the ring on the device
-// Nothing here synchronises with the host.
+// The network learns the shape of A^-1 on the residuals this run produces;
+// the combination sets the scale.  Nothing here synchronises with the host.
 #include "hd_handles.h"
 
 #include <algorithm>
@@ -30,13 +35,22 @@ using hd_abi::as_stream;
 
 namespace hd_abi
 {
-hipError_t learned_before_solve(hd_learned_guess &lg, const double *rho,
-                                const double **guess, hipStream_t s)
+hipError_t learned_before_solve(hd_advection &op, const double **guess,
+                                hipStream_t s)
 {
+  hd_learned_guess &lg = *op.learned;
   *guess = nullptr;
+  const size_t bytes = (size_t)op.n_x_dofs * sizeof(double);
+  hipError_t e =
+    hipMemcpyAsync(lg.phi_prev, op.phi, bytes, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess)
+    e = hd::vp_poisson(op.xdev, op.mf->nd, op.rho, op.phi, nullptr,
+                       op.efield, op.work, op.poisson_tol, op.poisson_max_it,
+                       op.stats, s, op.ptab, lg.resid);
   float *slot = lg.ring_in + (long long)lg.head * lg.plane;
-  hipError_t e = hd::cnn_dofs_to_image(rho, lg.n0, lg.n1, lg.N, slot,
-                                       lg.scale, s);
+  if (e == hipSuccess)
+    e = hd::cnn_dofs_to_image(lg.resid, nullptr, lg.n0, lg.n1, lg.N, slot,
+                              lg.scale, s);
   if (e != hipSuccess || lg.cnn.net.steps == 0)
     return e;
   e = lg.cnn.net.forward(slot, lg.img_out, 1, s);
@@ -51,12 +65,12 @@ hipError_t learned_before_solve(hd_learned_guess &lg, const double *rho,
   return e;
 }
 
-hipError_t learned_after_solve(hd_learned_guess &lg, const double *phi,
-                               hipStream_t s)
+hipError_t learned_after_solve(hd_advection &op, hipStream_t s)
 {
-  hipError_t e =
-    hd::cnn_target_image(phi, lg.scale, lg.n0, lg.n1, lg.N,
-                         lg.ring_tgt + (long long)lg.head * lg.plane, s);
+  hd_learned_guess &lg = *op.learned;
+  hipError_t e = hd::cnn_dofs_to_image(
+    op.phi, lg.phi_prev, lg.n0, lg.n1, lg.N,
+    lg.ring_tgt + (long long)lg.head * lg.plane, lg.scale_t, s);
   if (e != hipSuccess)
     return e;
   lg.head = (lg.head + 1) % lg.capacity;
@@ -87,7 +101,9 @@ void learned_release(hd_learned_guess *lg)
   lg->cnn.net.release();
   for (void *p : {(void *)lg->ring_in, (void *)lg->ring_tgt,
                   (void *)lg->img_out, (void *)lg->batch_in,
-                  (void *)lg->batch_tgt, (void *)lg->scale, (void *)lg->guess})
+                  (void *)lg->batch_tgt, (void *)lg->scale,
+                  (void *)lg->scale_t, (void *)lg->guess,
+                  (void *)lg->phi_prev, (void *)lg->resid})
     (void)hipFree(p);
   delete lg;
 }
@@ -249,6 +265,10 @@ int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
     return fail(HD_ERR_UNSUPPORTED,
                 "the learned guess works on 2D x-spaces (an image of nodes)");
   const int n0 = mf->mesh.n[0], n1 = mf->mesh.n[1], N = mf->nd;
+  if (!op->ptab)
+    return fail(HD_ERR_UNSUPPORTED,
+                "the learned guess needs the 2D tensor-product Poisson "
+                "solver (k <= 3, at most 1024 x cells)");
   if (!hd::CnnNet::shape_ok(n1 * N, n0 * N))
     return fail(HD_ERR_UNSUPPORTED,
                 "the learned guess needs (cells x nodes) per x direction to "
@@ -288,7 +308,13 @@ int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
   if (e == hipSuccess)
     e = hipMalloc(&lg->scale, sizeof(double));
   if (e == hipSuccess)
+    e = hipMalloc(&lg->scale_t, sizeof(double));
+  if (e == hipSuccess)
     e = hipMalloc(&lg->guess, (size_t)op->n_x_dofs * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&lg->phi_prev, (size_t)op->n_x_dofs * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMalloc(&lg->resid, (size_t)op->n_x_dofs * sizeof(double));
   // the activation buffers for training and inference, up front
   if (e == hipSuccess)
     e = lg->cnn.net.reserve(train_batch);

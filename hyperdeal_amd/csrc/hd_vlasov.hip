@@ -16,6 +16,8 @@
 // on the device -- no host round trip inside a Runge-Kutta stage.
 #include "hd_device.h"
 
+#include <algorithm>
+
 namespace hd
 {
 namespace vlasov
@@ -461,6 +463,813 @@ __global__ __launch_bounds__(kSolverThreads) void poisson_kernel(
     }
 }
 
+// ---- the 2D solver: tensor-product SIPG blocks, registers and LDS ----
+// On a uniform periodic 2D mesh the SIPG matrix is A = L_0 (x) M_1 +
+// M_0 (x) L_1 with L_d the 1D SIPG matrix of direction d (blocks coupling a
+// cell to its left neighbour, itself, its right neighbour) and M_d the 1D
+// mass matrix.  Table (8 N^2 doubles): L_0 left, self, right, L_1 left,
+// self, right, M_0, M_1, each row-major (test node, trial node).
+template <class T>
+using cptr = const __attribute__((address_space(4))) T *;
+
+// LDS stride of the transposed direction (one row per node): a compile-time
+// constant, so every node's read is an immediate offset from one address
+constexpr int kPStride = kSolverThreads;
+
+// y = A p for the cell (cx, cy); p transposed in LDS: Pt[i * kPStride + cell]
+template <int N>
+__device__ inline void sipg2d(const double *Pt, int n0, int n1, int cx,
+                              int cy, cptr<double> tab, double *y)
+{
+  constexpr int NN = N * N;
+  // opaque to the optimiser: the table loads stay inside the CG loop as
+  // scalar loads instead of being hoisted into vector registers
+  asm volatile("" : "+s"(tab));
+  const int cs = cx + n0 * cy;
+  const int xn[3] = {(cx == 0 ? n0 - 1 : cx - 1) + n0 * cy, cs,
+                     (cx == n0 - 1 ? 0 : cx + 1) + n0 * cy};
+  const int yn[3] = {cx + n0 * (cy == 0 ? n1 - 1 : cy - 1), cs,
+                     cx + n0 * (cy == n1 - 1 ? 0 : cy + 1)};
+  double z[NN];
+  // direction 0: z = sum_k (L_0k (x) I) p_k, y = (I (x) M_1) z
+#pragma unroll
+  for (int iy = 0; iy < N; ++iy)
+    {
+#pragma unroll
+      for (int ix = 0; ix < N; ++ix)
+        z[ix + N * iy] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        {
+          double q[N];
+#pragma unroll
+          for (int jx = 0; jx < N; ++jx)
+            q[jx] = Pt[(jx + N * iy) * kPStride + xn[k]];
+#pragma unroll
+          for (int ix = 0; ix < N; ++ix)
+#pragma unroll
+            for (int jx = 0; jx < N; ++jx)
+              z[ix + N * iy] += tab[k * NN + ix * N + jx] * q[jx];
+        }
+    }
+#pragma unroll
+  for (int iy = 0; iy < N; ++iy)
+#pragma unroll
+    for (int ix = 0; ix < N; ++ix)
+      {
+        double a = 0.0;
+#pragma unroll
+        for (int jy = 0; jy < N; ++jy)
+          a += tab[7 * NN + iy * N + jy] * z[ix + N * jy];
+        y[ix + N * iy] = a;
+      }
+  // direction 1: z = sum_k (I (x) L_1k) p_k, y += (M_0 (x) I) z
+#pragma unroll
+  for (int ix = 0; ix < N; ++ix)
+    {
+#pragma unroll
+      for (int iy = 0; iy < N; ++iy)
+        z[ix + N * iy] = 0.0;
+#pragma unroll
+      for (int k = 0; k < 3; ++k)
+        {
+          double q[N];
+#pragma unroll
+          for (int jy = 0; jy < N; ++jy)
+            q[jy] = Pt[(ix + N * jy) * kPStride + yn[k]];
+#pragma unroll
+          for (int iy = 0; iy < N; ++iy)
+#pragma unroll
+            for (int jy = 0; jy < N; ++jy)
+              z[ix + N * iy] += tab[(3 + k) * NN + iy * N + jy] * q[jy];
+        }
+    }
+#pragma unroll
+  for (int iy = 0; iy < N; ++iy)
+#pragma unroll
+    for (int ix = 0; ix < N; ++ix)
+      {
+        double a = 0.0;
+#pragma unroll
+        for (int jx = 0; jx < N; ++jx)
+          a += tab[6 * NN + ix * N + jx] * z[jx + N * iy];
+        y[ix + N * iy] += a;
+      }
+}
+
+// sum over the block (1024 threads = 16 waves): wave shuffles, one barrier,
+// every thread adds the 16 wave sums in the same order (uniform result).
+// Consecutive calls use alternate slots, so a slot is rewritten only after
+// every thread has passed the barrier of the call in between.
+template <int slot>
+__device__ inline double block_sum16(double v, double *red)
+{
+#pragma unroll
+  for (int off = 32; off >= 1; off >>= 1)
+    v += __shfl_xor(v, off, 64);
+  if ((threadIdx.x & 63) == 0)
+    red[slot * 16 + (threadIdx.x >> 6)] = v;
+  __syncthreads();
+  double r = 0.0;
+#pragma unroll
+  for (int w = 0; w < 16; ++w)
+    r += red[slot * 16 + w];
+  return r;
+}
+
+// One block of 1024 threads, thread t owns x cell t (n_cells <= 1024): the
+// residual and A p in registers, the search direction in LDS (transposed,
+// so that a wave reads consecutive addresses), the iterate in global
+// scratch (transposed).  Same algorithm and outputs as poisson_kernel.
+template <int N>
+__global__ __launch_bounds__(kSolverThreads) void poisson2d_kernel(
+  XDev xd, const double *__restrict__ tab_in,
+  const double *__restrict__ rho_in, double *__restrict__ phi,
+  const double *__restrict__ guess, double *__restrict__ efield,
+  double *__restrict__ work, double rel_tol, int max_it,
+  long long *__restrict__ stats, double *__restrict__ residual_out)
+{
+  constexpr int NN = N * N;
+  extern __shared__ double sm[];
+  const int nc = (int)xd.n_cells, n0 = xd.n[0], n1 = xd.n[1];
+  const double nx = (double)nc * NN;
+  double *Pt = sm;                  // NN x kPStride
+  double *red = sm + NN * kPStride; // 2 x 16
+  double *phit = work;        // NN x nc
+  const cptr<double> tab = (cptr<double>)tab_in;
+  const int t = threadIdx.x;
+  const bool own = t < nc;
+  const int c = own ? t : 0;
+  const int cx = c % n0, cy = c / n0;
+
+  // b = -(M_0 (x) M_1)(rho - mean rho), then b -= mean b
+  double r[NN], y[NN];
+  double s = 0.0;
+#pragma unroll
+  for (int i = 0; i < NN; ++i)
+    {
+      r[i] = own ? rho_in[(long long)c * NN + i] : 0.0;
+      s += r[i];
+    }
+  const double mean_rho = block_sum16<0>(s, red) / nx;
+#pragma unroll
+  for (int iy = 0; iy < N; ++iy)
+#pragma unroll
+    for (int ix = 0; ix < N; ++ix)
+      {
+        double a = 0.0;
+#pragma unroll
+        for (int jx = 0; jx < N; ++jx)
+          a += tab[6 * NN + ix * N + jx] * (r[jx + N * iy] - mean_rho);
+        y[ix + N * iy] = a;
+      }
+  s = 0.0;
+#pragma unroll
+  for (int iy = 0; iy < N; ++iy)
+#pragma unroll
+    for (int ix = 0; ix < N; ++ix)
+      {
+        double a = 0.0;
+#pragma unroll
+        for (int jy = 0; jy < N; ++jy)
+          a += tab[7 * NN + iy * N + jy] * y[ix + N * jy];
+        r[ix + N * iy] = own ? -a : 0.0;
+        s += r[ix + N * iy];
+      }
+  const double mean_b = block_sum16<1>(s, red) / nx;
+  double bb = 0.0;
+#pragma unroll
+  for (int i = 0; i < NN; ++i)
+    {
+      r[i] = own ? r[i] - mean_b : 0.0; // r holds b
+      bb += r[i] * r[i];
+    }
+  bb = block_sum16<0>(bb, red);
+
+  // the start: r = b - A phi (or the combination with the guess)
+  double x[NN];
+#pragma unroll
+  for (int i = 0; i < NN; ++i)
+    {
+      x[i] = own ? phi[(long long)c * NN + i] : 0.0;
+      if (own)
+        Pt[i * kPStride + c] = x[i];
+    }
+  __syncthreads();
+  if (own)
+    sipg2d<N>(Pt, n0, n1, cx, cy, tab, y);
+  else
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+      y[i] = 0.0;
+  double alpha0 = 1.0, beta0 = 0.0, rr_warm = -1.0;
+  if (guess)
+    {
+      double g[NN], Ag[NN];
+      __syncthreads(); // every thread has read Pt
+#pragma unroll
+      for (int i = 0; i < NN; ++i)
+        {
+          g[i] = own ? guess[(long long)c * NN + i] : 0.0;
+          if (own)
+            Pt[i * kPStride + c] = g[i];
+        }
+      __syncthreads();
+      if (own)
+        sipg2d<N>(Pt, n0, n1, cx, cy, tab, Ag);
+      else
+#pragma unroll
+        for (int i = 0; i < NN; ++i)
+          Ag[i] = 0.0;
+      double d[6] = {0, 0, 0, 0, 0, 0};
+#pragma unroll
+      for (int i = 0; i < NN; ++i)
+        {
+          d[0] += x[i] * y[i];
+          d[1] += x[i] * Ag[i];
+          d[2] += g[i] * Ag[i];
+          d[3] += x[i] * r[i];
+          d[4] += g[i] * r[i];
+          d[5] += (r[i] - y[i]) * (r[i] - y[i]);
+        }
+      d[0] = block_sum16<1>(d[0], red);
+      d[1] = block_sum16<0>(d[1], red);
+      d[2] = block_sum16<1>(d[2], red);
+      d[3] = block_sum16<0>(d[3], red);
+      d[4] = block_sum16<1>(d[4], red);
+      d[5] = block_sum16<0>(d[5], red);
+      rr_warm = d[5];
+      const double det = d[0] * d[2] - d[1] * d[1];
+      if (d[0] > 0.0 && d[2] > 0.0 && det > 1e-12 * d[0] * d[2])
+        {
+          alpha0 = (d[3] * d[2] - d[4] * d[1]) / det;
+          beta0 = (d[0] * d[4] - d[1] * d[3]) / det;
+        }
+      else if (!(d[0] > 0.0) && d[2] > 0.0)
+        {
+          alpha0 = 0.0;
+          beta0 = d[4] / d[2];
+        }
+      if (!isfinite(alpha0) || !isfinite(beta0))
+        {
+          alpha0 = 1.0;
+          beta0 = 0.0;
+        }
+#pragma unroll
+      for (int i = 0; i < NN; ++i)
+        {
+          x[i] = alpha0 * x[i] + beta0 * g[i];
+          y[i] = alpha0 * y[i] + beta0 * Ag[i];
+        }
+    }
+  double rr = 0.0;
+#pragma unroll
+  for (int i = 0; i < NN; ++i)
+    {
+      r[i] -= y[i];
+      rr += r[i] * r[i];
+    }
+  if (residual_out)
+    {
+      // residual of the start only (no solve): b - A phi
+      if (own)
+#pragma unroll
+        for (int i = 0; i < NN; ++i)
+          residual_out[(long long)c * NN + i] = r[i];
+      return;
+    }
+  rr = block_sum16<1>(rr, red); // also: every thread is done with Pt
+  const double rr_start = rr;
+  if (own)
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+      {
+        phit[(long long)i * nc + c] = x[i];
+        Pt[i * kPStride + c] = r[i];
+      }
+  const double stop = rel_tol * sqrt(bb);
+  int it = 0;
+  while (sqrt(rr) > stop && it < max_it)
+    {
+      __syncthreads(); // the new direction is visible
+      if (own)
+        sipg2d<N>(Pt, n0, n1, cx, cy, tab, y);
+      double pAp = 0.0;
+      if (own)
+#pragma unroll
+        for (int i = 0; i < NN; ++i)
+          pAp += Pt[i * kPStride + c] * y[i];
+      pAp = block_sum16<0>(pAp, red);
+      if (!(pAp > 0.0))
+        break; // what is left of the residual lies in the null space
+      const double alpha = rr / pAp;
+      double rr2 = 0.0;
+      if (own)
+#pragma unroll
+        for (int i = 0; i < NN; ++i)
+          {
+            phit[(long long)i * nc + c] += alpha * Pt[i * kPStride + c];
+            r[i] -= alpha * y[i];
+            rr2 += r[i] * r[i];
+          }
+      rr2 = block_sum16<1>(rr2, red);
+      const double beta = rr2 / rr;
+      rr = rr2;
+      // every matrix-vector product of this iteration ended before the
+      // barrier in the last sum: the direction can be overwritten
+      if (own)
+#pragma unroll
+        for (int i = 0; i < NN; ++i)
+          Pt[i * kPStride + c] = r[i] + beta * Pt[i * kPStride + c];
+      ++it;
+    }
+
+  // phi back in the cell-major layout, then grad phi at the quadrature
+  // points of this thread's cell
+  if (own)
+    {
+#pragma unroll
+      for (int i = 0; i < NN; ++i)
+        x[i] = phit[(long long)i * nc + c];
+#pragma unroll
+      for (int i = 0; i < NN; ++i)
+        phi[(long long)c * NN + i] = x[i];
+      // d/dx0: G along x0, S along x1; d/dx1: S along x0, G along x1
+#pragma unroll
+      for (int e = 0; e < 2; ++e)
+        {
+#pragma unroll
+          for (int iy = 0; iy < N; ++iy)
+#pragma unroll
+            for (int qx = 0; qx < N; ++qx)
+              {
+                double a = 0.0;
+#pragma unroll
+                for (int jx = 0; jx < N; ++jx)
+                  a += (e == 0 ? xd.G[qx * N + jx] : xd.S[qx * N + jx]) *
+                       x[jx + N * iy];
+                y[qx + N * iy] = a;
+              }
+#pragma unroll
+          for (int qy = 0; qy < N; ++qy)
+#pragma unroll
+            for (int qx = 0; qx < N; ++qx)
+              {
+                double a = 0.0;
+#pragma unroll
+                for (int jy = 0; jy < N; ++jy)
+                  a += (e == 1 ? xd.G[qy * N + jy] : xd.S[qy * N + jy]) *
+                       y[qx + N * jy];
+                efield[((long long)c * NN + qx + N * qy) * 2 + e] =
+                  a / xd.h[e];
+              }
+        }
+    }
+  if (threadIdx.x == 0)
+    {
+      double *ds = reinterpret_cast<double *>(stats);
+      const double nb = sqrt(bb) + 1e-300;
+      stats[0] += it;
+      stats[1] += 1;
+      ds[2] = sqrt(rr) / nb;
+      ds[3] = sqrt(rr_start) / nb;
+      ds[4] = (guess ? sqrt(rr_warm) : sqrt(rr_start)) / nb;
+      if (guess)
+        {
+          stats[5] += 1;
+          ds[6] = alpha0;
+          ds[7] = beta0;
+        }
+    }
+}
+
+// ---- the 2+2 phase-space apply with the velocity (v, grad phi) ----
+// The reference's quadrature algorithm (advection_operation.h:221-566 with
+// the velocity of vlasov_poisson/include/velocity_field_view.h:111-175)
+// rewritten exactly per direction, for affine Cartesian cells and n_points
+// = k + 1:
+//   x_d (d = 0, 1): the speed v_d is constant along every x line, so the
+//     x_d term is the unit-speed 1D DG operator along x_d (the collapsed
+//     A, b of the constant-velocity kernels) tensored with the v_d factor
+//     S^-1 diag(v_d(q)) S, split by sign for the upwind choice:
+//       out += V_d+ (x) (A_d+ u + b_d+ u_left) + V_d- (x) (A_d- u + b_d- u_right)
+//   v_e (e = 0, 1): E_e is given at the x Gauss points; in the x Gauss basis
+//     (S (x) S along x0, x1) every v_e line has the constant speed
+//     E_e(q_x): the unit-speed operator scaled by E_e(q_x), upwind by its
+//     sign, then back with S^-1 (x) S^-1.
+// A_d+ is the collapsed operator for speed +1, A_d- minus the one for speed
+// -1, so that the operator for speed c is c A_d+ (c >= 0) or c A_d- (c < 0).
+// Every quadrature the reference applies is reproduced (Gauss k+1 points in
+// every factor), so the results agree to roundoff.
+//
+// One block of 256 threads per cell, one DoF per thread and step (N^4 <=
+// 625: threads loop), five LDS barriers per cell.
+// Table (doubles): per direction d: A+ (N^2), b+ (N), A- (N^2), b- (N);
+// then S (N^2, row q: Gauss point), S^-1 (N^2), Gauss points (N).
+namespace vpk
+{
+template <int N>
+struct Tab
+{
+  static constexpr int NN = N * N, PER = 2 * (NN + N);
+  static constexpr int Ap(int d) { return d * PER; }
+  static constexpr int bp(int d) { return d * PER + NN; }
+  static constexpr int An(int d) { return d * PER + NN + N; }
+  static constexpr int bn(int d) { return d * PER + 2 * NN + N; }
+  static constexpr int S = 4 * PER, Si = S + NN, qp = Si + NN;
+  static constexpr int size = qp + N;
+};
+
+// Work mapping of every contraction step: element e = i * FP + l with i
+// the output value of the contracted digit (uniform over a wave: FP is a
+// multiple of 64) and l the line (the other three digits, ascending), so
+// the coefficient rows are scalar loads.
+template <int N>
+struct Map
+{
+  static constexpr int F = N * N * N;
+  static constexpr int FP = (F + 63) / 64 * 64;
+  static constexpr int E = N * FP;
+};
+
+// base index (digit d = 0) of line l along direction d
+template <int N>
+__device__ inline int line_base(int l, int d)
+{
+  int r = 0, p = 1;
+#pragma unroll
+  for (int k = 0; k < 4; ++k)
+    {
+      if (k != d)
+        {
+          r += (l % N) * p;
+          l /= N;
+        }
+      p *= N;
+    }
+  return r;
+}
+
+__device__ inline int uniform(int v)
+{
+  return __builtin_amdgcn_readfirstlane(v);
+}
+
+template <int N, typename T, int MODE>
+__global__ __launch_bounds__(256) void apply_kernel(
+  const T *__restrict__ src, T *__restrict__ dst,
+  const double *__restrict__ tab_in, MeshDev mesh, long long n_x_cells,
+  const double *__restrict__ efield, StageDev stage)
+{
+  using TB = Tab<N>;
+  using MP = Map<N>;
+  constexpr int NN = N * N, C = NN * NN, F = MP::F, FP = MP::FP;
+  constexpr int s1 = N, s2 = NN, s3 = N * NN; // digit strides
+  const cptr<double> tab = (cptr<double>)tab_in;
+  __shared__ T U[C], T0p[C], T0n[C], T1p[C], T1n[C], Q1[C], Q2[C];
+  __shared__ T FL[4][F], FR[4][F], G1[2][2][F], G2[2][2][F];
+  __shared__ T V[2][2][NN]; // [d][+/-][i][l]
+  __shared__ T E[2][NN];    // E_e at the x Gauss points
+  __shared__ int need[4][2];
+  // reuse: R0 = U (after S2), R1 = Q1 (after S2), Y0 = T0p, Y1 = T0n,
+  // Z = T1p, the result = T1n
+  T *R0 = U, *R1 = Q1, *Y0 = T0p, *Y1 = T0n, *Z = T1p, *O = T1n;
+
+  for (long long cell = blockIdx.x; cell < mesh.n_cells; cell += gridDim.x)
+    {
+      const long long xc = cell % n_x_cells;
+      long long nbl[4], nbr[4];
+      int cc[4];
+      {
+        unsigned rem = (unsigned)cell;
+#pragma unroll
+        for (int d = 0; d < 4; ++d)
+          {
+            const unsigned n = (unsigned)mesh.n[d];
+            const unsigned q = rem / n;
+            cc[d] = (int)(rem - q * n);
+            rem = q;
+            const long long st = mesh.stride[d];
+            nbl[d] = cc[d] == 0 ? cell + (long long)(n - 1) * st : cell - st;
+            nbr[d] =
+              cc[d] == (int)n - 1 ? cell - (long long)(n - 1) * st : cell + st;
+          }
+      }
+      // S0: the cell, E at the x Gauss points, the v_d factors
+      for (int t = threadIdx.x; t < C; t += blockDim.x)
+        U[t] = src[cell * C + t];
+      for (int t = threadIdx.x; t < 2 * NN; t += blockDim.x)
+        {
+          const int e = t / NN, q = t % NN;
+          E[e][q] = (T)efield[(xc * NN + q) * 2 + e];
+        }
+      for (int t = threadIdx.x; t < 4 * NN; t += blockDim.x)
+        {
+          const int d = t / (2 * NN), sg = (t / NN) % 2, r = t % NN;
+          const int i = r / N, l = r % N;
+          const double lo = mesh.lo[2 + d] + cc[2 + d] * mesh.h[2 + d];
+          double acc = 0.0;
+#pragma unroll
+          for (int q = 0; q < N; ++q)
+            {
+              const double v = lo + tab[TB::qp + q] * mesh.h[2 + d];
+              const double w = sg == 0 ? fmax(v, 0.0) : fmin(v, 0.0);
+              acc += tab[TB::Si + i * N + q] * w * tab[TB::S + q * N + l];
+            }
+          V[d][sg][r] = (T)acc;
+        }
+      __syncthreads();
+      // which faces carry inflow: x_d by the sign of v_d over the cell, v_e
+      // by the signs of E_e at the x Gauss points
+      if (threadIdx.x < 4)
+        {
+          const int d = threadIdx.x;
+          bool pos = false, neg = false;
+          if (d < 2)
+            {
+              const double lo = mesh.lo[2 + d] + cc[2 + d] * mesh.h[2 + d];
+              pos = lo + tab[TB::qp + N - 1] * mesh.h[2 + d] > 0.0;
+              neg = lo + tab[TB::qp] * mesh.h[2 + d] < 0.0;
+            }
+          else
+            for (int q = 0; q < NN; ++q)
+              {
+                pos = pos || E[d - 2][q] >= (T)0;
+                neg = neg || E[d - 2][q] < (T)0;
+              }
+          need[d][0] = pos;
+          need[d][1] = neg;
+        }
+      __syncthreads();
+#pragma unroll
+      for (int d = 0; d < 4; ++d)
+#pragma unroll
+        for (int side = 0; side < 2; ++side)
+          {
+            if (!need[d][side])
+              continue;
+            const long long nb = side == 0 ? nbl[d] : nbr[d];
+            const int node = side == 0 ? N - 1 : 0;
+            const int sd = d == 0 ? 1 : d == 1 ? s1 : d == 2 ? s2 : s3;
+            for (int f = threadIdx.x; f < F; f += blockDim.x)
+              {
+                const T v = src[nb * C + line_base<N>(f, d) + node * sd];
+                if (side == 0)
+                  FL[d][f] = v;
+                else
+                  FR[d][f] = v;
+              }
+          }
+      __syncthreads();
+      // S1: x0 operators and the x0 transform (i0 uniform), x1 operators
+      // (i1 uniform), the x0 transform of the v faces
+      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+        {
+          const int i = uniform(e / FP), l = e % FP;
+          if (l >= F)
+            continue;
+          const int b = line_base<N>(l, 0);
+          T u[N];
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            u[j] = U[b + j];
+          if (need[0][0])
+            {
+              T a = (T)tab[TB::bp(0) + i] * FL[0][l];
+#pragma unroll
+              for (int j = 0; j < N; ++j)
+                a += (T)tab[TB::Ap(0) + i * N + j] * u[j];
+              T0p[b + i] = a;
+            }
+          if (need[0][1])
+            {
+              T a = (T)tab[TB::bn(0) + i] * FR[0][l];
+#pragma unroll
+              for (int j = 0; j < N; ++j)
+                a += (T)tab[TB::An(0) + i * N + j] * u[j];
+              T0n[b + i] = a;
+            }
+          T a = 0;
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            a += (T)tab[TB::S + i * N + j] * u[j];
+          Q1[b + i] = a;
+        }
+      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+        {
+          const int i = uniform(e / FP), l = e % FP;
+          if (l >= F)
+            continue;
+          const int b = line_base<N>(l, 1);
+          T u[N];
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            u[j] = U[b + j * s1];
+          if (need[1][0])
+            {
+              T a = (T)tab[TB::bp(1) + i] * FL[1][l];
+#pragma unroll
+              for (int j = 0; j < N; ++j)
+                a += (T)tab[TB::Ap(1) + i * N + j] * u[j];
+              T1p[b + i * s1] = a;
+            }
+          if (need[1][1])
+            {
+              T a = (T)tab[TB::bn(1) + i] * FR[1][l];
+#pragma unroll
+              for (int j = 0; j < N; ++j)
+                a += (T)tab[TB::An(1) + i * N + j] * u[j];
+              T1n[b + i * s1] = a;
+            }
+        }
+      // v faces (digits x0, x1, other v): along x0, i0 uniform; lines are
+      // (face, side, x1, other v)
+      for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
+        {
+          const int i = e / (4 * NN), r = e % (4 * NN);
+          const int fs = r / NN, rest = r % NN; // rest = x1 + N * v_other
+          const int ee = fs / 2, side = fs % 2;
+          if (!need[2 + ee][side])
+            continue;
+          const T *fsrc = side == 0 ? FL[2 + ee] : FR[2 + ee];
+          T a = 0;
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            a += (T)tab[TB::S + i * N + j] * fsrc[j + N * rest];
+          G1[ee][side][i + N * rest] = a;
+        }
+      __syncthreads();
+      // S2: the v_d factors of the x part (i2, i3 uniform), the x1
+      // transform of Q1 and of the v faces (i1 uniform)
+      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+        {
+          const int i = uniform(e / FP), l = e % FP;
+          if (l >= F)
+            continue;
+          const int b = line_base<N>(l, 1);
+          T a = 0;
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            a += (T)tab[TB::S + i * N + j] * Q1[b + j * s1];
+          Q2[b + i * s1] = a;
+        }
+      for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
+        {
+          const int i = e / (4 * NN), r = e % (4 * NN);
+          const int fs = r / NN, rest = r % NN; // rest = x0 + N * v_other
+          const int ee = fs / 2, side = fs % 2;
+          if (!need[2 + ee][side])
+            continue;
+          const int x0 = rest % N, vo = rest / N;
+          T a = 0;
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            a += (T)tab[TB::S + i * N + j] * G1[ee][side][x0 + N * (j + N * vo)];
+          G2[ee][side][x0 + N * (i + N * vo)] = a;
+        }
+      __syncthreads();
+      // (R0, R1 overwrite U and Q1, which S2 has consumed)
+      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+        {
+          const int i = uniform(e / FP), l = e % FP;
+          if (l >= F)
+            continue;
+          const int b = line_base<N>(l, 2);
+          T a = 0;
+          if (need[0][0])
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+              a += V[0][0][i * N + j] * T0p[b + j * s2];
+          if (need[0][1])
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+              a += V[0][1][i * N + j] * T0n[b + j * s2];
+          R0[b + i * s2] = a;
+        }
+      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+        {
+          const int i = uniform(e / FP), l = e % FP;
+          if (l >= F)
+            continue;
+          const int b = line_base<N>(l, 3);
+          T a = 0;
+          if (need[1][0])
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+              a += V[1][0][i * N + j] * T1p[b + j * s3];
+          if (need[1][1])
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+              a += V[1][1][i * N + j] * T1n[b + j * s3];
+          R1[b + i * s3] = a;
+        }
+      __syncthreads();
+      // S3: the v_e operators at the x Gauss points (i2 / i3 uniform), into
+      // Y0, Y1 (T0p, T0n: consumed)
+#pragma unroll
+      for (int ee = 0; ee < 2; ++ee)
+        {
+          const int d = 2 + ee, sd = ee == 0 ? s2 : s3;
+          T *Yd = ee == 0 ? Y0 : Y1;
+          for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+            {
+              const int i = uniform(e / FP), l = e % FP;
+              if (l >= F)
+                continue;
+              const int b = line_base<N>(l, d);
+              const int qx = b % NN; // digits x0, x1 of the line
+              const T c = E[ee][qx];
+              T q[N];
+#pragma unroll
+              for (int j = 0; j < N; ++j)
+                q[j] = Q2[b + j * sd];
+              // face index of the line: (x0, x1, other v) = (qx, v_other)
+              const int f = qx + NN * (b / (ee == 0 ? s3 : s2) % N);
+              T yp = 0, yn = 0;
+              if (need[d][0])
+                {
+                  yp = (T)tab[TB::bp(d) + i] * G2[ee][0][f];
+#pragma unroll
+                  for (int j = 0; j < N; ++j)
+                    yp += (T)tab[TB::Ap(d) + i * N + j] * q[j];
+                }
+              if (need[d][1])
+                {
+                  yn = (T)tab[TB::bn(d) + i] * G2[ee][1][f];
+#pragma unroll
+                  for (int j = 0; j < N; ++j)
+                    yn += (T)tab[TB::An(d) + i * N + j] * q[j];
+                }
+              Yd[b + i * sd] = c * (c >= (T)0 ? yp : yn);
+            }
+        }
+      __syncthreads();
+      // S4: back along x0 (i0 uniform) into Z
+      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+        {
+          const int i = uniform(e / FP), l = e % FP;
+          if (l >= F)
+            continue;
+          const int b = line_base<N>(l, 0);
+          T a = 0;
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            a += (T)tab[TB::Si + i * N + j] * (Y0[b + j] + Y1[b + j]);
+          Z[b + i] = a;
+        }
+      __syncthreads();
+      // S5: back along x1 (i1 uniform), plus the x part, into O
+      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+        {
+          const int i = uniform(e / FP), l = e % FP;
+          if (l >= F)
+            continue;
+          const int b = line_base<N>(l, 1);
+          T a = R0[b + i * s1] + R1[b + i * s1];
+#pragma unroll
+          for (int j = 0; j < N; ++j)
+            a += (T)tab[TB::Si + i * N + j] * Z[b + j * s1];
+          O[b + i * s1] = a;
+        }
+      __syncthreads();
+      // coalesced write
+      for (int t = threadIdx.x; t < C; t += blockDim.x)
+        {
+          const T a = O[t];
+          const long long g = cell * C + t;
+          if constexpr (MODE == 0)
+            dst[g] = a;
+          else
+            {
+              const double so = stage.sol[g];
+              stage.sol[g] = so + stage.b_dt * (double)a;
+              if (stage.ti_out)
+                static_cast<T *>(stage.ti_out)[g] =
+                  (T)(so + stage.a_dt * (double)a);
+            }
+        }
+      __syncthreads(); // the LDS buffers are reused by the next cell
+    }
+}
+
+template <int N, typename T>
+hipError_t launch(int mode, const T *src, T *dst, const double *tab,
+                  const MeshDev &mesh, long long n_x_cells,
+                  const double *efield, const StageDev &stage,
+                  hipStream_t stream)
+{
+  const unsigned grid =
+    (unsigned)std::min<long long>(mesh.n_cells, 256LL * 64);
+  if (mode == 0)
+    hipLaunchKernelGGL((apply_kernel<N, T, 0>), dim3(grid), dim3(256), 0,
+                       stream, src, dst, tab, mesh, n_x_cells, efield, stage);
+  else
+    hipLaunchKernelGGL((apply_kernel<N, T, 1>), dim3(grid), dim3(256), 0,
+                       stream, src, dst, tab, mesh, n_x_cells, efield, stage);
+  return hipGetLastError();
+}
+} // namespace vpk
+
 // ---- diagnostics: mass, ||f||^2, kinetic energy, momentum (3), per cell ----
 __global__ __launch_bounds__(256) void moments_kernel(
   const double *__restrict__ f, MeshDev mesh, NormDev nt, int dim_x,
@@ -576,11 +1385,77 @@ bool vp_poisson_supported(int dim_x, int N, long long n_x_cells)
          (dim_x == 3 && N <= 4);
 }
 
+bool vp_poisson2d_supported(int dim_x, int N, long long n_x_cells)
+{
+  // the direction in LDS: N^2 rows of kPStride doubles (128 KB at N = 4)
+  return dim_x == 2 && N >= 2 && N <= 4 &&
+         n_x_cells <= vlasov::kSolverThreads;
+}
+
+void vp_poisson2d_table(const XDev &xd, int N, double *tab)
+{
+  // L_d (left, self, right) and M_d per direction, from the 1D mass M,
+  // stiffness K and end-point derivatives d0, d1 of the reference cell;
+  // SIPG penalty sigma = k (k + 1) * 2 / h (poisson.h:193-254)
+  const int NN = N * N;
+  for (int d = 0; d < 2; ++d)
+    {
+      const double h = xd.h[d], sg = xd.kk * (2.0 / h);
+      double *Ll = tab + (3 * d) * NN, *Ls = Ll + NN, *Lr = Ls + NN;
+      double *M = tab + (6 + d) * NN;
+      for (int i = 0; i < N; ++i)
+        for (int j = 0; j < N; ++j)
+          {
+            const double di0 = xd.d0[i] / h, di1 = xd.d1[i] / h;
+            const double dj0 = xd.d0[j] / h, dj1 = xd.d1[j] / h;
+            const bool i0 = i == 0, iN = i == N - 1, j0 = j == 0,
+                       jN = j == N - 1;
+            // cell stiffness; the right face (this cell minus), the left
+            // face (this cell plus): sigma [u][v] - {u'}[v] - [u]{v'}
+            Ls[i * N + j] = xd.K[i * N + j] / h +
+                            (iN ? sg * jN - 0.5 * dj1 : 0.0) -
+                            (jN ? 0.5 * di1 : 0.0) +
+                            (i0 ? sg * j0 + 0.5 * dj0 : 0.0) +
+                            (j0 ? 0.5 * di0 : 0.0);
+            Lr[i * N + j] = (iN ? -sg * j0 - 0.5 * dj0 : 0.0) +
+                            (j0 ? 0.5 * di1 : 0.0);
+            Ll[i * N + j] = (i0 ? -(sg * jN - 0.5 * dj1) : 0.0) -
+                            (jN ? 0.5 * di0 : 0.0);
+            M[i * N + j] = xd.M[i * N + j] * h;
+          }
+    }
+}
+
 hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
                       const double *guess, double *efield, double *work,
                       double rel_tol, int max_it, long long *stats,
-                      hipStream_t stream)
+                      hipStream_t stream, const double *tab2d,
+                      double *residual_out)
 {
+  if (residual_out &&
+      !(tab2d && vp_poisson2d_supported(xd.dim, N, xd.n_cells)))
+    return hipErrorInvalidValue;
+  if (tab2d && vp_poisson2d_supported(xd.dim, N, xd.n_cells))
+    {
+      const size_t lds =
+        (size_t)(vlasov::kPStride * N * N + 32) * sizeof(double);
+#define HD_VP2(NN_)                                                            \
+  if (N == NN_)                                                                \
+    {                                                                          \
+      hipError_t e = hipFuncSetAttribute(                                      \
+        reinterpret_cast<const void *>(&vlasov::poisson2d_kernel<NN_>),        \
+        hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                 \
+      if (e != hipSuccess)                                                     \
+        return e;                                                              \
+      hipLaunchKernelGGL((vlasov::poisson2d_kernel<NN_>), dim3(1),             \
+                         dim3(vlasov::kSolverThreads), lds, stream, xd, tab2d, \
+                         rho, phi, guess, efield, work, rel_tol, max_it,       \
+                         stats, residual_out);                                 \
+      return hipGetLastError();                                                \
+    }
+      HD_VP2(2) HD_VP2(3) HD_VP2(4)
+#undef HD_VP2
+    }
   const long long dofs = xd.n_cells * ipow(N, xd.dim);
   const size_t lds = (size_t)(dofs + vlasov::kSolverThreads) * sizeof(double);
 #define HD_VP(DX, NN_)                                                         \
@@ -615,5 +1490,46 @@ hipError_t vp_moments(const double *f, const MeshDev &mesh, const NormDev &nt,
   hipLaunchKernelGGL(vlasov::moments_kernel, dim3((unsigned)mesh.n_cells),
                      dim3(256), lds, stream, f, mesh, nt, dim_x, partial);
   return hipGetLastError();
+}
+
+bool vp_apply_supported(int dim_x, int dim_v, int N)
+{
+  return dim_x == 2 && dim_v == 2 && N >= 2 && N <= 5;
+}
+
+int vp_apply_table_size(int N)
+{
+  switch (N)
+    {
+      case 2:
+        return vlasov::vpk::Tab<2>::size;
+      case 3:
+        return vlasov::vpk::Tab<3>::size;
+      case 4:
+        return vlasov::vpk::Tab<4>::size;
+      case 5:
+        return vlasov::vpk::Tab<5>::size;
+      default:
+        return 0;
+    }
+}
+
+hipError_t vp_apply(int N, int mode, int precision, const void *src, void *dst,
+                    const double *tab, const MeshDev &mesh,
+                    long long n_x_cells, const double *efield,
+                    const StageDev &stage, hipStream_t stream)
+{
+#define HD_VPK(NN_)                                                            \
+  if (N == NN_)                                                                \
+    return precision == 0                                                      \
+             ? vlasov::vpk::launch<NN_, double>(mode, (const double *)src,             \
+                                        (double *)dst, tab, mesh, n_x_cells,   \
+                                        efield, stage, stream)                 \
+             : vlasov::vpk::launch<NN_, float>(mode, (const float *)src,               \
+                                       (float *)dst, tab, mesh, n_x_cells,     \
+                                       efield, stage, stream);
+  HD_VPK(2) HD_VPK(3) HD_VPK(4) HD_VPK(5)
+#undef HD_VPK
+  return hipErrorInvalidValue;
 }
 } // namespace hd

@@ -17,8 +17,9 @@ Runs (any subset, in this order):
   fp32          FP32-mixed apply, CG from the previous phi (the method
                 without the learned part)
   fp32-learned  FP32-mixed apply, CG from the combination of the previous
-                phi and the network's guess; the network trains online on
-                the run's own (rho, phi) pairs
+                phi and the network's correction for its residual; the
+                network trains online on the run's own solves
+  fp64-learned  the same with the FP64 apply
 
 Reported per run: wall time per step, CG iterations per solve, the share
 of the step spent on the field update (density, network, CG, training:
@@ -73,8 +74,8 @@ def run(H, torch, name, args, ref_f=None):
     n, k = args.cells, args.degree
     mf = H.MatrixFree(2, 2, k, [n] * 4, [0, 0, -6, -6], [L, L, 6, 6])
     op = H.VlasovPoissonOperation(mf, poisson_tol=1e-10)
-    prec = H.FP64 if name == "fp64" else H.FP32
-    if name == "fp32-learned":
+    prec = H.FP64 if name.startswith("fp64") else H.FP32
+    if name.endswith("-learned"):
         op.enable_learned_guess(seed=20261205, capacity=64, train_every=1,
                                 train_batch=8, train_steps=1,
                                 learning_rate=1e-3, min_pairs=8)
@@ -129,7 +130,7 @@ def run(H, torch, name, args, ref_f=None):
         "mass_drift": (H.vp_moments(mf, f)[0] - m0) / m0,
         "gdof_per_s_stage": 5 * n_steps * mf.n_dofs() / wall / 1e9,
     }
-    if name == "fp32-learned":
+    if name.endswith("-learned"):
         st, guessed, pairs, loss = op.learned_guess_stats()
         r0, rp, w, _ = op.solver_start()
         out.update({"train_steps": st, "guessed_solves": guessed,
@@ -137,7 +138,7 @@ def run(H, torch, name, args, ref_f=None):
                     "last_start_rel_residual": r0,
                     "last_previous_phi_rel_residual": rp,
                     "last_start_weights": list(w)})
-    if ref_f is not None:
+    if ref_f is not None and name != "fp64":
         out["rel_l2_diff_vs_fp64"] = float(
             torch.linalg.vector_norm(f - ref_f) /
             torch.linalg.vector_norm(ref_f))
@@ -167,7 +168,7 @@ def main():
     for name in args.runs.split(","):
         print("run", name, flush=True)
         out, f = run(H, torch, name, args, ref)
-        if name == "fp64":
+        if name == "fp64" and ref is None:
             ref = f
         else:
             del f

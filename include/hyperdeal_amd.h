@@ -247,15 +247,17 @@ int hd_vp_moments(hd_matrix_free *mf, const double *f, double *out,
  * application.h:575-576); for callers that solve many Poisson problems along
  * one trajectory.  A convolutional network (four 3x3 periodic convolutions,
  * 1 -> 32 -> 32 -> 32 -> 1 channels, leaky ReLU 1/8, BF16 weights, FP32
- * accumulation and Adam state) maps the normalised density image (x cells x
- * nodes per direction, a multiple of 16 each; dim_x == 2) to a potential
- * guess; it trains on the operator's own converged (rho, phi) pairs, kept in
- * a ring of `capacity`, with train_steps Adam steps on batches of
- * train_batch every train_every solves once min_pairs are stored
- * (train_every 0: no training).  CG starts from the energy-minimising
- * combination of the previous phi and the guess and still runs to
- * poisson_tol, so the guess changes the iteration count, not the result's
- * tolerance.
+ * accumulation and Adam state) maps the residual of the previous phi,
+ * b - A phi_prev, as a normalised image of the x nodes (x cells x nodes per
+ * direction, a multiple of 16 each; dim_x == 2, the 2D tensor-product
+ * solver: k <= 3, at most 1024 x cells) to the shape of the correction CG
+ * will find.  It trains on the operator's own converged solves, pairs
+ * (b - A phi_prev, phi - phi_prev) kept in a ring of `capacity`, with
+ * train_steps Adam steps on batches of train_batch every train_every solves
+ * once min_pairs are stored (train_every 0: no training).  CG starts from
+ * the energy-minimising combination of phi_prev and the guess and still runs
+ * to poisson_tol, so the guess changes the iteration count, not the
+ * result's tolerance.
  */
 int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
                                int capacity, int train_every, int train_batch,

@@ -187,6 +187,20 @@ def dofs_to_image(u, n0, n1, N):
     return (d / rms if rms > 0 else 0 * d).astype(np.float32), rms
 
 
-def image_to_dofs(img, n0, n1, N, scale):
-    img = np.asarray(img, dtype=np.float64).reshape(n1, N, n0, N)
+def image_to_dofs(img, n0, n1, N, scale, continuous=False):
+    """image -> 2D x-space nodal values (scale * image); continuous: the
+    end nodes of neighbouring cells (one point, two pixels, four at a
+    corner) get the mean of their pixels."""
+    img = np.asarray(img, dtype=np.float64)
+    if continuous and N > 1:
+        img = img.copy()
+        for axis in (1, 0):
+            a = np.moveaxis(img, axis, 0)
+            last = np.arange(N - 1, a.shape[0], N)
+            first = (last + 1) % a.shape[0]
+            m = 0.5 * (a[last] + a[first])
+            a[last] = m
+            a[first] = m
+            img = np.moveaxis(a, 0, axis)
+    img = img.reshape(n1, N, n0, N)
     return (scale * img.transpose(0, 2, 1, 3)).reshape(-1)
