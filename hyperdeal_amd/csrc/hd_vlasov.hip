@@ -880,6 +880,15 @@ struct Tab
   static constexpr int size = qp + N;
 };
 
+// a workgroup barrier that orders LDS only: the next cell's global loads
+// stay in flight across it (__syncthreads would wait for them)
+__device__ inline void lds_sync()
+{
+  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
+  __builtin_amdgcn_s_barrier();
+  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
+}
+
 // Work mapping of every contraction step: element e = i * FP + l with i
 // the output value of the contracted digit (uniform over a wave: FP is a
 // multiple of 64) and l the line (the other three digits, ascending), so
@@ -930,15 +939,71 @@ __global__ __launch_bounds__(256) void apply_kernel(
   __shared__ T FL[4][F], FR[4][F], G1[2][2][F], G2[2][2][F];
   __shared__ T V[2][2][NN]; // [d][+/-][i][l]
   __shared__ T E[2][NN];    // E_e at the x Gauss points
-  __shared__ int need[4][2];
+  // the next cell's values, field and eight inflow faces (both sides of
+  // every direction) travel into registers while this cell computes
+  constexpr int PU = (C + 255) / 256, PE = (2 * NN + 255) / 256;
+  constexpr int PF = (8 * F + 255) / 256;
+  T pu[PU], pe[PE], pf[PF];
+  auto prefetch = [&](long long cell) {
+#pragma unroll
+    for (int k = 0; k < PU; ++k)
+      {
+        const int t = threadIdx.x + 256 * k;
+        if (t < C)
+          pu[k] = src[cell * C + t];
+      }
+    const long long xc = cell % n_x_cells;
+#pragma unroll
+    for (int k = 0; k < PE; ++k)
+      {
+        const int t = threadIdx.x + 256 * k;
+        if (t < 2 * NN)
+          pe[k] = (T)efield[(xc * NN + t % NN) * 2 + t / NN];
+      }
+    long long nbl[4], nbr[4];
+    {
+      unsigned rem = (unsigned)cell;
+#pragma unroll
+      for (int d = 0; d < 4; ++d)
+        {
+          const unsigned n = (unsigned)mesh.n[d];
+          const unsigned q = rem / n;
+          const unsigned c = rem - q * n;
+          rem = q;
+          const long long st = mesh.stride[d];
+          nbl[d] = c == 0 ? cell + (long long)(n - 1) * st : cell - st;
+          nbr[d] = c == n - 1 ? cell - (long long)(n - 1) * st : cell + st;
+        }
+    }
+#pragma unroll
+    for (int k = 0; k < PF; ++k)
+      {
+        const int t = threadIdx.x + 256 * k;
+        if (t < 8 * F)
+          {
+            const int ds = t / F, f = t % F, d = ds / 2, side = ds % 2;
+            long long nb = 0;
+            int sd = 1;
+#pragma unroll
+            for (int dd = 0; dd < 4; ++dd)
+              if (dd == d)
+                {
+                  nb = side == 0 ? nbl[dd] : nbr[dd];
+                  sd = dd == 0 ? 1 : dd == 1 ? s1 : dd == 2 ? s2 : s3;
+                }
+            const int node = side == 0 ? N - 1 : 0;
+            pf[k] = src[nb * C + line_base<N>(f, d) + node * sd];
+          }
+      }
+  };
   // reuse: R0 = U (after S2), R1 = Q1 (after S2), Y0 = T0p, Y1 = T0n,
   // Z = T1p, the result = T1n
   T *R0 = U, *R1 = Q1, *Y0 = T0p, *Y1 = T0n, *Z = T1p, *O = T1n;
 
+  if ((long long)blockIdx.x < mesh.n_cells)
+    prefetch(blockIdx.x);
   for (long long cell = blockIdx.x; cell < mesh.n_cells; cell += gridDim.x)
     {
-      const long long xc = cell % n_x_cells;
-      long long nbl[4], nbr[4];
       int cc[4];
       {
         unsigned rem = (unsigned)cell;
@@ -949,19 +1014,35 @@ __global__ __launch_bounds__(256) void apply_kernel(
             const unsigned q = rem / n;
             cc[d] = (int)(rem - q * n);
             rem = q;
-            const long long st = mesh.stride[d];
-            nbl[d] = cc[d] == 0 ? cell + (long long)(n - 1) * st : cell - st;
-            nbr[d] =
-              cc[d] == (int)n - 1 ? cell - (long long)(n - 1) * st : cell + st;
           }
       }
-      // S0: the cell, E at the x Gauss points, the v_d factors
-      for (int t = threadIdx.x; t < C; t += blockDim.x)
-        U[t] = src[cell * C + t];
-      for (int t = threadIdx.x; t < 2 * NN; t += blockDim.x)
+      // S0: the prefetched cell, field and faces into LDS; the v_d factors
+#pragma unroll
+      for (int k = 0; k < PU; ++k)
         {
-          const int e = t / NN, q = t % NN;
-          E[e][q] = (T)efield[(xc * NN + q) * 2 + e];
+          const int t = threadIdx.x + 256 * k;
+          if (t < C)
+            U[t] = pu[k];
+        }
+#pragma unroll
+      for (int k = 0; k < PE; ++k)
+        {
+          const int t = threadIdx.x + 256 * k;
+          if (t < 2 * NN)
+            E[t / NN][t % NN] = pe[k];
+        }
+#pragma unroll
+      for (int k = 0; k < PF; ++k)
+        {
+          const int t = threadIdx.x + 256 * k;
+          if (t < 8 * F)
+            {
+              const int ds = t / F, f = t % F;
+              if (ds % 2 == 0)
+                FL[ds / 2][f] = pf[k];
+              else
+                FR[ds / 2][f] = pf[k];
+            }
         }
       for (int t = threadIdx.x; t < 4 * NN; t += blockDim.x)
         {
@@ -978,49 +1059,13 @@ __global__ __launch_bounds__(256) void apply_kernel(
             }
           V[d][sg][r] = (T)acc;
         }
-      __syncthreads();
-      // which faces carry inflow: x_d by the sign of v_d over the cell, v_e
-      // by the signs of E_e at the x Gauss points
-      if (threadIdx.x < 4)
-        {
-          const int d = threadIdx.x;
-          bool pos = false, neg = false;
-          if (d < 2)
-            {
-              const double lo = mesh.lo[2 + d] + cc[2 + d] * mesh.h[2 + d];
-              pos = lo + tab[TB::qp + N - 1] * mesh.h[2 + d] > 0.0;
-              neg = lo + tab[TB::qp] * mesh.h[2 + d] < 0.0;
-            }
-          else
-            for (int q = 0; q < NN; ++q)
-              {
-                pos = pos || E[d - 2][q] >= (T)0;
-                neg = neg || E[d - 2][q] < (T)0;
-              }
-          need[d][0] = pos;
-          need[d][1] = neg;
-        }
-      __syncthreads();
-#pragma unroll
-      for (int d = 0; d < 4; ++d)
-#pragma unroll
-        for (int side = 0; side < 2; ++side)
-          {
-            if (!need[d][side])
-              continue;
-            const long long nb = side == 0 ? nbl[d] : nbr[d];
-            const int node = side == 0 ? N - 1 : 0;
-            const int sd = d == 0 ? 1 : d == 1 ? s1 : d == 2 ? s2 : s3;
-            for (int f = threadIdx.x; f < F; f += blockDim.x)
-              {
-                const T v = src[nb * C + line_base<N>(f, d) + node * sd];
-                if (side == 0)
-                  FL[d][f] = v;
-                else
-                  FR[d][f] = v;
-              }
-          }
-      __syncthreads();
+      lds_sync();
+      // the next cell's loads go out now and land during the arithmetic
+      {
+        const long long next = cell + gridDim.x;
+        if (next < mesh.n_cells)
+          prefetch(next);
+      }
       // S1: x0 operators and the x0 transform (i0 uniform), x1 operators
       // (i1 uniform), the x0 transform of the v faces
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
@@ -1033,16 +1078,14 @@ __global__ __launch_bounds__(256) void apply_kernel(
 #pragma unroll
           for (int j = 0; j < N; ++j)
             u[j] = U[b + j];
-          if (need[0][0])
-            {
+          {
               T a = (T)tab[TB::bp(0) + i] * FL[0][l];
 #pragma unroll
               for (int j = 0; j < N; ++j)
                 a += (T)tab[TB::Ap(0) + i * N + j] * u[j];
               T0p[b + i] = a;
             }
-          if (need[0][1])
-            {
+          {
               T a = (T)tab[TB::bn(0) + i] * FR[0][l];
 #pragma unroll
               for (int j = 0; j < N; ++j)
@@ -1065,16 +1108,14 @@ __global__ __launch_bounds__(256) void apply_kernel(
 #pragma unroll
           for (int j = 0; j < N; ++j)
             u[j] = U[b + j * s1];
-          if (need[1][0])
-            {
+          {
               T a = (T)tab[TB::bp(1) + i] * FL[1][l];
 #pragma unroll
               for (int j = 0; j < N; ++j)
                 a += (T)tab[TB::Ap(1) + i * N + j] * u[j];
               T1p[b + i * s1] = a;
             }
-          if (need[1][1])
-            {
+          {
               T a = (T)tab[TB::bn(1) + i] * FR[1][l];
 #pragma unroll
               for (int j = 0; j < N; ++j)
@@ -1089,8 +1130,6 @@ __global__ __launch_bounds__(256) void apply_kernel(
           const int i = e / (4 * NN), r = e % (4 * NN);
           const int fs = r / NN, rest = r % NN; // rest = x1 + N * v_other
           const int ee = fs / 2, side = fs % 2;
-          if (!need[2 + ee][side])
-            continue;
           const T *fsrc = side == 0 ? FL[2 + ee] : FR[2 + ee];
           T a = 0;
 #pragma unroll
@@ -1098,7 +1137,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
             a += (T)tab[TB::S + i * N + j] * fsrc[j + N * rest];
           G1[ee][side][i + N * rest] = a;
         }
-      __syncthreads();
+      lds_sync();
       // S2: the v_d factors of the x part (i2, i3 uniform), the x1
       // transform of Q1 and of the v faces (i1 uniform)
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
@@ -1118,8 +1157,6 @@ __global__ __launch_bounds__(256) void apply_kernel(
           const int i = e / (4 * NN), r = e % (4 * NN);
           const int fs = r / NN, rest = r % NN; // rest = x0 + N * v_other
           const int ee = fs / 2, side = fs % 2;
-          if (!need[2 + ee][side])
-            continue;
           const int x0 = rest % N, vo = rest / N;
           T a = 0;
 #pragma unroll
@@ -1127,7 +1164,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
             a += (T)tab[TB::S + i * N + j] * G1[ee][side][x0 + N * (j + N * vo)];
           G2[ee][side][x0 + N * (i + N * vo)] = a;
         }
-      __syncthreads();
+      lds_sync();
       // (R0, R1 overwrite U and Q1, which S2 has consumed)
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
         {
@@ -1136,11 +1173,9 @@ __global__ __launch_bounds__(256) void apply_kernel(
             continue;
           const int b = line_base<N>(l, 2);
           T a = 0;
-          if (need[0][0])
 #pragma unroll
             for (int j = 0; j < N; ++j)
               a += V[0][0][i * N + j] * T0p[b + j * s2];
-          if (need[0][1])
 #pragma unroll
             for (int j = 0; j < N; ++j)
               a += V[0][1][i * N + j] * T0n[b + j * s2];
@@ -1153,17 +1188,15 @@ __global__ __launch_bounds__(256) void apply_kernel(
             continue;
           const int b = line_base<N>(l, 3);
           T a = 0;
-          if (need[1][0])
 #pragma unroll
             for (int j = 0; j < N; ++j)
               a += V[1][0][i * N + j] * T1p[b + j * s3];
-          if (need[1][1])
 #pragma unroll
             for (int j = 0; j < N; ++j)
               a += V[1][1][i * N + j] * T1n[b + j * s3];
           R1[b + i * s3] = a;
         }
-      __syncthreads();
+      lds_sync();
       // S3: the v_e operators at the x Gauss points (i2 / i3 uniform), into
       // Y0, Y1 (T0p, T0n: consumed)
 #pragma unroll
@@ -1185,25 +1218,18 @@ __global__ __launch_bounds__(256) void apply_kernel(
                 q[j] = Q2[b + j * sd];
               // face index of the line: (x0, x1, other v) = (qx, v_other)
               const int f = qx + NN * (b / (ee == 0 ? s3 : s2) % N);
-              T yp = 0, yn = 0;
-              if (need[d][0])
-                {
-                  yp = (T)tab[TB::bp(d) + i] * G2[ee][0][f];
+              T yp = (T)tab[TB::bp(d) + i] * G2[ee][0][f];
+              T yn = (T)tab[TB::bn(d) + i] * G2[ee][1][f];
 #pragma unroll
-                  for (int j = 0; j < N; ++j)
-                    yp += (T)tab[TB::Ap(d) + i * N + j] * q[j];
-                }
-              if (need[d][1])
+              for (int j = 0; j < N; ++j)
                 {
-                  yn = (T)tab[TB::bn(d) + i] * G2[ee][1][f];
-#pragma unroll
-                  for (int j = 0; j < N; ++j)
-                    yn += (T)tab[TB::An(d) + i * N + j] * q[j];
+                  yp += (T)tab[TB::Ap(d) + i * N + j] * q[j];
+                  yn += (T)tab[TB::An(d) + i * N + j] * q[j];
                 }
               Yd[b + i * sd] = c * (c >= (T)0 ? yp : yn);
             }
         }
-      __syncthreads();
+      lds_sync();
       // S4: back along x0 (i0 uniform) into Z
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
         {
@@ -1217,7 +1243,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
             a += (T)tab[TB::Si + i * N + j] * (Y0[b + j] + Y1[b + j]);
           Z[b + i] = a;
         }
-      __syncthreads();
+      lds_sync();
       // S5: back along x1 (i1 uniform), plus the x part, into O
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
         {
@@ -1231,7 +1257,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
             a += (T)tab[TB::Si + i * N + j] * Z[b + j * s1];
           O[b + i * s1] = a;
         }
-      __syncthreads();
+      lds_sync();
       // coalesced write
       for (int t = threadIdx.x; t < C; t += blockDim.x)
         {
@@ -1248,7 +1274,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
                   (T)(so + stage.a_dt * (double)a);
             }
         }
-      __syncthreads(); // the LDS buffers are reused by the next cell
+      lds_sync(); // the LDS buffers are reused by the next cell
     }
 }
 
