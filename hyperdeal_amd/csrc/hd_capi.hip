@@ -45,14 +45,31 @@ int choose_algorithm(const hd_advection *op, int precision)
     }
 }
 
-// rho from f (FP64, or the FP32 stage input of the mixed form), the
-// potential solve (with the learned guess when it is on), grad phi
-int vp_update(hd_advection *op, const double *f, const float *f32,
-              hipStream_t stream)
+// where a Vlasov-Poisson stage takes its density from
+enum VpDensity
 {
-  hipError_t e = hd::vp_density(f, f32, op->vpdev, op->n_x_cells,
-                                op->n_v_cells, op->partial,
-                                op->density_chunks, op->rho, stream);
+  kDensityInput = 0,    // the stage input (FP64, or FP32 in the mixed form)
+  kDensitySolution = 1, // the FP64 solution (first stage of an LSRK step)
+  kDensityPartials = 2  // the partials the previous stage's kernel wrote
+};
+
+// the phase-space kernel runs (not the quadrature kernel)
+bool vp_kernel_active(const hd_advection *op)
+{
+  return op->vp && op->vtab && op->algorithm != HD_ALGO_QUADRATURE;
+}
+
+// rho (from f, or from the partials of the previous stage), the potential
+// solve (with the learned guess when it is on), grad phi
+int vp_update(hd_advection *op, const double *f, const float *f32,
+              hipStream_t stream, bool from_partials = false)
+{
+  hipError_t e =
+    from_partials
+      ? hd::vp_density_from_partials(op->rho_part, op->n_v_cells,
+                                     op->n_x_dofs, op->rho, stream)
+      : hd::vp_density(f, f32, op->vpdev, op->n_x_cells, op->n_v_cells,
+                       op->partial, op->density_chunks, op->rho, stream);
   const double *guess = nullptr;
   if (e == hipSuccess && op->learned)
     e = hd_abi::learned_before_solve(*op, &guess, stream);
@@ -65,21 +82,29 @@ int vp_update(hd_advection *op, const double *f, const float *f32,
   return e == hipSuccess ? HD_OK : hip_fail(e, "Vlasov-Poisson field update");
 }
 
+// density: where a Vlasov-Poisson operator takes rho from (VpDensity);
+// sol: the FP64 solution (kDensitySolution)
 int run_operator(hd_advection *op, int mode, int precision, const void *src,
                  void *dst, const hd::StageDev &stage, double time,
-                 hipStream_t stream)
+                 hipStream_t stream, int density = kDensityInput,
+                 const double *sol = nullptr)
 {
   hd_matrix_free *mf = op->mf;
   if (op->vp && op->vp_auto)
     {
-      const int rc =
-        precision == HD_FP64
-          ? vp_update(op, static_cast<const double *>(src), nullptr, stream)
-          : vp_update(op, nullptr, static_cast<const float *>(src), stream);
+      int rc;
+      if (density == kDensityPartials)
+        rc = vp_update(op, nullptr, nullptr, stream, true);
+      else if (density == kDensitySolution)
+        rc = vp_update(op, sol, nullptr, stream);
+      else if (precision == HD_FP64)
+        rc = vp_update(op, static_cast<const double *>(src), nullptr, stream);
+      else
+        rc = vp_update(op, nullptr, static_cast<const float *>(src), stream);
       if (rc != HD_OK)
         return rc;
     }
-  if (op->vp && op->vtab && op->algorithm != HD_ALGO_QUADRATURE)
+  if (vp_kernel_active(op))
     {
       const hipError_t e =
         hd::vp_apply(mf->nd, mode, precision == HD_FP64 ? 0 : 1, src, dst,
@@ -400,7 +425,8 @@ int hd_advection_destroy(hd_advection *op)
   for (void *p : {(void *)op->coef64, (void *)op->coef32, (void *)op->efield,
                   (void *)op->rho, (void *)op->phi, (void *)op->work,
                   (void *)op->partial, (void *)op->stats,
-                  (void *)op->ptab, (void *)op->vtab, op->split_scratch})
+                  (void *)op->ptab, (void *)op->vtab, (void *)op->rho_part,
+                  op->split_scratch})
     if (p)
       (void)hipFree(p);
   hd_abi::learned_release(op->learned);
@@ -543,10 +569,16 @@ int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
       std::copy(t.S.begin(), t.S.begin() + nd * nd, o);
       std::copy(t.Sinv.begin(), t.Sinv.begin() + nd * nd, o + nd * nd);
       std::copy(t.qpoints.begin(), t.qpoints.begin() + nd, o + 2 * nd * nd);
+      std::copy(op->vpdev.wv, op->vpdev.wv + nd * nd,
+                o + 2 * nd * nd + nd);
       e = hipMalloc(&op->vtab, tab.size() * sizeof(double));
       if (e == hipSuccess)
         e = hipMemcpy(op->vtab, tab.data(), tab.size() * sizeof(double),
                       hipMemcpyHostToDevice);
+      // density partials of a stage's output: one per v cell and x node
+      if (e == hipSuccess)
+        e = hipMalloc(&op->rho_part, (size_t)op->n_v_cells * op->n_x_dofs *
+                                       sizeof(double));
     }
   if (e == hipSuccess && hd::vp_poisson2d_supported(mf->dim_x, nd, ncx))
     {
@@ -722,7 +754,8 @@ namespace
 // scratch: a vector the last stage may use (the free LSRK register)
 int lsrk_stage(hd_advection *op, int precision, const void *ti_in,
                void *ti_out, void *scratch, double *sol, double time,
-               double a_dt, double b_dt, void *stream);
+               double a_dt, double b_dt, void *stream,
+               int density = kDensityInput, bool write_partials = false);
 } // namespace
 
 int hd_lsrk_stage(hd_advection *op, int precision, const void *ti_in,
@@ -737,7 +770,8 @@ namespace
 {
 int lsrk_stage(hd_advection *op, int precision, const void *ti_in,
                void *ti_out, void *scratch, double *sol, double time,
-               double a_dt, double b_dt, void *stream)
+               double a_dt, double b_dt, void *stream, int density,
+               bool write_partials)
 {
   if (!op || !ti_in || !sol)
     return fail(HD_ERR_INVALID, "hd_lsrk_stage: null argument");
@@ -751,8 +785,9 @@ int lsrk_stage(hd_advection *op, int precision, const void *ti_in,
   stage.sol = sol;
   stage.ti_out = ti_out;
   stage.scratch = ti_out ? nullptr : scratch;
+  stage.rho_part = write_partials && ti_out ? op->rho_part : nullptr;
   return run_operator(op, 1, precision, ti_in, nullptr, stage, time,
-                      as_stream(stream));
+                      as_stream(stream), density, sol);
 }
 } // namespace
 
@@ -792,9 +827,16 @@ int hd_lsrk_step(hd_advection *op, const char *type, int precision,
       if (s > 0)
         sum_b += b[s - 1];
       const bool last = s == n - 1;
+      // Vlasov-Poisson with the phase-space kernel: the first stage takes
+      // rho from the FP64 solution, the others from the partials the
+      // previous stage wrote of its unrounded output (no pass over f, and
+      // in the mixed form no FP32 rounding in the Poisson right-hand side)
+      const bool fused = vp_kernel_active(op) && op->vp_auto;
+      const int density =
+        !fused ? kDensityInput : (s == 0 ? kDensitySolution : kDensityPartials);
       rc = lsrk_stage(op, precision, in, last ? nullptr : out, out, sol,
-                      time + c * dt, last ? 0.0 : a[s] * dt, b[s] * dt,
-                      stream);
+                      time + c * dt, last ? 0.0 : a[s] * dt, b[s] * dt, stream,
+                      density, fused && !last);
       if (rc != HD_OK)
         return rc;
       std::swap(in, out);

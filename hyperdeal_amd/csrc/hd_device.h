@@ -44,6 +44,9 @@ struct StageDev
   double *sol;   // FP64 solution, updated in place
   void *ti_out;  // next stage input (Number), may be null (last stage)
   void *scratch; // a free Number vector for the last stage, or null
+  // Vlasov-Poisson stages: the kernel writes the density partials of
+  // ti_out here (one per v cell and x node), or null
+  double *rho_part;
 };
 
 // blocks are dispatched round-robin over the 8 XCDs; give each XCD a
@@ -247,6 +250,10 @@ bool vp_poisson2d_supported(int dim_x, int N, long long n_x_cells);
 // directions for both signs, S, S^-1, Gauss points), precision 0 FP64, 1
 // FP32-mixed; mode 0 apply, 1 LSRK stage
 bool vp_apply_supported(int dim_x, int dim_v, int N);
+// rho = the sum over v cells of the partials a phase-space stage wrote
+hipError_t vp_density_from_partials(const double *part, long long n_v_cells,
+                                    long long n_x_dofs, double *rho,
+                                    hipStream_t stream);
 int vp_apply_table_size(int N);
 hipError_t vp_apply(int N, int mode, int precision, const void *src, void *dst,
                     const double *tab, const MeshDev &mesh,

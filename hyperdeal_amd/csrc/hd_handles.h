@@ -104,6 +104,7 @@ struct hd_advection
   long long *stats = nullptr; // solver statistics, see hd::vp_poisson
   double *ptab = nullptr;     // 2D SIPG table of the fast solver, or null
   double *vtab = nullptr;     // tables of the 2+2 phase-space apply, or null
+  double *rho_part = nullptr; // density partials written by a stage
   hd_learned_guess *learned = nullptr;
 };
 

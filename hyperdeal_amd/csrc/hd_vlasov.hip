@@ -877,7 +877,8 @@ struct Tab
   static constexpr int An(int d) { return d * PER + NN + N; }
   static constexpr int bn(int d) { return d * PER + 2 * NN + N; }
   static constexpr int S = 4 * PER, Si = S + NN, qp = Si + NN;
-  static constexpr int size = qp + N;
+  static constexpr int wv = qp + N;          // JxW_v of the v nodes (N^2)
+  static constexpr int size = wv + NN;
 };
 
 // a workgroup barrier that orders LDS only: the next cell's global loads
@@ -939,6 +940,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
   __shared__ T FL[4][F], FR[4][F], G1[2][2][F], G2[2][2][F];
   __shared__ T V[2][2][NN]; // [d][+/-][i][l]
   __shared__ T E[2][NN];    // E_e at the x Gauss points
+  __shared__ double RD[MODE == 1 ? C : 1]; // the stage value, for rho
   // the next cell's values, field and eight inflow faces (both sides of
   // every direction) travel into registers while this cell computes
   constexpr int PU = (C + 255) / 256, PE = (2 * NN + 255) / 256;
@@ -1258,7 +1260,9 @@ __global__ __launch_bounds__(256) void apply_kernel(
           O[b + i * s1] = a;
         }
       lds_sync();
-      // coalesced write
+      // coalesced write; with rho_part set, the next stage input also goes
+      // to LDS unrounded, for its density
+      const bool dens = MODE == 1 && stage.rho_part && stage.ti_out;
       for (int t = threadIdx.x; t < C; t += blockDim.x)
         {
           const T a = O[t];
@@ -1270,11 +1274,30 @@ __global__ __launch_bounds__(256) void apply_kernel(
               const double so = stage.sol[g];
               stage.sol[g] = so + stage.b_dt * (double)a;
               if (stage.ti_out)
-                static_cast<T *>(stage.ti_out)[g] =
-                  (T)(so + stage.a_dt * (double)a);
+                {
+                  const double tn = so + stage.a_dt * (double)a;
+                  static_cast<T *>(stage.ti_out)[g] = (T)tn;
+                  if (dens)
+                    RD[t] = tn;
+                }
             }
         }
       lds_sync(); // the LDS buffers are reused by the next cell
+      if (dens)
+        {
+          // int f dv over this cell per x node (the Gauss-Lobatto v rule on
+          // the nodes, numerics/vector_tools.h:230-315), in FP64 from the
+          // unrounded stage value: one partial per (v cell, x node)
+          const long long vc = cell / n_x_cells, xc = cell % n_x_cells;
+          for (int t = threadIdx.x; t < NN; t += blockDim.x)
+            {
+              double acc = 0.0;
+#pragma unroll
+              for (int q = 0; q < NN; ++q)
+                acc += tab[TB::wv + q] * RD[t + NN * q];
+              stage.rho_part[vc * n_x_cells * NN + xc * NN + t] = acc;
+            }
+        }
     }
 }
 
@@ -1515,6 +1538,33 @@ hipError_t vp_moments(const double *f, const MeshDev &mesh, const NormDev &nt,
     return e;
   hipLaunchKernelGGL(vlasov::moments_kernel, dim3((unsigned)mesh.n_cells),
                      dim3(256), lds, stream, f, mesh, nt, dim_x, partial);
+  return hipGetLastError();
+}
+
+namespace vlasov
+{
+// rho[i] = sum over v cells of part[vc][i], in v-cell order
+__global__ void density_from_partials(const double *__restrict__ part,
+                                      long long n_v_cells, long long n_x_dofs,
+                                      double *__restrict__ rho)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n_x_dofs)
+    return;
+  double s = 0.0;
+  for (long long vc = 0; vc < n_v_cells; ++vc)
+    s += part[vc * n_x_dofs + i];
+  rho[i] = s;
+}
+} // namespace vlasov
+
+hipError_t vp_density_from_partials(const double *part, long long n_v_cells,
+                                    long long n_x_dofs, double *rho,
+                                    hipStream_t stream)
+{
+  hipLaunchKernelGGL(vlasov::density_from_partials,
+                     dim3((unsigned)((n_x_dofs + 255) / 256)), dim3(256), 0,
+                     stream, part, n_v_cells, n_x_dofs, rho);
   return hipGetLastError();
 }
 
