@@ -881,6 +881,20 @@ struct Tab
   static constexpr int size = wv + NN;
 };
 
+// An LDS read of the line mappings below.  FP64: a single ds_read_b64 the
+// compiler neither pairs (ds_read2) nor widens (b128) -- on gfx950 those
+// forms conflict for these lane strides, the single form does not (bank
+// conflicts measured at about 6 cycles per LDS instruction before).  FP32:
+// the compiler's wide reads measured faster than single ones.
+template <typename T>
+__device__ inline T ld(const T *p, int i)
+{
+  if constexpr (sizeof(T) == 8)
+    return ((const volatile __attribute__((address_space(3))) T *)p)[i];
+  else
+    return p[i];
+}
+
 // a workgroup barrier that orders LDS only: the next cell's global loads
 // stay in flight across it (__syncthreads would wait for them)
 __device__ inline void lds_sync()
@@ -1079,7 +1093,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T u[N];
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            u[j] = U[b + j];
+            u[j] = ld(U, b + j);
           {
               T a = (T)tab[TB::bp(0) + i] * FL[0][l];
 #pragma unroll
@@ -1109,7 +1123,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T u[N];
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            u[j] = U[b + j * s1];
+            u[j] = ld(U, b + j * s1);
           {
               T a = (T)tab[TB::bp(1) + i] * FL[1][l];
 #pragma unroll
@@ -1136,7 +1150,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = 0;
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::S + i * N + j] * fsrc[j + N * rest];
+            a += (T)tab[TB::S + i * N + j] * ld(fsrc, j + N * rest);
           G1[ee][side][i + N * rest] = a;
         }
       lds_sync();
@@ -1151,7 +1165,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = 0;
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::S + i * N + j] * Q1[b + j * s1];
+            a += (T)tab[TB::S + i * N + j] * ld(Q1, b + j * s1);
           Q2[b + i * s1] = a;
         }
       for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
@@ -1163,7 +1177,8 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = 0;
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::S + i * N + j] * G1[ee][side][x0 + N * (j + N * vo)];
+            a += (T)tab[TB::S + i * N + j] *
+                 ld(G1[ee][side], x0 + N * (j + N * vo));
           G2[ee][side][x0 + N * (i + N * vo)] = a;
         }
       lds_sync();
@@ -1177,10 +1192,10 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = 0;
 #pragma unroll
             for (int j = 0; j < N; ++j)
-              a += V[0][0][i * N + j] * T0p[b + j * s2];
+              a += V[0][0][i * N + j] * ld(T0p, b + j * s2);
 #pragma unroll
             for (int j = 0; j < N; ++j)
-              a += V[0][1][i * N + j] * T0n[b + j * s2];
+              a += V[0][1][i * N + j] * ld(T0n, b + j * s2);
           R0[b + i * s2] = a;
         }
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
@@ -1192,10 +1207,10 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = 0;
 #pragma unroll
             for (int j = 0; j < N; ++j)
-              a += V[1][0][i * N + j] * T1p[b + j * s3];
+              a += V[1][0][i * N + j] * ld(T1p, b + j * s3);
 #pragma unroll
             for (int j = 0; j < N; ++j)
-              a += V[1][1][i * N + j] * T1n[b + j * s3];
+              a += V[1][1][i * N + j] * ld(T1n, b + j * s3);
           R1[b + i * s3] = a;
         }
       lds_sync();
@@ -1217,7 +1232,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
               T q[N];
 #pragma unroll
               for (int j = 0; j < N; ++j)
-                q[j] = Q2[b + j * sd];
+                q[j] = ld(Q2, b + j * sd);
               // face index of the line: (x0, x1, other v) = (qx, v_other)
               const int f = qx + NN * (b / (ee == 0 ? s3 : s2) % N);
               T yp = (T)tab[TB::bp(d) + i] * G2[ee][0][f];
@@ -1242,7 +1257,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = 0;
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::Si + i * N + j] * (Y0[b + j] + Y1[b + j]);
+            a += (T)tab[TB::Si + i * N + j] * (ld(Y0, b + j) + ld(Y1, b + j));
           Z[b + i] = a;
         }
       lds_sync();
@@ -1253,10 +1268,10 @@ __global__ __launch_bounds__(256) void apply_kernel(
           if (l >= F)
             continue;
           const int b = line_base<N>(l, 1);
-          T a = R0[b + i * s1] + R1[b + i * s1];
+          T a = ld(R0, b + i * s1) + ld(R1, b + i * s1);
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::Si + i * N + j] * Z[b + j * s1];
+            a += (T)tab[TB::Si + i * N + j] * ld(Z, b + j * s1);
           O[b + i * s1] = a;
         }
       lds_sync();
