@@ -557,12 +557,12 @@ __device__ inline void sipg2d(const double *Pt, int n0, int n1, int cx,
       }
 }
 
-// sum over the block (1024 threads = 16 waves): wave shuffles, one barrier,
-// every thread adds the 16 wave sums in the same order (uniform result).
-// Consecutive calls use alternate slots, so a slot is rewritten only after
-// every thread has passed the barrier of the call in between.
-template <int slot>
-__device__ inline double block_sum16(double v, double *red)
+// sum over the block (W waves): wave shuffles, one barrier, every thread
+// adds the W wave sums in the same order (uniform result).  Consecutive
+// calls use alternate slots, so a slot is rewritten only after every thread
+// has passed the barrier of the call in between.
+template <int W, int slot>
+__device__ inline double block_sum(double v, double *red)
 {
 #pragma unroll
   for (int off = 32; off >= 1; off >>= 1)
@@ -572,132 +572,158 @@ __device__ inline double block_sum16(double v, double *red)
   __syncthreads();
   double r = 0.0;
 #pragma unroll
-  for (int w = 0; w < 16; ++w)
+  for (int w = 0; w < W; ++w)
     r += red[slot * 16 + w];
   return r;
 }
 
-// One block of 1024 threads, thread t owns x cell t (n_cells <= 1024): the
-// residual and A p in registers, the search direction in LDS (transposed,
-// so that a wave reads consecutive addresses), the iterate in global
-// scratch (transposed).  Same algorithm and outputs as poisson_kernel.
-template <int N>
-__global__ __launch_bounds__(kSolverThreads) void poisson2d_kernel(
+// One block of 1024 / CPT threads; thread t owns the x cells t + k 1024/CPT,
+// k < CPT (n_cells <= 1024): residuals and A p in registers, the search
+// direction in LDS (transposed, so that a wave reads consecutive addresses),
+// the iterate in global scratch (transposed).  Two cells per thread leave
+// 256 registers per thread (no spills) and two independent matrix-vector
+// products in flight.  Same algorithm and outputs as poisson_kernel.
+template <int N, int CPT>
+__global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
   XDev xd, const double *__restrict__ tab_in,
   const double *__restrict__ rho_in, double *__restrict__ phi,
   const double *__restrict__ guess, double *__restrict__ efield,
   double *__restrict__ work, double rel_tol, int max_it,
   long long *__restrict__ stats, double *__restrict__ residual_out)
 {
-  constexpr int NN = N * N;
+  constexpr int NN = N * N, T = kSolverThreads / CPT, W = T / 64;
   extern __shared__ double sm[];
   const int nc = (int)xd.n_cells, n0 = xd.n[0], n1 = xd.n[1];
   const double nx = (double)nc * NN;
   double *Pt = sm;                  // NN x kPStride
   double *red = sm + NN * kPStride; // 2 x 16
-  double *phit = work;        // NN x nc
+  double *phit = work;              // NN x nc
   const cptr<double> tab = (cptr<double>)tab_in;
   const int t = threadIdx.x;
-  const bool own = t < nc;
-  const int c = own ? t : 0;
-  const int cx = c % n0, cy = c / n0;
+  bool own[CPT];
+  int cl[CPT], cx[CPT], cy[CPT];
+#pragma unroll
+  for (int k = 0; k < CPT; ++k)
+    {
+      const int c = t + k * T;
+      own[k] = c < nc;
+      cl[k] = own[k] ? c : 0;
+      cx[k] = cl[k] % n0;
+      cy[k] = cl[k] / n0;
+    }
 
   // b = -(M_0 (x) M_1)(rho - mean rho), then b -= mean b
-  double r[NN], y[NN];
+  double r[CPT][NN], y[CPT][NN];
   double s = 0.0;
 #pragma unroll
-  for (int i = 0; i < NN; ++i)
-    {
-      r[i] = own ? rho_in[(long long)c * NN + i] : 0.0;
-      s += r[i];
-    }
-  const double mean_rho = block_sum16<0>(s, red) / nx;
-#pragma unroll
-  for (int iy = 0; iy < N; ++iy)
-#pragma unroll
-    for (int ix = 0; ix < N; ++ix)
-      {
-        double a = 0.0;
-#pragma unroll
-        for (int jx = 0; jx < N; ++jx)
-          a += tab[6 * NN + ix * N + jx] * (r[jx + N * iy] - mean_rho);
-        y[ix + N * iy] = a;
-      }
-  s = 0.0;
-#pragma unroll
-  for (int iy = 0; iy < N; ++iy)
-#pragma unroll
-    for (int ix = 0; ix < N; ++ix)
-      {
-        double a = 0.0;
-#pragma unroll
-        for (int jy = 0; jy < N; ++jy)
-          a += tab[7 * NN + iy * N + jy] * y[ix + N * jy];
-        r[ix + N * iy] = own ? -a : 0.0;
-        s += r[ix + N * iy];
-      }
-  const double mean_b = block_sum16<1>(s, red) / nx;
-  double bb = 0.0;
-#pragma unroll
-  for (int i = 0; i < NN; ++i)
-    {
-      r[i] = own ? r[i] - mean_b : 0.0; // r holds b
-      bb += r[i] * r[i];
-    }
-  bb = block_sum16<0>(bb, red);
-
-  // the start: r = b - A phi (or the combination with the guess)
-  double x[NN];
-#pragma unroll
-  for (int i = 0; i < NN; ++i)
-    {
-      x[i] = own ? phi[(long long)c * NN + i] : 0.0;
-      if (own)
-        Pt[i * kPStride + c] = x[i];
-    }
-  __syncthreads();
-  if (own)
-    sipg2d<N>(Pt, n0, n1, cx, cy, tab, y);
-  else
+  for (int k = 0; k < CPT; ++k)
 #pragma unroll
     for (int i = 0; i < NN; ++i)
-      y[i] = 0.0;
-  double alpha0 = 1.0, beta0 = 0.0, rr_warm = -1.0;
-  if (guess)
+      {
+        r[k][i] = own[k] ? rho_in[(long long)cl[k] * NN + i] : 0.0;
+        s += r[k][i];
+      }
+  const double mean_rho = block_sum<W, 0>(s, red) / nx;
+  s = 0.0;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k)
     {
-      double g[NN], Ag[NN];
-      __syncthreads(); // every thread has read Pt
+#pragma unroll
+      for (int iy = 0; iy < N; ++iy)
+#pragma unroll
+        for (int ix = 0; ix < N; ++ix)
+          {
+            double a = 0.0;
+#pragma unroll
+            for (int jx = 0; jx < N; ++jx)
+              a += tab[6 * NN + ix * N + jx] * (r[k][jx + N * iy] - mean_rho);
+            y[k][ix + N * iy] = a;
+          }
+#pragma unroll
+      for (int iy = 0; iy < N; ++iy)
+#pragma unroll
+        for (int ix = 0; ix < N; ++ix)
+          {
+            double a = 0.0;
+#pragma unroll
+            for (int jy = 0; jy < N; ++jy)
+              a += tab[7 * NN + iy * N + jy] * y[k][ix + N * jy];
+            r[k][ix + N * iy] = own[k] ? -a : 0.0;
+            s += r[k][ix + N * iy];
+          }
+    }
+  const double mean_b = block_sum<W, 1>(s, red) / nx;
+  double bb = 0.0;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k)
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+      {
+        r[k][i] = own[k] ? r[k][i] - mean_b : 0.0; // r holds b
+        bb += r[k][i] * r[k][i];
+      }
+  bb = block_sum<W, 0>(bb, red);
+
+  // the start: r = b - A phi (or the combination with the guess); phi goes
+  // to the transposed iterate right away
+#pragma unroll
+  for (int k = 0; k < CPT; ++k)
+    if (own[k])
 #pragma unroll
       for (int i = 0; i < NN; ++i)
         {
-          g[i] = own ? guess[(long long)c * NN + i] : 0.0;
-          if (own)
-            Pt[i * kPStride + c] = g[i];
+          const double v = phi[(long long)cl[k] * NN + i];
+          Pt[i * kPStride + cl[k]] = v;
+          phit[(long long)i * nc + cl[k]] = v;
         }
-      __syncthreads();
-      if (own)
-        sipg2d<N>(Pt, n0, n1, cx, cy, tab, Ag);
+  __syncthreads();
+#pragma unroll
+  for (int k = 0; k < CPT; ++k)
+    {
+      if (own[k])
+        sipg2d<N>(Pt, n0, n1, cx[k], cy[k], tab, y[k]);
       else
 #pragma unroll
         for (int i = 0; i < NN; ++i)
-          Ag[i] = 0.0;
+          y[k][i] = 0.0;
+    }
+  double alpha0 = 1.0, beta0 = 0.0, rr_warm = -1.0;
+  if (guess)
+    {
       double d[6] = {0, 0, 0, 0, 0, 0};
+      __syncthreads(); // every thread has read Pt
 #pragma unroll
-      for (int i = 0; i < NN; ++i)
-        {
-          d[0] += x[i] * y[i];
-          d[1] += x[i] * Ag[i];
-          d[2] += g[i] * Ag[i];
-          d[3] += x[i] * r[i];
-          d[4] += g[i] * r[i];
-          d[5] += (r[i] - y[i]) * (r[i] - y[i]);
-        }
-      d[0] = block_sum16<1>(d[0], red);
-      d[1] = block_sum16<0>(d[1], red);
-      d[2] = block_sum16<1>(d[2], red);
-      d[3] = block_sum16<0>(d[3], red);
-      d[4] = block_sum16<1>(d[4], red);
-      d[5] = block_sum16<0>(d[5], red);
+      for (int k = 0; k < CPT; ++k)
+        if (own[k])
+#pragma unroll
+          for (int i = 0; i < NN; ++i)
+            Pt[i * kPStride + cl[k]] = guess[(long long)cl[k] * NN + i];
+      __syncthreads();
+#pragma unroll
+      for (int k = 0; k < CPT; ++k)
+        if (own[k])
+          {
+            double Ag[NN];
+            sipg2d<N>(Pt, n0, n1, cx[k], cy[k], tab, Ag);
+#pragma unroll
+            for (int i = 0; i < NN; ++i)
+              {
+                const double x = phit[(long long)i * nc + cl[k]];
+                const double g = guess[(long long)cl[k] * NN + i];
+                d[0] += x * y[k][i];
+                d[1] += x * Ag[i];
+                d[2] += g * Ag[i];
+                d[3] += x * r[k][i];
+                d[4] += g * r[k][i];
+                d[5] += (r[k][i] - y[k][i]) * (r[k][i] - y[k][i]);
+              }
+          }
+      d[0] = block_sum<W, 1>(d[0], red);
+      d[1] = block_sum<W, 0>(d[1], red);
+      d[2] = block_sum<W, 1>(d[2], red);
+      d[3] = block_sum<W, 0>(d[3], red);
+      d[4] = block_sum<W, 1>(d[4], red);
+      d[5] = block_sum<W, 0>(d[5], red);
       rr_warm = d[5];
       const double det = d[0] * d[2] - d[1] * d[1];
       if (d[0] > 0.0 && d[2] > 0.0 && det > 1e-12 * d[0] * d[2])
@@ -715,116 +741,142 @@ __global__ __launch_bounds__(kSolverThreads) void poisson2d_kernel(
           alpha0 = 1.0;
           beta0 = 0.0;
         }
+      // x = alpha phi + beta g, A x = alpha A phi + beta A g (A g again:
+      // the direction buffer still holds g)
 #pragma unroll
-      for (int i = 0; i < NN; ++i)
-        {
-          x[i] = alpha0 * x[i] + beta0 * g[i];
-          y[i] = alpha0 * y[i] + beta0 * Ag[i];
-        }
+      for (int k = 0; k < CPT; ++k)
+        if (own[k])
+          {
+            double Ag[NN];
+            sipg2d<N>(Pt, n0, n1, cx[k], cy[k], tab, Ag);
+#pragma unroll
+            for (int i = 0; i < NN; ++i)
+              {
+                const long long gi = (long long)i * nc + cl[k];
+                phit[gi] = alpha0 * phit[gi] +
+                           beta0 * guess[(long long)cl[k] * NN + i];
+                y[k][i] = alpha0 * y[k][i] + beta0 * Ag[i];
+              }
+          }
     }
   double rr = 0.0;
 #pragma unroll
-  for (int i = 0; i < NN; ++i)
-    {
-      r[i] -= y[i];
-      rr += r[i] * r[i];
-    }
-  if (residual_out)
-    {
-      // residual of the start only (no solve): b - A phi
-      if (own)
-#pragma unroll
-        for (int i = 0; i < NN; ++i)
-          residual_out[(long long)c * NN + i] = r[i];
-      return;
-    }
-  rr = block_sum16<1>(rr, red); // also: every thread is done with Pt
-  const double rr_start = rr;
-  if (own)
+  for (int k = 0; k < CPT; ++k)
 #pragma unroll
     for (int i = 0; i < NN; ++i)
       {
-        phit[(long long)i * nc + c] = x[i];
-        Pt[i * kPStride + c] = r[i];
+        r[k][i] -= y[k][i];
+        rr += r[k][i] * r[k][i];
       }
+  if (residual_out)
+    {
+      // residual of the start only (no solve): b - A phi
+#pragma unroll
+      for (int k = 0; k < CPT; ++k)
+        if (own[k])
+#pragma unroll
+          for (int i = 0; i < NN; ++i)
+            residual_out[(long long)cl[k] * NN + i] = r[k][i];
+      return;
+    }
+  rr = block_sum<W, 1>(rr, red); // also: every thread is done with Pt
+  const double rr_start = rr;
+#pragma unroll
+  for (int k = 0; k < CPT; ++k)
+    if (own[k])
+#pragma unroll
+      for (int i = 0; i < NN; ++i)
+        Pt[i * kPStride + cl[k]] = r[k][i];
   const double stop = rel_tol * sqrt(bb);
   int it = 0;
   while (sqrt(rr) > stop && it < max_it)
     {
       __syncthreads(); // the new direction is visible
-      if (own)
-        sipg2d<N>(Pt, n0, n1, cx, cy, tab, y);
       double pAp = 0.0;
-      if (own)
 #pragma unroll
-        for (int i = 0; i < NN; ++i)
-          pAp += Pt[i * kPStride + c] * y[i];
-      pAp = block_sum16<0>(pAp, red);
+      for (int k = 0; k < CPT; ++k)
+        if (own[k])
+          {
+            sipg2d<N>(Pt, n0, n1, cx[k], cy[k], tab, y[k]);
+#pragma unroll
+            for (int i = 0; i < NN; ++i)
+              pAp += Pt[i * kPStride + cl[k]] * y[k][i];
+          }
+      pAp = block_sum<W, 0>(pAp, red);
       if (!(pAp > 0.0))
         break; // what is left of the residual lies in the null space
       const double alpha = rr / pAp;
       double rr2 = 0.0;
-      if (own)
 #pragma unroll
-        for (int i = 0; i < NN; ++i)
-          {
-            phit[(long long)i * nc + c] += alpha * Pt[i * kPStride + c];
-            r[i] -= alpha * y[i];
-            rr2 += r[i] * r[i];
-          }
-      rr2 = block_sum16<1>(rr2, red);
+      for (int k = 0; k < CPT; ++k)
+        if (own[k])
+#pragma unroll
+          for (int i = 0; i < NN; ++i)
+            {
+              phit[(long long)i * nc + cl[k]] +=
+                alpha * Pt[i * kPStride + cl[k]];
+              r[k][i] -= alpha * y[k][i];
+              rr2 += r[k][i] * r[k][i];
+            }
+      rr2 = block_sum<W, 1>(rr2, red);
       const double beta = rr2 / rr;
       rr = rr2;
       // every matrix-vector product of this iteration ended before the
       // barrier in the last sum: the direction can be overwritten
-      if (own)
 #pragma unroll
-        for (int i = 0; i < NN; ++i)
-          Pt[i * kPStride + c] = r[i] + beta * Pt[i * kPStride + c];
+      for (int k = 0; k < CPT; ++k)
+        if (own[k])
+#pragma unroll
+          for (int i = 0; i < NN; ++i)
+            Pt[i * kPStride + cl[k]] =
+              r[k][i] + beta * Pt[i * kPStride + cl[k]];
       ++it;
     }
 
   // phi back in the cell-major layout, then grad phi at the quadrature
-  // points of this thread's cell
-  if (own)
-    {
+  // points of this thread's cells
 #pragma unroll
-      for (int i = 0; i < NN; ++i)
-        x[i] = phit[(long long)i * nc + c];
+  for (int k = 0; k < CPT; ++k)
+    if (own[k])
+      {
+        double x[NN], g[NN];
 #pragma unroll
-      for (int i = 0; i < NN; ++i)
-        phi[(long long)c * NN + i] = x[i];
-      // d/dx0: G along x0, S along x1; d/dx1: S along x0, G along x1
+        for (int i = 0; i < NN; ++i)
+          x[i] = phit[(long long)i * nc + cl[k]];
 #pragma unroll
-      for (int e = 0; e < 2; ++e)
-        {
+        for (int i = 0; i < NN; ++i)
+          phi[(long long)cl[k] * NN + i] = x[i];
+        // d/dx0: G along x0, S along x1; d/dx1: S along x0, G along x1
 #pragma unroll
-          for (int iy = 0; iy < N; ++iy)
+        for (int e = 0; e < 2; ++e)
+          {
 #pragma unroll
-            for (int qx = 0; qx < N; ++qx)
-              {
-                double a = 0.0;
+            for (int iy = 0; iy < N; ++iy)
 #pragma unroll
-                for (int jx = 0; jx < N; ++jx)
-                  a += (e == 0 ? xd.G[qx * N + jx] : xd.S[qx * N + jx]) *
-                       x[jx + N * iy];
-                y[qx + N * iy] = a;
-              }
+              for (int qx = 0; qx < N; ++qx)
+                {
+                  double a = 0.0;
 #pragma unroll
-          for (int qy = 0; qy < N; ++qy)
+                  for (int jx = 0; jx < N; ++jx)
+                    a += (e == 0 ? xd.G[qx * N + jx] : xd.S[qx * N + jx]) *
+                         x[jx + N * iy];
+                  g[qx + N * iy] = a;
+                }
 #pragma unroll
-            for (int qx = 0; qx < N; ++qx)
-              {
-                double a = 0.0;
+            for (int qy = 0; qy < N; ++qy)
 #pragma unroll
-                for (int jy = 0; jy < N; ++jy)
-                  a += (e == 1 ? xd.G[qy * N + jy] : xd.S[qy * N + jy]) *
-                       y[qx + N * jy];
-                efield[((long long)c * NN + qx + N * qy) * 2 + e] =
-                  a / xd.h[e];
-              }
-        }
-    }
+              for (int qx = 0; qx < N; ++qx)
+                {
+                  double a = 0.0;
+#pragma unroll
+                  for (int jy = 0; jy < N; ++jy)
+                    a += (e == 1 ? xd.G[qy * N + jy] : xd.S[qy * N + jy]) *
+                         g[qx + N * jy];
+                  efield[((long long)cl[k] * NN + qx + N * qy) * 2 + e] =
+                    a / xd.h[e];
+                }
+          }
+      }
   if (threadIdx.x == 0)
     {
       double *ds = reinterpret_cast<double *>(stats);
@@ -1506,15 +1558,17 @@ hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
 #define HD_VP2(NN_)                                                            \
   if (N == NN_)                                                                \
     {                                                                          \
+      constexpr int CPT = 2;                                                   \
       hipError_t e = hipFuncSetAttribute(                                      \
-        reinterpret_cast<const void *>(&vlasov::poisson2d_kernel<NN_>),        \
+        reinterpret_cast<const void *>(                                        \
+          &vlasov::poisson2d_kernel<NN_, CPT>),                                \
         hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);                 \
       if (e != hipSuccess)                                                     \
         return e;                                                              \
-      hipLaunchKernelGGL((vlasov::poisson2d_kernel<NN_>), dim3(1),             \
-                         dim3(vlasov::kSolverThreads), lds, stream, xd, tab2d, \
-                         rho, phi, guess, efield, work, rel_tol, max_it,       \
-                         stats, residual_out);                                 \
+      hipLaunchKernelGGL((vlasov::poisson2d_kernel<NN_, CPT>), dim3(1),        \
+                         dim3(vlasov::kSolverThreads / CPT), lds, stream, xd,  \
+                         tab2d, rho, phi, guess, efield, work, rel_tol,        \
+                         max_it, stats, residual_out);                         \
       return hipGetLastError();                                                \
     }
       HD_VP2(2) HD_VP2(3) HD_VP2(4)
