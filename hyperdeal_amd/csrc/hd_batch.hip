@@ -37,7 +37,7 @@ struct hd_batch
   hd::BatchOp *dev_ops[2] = {nullptr, nullptr}; // FP64 / FP32 coefficients
   int units[2][kMaxN + 1];                      // units per task, per N
   hd::BatchTask *dev_tasks[2] = {nullptr, nullptr};
-  int n_tasks[2] = {0, 0};
+  hd::BatchSegments seg[2] = {};
   bool dirty[2] = {true, true};
 };
 
@@ -50,11 +50,16 @@ struct hd_graph
 
 namespace
 {
-// tasks of the operators in `which` (all if empty), longest first
+// tasks of the operators in `which` (all if empty) in one segment per XCD:
+// each operator's logical units are cut into 8 contiguous ranges, range x
+// goes to XCD x (so the face neighbours of a range's cells sit in that XCD's
+// L2, as in the single-operator kernel) in tasks of units[N] units; each
+// segment runs its longest tasks first
 void make_tasks(const hd_batch *b, int pi, const std::vector<int> &which,
-                const int *units, std::vector<hd::BatchTask> &tasks)
+                const int *units, std::vector<hd::BatchTask> &tasks,
+                hd::BatchSegments &seg)
 {
-  std::vector<std::pair<double, hd::BatchTask>> t;
+  std::vector<std::pair<double, hd::BatchTask>> t[8];
   for (int i = 0; i < (int)b->ops.size(); ++i)
     {
       if (!which.empty() &&
@@ -65,24 +70,34 @@ void make_tasks(const hd_batch *b, int pi, const std::vector<int> &which,
       const int cpb = hd::collapsed_batch_cells_per_block(b->D, N, pi);
       const long long n_logical = (mf->mesh.n_cells + cpb - 1) / cpb;
       const int u = units[N];
-      for (long long lb = 0; lb < n_logical; lb += u)
+      for (int x = 0; x < 8; ++x)
         {
-          hd::BatchTask task{};
-          task.op = i;
-          task.lb = lb;
-          task.le = std::min(lb + u, n_logical);
-          // cost: the DoF the task touches
-          const double cost =
-            (double)(task.le - task.lb) * cpb * mf->dofs_cell;
-          t.push_back({cost, task});
+          const long long xb = n_logical * x / 8, xe = n_logical * (x + 1) / 8;
+          for (long long lb = xb; lb < xe; lb += u)
+            {
+              hd::BatchTask task{};
+              task.op = i;
+              task.lb = lb;
+              task.le = std::min(lb + u, xe);
+              // cost: the DoF the task touches
+              const double cost =
+                (double)(task.le - task.lb) * cpb * mf->dofs_cell;
+              t[x].push_back({cost, task});
+            }
         }
     }
-  std::stable_sort(t.begin(), t.end(), [](const auto &x, const auto &y) {
-    return x.first > y.first;
-  });
   tasks.clear();
-  for (const auto &x : t)
-    tasks.push_back(x.second);
+  for (int x = 0; x < 8; ++x)
+    {
+      std::stable_sort(t[x].begin(), t[x].end(),
+                       [](const auto &p, const auto &q) {
+                         return p.first > q.first;
+                       });
+      seg.start[x] = (int)tasks.size();
+      for (const auto &y : t[x])
+        tasks.push_back(y.second);
+    }
+  seg.start[8] = (int)tasks.size();
 }
 
 int upload_tasks(const std::vector<hd::BatchTask> &tasks,
@@ -109,14 +124,15 @@ int ensure_tasks(hd_batch *b, int pi)
   if (!b->dirty[pi])
     return ok();
   std::vector<hd::BatchTask> tasks;
-  make_tasks(b, pi, {}, b->units[pi], tasks);
+  hd::BatchSegments seg{};
+  make_tasks(b, pi, {}, b->units[pi], tasks, seg);
   if (b->dev_tasks[pi])
     (void)hipFree(b->dev_tasks[pi]);
   b->dev_tasks[pi] = nullptr;
   const int rc = upload_tasks(tasks, &b->dev_tasks[pi]);
   if (rc != HD_OK)
     return rc;
-  b->n_tasks[pi] = (int)tasks.size();
+  b->seg[pi] = seg;
   b->dirty[pi] = false;
   return ok();
 }
@@ -134,8 +150,8 @@ bool valid_ptrs(const hd_batch *b, const void *const *p)
 // the step: copy sol into ti_a, then the stages ping-ponging ti_a / ti_b
 int batch_step(hd_batch *b, const char *type, int precision,
                double *const *sol, void *const *ti_a, void *const *ti_b,
-               double dt, const hd::BatchTask *tasks, int n_tasks,
-               hipStream_t stream)
+               double dt, const hd::BatchTask *tasks,
+               const hd::BatchSegments &seg, hipStream_t stream)
 {
   double a[16], bb[16];
   const int n_stages = hd_lsrk_coefficients(type, a, bb);
@@ -168,7 +184,7 @@ int batch_step(hd_batch *b, const char *type, int precision,
       p.a_dt = last ? 0.0 : a[s] * dt;
       p.b_dt = bb[s] * dt;
       e = hd::collapsed_batch(b->D, pi, last ? 2 : 1, b->dev_ops[pi], tasks,
-                              n_tasks, p, stream);
+                              seg, p, stream);
       if (e != hipSuccess)
         return hip_fail(e, "batch stage");
       in_a = !in_a;
@@ -278,7 +294,7 @@ int hd_batch_apply(hd_batch *batch, const void *const *src, void *const *dst,
     }
   const hipError_t e =
     hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi],
-                        batch->dev_tasks[pi], batch->n_tasks[pi], p,
+                        batch->dev_tasks[pi], batch->seg[pi], p,
                         as_stream(stream));
   return e == hipSuccess ? ok() : hip_fail(e, "hd_batch_apply");
 }
@@ -303,8 +319,7 @@ int hd_batch_lsrk_step(hd_batch *batch, const char *type, int precision,
   if (rc != HD_OK)
     return rc;
   return batch_step(batch, type, precision, sol, ti_a, ti_b, dt,
-                    batch->dev_tasks[pi], batch->n_tasks[pi],
-                    as_stream(stream));
+                    batch->dev_tasks[pi], batch->seg[pi], as_stream(stream));
 }
 
 int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
@@ -345,19 +360,20 @@ int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
           int units[kMaxN + 1];
           std::fill(units, units + kMaxN + 1, u);
           std::vector<hd::BatchTask> tasks;
-          make_tasks(batch, pi, which, units, tasks);
+          hd::BatchSegments seg{};
+          make_tasks(batch, pi, which, units, tasks, seg);
           hd::BatchTask *dev = nullptr;
           rc = upload_tasks(tasks, &dev);
           if (rc != HD_OK)
             break;
           float ms = 0.0f;
           e = hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi], dev,
-                                  (int)tasks.size(), p, s); // warm-up
+                                  seg, p, s); // warm-up
           if (e == hipSuccess)
             e = hipEventRecord(t0, s);
           for (int r = 0; r < 3 && e == hipSuccess; ++r)
             e = hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi], dev,
-                                    (int)tasks.size(), p, s);
+                                    seg, p, s);
           if (e == hipSuccess)
             e = hipEventRecord(t1, s);
           if (e == hipSuccess)
@@ -437,12 +453,12 @@ int hd_batch_graph_create(hd_batch *batch, const char *type, int precision,
   if (!g)
     return fail(HD_ERR_NOMEM, "out of host memory");
   // the graph keeps its own task list: later tuning does not change it
-  hipError_t e = hipMalloc(&g->tasks, batch->n_tasks[pi] *
+  const int n_tasks = batch->seg[pi].start[8];
+  hipError_t e = hipMalloc(&g->tasks, std::max(n_tasks, 1) *
                                         sizeof(hd::BatchTask));
-  if (e == hipSuccess)
+  if (e == hipSuccess && n_tasks > 0)
     e = hipMemcpy(g->tasks, batch->dev_tasks[pi],
-                  batch->n_tasks[pi] * sizeof(hd::BatchTask),
-                  hipMemcpyDeviceToDevice);
+                  n_tasks * sizeof(hd::BatchTask), hipMemcpyDeviceToDevice);
   hipStream_t s = nullptr;
   if (e == hipSuccess)
     e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
@@ -455,7 +471,7 @@ int hd_batch_graph_create(hd_batch *batch, const char *type, int precision,
   if (e == hipSuccess)
     {
       rc = batch_step(batch, type, precision, sol, ti_a, ti_b, dt, g->tasks,
-                      batch->n_tasks[pi], s);
+                      batch->seg[pi], s);
       const hipError_t e2 = hipStreamEndCapture(s, &g->graph);
       if (rc == HD_OK && e2 != hipSuccess)
         rc = hip_fail(e2, "hd_batch_graph_create: capture");

@@ -131,8 +131,13 @@ struct Plan
   static constexpr bool kRowWalk = kTuned;
 };
 
-// G: cells per block along direction 1 (row walk only)
-template <int D, int N, typename T, int G = 1>
+// G: cells per block along direction 1 (row walk only).  BT: threads per
+// block; 0 trims the block to the waves that own points (a cell group that
+// LDS limits to fewer than kBlock threads then runs no shadow waves and its
+// launch bounds leave the tiles more registers); kBlock for kernels whose
+// block size is fixed (batches).  LB: blocks per CU that share the LDS (0:
+// two for FP64, four for FP32).
+template <int D, int N, typename T, int G = 1, int BT = 0, int LB = 0>
 struct Shape
 {
   using PL = Plan<D, N>;
@@ -144,17 +149,19 @@ struct Shape
   static constexpr bool kGroup = PL::kRowWalk;
   static_assert(kGroup || G == 1, "cell groups need the row walk");
   // LDS budget per block: that of the most resident blocks per CU
-  static constexpr int kLds = 160 * 1024 / (sizeof(T) == 8 ? 2 : 4) - 2048;
+  static constexpr int kLds =
+    160 * 1024 / (LB > 0 ? LB : (sizeof(T) == 8 ? 2 : 4)) - 2048;
   static constexpr int NBUF = P > 1 ? 2 : 1;
-  static constexpr int CPB_THREADS = kBlock / TPC;
+  static constexpr int CPB_THREADS = (BT > 0 ? BT : kBlock) / TPC;
   static constexpr int CPB_LDS =
     kLds / (NBUF * CELLP * (int)sizeof(T)) > 0
       ? kLds / (NBUF * CELLP * (int)sizeof(T))
       : 1;
   static constexpr int CPB =
     kGroup ? G : (CPB_THREADS < CPB_LDS ? CPB_THREADS : CPB_LDS);
-  static constexpr int BLOCK = kGroup ? G * TPC : kBlock;
   static constexpr int THREADS = CPB * TPC; // threads that own points
+  static constexpr int BLOCK =
+    kGroup ? G * TPC : (BT > 0 ? BT : (THREADS + 63) / 64 * 64);
   static constexpr bool kAllOwn = THREADS == BLOCK;
   // 16-byte chunks of the coalesced cell loads: per cell (groups) or over
   // the block's consecutive cells
@@ -162,7 +169,7 @@ struct Shape
     CELL % (16 / (int)sizeof(T)) == 0 ? 16 / (int)sizeof(T) : 1;
   static constexpr int CHUNKS = CPB * CELL / VEC;
   static constexpr int CPT =
-    kGroup ? CELL / VEC / TPC : (CHUNKS + kBlock - 1) / kBlock;
+    kGroup ? CELL / VEC / TPC : (CHUNKS + BLOCK - 1) / BLOCK;
   // a chunk is one aligned run in LDS
   static constexpr bool kChunkRuns = PL::kAlignedRows && N % VEC == 0;
   // rows along direction 0 as 16-byte LDS / global vectors
@@ -170,6 +177,7 @@ struct Shape
   static constexpr bool kRowVec = PL::kAlignedRows && N % RV == 0;
   static constexpr bool kFaceVec = N % RV == 0;
   static_assert(TPC <= kBlock, "cell too large for one block");
+  static_assert(THREADS <= BLOCK, "block smaller than its cells");
   static_assert(!kGroup || CELL % (VEC * TPC) == 0, "chunks per cell");
 };
 
@@ -286,6 +294,10 @@ __device__ inline void tile_write(T *buf, int base, const T *u)
 // scalar loads through the scalar cache
 template <typename T>
 using cptr = const __attribute__((address_space(4))) T *;
+// global memory: kernel arguments are known to point there, but a device
+// function's pointer arguments are generic (FLAT accesses) unless typed so
+template <typename T>
+using gptr = __attribute__((address_space(1))) T *;
 
 template <int D, int N, typename T, int p>
 __device__ inline void tile_contrib(cptr<T> coef, const T *u,
@@ -337,7 +349,7 @@ __device__ inline void tile_contrib(cptr<T> coef, const T *u,
 //   skip_a: the direction-a face will be taken from LDS (a wave-uniform
 //   condition): no loads for it
 template <int D, int N, typename T, int p>
-__device__ inline void face_load(const T *__restrict__ src, int gbase,
+__device__ inline void face_load(gptr<const T> __restrict__ src, int gbase,
                                  const long long *nb, const int *fnode,
                                  T (*face)[N], bool skip_a = false)
 {
@@ -348,14 +360,14 @@ __device__ inline void face_load(const T *__restrict__ src, int gbase,
   constexpr int sa = ipow(N, a), sb = ipow(N, b);
   if (!skip_a)
     {
-      const T *s0 = src + nb[a] * CELL + gbase + fnode[a] * sa;
+      gptr<const T> s0 = src + nb[a] * CELL + gbase + fnode[a] * sa;
 #pragma unroll
       for (int l = 0; l < N; ++l)
         face[0][l] = s0[l * sb];
     }
   if constexpr (PL::b_active(p))
     {
-      const T *s1 = src + nb[b] * CELL + gbase + fnode[b] * sb;
+      gptr<const T> s1 = src + nb[b] * CELL + gbase + fnode[b] * sb;
       if constexpr (a == 0 && S::kFaceVec)
         {
           // the N values of the b face along direction 0 are contiguous and
@@ -364,11 +376,11 @@ __device__ inline void face_load(const T *__restrict__ src, int gbase,
 #pragma unroll
           for (int c = 0; c < N; c += RV)
             {
-              const Vec<T, RV> v =
-                *reinterpret_cast<const Vec<T, RV> *>(s1 + c);
+              gptr<const Vec<T, RV>> v =
+                reinterpret_cast<gptr<const Vec<T, RV>>>(s1 + c);
 #pragma unroll
               for (int e = 0; e < RV; ++e)
-                face[1][c + e] = v.v[e];
+                face[1][c + e] = v->v[e];
             }
         }
       else
@@ -381,50 +393,51 @@ __device__ inline void face_load(const T *__restrict__ src, int gbase,
 }
 
 // this block's cells (from c0, ncell of them) as 16-byte chunks, chunk
-// tid + k kBlock into ch[k]; indices past the last chunk re-read it
+// tid + k BLOCK into ch[k]; indices past the last chunk re-read it
 //   groups: the chunks of cell `mycell` (the thread's own), tau + k TPC
-template <int D, int N, typename T, int G, typename Chunks>
+template <int D, int N, typename T, int G, int BT, int LB, typename Chunks>
 __device__ __forceinline__ void load_chunks(Chunks &ch,
-                                            const T *__restrict__ src,
+                                            gptr<const T> __restrict__ src,
                                             long long c0, int ncell,
                                             long long mycell, int tau)
 {
-  using S = Shape<D, N, T, G>;
+  using S = Shape<D, N, T, G, BT, LB>;
   constexpr int VEC = S::VEC;
   if constexpr (S::kGroup)
     {
-      const Vec<T, VEC> *g =
-        reinterpret_cast<const Vec<T, VEC> *>(src + mycell * S::CELL);
+      gptr<const Vec<T, VEC>> g =
+        reinterpret_cast<gptr<const Vec<T, VEC>>>(src + mycell * S::CELL);
 #pragma unroll
       for (int k = 0; k < S::CPT; ++k)
         {
-          const Vec<T, VEC> v = g[tau + k * S::TPC];
+          gptr<const Vec<T, VEC>> v = g + tau + k * S::TPC;
 #pragma unroll
           for (int e = 0; e < VEC; ++e)
-            ch[k * VEC + e] = v.v[e];
+            ch[k * VEC + e] = v->v[e];
         }
     }
   else
     {
-      const Vec<T, VEC> *g =
-        reinterpret_cast<const Vec<T, VEC> *>(src + c0 * S::CELL);
+      gptr<const Vec<T, VEC>> g =
+        reinterpret_cast<gptr<const Vec<T, VEC>>>(src + c0 * S::CELL);
       const int last = ncell * S::CELL / VEC - 1;
 #pragma unroll
       for (int k = 0; k < S::CPT; ++k)
         {
-          const Vec<T, VEC> v = g[min((int)threadIdx.x + k * kBlock, last)];
+          gptr<const Vec<T, VEC>> v =
+            g + min((int)threadIdx.x + k * S::BLOCK, last);
 #pragma unroll
           for (int e = 0; e < VEC; ++e)
-            ch[k * VEC + e] = v.v[e];
+            ch[k * VEC + e] = v->v[e];
         }
     }
 }
 
-template <int D, int N, typename T, int G, typename Chunks>
+template <int D, int N, typename T, int G, int BT, int LB, typename Chunks>
 __device__ __forceinline__ void store_chunks(const Chunks &ch, T *U, int lc,
                                              int tau)
 {
-  using S = Shape<D, N, T, G>;
+  using S = Shape<D, N, T, G, BT, LB>;
   constexpr int VEC = S::VEC, CELL = S::CELL, CELLP = S::CELLP;
 #pragma unroll
   for (int k = 0; k < S::CPT; ++k)
@@ -437,7 +450,7 @@ __device__ __forceinline__ void store_chunks(const Chunks &ch, T *U, int lc,
         }
       else
         {
-          const int c = min((int)threadIdx.x + k * kBlock, S::CHUNKS - 1);
+          const int c = min((int)threadIdx.x + k * S::BLOCK, S::CHUNKS - 1);
           const int i = c * VEC;
           l = i / CELL;
           li = i - l * CELL;
@@ -494,18 +507,43 @@ constexpr int kBlocksPerCu =
     ? (sizeof(T) == 8 ? 2 : (MODE == 0 ? 4 : 3)) / G
     : 1;
 
+// Residency of the single-operator kernel.  A full block (kBlock threads, one
+// wave per SIMD) runs kBlocksPerCu blocks per CU; a trimmed block runs as
+// many as LDS holds, at most that count, and its launch bound asks for the
+// waves per SIMD those blocks need (HIP's second launch bound is the minimum
+// number of waves per SIMD, which fixes the register budget: 512 / waves).
+template <int D, int N, typename T, int MODE, int G>
+struct Residency
+{
+  using S = Shape<D, N, T, G>;
+  static constexpr int kFull = kBlocksPerCu<T, MODE, G>;
+  static constexpr bool kTrim = !S::kGroup && S::BLOCK < kBlock;
+  static constexpr int kLdsBytes =
+    S::NBUF * S::CPB * S::CELLP * (int)sizeof(T);
+  static constexpr int kLdsBlocks = 160 * 1024 / kLdsBytes;
+  static constexpr int kBlocks =
+    kTrim ? (kLdsBlocks < kFull ? kLdsBlocks : kFull) : kFull;
+  static constexpr int kWavesPerEu =
+    kTrim ? (kBlocks * (S::BLOCK / 64) + 3) / 4 : kFull;
+  static_assert(kBlocks >= 1 && kWavesPerEu >= 1, "residency");
+};
+
 // One block's work on one operator: the logical units lb, lb + slots, ...
 // below lend, with the cells in U and the partial sums in R (LDS buffers of
 // CPB * CELLP values each).  Logical units: G rows of cells along direction
-// 0 (row walk) or groups of CPB cells.
-template <int D, int N, typename T, int MODE, int G>
+// 0 (row walk) or groups of CPB cells.  BT: threads per block (see Shape).
+template <int D, int N, typename T, int MODE, int G, int BT = 0, int LB = 0>
 __device__ __forceinline__ void
-apply_block(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ src,
-            T *__restrict__ dst, const T *__restrict__ coef,
+apply_block(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ src_,
+            T *__restrict__ dst_, const T *__restrict__ coef,
             const MeshDev &mesh, const InflowDev &inflow,
             const StageDev &stage, long long lb, long long lend, int slots)
 {
-  using S = Shape<D, N, T, G>;
+  const gptr<const T> src = (gptr<const T>)src_;
+  const gptr<T> dst = (gptr<T>)dst_;
+  const gptr<double> sol = (gptr<double>)stage.sol;
+  const gptr<T> ti_out = (gptr<T>)static_cast<T *>(stage.ti_out);
+  using S = Shape<D, N, T, G, BT, LB>;
   using PL = Plan<D, N>;
   constexpr int P = S::P, NN = S::NN, CELL = S::CELL, TPC = S::TPC;
   constexpr int CPB = S::CPB, CELLP = S::CELLP, VEC = S::VEC, CPT = S::CPT;
@@ -573,7 +611,7 @@ apply_block(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ src,
     long long nb[D];
     const long long mc = my_cell(c0);
     inflow_cells<D>(mc, mesh, inflow, nb);
-    load_chunks<D, N, T, G>(ch, src, c0, ncell, mc, tau);
+    load_chunks<D, N, T, G, BT, LB>(ch, src, c0, ncell, mc, tau);
     static_for<P>([&](auto pc) {
       constexpr int p = decltype(pc)::value;
       face_load<D, N, T, p>(src, gbase[p], nb, fnode, fc[p], p == P1 && d1_lds);
@@ -583,7 +621,7 @@ apply_block(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ src,
   for (;;)
     {
       // 1) this step's cells into U
-      store_chunks<D, N, T, G>(ch, U, lc, tau);
+      store_chunks<D, N, T, G, BT, LB>(ch, U, lc, tau);
       lds_barrier();
 
       // 2) the next step's values go out now and land during the arithmetic
@@ -596,7 +634,7 @@ apply_block(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ src,
       const long long nc0 = more ? first_cell(ln, nk) : c0;
       const int nncell = (int)min((long long)CPB, mesh.n_cells - nc0);
       const long long nmc = my_cell(nc0);
-      load_chunks<D, N, T, G>(ch, src, nc0, nncell, nmc, tau);
+      load_chunks<D, N, T, G, BT, LB>(ch, src, nc0, nncell, nmc, tau);
       long long nnb[D];
       inflow_cells<D>(nmc, mesh, inflow, nnb);
 
@@ -675,8 +713,7 @@ apply_block(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ src,
           {
 #pragma unroll
             for (int r = 0; r < NN; ++r)
-              sv[r] = stage.sol[g0 + (r % N) * ipow(N, a) +
-                                (r / N) * ipow(N, b)];
+              sv[r] = sol[g0 + (r % N) * ipow(N, a) + (r / N) * ipow(N, b)];
           }
         T out[NN];
         if constexpr (kLateLast)
@@ -698,8 +735,8 @@ apply_block(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ src,
               {
 #pragma unroll
                 for (int r = 0; r < NN; ++r)
-                  sv[r] = stage.sol[g0 + (r % N) * ipow(N, a) +
-                                    (r / N) * ipow(N, b)];
+                  sv[r] =
+                    sol[g0 + (r % N) * ipow(N, a) + (r / N) * ipow(N, b)];
               }
 #pragma unroll
             for (int r = 0; r < NN; ++r)
@@ -711,11 +748,11 @@ apply_block(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ src,
                 else
                   {
                     __builtin_nontemporal_store(
-                      sv[r] + stage.b_dt * (double)out[r], &stage.sol[g]);
+                      sv[r] + stage.b_dt * (double)out[r], &sol[g]);
                     if constexpr (MODE == 1)
                       __builtin_nontemporal_store(
                         (T)(sv[r] + stage.a_dt * (double)out[r]),
-                        &static_cast<T *>(stage.ti_out)[g]);
+                        &ti_out[g]);
                   }
               }
           }
@@ -736,7 +773,8 @@ apply_block(T *__restrict__ U, T *__restrict__ R, const T *__restrict__ src,
 
 template <int D, int N, typename T, int MODE, int G>
 __global__ __launch_bounds__((Shape<D, N, T, G>::BLOCK),
-                             (kBlocksPerCu<T, MODE, G>)) void apply_kernel(
+                             (Residency<D, N, T, MODE, G>::kWavesPerEu)) void
+  apply_kernel(
   const T *__restrict__ src, T *__restrict__ dst, const T *__restrict__ coef,
   MeshDev mesh, InflowDev inflow, StageDev stage, long long n_logical,
   int slots)
@@ -795,11 +833,11 @@ hipError_t launch_g(int mode, const T *src, T *dst, const T *coef,
   };
   const int m = mode == 0 ? 0 : (stage.ti_out ? 1 : 2);
   if (m == 0)
-    go(apply_kernel<D, N, T, 0, G>, kBlocksPerCu<T, 0, G>);
+    go(apply_kernel<D, N, T, 0, G>, Residency<D, N, T, 0, G>::kBlocks);
   else if (m == 1)
-    go(apply_kernel<D, N, T, 1, G>, kBlocksPerCu<T, 1, G>);
+    go(apply_kernel<D, N, T, 1, G>, Residency<D, N, T, 1, G>::kBlocks);
   else
-    go(apply_kernel<D, N, T, 2, G>, kBlocksPerCu<T, 2, G>);
+    go(apply_kernel<D, N, T, 2, G>, Residency<D, N, T, 2, G>::kBlocks);
   return hipGetLastError();
 }
 
@@ -863,10 +901,25 @@ hipError_t dispatch(int D, int N, int mode, const T *src, T *dst,
 // block loops over tasks, switching on the operator's N into the same block
 // code as the single-operator kernel, with one LDS buffer sized for the
 // largest cell.
+// The batch kernel's blocks: one size for every degree of a launch.  FP32:
+// two waves, four blocks per CU (a quarter of the LDS each: 3 cells at N = 6,
+// 5 at N = 5, 8 at N = 4, the groups the single-operator kernels trim to);
+// FP64: four waves, two blocks per CU.  Both leave two waves per SIMD, 256
+// VGPRs for the 6 x 6 tiles of N = 6.
+template <typename T>
+constexpr int kBatchBlock = sizeof(T) == 4 ? 128 : kBlock;
+template <typename T>
+constexpr int kBatchBlocksPerCu = sizeof(T) == 4 ? 4 : 2;
+template <typename T>
+constexpr int kBatchWavesPerEu =
+  (kBatchBlocksPerCu<T> * kBatchBlock<T> / 64 + 3) / 4;
+template <int N, typename T>
+using BatchShape = Shape<4, N, T, 1, kBatchBlock<T>, kBatchBlocksPerCu<T>>;
+
 template <int D, typename T, int N>
 constexpr int lds_elems()
 {
-  using S = Shape<D, N, T>;
+  using S = BatchShape<N, T>;
   return S::NBUF * S::CPB * S::CELLP;
 }
 
@@ -882,33 +935,94 @@ constexpr int batch_lds_elems()
   return m;
 }
 
-// The batch kernel's registers are those of its largest cell (6 x 6 tiles
-// at N = 6): two blocks per CU leave it 256 VGPRs.
-constexpr int kBatchBlocksPerCu = 2;
+// 64-bit values the caller holds uniform over the block: function arguments
+// arrive in VGPRs, readfirstlane puts them back on the scalar unit
+__device__ inline unsigned long long uniform64(unsigned long long v)
+{
+  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
+  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
+  return (unsigned long long)hi << 32 | lo;
+}
+template <typename P>
+__device__ inline P *uniform_ptr(P *p)
+{
+  return reinterpret_cast<P *>(uniform64(reinterpret_cast<uintptr_t>(p)));
+}
+__device__ inline double uniform_f64(double x)
+{
+  return __builtin_bit_cast(double,
+                            uniform64(__builtin_bit_cast(unsigned long long,
+                                                         x)));
+}
+
+// One task of a batch.  Kept out of line: each degree's block code then gets
+// its own register allocation (inlined into one body, the five degrees
+// spilled where each alone does not); the call costs once per task.  Its
+// arguments arrive in VGPRs and are made uniform again; apply_block types
+// the global pointers itself.
+template <int N, typename T, int MODE>
+__device__ __noinline__ void
+batch_task(T *lds, const T *src, T *dst, const T *coef, const BatchOp *op,
+           StageDev stage, long long lb, long long le)
+{
+  using S = BatchShape<N, T>;
+  stage.a_dt = uniform_f64(stage.a_dt);
+  stage.b_dt = uniform_f64(stage.b_dt);
+  stage.sol = MODE == 0 ? nullptr : uniform_ptr(stage.sol);
+  stage.ti_out = MODE == 1 ? uniform_ptr(stage.ti_out) : nullptr;
+  stage.scratch = nullptr;
+  stage.rho_part = nullptr;
+  // the operator's mesh and inflow sides, field by field (uniform loads)
+  const gptr<const BatchOp> o = (gptr<const BatchOp>)uniform_ptr(op);
+  MeshDev mesh;
+  InflowDev inflow;
+  mesh.dim = o->mesh.dim;
+  mesh.n_cells = o->mesh.n_cells;
+  mesh.inner = o->mesh.inner;
+#pragma unroll
+  for (int d = 0; d < kMaxDim; ++d)
+    {
+      mesh.n[d] = o->mesh.n[d];
+      mesh.stride[d] = o->mesh.stride[d];
+      mesh.lo[d] = o->mesh.lo[d];
+      mesh.h[d] = o->mesh.h[d];
+      inflow.side[d] = o->inflow.side[d];
+    }
+  lds = uniform_ptr(lds);
+  apply_block<4, N, T, MODE, 1, kBatchBlock<T>, kBatchBlocksPerCu<T>>(
+    lds, lds + S::CPB * S::CELLP, uniform_ptr(src),
+    MODE == 0 ? uniform_ptr(dst) : nullptr, uniform_ptr(coef), mesh, inflow,
+    stage,
+    (long long)uniform64((unsigned long long)lb),
+    (long long)uniform64((unsigned long long)le), 1);
+}
 
 template <int D, typename T, int MODE>
-__global__ __launch_bounds__(kBlock, kBatchBlocksPerCu) void
+__global__ __launch_bounds__(kBatchBlock<T>, kBatchWavesPerEu<T>) void
   batch_kernel(const BatchOp *__restrict__ ops,
-               const BatchTask *__restrict__ tasks, int n_tasks,
+               const BatchTask *__restrict__ tasks, BatchSegments seg,
                BatchPtrs ptrs)
 {
   __shared__ __align__(16) T lds[batch_lds_elems<D, T>()];
-  for (int t = blockIdx.x; t < n_tasks; t += gridDim.x)
+  // this XCD's segment of the task list, strided over its blocks
+  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
+  const int slots = gridDim.x / 8;
+  for (int t = seg.start[xcd] + slot; t < seg.start[xcd + 1]; t += slots)
     {
       const BatchTask task = tasks[t];
-      const BatchOp &op = ops[task.op];
+      const BatchOp *op = ops + task.op;
       const StageDev stage{ptrs.a_dt, ptrs.b_dt, ptrs.sol[task.op],
                            ptrs.ti[task.op]};
       const T *src = static_cast<const T *>(ptrs.src[task.op]);
       T *dst = static_cast<T *>(ptrs.dst[task.op]);
-      const T *coef = static_cast<const T *>(op.coef);
-      switch (op.N)
+      const T *coef = static_cast<const T *>(op->coef);
+      static_assert(D == 4, "batches of 2+2 operators");
+      switch (op->N)
         {
 #define HD_BCASE(NN_)                                                        \
   case NN_:                                                                  \
-    apply_block<D, NN_, T, MODE, 1>(                                         \
-      lds, lds + Shape<D, NN_, T>::CPB * Shape<D, NN_, T>::CELLP, src, dst,  \
-      coef, op.mesh, op.inflow, stage, task.lb, task.le, 1);                 \
+    batch_task<NN_, T, MODE>(lds, src, dst, coef, op, stage, task.lb,        \
+                             task.le);                                       \
     break;
           HD_BCASE(2) HD_BCASE(3) HD_BCASE(4) HD_BCASE(5) HD_BCASE(6)
 #undef HD_BCASE
@@ -938,23 +1052,26 @@ __global__ void batch_convert_kernel(BatchPtrs ptrs, BatchSizes sizes)
 
 template <typename T>
 hipError_t batch_launch(int D, int mode, const BatchOp *ops,
-                        const BatchTask *tasks, int n_tasks,
+                        const BatchTask *tasks, const BatchSegments &seg,
                         const BatchPtrs &ptrs, hipStream_t stream)
 {
   if (D != 4)
     return hipErrorInvalidValue;
+  int most = 0; // the longest segment
+  for (int x = 0; x < 8; ++x)
+    most = std::max(most, seg.start[x + 1] - seg.start[x]);
   auto go = [&](auto kernel, int bpc) {
-    const int grid = (int)std::min<long long>(n_tasks, (long long)bpc * 8 *
-                                                         cus_per_xcd());
-    hipLaunchKernelGGL(kernel, dim3(grid), dim3(kBlock), 0, stream, ops, tasks,
-                       n_tasks, ptrs);
+    // the same number of blocks on every XCD, no more than its segment needs
+    const int per = std::min(most, bpc * cus_per_xcd());
+    hipLaunchKernelGGL(kernel, dim3(8 * per), dim3(kBatchBlock<T>), 0, stream,
+                       ops, tasks, seg, ptrs);
   };
   if (mode == 0)
-    go(batch_kernel<4, T, 0>, kBatchBlocksPerCu);
+    go(batch_kernel<4, T, 0>, kBatchBlocksPerCu<T>);
   else if (mode == 1)
-    go(batch_kernel<4, T, 1>, kBatchBlocksPerCu);
+    go(batch_kernel<4, T, 1>, kBatchBlocksPerCu<T>);
   else
-    go(batch_kernel<4, T, 2>, kBatchBlocksPerCu);
+    go(batch_kernel<4, T, 2>, kBatchBlocksPerCu<T>);
   return hipGetLastError();
 }
 
@@ -994,37 +1111,33 @@ int collapsed_batch_cells_per_block(int D, int N, int precision)
 {
   if (!collapsed_batch_supported(D, N))
     return 0;
+  using collapsed::BatchShape;
   const bool f64 = precision == 0;
   switch (N)
     {
       case 2:
-        return f64 ? collapsed::Shape<4, 2, double>::CPB
-                   : collapsed::Shape<4, 2, float>::CPB;
+        return f64 ? BatchShape<2, double>::CPB : BatchShape<2, float>::CPB;
       case 3:
-        return f64 ? collapsed::Shape<4, 3, double>::CPB
-                   : collapsed::Shape<4, 3, float>::CPB;
+        return f64 ? BatchShape<3, double>::CPB : BatchShape<3, float>::CPB;
       case 4:
-        return f64 ? collapsed::Shape<4, 4, double>::CPB
-                   : collapsed::Shape<4, 4, float>::CPB;
+        return f64 ? BatchShape<4, double>::CPB : BatchShape<4, float>::CPB;
       case 5:
-        return f64 ? collapsed::Shape<4, 5, double>::CPB
-                   : collapsed::Shape<4, 5, float>::CPB;
+        return f64 ? BatchShape<5, double>::CPB : BatchShape<5, float>::CPB;
       default:
-        return f64 ? collapsed::Shape<4, 6, double>::CPB
-                   : collapsed::Shape<4, 6, float>::CPB;
+        return f64 ? BatchShape<6, double>::CPB : BatchShape<6, float>::CPB;
     }
 }
 
 hipError_t collapsed_batch(int D, int precision, int mode, const BatchOp *ops,
-                           const BatchTask *tasks, int n_tasks,
+                           const BatchTask *tasks, const BatchSegments &seg,
                            const BatchPtrs &ptrs, hipStream_t stream)
 {
-  if (n_tasks <= 0)
+  if (seg.start[8] <= seg.start[0])
     return hipSuccess;
   return precision == 0
-           ? collapsed::batch_launch<double>(D, mode, ops, tasks, n_tasks,
-                                             ptrs, stream)
-           : collapsed::batch_launch<float>(D, mode, ops, tasks, n_tasks, ptrs,
+           ? collapsed::batch_launch<double>(D, mode, ops, tasks, seg, ptrs,
+                                             stream)
+           : collapsed::batch_launch<float>(D, mode, ops, tasks, seg, ptrs,
                                             stream);
 }
 

@@ -155,6 +155,13 @@ struct BatchTask
   long long lb, le;
 };
 
+// the task list in one segment per XCD: tasks [start[x], start[x + 1]) run
+// on the blocks of XCD x (blocks are dispatched round-robin over the XCDs)
+struct BatchSegments
+{
+  int start[9];
+};
+
 // per call: the vectors of every operator (kernel argument) and the stage
 // coefficients
 struct BatchPtrs
@@ -190,7 +197,7 @@ hipError_t collapsed_apply_f32(int D, int N, int mode, const float *src,
 bool collapsed_batch_supported(int D, int N);
 int collapsed_batch_cells_per_block(int D, int N, int precision);
 hipError_t collapsed_batch(int D, int precision, int mode, const BatchOp *ops,
-                           const BatchTask *tasks, int n_tasks,
+                           const BatchTask *tasks, const BatchSegments &seg,
                            const BatchPtrs &ptrs, hipStream_t stream);
 hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,
                          const BatchPtrs &ptrs, hipStream_t stream);
