@@ -6,6 +6,7 @@
 #include "hd_handles.h"
 
 #include <algorithm>
+#include <cmath>
 #include <new>
 #include <string>
 #include <vector>
@@ -18,10 +19,8 @@ using hd_abi::ok;
 namespace
 {
 constexpr int kMaxN = 8;
-// candidate work items per task, in units of the cells one block handles at a
-// time; the autotuner times each
-constexpr int kCandidates[] = {1, 2, 4, 8, 16, 32};
-constexpr int kDefaultUnits = 4;
+// the autotuner's refinement: factors tried on each degree's cost weight
+constexpr double kRefine[] = {0.8, 1.25};
 
 int prec_index(int precision)
 {
@@ -29,15 +28,28 @@ int prec_index(int precision)
 }
 } // namespace
 
+// a work split on the device: the task list, then the blocks' task ranges
+struct DevPlan
+{
+  void *mem = nullptr;
+  size_t bytes = 0;
+  const hd::BatchTask *tasks = nullptr;
+  hd::BatchPlan plan{};
+};
+
 struct hd_batch
 {
   std::vector<hd_advection *> ops;
   std::vector<long long> n_dofs;
   int D = 0;
   hd::BatchOp *dev_ops[2] = {nullptr, nullptr}; // FP64 / FP32 coefficients
-  int units[2][kMaxN + 1];                      // units per task, per N
-  hd::BatchTask *dev_tasks[2] = {nullptr, nullptr};
-  hd::BatchSegments seg[2] = {};
+  // per precision and N: the cost of one DoF in the work split (seconds per
+  // DoF from the autotuner, 1 before it ran) and the most units per task
+  // (0: no limit)
+  double weight[2][kMaxN + 1];
+  int cap[2][kMaxN + 1];
+  DevPlan plan[2];
+  std::vector<hd::BatchTask> host_tasks[2];
   bool dirty[2] = {true, true};
 };
 
@@ -45,21 +57,44 @@ struct hd_graph
 {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  hd::BatchTask *tasks = nullptr; // the graph's own copy of the task list
+  void *plan = nullptr; // the graph's own copy of the work split
 };
 
 namespace
 {
-// tasks of the operators in `which` (all if empty) in one segment per XCD:
-// each operator's logical units are cut into 8 contiguous ranges, range x
-// goes to XCD x (so the face neighbours of a range's cells sit in that XCD's
-// L2, as in the single-operator kernel) in tasks of units[N] units; each
-// segment runs its longest tasks first
+// The work split of the operators in `which` (all if empty) over 8 x per
+// blocks.  Operators are cut into fragments of contiguous logical units, no
+// more than about a sixteenth of the total cost each (cost: weight[N] per
+// DoF), and the fragments go to the XCDs, largest first, each to the least
+// loaded one.  An XCD's blocks walk its fragments the way the single-operator
+// kernel walks its range: interleaved, unit u of the XCD's concatenated list
+// on block u mod per, so that the blocks of an XCD work on a window of
+// neighbouring cells (their face neighbours in that XCD's L2) and each block
+// restarts its load pipeline once per fragment.  Tasks hold at most cap[N]
+// units when a cap is set.
 void make_tasks(const hd_batch *b, int pi, const std::vector<int> &which,
-                const int *units, std::vector<hd::BatchTask> &tasks,
-                hd::BatchSegments &seg)
+                int per, std::vector<hd::BatchTask> &tasks,
+                std::vector<int> &block_start)
 {
-  std::vector<std::pair<double, hd::BatchTask>> t[8];
+  struct Frag
+  {
+    int op, N;
+    long long lb, le;
+    double cost;
+  };
+  std::vector<Frag> frags;
+  double total = 0.0;
+  for (int i = 0; i < (int)b->ops.size(); ++i)
+    {
+      if (!which.empty() &&
+          std::find(which.begin(), which.end(), i) == which.end())
+        continue;
+      const hd_matrix_free *mf = b->ops[i]->mf;
+      const int cpb = hd::collapsed_batch_cells_per_block(b->D, mf->nd, pi);
+      const long long nl = (mf->mesh.n_cells + cpb - 1) / cpb;
+      total += b->weight[pi][mf->nd] * (double)nl * cpb * mf->dofs_cell;
+    }
+  const double frag_cost = total / 16.0;
   for (int i = 0; i < (int)b->ops.size(); ++i)
     {
       if (!which.empty() &&
@@ -68,54 +103,102 @@ void make_tasks(const hd_batch *b, int pi, const std::vector<int> &which,
       const hd_matrix_free *mf = b->ops[i]->mf;
       const int N = mf->nd;
       const int cpb = hd::collapsed_batch_cells_per_block(b->D, N, pi);
-      const long long n_logical = (mf->mesh.n_cells + cpb - 1) / cpb;
-      const int u = units[N];
-      for (int x = 0; x < 8; ++x)
+      const long long nl = (mf->mesh.n_cells + cpb - 1) / cpb;
+      const double uc = b->weight[pi][N] * (double)cpb * mf->dofs_cell;
+      const long long nf = std::max(
+        1LL, std::min(nl, (long long)std::ceil(nl * uc / frag_cost - 1e-9)));
+      for (long long f = 0; f < nf; ++f)
         {
-          const long long xb = n_logical * x / 8, xe = n_logical * (x + 1) / 8;
-          for (long long lb = xb; lb < xe; lb += u)
+          const long long lb = nl * f / nf, le = nl * (f + 1) / nf;
+          frags.push_back({i, N, lb, le, (double)(le - lb) * uc});
+        }
+    }
+  // largest first, each to the least loaded XCD
+  std::vector<int> order(frags.size());
+  for (size_t f = 0; f < frags.size(); ++f)
+    order[f] = (int)f;
+  std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
+    return frags[p].cost > frags[q].cost;
+  });
+  std::vector<int> owner(frags.size(), 0);
+  double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
+  for (int f : order)
+    {
+      const int x = (int)(std::min_element(load, load + 8) - load);
+      owner[f] = x;
+      load[x] += frags[f].cost;
+    }
+  tasks.clear();
+  block_start.assign(8 * (size_t)per + 1, 0);
+  for (int x = 0; x < 8; ++x)
+    {
+      // this XCD's fragments by degree, then operator and position
+      std::vector<int> mine;
+      for (size_t f = 0; f < frags.size(); ++f)
+        if (owner[f] == x)
+          mine.push_back((int)f);
+      std::stable_sort(mine.begin(), mine.end(), [&](int p, int q) {
+        return frags[p].N < frags[q].N;
+      });
+      for (int j = 0; j < per; ++j)
+        {
+          block_start[(size_t)x * per + j] = (int)tasks.size();
+          long long offset = 0; // units of the XCD's list before fragment
+          for (int f : mine)
             {
-              hd::BatchTask task{};
-              task.op = i;
-              task.lb = lb;
-              task.le = std::min(lb + u, xe);
-              // cost: the DoF the task touches
-              const double cost =
-                (double)(task.le - task.lb) * cpb * mf->dofs_cell;
-              t[x].push_back({cost, task});
+              const Frag &fr = frags[f];
+              const long long n = fr.le - fr.lb;
+              // the fragment's first unit on block j
+              const long long k0 = ((j - offset) % per + per) % per;
+              offset += n;
+              if (k0 >= n)
+                continue;
+              const long long n_mine = (n - 1 - k0) / per + 1;
+              const int cap = b->cap[pi][fr.N];
+              const long long piece = cap > 0 ? cap : n_mine;
+              for (long long m = 0; m < n_mine; m += piece)
+                {
+                  hd::BatchTask task{};
+                  task.op = fr.op;
+                  task.stride = per;
+                  task.lb = fr.lb + k0 + m * per;
+                  task.le = std::min(fr.le, task.lb + piece * per);
+                  tasks.push_back(task);
+                }
             }
         }
     }
-  tasks.clear();
-  for (int x = 0; x < 8; ++x)
-    {
-      std::stable_sort(t[x].begin(), t[x].end(),
-                       [](const auto &p, const auto &q) {
-                         return p.first > q.first;
-                       });
-      seg.start[x] = (int)tasks.size();
-      for (const auto &y : t[x])
-        tasks.push_back(y.second);
-    }
-  seg.start[8] = (int)tasks.size();
+  block_start[8 * (size_t)per] = (int)tasks.size();
 }
 
-int upload_tasks(const std::vector<hd::BatchTask> &tasks,
-                 hd::BatchTask **dev)
+void free_plan(DevPlan &p)
 {
-  *dev = nullptr;
-  if (tasks.empty())
-    return ok();
-  hipError_t e = hipMalloc(dev, tasks.size() * sizeof(hd::BatchTask));
-  if (e == hipSuccess)
-    e = hipMemcpy(*dev, tasks.data(), tasks.size() * sizeof(hd::BatchTask),
+  (void)hipFree(p.mem);
+  p = DevPlan{};
+}
+
+int upload_plan(const std::vector<hd::BatchTask> &tasks,
+                const std::vector<int> &block_start, int per, DevPlan &p)
+{
+  free_plan(p);
+  const size_t tb = (tasks.size() * sizeof(hd::BatchTask) + 15) / 16 * 16;
+  p.bytes = tb + block_start.size() * sizeof(int);
+  hipError_t e = hipMalloc(&p.mem, p.bytes);
+  if (e == hipSuccess && !tasks.empty())
+    e = hipMemcpy(p.mem, tasks.data(), tasks.size() * sizeof(hd::BatchTask),
                   hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(static_cast<char *>(p.mem) + tb, block_start.data(),
+                  block_start.size() * sizeof(int), hipMemcpyHostToDevice);
   if (e != hipSuccess)
     {
-      (void)hipFree(*dev);
-      *dev = nullptr;
-      return hip_fail(e, "batch task list");
+      free_plan(p);
+      return hip_fail(e, "batch work split");
     }
+  p.tasks = static_cast<const hd::BatchTask *>(p.mem);
+  p.plan.block_start =
+    reinterpret_cast<const int *>(static_cast<char *>(p.mem) + tb);
+  p.plan.per = per;
   return ok();
 }
 
@@ -123,16 +206,12 @@ int ensure_tasks(hd_batch *b, int pi)
 {
   if (!b->dirty[pi])
     return ok();
-  std::vector<hd::BatchTask> tasks;
-  hd::BatchSegments seg{};
-  make_tasks(b, pi, {}, b->units[pi], tasks, seg);
-  if (b->dev_tasks[pi])
-    (void)hipFree(b->dev_tasks[pi]);
-  b->dev_tasks[pi] = nullptr;
-  const int rc = upload_tasks(tasks, &b->dev_tasks[pi]);
+  const int per = hd::collapsed_batch_blocks_per_xcd(pi);
+  std::vector<int> block_start;
+  make_tasks(b, pi, {}, per, b->host_tasks[pi], block_start);
+  const int rc = upload_plan(b->host_tasks[pi], block_start, per, b->plan[pi]);
   if (rc != HD_OK)
     return rc;
-  b->seg[pi] = seg;
   b->dirty[pi] = false;
   return ok();
 }
@@ -151,7 +230,7 @@ bool valid_ptrs(const hd_batch *b, const void *const *p)
 int batch_step(hd_batch *b, const char *type, int precision,
                double *const *sol, void *const *ti_a, void *const *ti_b,
                double dt, const hd::BatchTask *tasks,
-               const hd::BatchSegments &seg, hipStream_t stream)
+               const hd::BatchPlan &plan, hipStream_t stream)
 {
   double a[16], bb[16];
   const int n_stages = hd_lsrk_coefficients(type, a, bb);
@@ -184,7 +263,7 @@ int batch_step(hd_batch *b, const char *type, int precision,
       p.a_dt = last ? 0.0 : a[s] * dt;
       p.b_dt = bb[s] * dt;
       e = hd::collapsed_batch(b->D, pi, last ? 2 : 1, b->dev_ops[pi], tasks,
-                              seg, p, stream);
+                              plan, p, stream);
       if (e != hipSuccess)
         return hip_fail(e, "batch stage");
       in_a = !in_a;
@@ -220,7 +299,10 @@ int hd_batch_create(hd_advection *const *ops, int n_ops, hd_batch **out)
   b->D = D;
   for (int p = 0; p < 2; ++p)
     for (int N = 0; N <= kMaxN; ++N)
-      b->units[p][N] = kDefaultUnits;
+      {
+        b->weight[p][N] = 1.0;
+        b->cap[p][N] = 0;
+      }
   std::vector<hd::BatchOp> table[2];
   for (int i = 0; i < n_ops; ++i)
     {
@@ -261,7 +343,7 @@ int hd_batch_destroy(hd_batch *batch)
   for (int p = 0; p < 2; ++p)
     {
       (void)hipFree(batch->dev_ops[p]);
-      (void)hipFree(batch->dev_tasks[p]);
+      free_plan(batch->plan[p]);
     }
   delete batch;
   return ok();
@@ -294,7 +376,7 @@ int hd_batch_apply(hd_batch *batch, const void *const *src, void *const *dst,
     }
   const hipError_t e =
     hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi],
-                        batch->dev_tasks[pi], batch->seg[pi], p,
+                        batch->plan[pi].tasks, batch->plan[pi].plan, p,
                         as_stream(stream));
   return e == hipSuccess ? ok() : hip_fail(e, "hd_batch_apply");
 }
@@ -319,7 +401,8 @@ int hd_batch_lsrk_step(hd_batch *batch, const char *type, int precision,
   if (rc != HD_OK)
     return rc;
   return batch_step(batch, type, precision, sol, ti_a, ti_b, dt,
-                    batch->dev_tasks[pi], batch->seg[pi], as_stream(stream));
+                    batch->plan[pi].tasks, batch->plan[pi].plan,
+                    as_stream(stream));
 }
 
 int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
@@ -330,6 +413,7 @@ int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
       (precision != HD_FP64 && precision != HD_FP32))
     return fail(HD_ERR_INVALID, "hd_batch_autotune: invalid argument");
   const int pi = prec_index(precision);
+  const int per = hd::collapsed_batch_blocks_per_xcd(pi);
   hipStream_t s = as_stream(stream);
   hd::BatchPtrs p{};
   for (size_t i = 0; i < batch->ops.size(); ++i)
@@ -343,57 +427,75 @@ int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
     e = hipEventCreate(&t1);
   if (e != hipSuccess)
     return hip_fail(e, "hd_batch_autotune");
+  // milliseconds of one apply of the operators in `which` under the current
+  // weights (one warm-up, the mean of three)
+  auto time_split = [&](const std::vector<int> &which, float &ms) -> int {
+    std::vector<hd::BatchTask> tasks;
+    std::vector<int> block_start;
+    make_tasks(batch, pi, which, per, tasks, block_start);
+    DevPlan dp;
+    int rc = upload_plan(tasks, block_start, per, dp);
+    if (rc != HD_OK)
+      return rc;
+    hipError_t err = hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi],
+                                         dp.tasks, dp.plan, p, s);
+    if (err == hipSuccess)
+      err = hipEventRecord(t0, s);
+    for (int r = 0; r < 3 && err == hipSuccess; ++r)
+      err = hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi], dp.tasks,
+                                dp.plan, p, s);
+    if (err == hipSuccess)
+      err = hipEventRecord(t1, s);
+    if (err == hipSuccess)
+      err = hipEventSynchronize(t1);
+    if (err == hipSuccess)
+      err = hipEventElapsedTime(&ms, t0, t1);
+    free_plan(dp);
+    ms /= 3.0f;
+    return err == hipSuccess ? ok() : hip_fail(err, "hd_batch_autotune");
+  };
   int rc = ok();
-  // per degree: the operators of that degree alone, each candidate timed
+  // 1) each degree alone: its time per DoF is its cost weight
+  std::vector<int> degrees;
   for (int N = 2; N <= kMaxN && rc == HD_OK; ++N)
     {
       std::vector<int> which;
+      double dofs = 0.0;
       for (size_t i = 0; i < batch->ops.size(); ++i)
         if (batch->ops[i]->mf->nd == N)
-          which.push_back((int)i);
+          {
+            which.push_back((int)i);
+            dofs += (double)batch->ops[i]->mf->n_dofs;
+          }
       if (which.empty())
         continue;
-      float best = 1e30f;
-      int best_u = kDefaultUnits;
-      for (int u : kCandidates)
-        {
-          int units[kMaxN + 1];
-          std::fill(units, units + kMaxN + 1, u);
-          std::vector<hd::BatchTask> tasks;
-          hd::BatchSegments seg{};
-          make_tasks(batch, pi, which, units, tasks, seg);
-          hd::BatchTask *dev = nullptr;
-          rc = upload_tasks(tasks, &dev);
-          if (rc != HD_OK)
-            break;
-          float ms = 0.0f;
-          e = hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi], dev,
-                                  seg, p, s); // warm-up
-          if (e == hipSuccess)
-            e = hipEventRecord(t0, s);
-          for (int r = 0; r < 3 && e == hipSuccess; ++r)
-            e = hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi], dev,
-                                    seg, p, s);
-          if (e == hipSuccess)
-            e = hipEventRecord(t1, s);
-          if (e == hipSuccess)
-            e = hipEventSynchronize(t1);
-          if (e == hipSuccess)
-            e = hipEventElapsedTime(&ms, t0, t1);
-          (void)hipFree(dev);
-          if (e != hipSuccess)
-            {
-              rc = hip_fail(e, "hd_batch_autotune");
-              break;
-            }
-          if (ms < best)
-            {
-              best = ms;
-              best_u = u;
-            }
-        }
+      float ms = 0.0f;
+      rc = time_split(which, ms);
       if (rc == HD_OK)
-        batch->units[pi][N] = best_u;
+        {
+          batch->weight[pi][N] = std::max(1e-3 * ms / dofs, 1e-18);
+          degrees.push_back(N);
+        }
+    }
+  // 2) the whole batch: scale each degree's weight while that helps
+  if (rc == HD_OK && degrees.size() > 1)
+    {
+      float best = 0.0f;
+      rc = time_split({}, best);
+      for (int N : degrees)
+        for (double f : kRefine)
+          {
+            if (rc != HD_OK)
+              break;
+            const double w0 = batch->weight[pi][N];
+            batch->weight[pi][N] = w0 * f;
+            float ms = 0.0f;
+            rc = time_split({}, ms);
+            if (rc == HD_OK && ms < best)
+              best = ms;
+            else
+              batch->weight[pi][N] = w0;
+          }
     }
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
@@ -411,7 +513,26 @@ int hd_batch_cells_per_task(const hd_batch *batch, int degree, int precision)
   const int cpb = hd::collapsed_batch_cells_per_block(batch->D, N, pi);
   if (cpb <= 0)
     return fail(HD_ERR_UNSUPPORTED, "no batch kernel for this degree");
-  return batch->units[pi][N] * cpb;
+  // the mean task of this degree in the current work split
+  std::vector<hd::BatchTask> fresh;
+  const std::vector<hd::BatchTask> *tasks = &batch->host_tasks[pi];
+  if (batch->dirty[pi])
+    {
+      std::vector<int> block_start;
+      make_tasks(batch, pi, {}, hd::collapsed_batch_blocks_per_xcd(pi), fresh,
+                 block_start);
+      tasks = &fresh;
+    }
+  long long units = 0, count = 0;
+  for (const hd::BatchTask &t : *tasks)
+    if (batch->ops[t.op]->mf->nd == N)
+      {
+        units += (t.le - t.lb + t.stride - 1) / t.stride;
+        ++count;
+      }
+  if (count == 0)
+    return (batch->cap[pi][N] > 0 ? batch->cap[pi][N] : 1) * cpb;
+  return (int)std::max(1LL, units * cpb / count);
 }
 
 int hd_batch_set_cells_per_task(hd_batch *batch, int degree, int precision,
@@ -426,7 +547,7 @@ int hd_batch_set_cells_per_task(hd_batch *batch, int degree, int precision,
   const int cpb = hd::collapsed_batch_cells_per_block(batch->D, N, pi);
   if (cpb <= 0)
     return fail(HD_ERR_UNSUPPORTED, "no batch kernel for this degree");
-  batch->units[pi][N] = std::max(1, (cells + cpb - 1) / cpb);
+  batch->cap[pi][N] = std::max(1, (cells + cpb - 1) / cpb);
   batch->dirty[pi] = true;
   return ok();
 }
@@ -452,13 +573,19 @@ int hd_batch_graph_create(hd_batch *batch, const char *type, int precision,
   auto *g = new (std::nothrow) hd_graph();
   if (!g)
     return fail(HD_ERR_NOMEM, "out of host memory");
-  // the graph keeps its own task list: later tuning does not change it
-  const int n_tasks = batch->seg[pi].start[8];
-  hipError_t e = hipMalloc(&g->tasks, std::max(n_tasks, 1) *
-                                        sizeof(hd::BatchTask));
-  if (e == hipSuccess && n_tasks > 0)
-    e = hipMemcpy(g->tasks, batch->dev_tasks[pi],
-                  n_tasks * sizeof(hd::BatchTask), hipMemcpyDeviceToDevice);
+  // the graph keeps its own work split: later tuning does not change it
+  const DevPlan &bp = batch->plan[pi];
+  hipError_t e = hipMalloc(&g->plan, bp.bytes);
+  if (e == hipSuccess)
+    e = hipMemcpy(g->plan, bp.mem, bp.bytes, hipMemcpyDeviceToDevice);
+  const char *base = static_cast<const char *>(g->plan);
+  const auto *gtasks = reinterpret_cast<const hd::BatchTask *>(
+    base + (reinterpret_cast<const char *>(bp.tasks) -
+            static_cast<const char *>(bp.mem)));
+  hd::BatchPlan gplan = bp.plan;
+  gplan.block_start = reinterpret_cast<const int *>(
+    base + (reinterpret_cast<const char *>(bp.plan.block_start) -
+            static_cast<const char *>(bp.mem)));
   hipStream_t s = nullptr;
   if (e == hipSuccess)
     e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
@@ -470,8 +597,8 @@ int hd_batch_graph_create(hd_batch *batch, const char *type, int precision,
   e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess)
     {
-      rc = batch_step(batch, type, precision, sol, ti_a, ti_b, dt, g->tasks,
-                      batch->seg[pi], s);
+      rc = batch_step(batch, type, precision, sol, ti_a, ti_b, dt, gtasks,
+                      gplan, s);
       const hipError_t e2 = hipStreamEndCapture(s, &g->graph);
       if (rc == HD_OK && e2 != hipSuccess)
         rc = hip_fail(e2, "hd_batch_graph_create: capture");
@@ -510,7 +637,7 @@ int hd_graph_destroy(hd_graph *graph)
     (void)hipGraphExecDestroy(graph->exec);
   if (graph->graph)
     (void)hipGraphDestroy(graph->graph);
-  (void)hipFree(graph->tasks);
+  (void)hipFree(graph->plan);
   delete graph;
   return ok();
 }

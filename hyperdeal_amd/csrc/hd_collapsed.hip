@@ -901,15 +901,15 @@ hipError_t dispatch(int D, int N, int mode, const T *src, T *dst,
 // block loops over tasks, switching on the operator's N into the same block
 // code as the single-operator kernel, with one LDS buffer sized for the
 // largest cell.
-// The batch kernel's blocks: one size for every degree of a launch.  FP32:
-// two waves, four blocks per CU (a quarter of the LDS each: 3 cells at N = 6,
-// 5 at N = 5, 8 at N = 4, the groups the single-operator kernels trim to);
-// FP64: four waves, two blocks per CU.  Both leave two waves per SIMD, 256
-// VGPRs for the 6 x 6 tiles of N = 6.
+// The batch kernel's blocks: one size for every degree of a launch, kBlock
+// threads and two blocks per CU (two waves per SIMD: 256 VGPRs for the 6 x 6
+// tiles of N = 6), each with half of the LDS (FP32: 7 cells at N = 6).  On
+// MI355X, 64 FP32-mixed 2+2 operators of degrees 2..5 stepped 2% faster so
+// than with four blocks of 128 threads (a quarter of the LDS each).
 template <typename T>
-constexpr int kBatchBlock = sizeof(T) == 4 ? 128 : kBlock;
+constexpr int kBatchBlock = kBlock;
 template <typename T>
-constexpr int kBatchBlocksPerCu = sizeof(T) == 4 ? 4 : 2;
+constexpr int kBatchBlocksPerCu = 2;
 template <typename T>
 constexpr int kBatchWavesPerEu =
   (kBatchBlocksPerCu<T> * kBatchBlock<T> / 64 + 3) / 4;
@@ -963,7 +963,7 @@ __device__ inline double uniform_f64(double x)
 template <int N, typename T, int MODE>
 __device__ __noinline__ void
 batch_task(T *lds, const T *src, T *dst, const T *coef, const BatchOp *op,
-           StageDev stage, long long lb, long long le)
+           StageDev stage, long long lb, long long le, int stride)
 {
   using S = BatchShape<N, T>;
   stage.a_dt = uniform_f64(stage.a_dt);
@@ -994,20 +994,22 @@ batch_task(T *lds, const T *src, T *dst, const T *coef, const BatchOp *op,
     MODE == 0 ? uniform_ptr(dst) : nullptr, uniform_ptr(coef), mesh, inflow,
     stage,
     (long long)uniform64((unsigned long long)lb),
-    (long long)uniform64((unsigned long long)le), 1);
+    (long long)uniform64((unsigned long long)le),
+    __builtin_amdgcn_readfirstlane(stride));
 }
 
 template <int D, typename T, int MODE>
 __global__ __launch_bounds__(kBatchBlock<T>, kBatchWavesPerEu<T>) void
   batch_kernel(const BatchOp *__restrict__ ops,
-               const BatchTask *__restrict__ tasks, BatchSegments seg,
+               const BatchTask *__restrict__ tasks, BatchPlan plan,
                BatchPtrs ptrs)
 {
   __shared__ __align__(16) T lds[batch_lds_elems<D, T>()];
-  // this XCD's segment of the task list, strided over its blocks
-  const int xcd = blockIdx.x % 8, slot = blockIdx.x / 8;
-  const int slots = gridDim.x / 8;
-  for (int t = seg.start[xcd] + slot; t < seg.start[xcd + 1]; t += slots)
+  // this block's tasks: its strided share of each operator fragment its
+  // XCD walks
+  const int b = (blockIdx.x % 8) * plan.per + blockIdx.x / 8;
+  const int t_end = plan.block_start[b + 1];
+  for (int t = plan.block_start[b]; t < t_end; ++t)
     {
       const BatchTask task = tasks[t];
       const BatchOp *op = ops + task.op;
@@ -1022,7 +1024,7 @@ __global__ __launch_bounds__(kBatchBlock<T>, kBatchWavesPerEu<T>) void
 #define HD_BCASE(NN_)                                                        \
   case NN_:                                                                  \
     batch_task<NN_, T, MODE>(lds, src, dst, coef, op, stage, task.lb,        \
-                             task.le);                                       \
+                             task.le, task.stride);                          \
     break;
           HD_BCASE(2) HD_BCASE(3) HD_BCASE(4) HD_BCASE(5) HD_BCASE(6)
 #undef HD_BCASE
@@ -1052,26 +1054,21 @@ __global__ void batch_convert_kernel(BatchPtrs ptrs, BatchSizes sizes)
 
 template <typename T>
 hipError_t batch_launch(int D, int mode, const BatchOp *ops,
-                        const BatchTask *tasks, const BatchSegments &seg,
+                        const BatchTask *tasks, const BatchPlan &plan,
                         const BatchPtrs &ptrs, hipStream_t stream)
 {
-  if (D != 4)
+  if (D != 4 || plan.per < 1 || !plan.block_start)
     return hipErrorInvalidValue;
-  int most = 0; // the longest segment
-  for (int x = 0; x < 8; ++x)
-    most = std::max(most, seg.start[x + 1] - seg.start[x]);
-  auto go = [&](auto kernel, int bpc) {
-    // the same number of blocks on every XCD, no more than its segment needs
-    const int per = std::min(most, bpc * cus_per_xcd());
-    hipLaunchKernelGGL(kernel, dim3(8 * per), dim3(kBatchBlock<T>), 0, stream,
-                       ops, tasks, seg, ptrs);
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(8 * plan.per), dim3(kBatchBlock<T>), 0,
+                       stream, ops, tasks, plan, ptrs);
   };
   if (mode == 0)
-    go(batch_kernel<4, T, 0>, kBatchBlocksPerCu<T>);
+    go(batch_kernel<4, T, 0>);
   else if (mode == 1)
-    go(batch_kernel<4, T, 1>, kBatchBlocksPerCu<T>);
+    go(batch_kernel<4, T, 1>);
   else
-    go(batch_kernel<4, T, 2>, kBatchBlocksPerCu<T>);
+    go(batch_kernel<4, T, 2>);
   return hipGetLastError();
 }
 
@@ -1128,16 +1125,21 @@ int collapsed_batch_cells_per_block(int D, int N, int precision)
     }
 }
 
+int collapsed_batch_blocks_per_xcd(int precision)
+{
+  return (precision == 0 ? collapsed::kBatchBlocksPerCu<double>
+                         : collapsed::kBatchBlocksPerCu<float>) *
+         collapsed::cus_per_xcd();
+}
+
 hipError_t collapsed_batch(int D, int precision, int mode, const BatchOp *ops,
-                           const BatchTask *tasks, const BatchSegments &seg,
+                           const BatchTask *tasks, const BatchPlan &plan,
                            const BatchPtrs &ptrs, hipStream_t stream)
 {
-  if (seg.start[8] <= seg.start[0])
-    return hipSuccess;
   return precision == 0
-           ? collapsed::batch_launch<double>(D, mode, ops, tasks, seg, ptrs,
+           ? collapsed::batch_launch<double>(D, mode, ops, tasks, plan, ptrs,
                                              stream)
-           : collapsed::batch_launch<float>(D, mode, ops, tasks, seg, ptrs,
+           : collapsed::batch_launch<float>(D, mode, ops, tasks, plan, ptrs,
                                             stream);
 }
 

@@ -147,19 +147,22 @@ struct BatchOp
   long long n_dofs;
 };
 
-// a block's work item: logical units [lb, le) of operator op
+// a block's work item: logical units lb, lb + stride, ... below le of
+// operator op
 struct BatchTask
 {
   int op;
-  int pad;
+  int stride;
   long long lb, le;
 };
 
-// the task list in one segment per XCD: tasks [start[x], start[x + 1]) run
-// on the blocks of XCD x (blocks are dispatched round-robin over the XCDs)
-struct BatchSegments
+// the work split of a batch launch: 8 x per blocks (blocks are dispatched
+// round-robin over the 8 XCDs); block `slot` of XCD x runs the tasks
+// [block_start[x per + slot], block_start[x per + slot + 1]) in order
+struct BatchPlan
 {
-  int start[9];
+  const int *block_start; // device array of 8 per + 1 entries
+  int per;                // blocks per XCD
 };
 
 // per call: the vectors of every operator (kernel argument) and the stage
@@ -196,8 +199,9 @@ hipError_t collapsed_apply_f32(int D, int N, int mode, const float *src,
 // every operator) and the batched FP64 -> precision copy of sol into dst
 bool collapsed_batch_supported(int D, int N);
 int collapsed_batch_cells_per_block(int D, int N, int precision);
+int collapsed_batch_blocks_per_xcd(int precision);
 hipError_t collapsed_batch(int D, int precision, int mode, const BatchOp *ops,
-                           const BatchTask *tasks, const BatchSegments &seg,
+                           const BatchTask *tasks, const BatchPlan &plan,
                            const BatchPtrs &ptrs, hipStream_t stream);
 hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,
                          const BatchPtrs &ptrs, hipStream_t stream);

@@ -185,13 +185,19 @@ int hd_batch_apply(hd_batch *batch, const void *const *src, void *const *dst,
 int hd_batch_lsrk_step(hd_batch *batch, const char *type, int precision,
                        double *const *sol, void *const *ti_a,
                        void *const *ti_b, double dt, void *stream);
-/* Autotuner: per degree, times the batched apply of that degree's operators
- * on src -> dst for each candidate work split (cells per task) and keeps the
- * fastest.  Synchronous on `stream`. */
+/* Work split: the operators are cut into fragments of contiguous cells
+ * (about 1/16 of the batch's cost each), the fragments are spread over the
+ * XCDs by cost, and each XCD's blocks walk its fragments interleaved.  The
+ * cost of a DoF depends on its degree; the autotuner measures it: it times
+ * the batched apply of each degree's operators alone on src -> dst, then
+ * scales each degree's cost while that makes the whole batch faster.
+ * Synchronous on `stream`. */
 int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
                       void *const *dst, void *stream);
-/* cells per task used for a degree (tuned or set), or an error */
+/* the mean cells per task of a degree in the current work split (a task: one
+ * block's share of one fragment), or an error */
 int hd_batch_cells_per_task(const hd_batch *batch, int degree, int precision);
+/* caps a degree's tasks at `cells` cells (rounded up to whole cell groups) */
 int hd_batch_set_cells_per_task(hd_batch *batch, int degree, int precision,
                                 int cells);
 /* a hipGraph of hd_batch_lsrk_step with these vectors and dt: each launch
