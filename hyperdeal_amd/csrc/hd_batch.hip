@@ -216,6 +216,19 @@ int ensure_tasks(hd_batch *b, int pi)
   return ok();
 }
 
+// the vectors of an LSRK step: distinct stage registers, 16-byte aligned
+// (the batched copy and the stage kernels use 16-byte accesses)
+bool step_vectors_ok(const hd_batch *b, double *const *sol,
+                     void *const *ti_a, void *const *ti_b)
+{
+  for (size_t i = 0; i < b->ops.size(); ++i)
+    if (ti_a[i] == ti_b[i] || reinterpret_cast<uintptr_t>(sol[i]) % 16 ||
+        reinterpret_cast<uintptr_t>(ti_a[i]) % 16 ||
+        reinterpret_cast<uintptr_t>(ti_b[i]) % 16)
+      return false;
+  return true;
+}
+
 bool valid_ptrs(const hd_batch *b, const void *const *p)
 {
   if (!p)
@@ -390,12 +403,10 @@ int hd_batch_lsrk_step(hd_batch *batch, const char *type, int precision,
       !valid_ptrs(batch, (const void *const *)ti_b) ||
       (precision != HD_FP64 && precision != HD_FP32))
     return fail(HD_ERR_INVALID, "hd_batch_lsrk_step: invalid argument");
-  for (size_t i = 0; i < batch->ops.size(); ++i)
-    if (ti_a[i] == ti_b[i] || reinterpret_cast<uintptr_t>(ti_a[i]) % 16 ||
-        reinterpret_cast<uintptr_t>(ti_b[i]) % 16)
-      return fail(HD_ERR_INVALID, "hd_batch_lsrk_step: the two stage "
-                                  "registers must differ and be 16-byte "
-                                  "aligned");
+  if (!step_vectors_ok(batch, sol, ti_a, ti_b))
+    return fail(HD_ERR_INVALID, "hd_batch_lsrk_step: the two stage "
+                                "registers must differ; sol and both "
+                                "registers must be 16-byte aligned");
   const int pi = prec_index(precision);
   const int rc = ensure_tasks(batch, pi);
   if (rc != HD_OK)
@@ -562,10 +573,10 @@ int hd_batch_graph_create(hd_batch *batch, const char *type, int precision,
       (precision != HD_FP64 && precision != HD_FP32))
     return fail(HD_ERR_INVALID, "hd_batch_graph_create: invalid argument");
   *out = nullptr;
-  for (size_t i = 0; i < batch->ops.size(); ++i)
-    if (ti_a[i] == ti_b[i])
-      return fail(HD_ERR_INVALID, "hd_batch_graph_create: the two stage "
-                                  "registers must differ");
+  if (!step_vectors_ok(batch, sol, ti_a, ti_b))
+    return fail(HD_ERR_INVALID, "hd_batch_graph_create: the two stage "
+                                "registers must differ; sol and both "
+                                "registers must be 16-byte aligned");
   const int pi = prec_index(precision);
   int rc = ensure_tasks(batch, pi);
   if (rc != HD_OK)

@@ -1037,19 +1037,32 @@ __global__ __launch_bounds__(kBatchBlock<T>, kBatchWavesPerEu<T>) void
     }
 }
 
+// dst[op] = (T) sol[op] for every operator: grid row op converts operator
+// op, four values per thread and step (two 16-byte loads, 16 or 32 bytes
+// stored), the last n % 4 values by block 0
 template <typename T>
 __global__ void batch_convert_kernel(BatchPtrs ptrs, BatchSizes sizes)
 {
-  const long long total = sizes.off[sizes.n];
-  int op = 0;
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x;
-       i < total; i += (long long)gridDim.x * blockDim.x)
+  const int op = blockIdx.y;
+  const long long n = sizes.off[op + 1] - sizes.off[op];
+  const double *__restrict__ src = ptrs.sol[op];
+  T *__restrict__ dst = static_cast<T *>(ptrs.dst[op]);
+  const long long n4 = n / 4;
+  for (long long q = blockIdx.x * (long long)blockDim.x + threadIdx.x; q < n4;
+       q += (long long)gridDim.x * blockDim.x)
     {
-      while (i >= sizes.off[op + 1])
-        ++op;
-      const long long j = i - sizes.off[op];
-      static_cast<T *>(ptrs.dst[op])[j] = (T)ptrs.sol[op][j];
+      const Vec<double, 2> a = reinterpret_cast<const Vec<double, 2> *>(src)[2 * q];
+      const Vec<double, 2> b =
+        reinterpret_cast<const Vec<double, 2> *>(src)[2 * q + 1];
+      Vec<T, 4> o;
+      o.v[0] = (T)a.v[0];
+      o.v[1] = (T)a.v[1];
+      o.v[2] = (T)b.v[0];
+      o.v[3] = (T)b.v[1];
+      reinterpret_cast<Vec<T, 4> *>(dst)[q] = o;
     }
+  if (blockIdx.x == 0 && threadIdx.x < n - 4 * n4)
+    dst[4 * n4 + threadIdx.x] = (T)src[4 * n4 + threadIdx.x];
 }
 
 template <typename T>
@@ -1148,17 +1161,25 @@ hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,
 {
   BatchSizes sizes{};
   sizes.n = n_ops;
+  long long most = 0;
   for (int i = 0; i < n_ops; ++i)
-    sizes.off[i + 1] = sizes.off[i] + n_dofs[i];
-  const long long total = sizes.off[n_ops];
-  if (total == 0)
+    {
+      sizes.off[i + 1] = sizes.off[i] + n_dofs[i];
+      most = std::max(most, n_dofs[i]);
+      // 16-byte accesses: every vector starts 16-byte aligned
+      if (reinterpret_cast<uintptr_t>(ptrs.sol[i]) % 16 != 0 ||
+          reinterpret_cast<uintptr_t>(ptrs.dst[i]) % 16 != 0)
+        return hipErrorInvalidValue;
+    }
+  if (n_ops == 0 || most == 0)
     return hipSuccess;
-  const int grid = (int)std::min<long long>((total + 255) / 256, 8192);
+  const dim3 grid((unsigned)std::min<long long>((most / 4 + 255) / 256, 1024),
+                  (unsigned)n_ops);
   if (precision == 0)
-    hipLaunchKernelGGL(collapsed::batch_convert_kernel<double>, dim3(grid),
+    hipLaunchKernelGGL(collapsed::batch_convert_kernel<double>, grid,
                        dim3(256), 0, stream, ptrs, sizes);
   else
-    hipLaunchKernelGGL(collapsed::batch_convert_kernel<float>, dim3(grid),
+    hipLaunchKernelGGL(collapsed::batch_convert_kernel<float>, grid,
                        dim3(256), 0, stream, ptrs, sizes);
   return hipGetLastError();
 }
