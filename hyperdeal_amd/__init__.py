@@ -56,7 +56,7 @@ EXPORTED = [
     "hd_vp_learned_guess_stats", "hd_vp_learned_guess_net",
     "hd_cnn_n_params", "hd_cnn_create", "hd_cnn_destroy", "hd_cnn_params",
     "hd_cnn_forward", "hd_cnn_loss_grad", "hd_cnn_train_step",
-    "hd_cnn_last_loss",
+    "hd_cnn_last_loss", "hd_vp_exact_start",
 ]
 BATCH_MAX = 64
 
@@ -136,6 +136,7 @@ def load_library(path=None):
                                        ctypes.POINTER(ctypes.c_longlong), _dp]
     _llp = ctypes.POINTER(ctypes.c_longlong)
     lib.hd_vp_solver_start.argtypes = [_vp, _dp, _dp, _dp, _llp]
+    lib.hd_vp_exact_start.argtypes = [_vp, ctypes.c_int]
     lib.hd_vp_learned_guess_enable.argtypes = [
         _vp, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         ctypes.c_int, ctypes.c_float, ctypes.c_int]
@@ -384,6 +385,13 @@ class VlasovPoissonOperation(AdvectionOperation):
                                             ctypes.byref(g)),
                "hd_vp_solver_start")
         return r0.value, rp.value, (w[0], w[1]), g.value
+
+    def exact_start(self, enable=True):
+        """Start the potential solves from the exact solution of the
+        periodic 2D SIPG system (fast diagonalisation); CG still runs to the
+        tolerance from there.  Disables a learned guess."""
+        _check(self._lib.hd_vp_exact_start(self.handle, 1 if enable else 0),
+               "hd_vp_exact_start")
 
     def enable_learned_guess(self, seed=20261205, capacity=64, train_every=1,
                              train_batch=8, train_steps=1,

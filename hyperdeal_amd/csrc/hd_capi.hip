@@ -73,10 +73,16 @@ int vp_update(hd_advection *op, const double *f, const float *f32,
   const double *guess = nullptr;
   if (e == hipSuccess && op->learned)
     e = hd_abi::learned_before_solve(*op, &guess, stream);
+  else if (e == hipSuccess && op->fdm_on)
+    {
+      e = hd::vp_fdm_solve(*op->fdm, op->ptab, op->rho, stream);
+      guess = op->fdm->x;
+    }
   if (e == hipSuccess)
     e = hd::vp_poisson(op->xdev, op->mf->nd, op->rho, op->phi, guess,
                        op->efield, op->work, op->poisson_tol,
-                       op->poisson_max_it, op->stats, stream, op->ptab);
+                       op->poisson_max_it, op->stats, stream, op->ptab,
+                       nullptr, !op->learned && op->fdm_on);
   if (e == hipSuccess && op->learned)
     e = hd_abi::learned_after_solve(*op, stream);
   return e == hipSuccess ? HD_OK : hip_fail(e, "Vlasov-Poisson field update");
@@ -430,6 +436,11 @@ int hd_advection_destroy(hd_advection *op)
     if (p)
       (void)hipFree(p);
   hd_abi::learned_release(op->learned);
+  if (op->fdm)
+    {
+      hd::vp_fdm_destroy(*op->fdm);
+      delete op->fdm;
+    }
   delete op;
   return ok();
 }
@@ -661,6 +672,50 @@ int hd_vp_solver_start(hd_advection *op, double *start_rel_residual,
     std::memcpy(weights, &s[6], 2 * sizeof(double));
   if (guessed_solves)
     *guessed_solves = s[5];
+  return ok();
+}
+
+int hd_vp_exact_start(hd_advection *op, int enable)
+{
+  if (!op || !op->vp)
+    return fail(HD_ERR_INVALID, "hd_vp_exact_start: not a Vlasov-Poisson "
+                                "operator");
+  if (!enable)
+    {
+      op->fdm_on = false;
+      return ok();
+    }
+  const hd_matrix_free *mf = op->mf;
+  const int N = mf->nd;
+  if (!op->ptab ||
+      !hd::vp_fdm_supported(mf->dim_x, N, mf->mesh.n, op->n_x_cells))
+    return fail(HD_ERR_UNSUPPORTED,
+                "the exact start needs the 2D tensor-product Poisson solver "
+                "(dim_x = 2, k <= 3, at most 1024 x cells) and at most 256 "
+                "x nodes per direction");
+  if (!op->fdm)
+    {
+      std::vector<double> tab(8 * N * N);
+      hd::vp_poisson2d_table(op->xdev, N, tab.data());
+      auto *f = new (std::nothrow) hd::FdmDev();
+      if (!f)
+        return fail(HD_ERR_NOMEM, "out of host memory");
+      const hipError_t e =
+        hd::vp_fdm_create(tab.data(), N, mf->mesh.n[0], mf->mesh.n[1], f);
+      if (e != hipSuccess)
+        {
+          delete f;
+          return hip_fail(e, "hd_vp_exact_start");
+        }
+      op->fdm = f;
+    }
+  if (op->learned)
+    {
+      const int rc = hd_vp_learned_guess_disable(op);
+      if (rc != HD_OK)
+        return rc;
+    }
+  op->fdm_on = true;
   return ok();
 }
 

@@ -249,12 +249,15 @@ bool vp_poisson_supported(int dim_x, int N, long long n_x_cells);
 // tab2d: the 2D tensor-product SIPG table (vp_poisson2d_table, 8 N^2
 // doubles on the device) selecting the fast 2D solver where it applies, or
 // null.  residual_out (2D solver only): write b - A phi there and stop
-// (phi, efield and stats untouched).
+// (phi, efield and stats untouched).  guess (2D solver: its nodal mean
+// removed): CG starts from the energy-minimising combination of phi and the
+// guess, or from the guess alone when guess_alone (an exact start).
 hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
                       const double *guess, double *efield, double *work,
                       double rel_tol, int max_it, long long *stats,
                       hipStream_t stream, const double *tab2d = nullptr,
-                      double *residual_out = nullptr);
+                      double *residual_out = nullptr,
+                      bool guess_alone = false);
 bool vp_poisson2d_supported(int dim_x, int N, long long n_x_cells);
 // the 2+2 Vlasov-Poisson apply with the velocity (v, grad phi): a table of
 // vp_apply_table_size(N) doubles (the unit-speed 1D operators of the four
@@ -271,6 +274,27 @@ hipError_t vp_apply(int N, int mode, int precision, const void *src, void *dst,
                     long long n_x_cells, const double *efield,
                     const StageDev &stage, hipStream_t stream);
 void vp_poisson2d_table(const XDev &xd, int N, double *tab);
+// The exact start of the 2D solver by fast diagonalisation (hd_fdm.hip):
+// the generalised eigenvectors of the two 1D periodic SIPG operators on the
+// device; vp_fdm_solve writes the solution of the system the 2D solver
+// forms from rho into x (cell-major; one plus four small launches)
+struct FdmDev
+{
+  double *mem = nullptr;
+  double *V0 = nullptr, *V1 = nullptr; // (n_d N)^2 each, row-major
+  double *inv = nullptr;               // 1 / (lambda1_r + lambda0_c)
+  double *B = nullptr, *T = nullptr;   // work arrays, n1 N x n0 N
+  double *x = nullptr;                 // the solution, cell-major
+  int N = 0, n0 = 0, n1 = 0;
+};
+bool vp_fdm_supported(int dim_x, int N, const int *n, long long n_x_cells);
+// tab_host: the table of vp_poisson2d_table
+hipError_t vp_fdm_create(const double *tab_host, int N, int n0, int n1,
+                         FdmDev *out);
+void vp_fdm_destroy(FdmDev &f);
+// tab: the same table on the device
+hipError_t vp_fdm_solve(const FdmDev &f, const double *tab, const double *rho,
+                        hipStream_t stream);
 hipError_t vp_moments(const double *f, const MeshDev &mesh, const NormDev &nt,
                       int dim_x, int vol, double *partial, hipStream_t stream);
 hipError_t launch_norm(const double *src, const MeshDev &mesh,

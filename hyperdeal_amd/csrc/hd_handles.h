@@ -106,6 +106,9 @@ struct hd_advection
   double *vtab = nullptr;     // tables of the 2+2 phase-space apply, or null
   double *rho_part = nullptr; // density partials written by a stage
   hd_learned_guess *learned = nullptr;
+  // the exact start of the 2D potential solve (fast diagonalisation), or null
+  hd::FdmDev *fdm = nullptr;
+  bool fdm_on = false;
 };
 
 namespace hd_abi

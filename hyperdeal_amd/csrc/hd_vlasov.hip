@@ -589,7 +589,8 @@ __global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
   const double *__restrict__ rho_in, double *__restrict__ phi,
   const double *__restrict__ guess, double *__restrict__ efield,
   double *__restrict__ work, double rel_tol, int max_it,
-  long long *__restrict__ stats, double *__restrict__ residual_out)
+  long long *__restrict__ stats, double *__restrict__ residual_out,
+  int guess_alone)
 {
   constexpr int NN = N * N, T = kSolverThreads / CPT, W = T / 64;
   extern __shared__ double sm[];
@@ -690,6 +691,17 @@ __global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
   double alpha0 = 1.0, beta0 = 0.0, rr_warm = -1.0;
   if (guess)
     {
+      // the guess without its nodal mean: A g is the same, and the start
+      // keeps the zero nodal mean of the iterates (Krylov updates lie in the
+      // range of A, orthogonal to the constants)
+      double sg = 0.0;
+#pragma unroll
+      for (int k = 0; k < CPT; ++k)
+        if (own[k])
+#pragma unroll
+          for (int i = 0; i < NN; ++i)
+            sg += guess[(long long)cl[k] * NN + i];
+      const double mean_g = block_sum<W, 1>(sg, red) / nx;
       double d[6] = {0, 0, 0, 0, 0, 0};
       __syncthreads(); // every thread has read Pt
 #pragma unroll
@@ -697,49 +709,68 @@ __global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
         if (own[k])
 #pragma unroll
           for (int i = 0; i < NN; ++i)
-            Pt[i * kPStride + cl[k]] = guess[(long long)cl[k] * NN + i];
+            Pt[i * kPStride + cl[k]] =
+              guess[(long long)cl[k] * NN + i] - mean_g;
       __syncthreads();
+      if (guess_alone)
+        {
+          // an exact guess: the start is the guess itself (a combination
+          // with a previous phi that is nearly parallel to it would be an
+          // ill-conditioned 2 x 2 system)
 #pragma unroll
-      for (int k = 0; k < CPT; ++k)
-        if (own[k])
-          {
-            double Ag[NN];
-            sipg2d<N>(Pt, n0, n1, cx[k], cy[k], tab, Ag);
+          for (int k = 0; k < CPT; ++k)
 #pragma unroll
             for (int i = 0; i < NN; ++i)
-              {
-                const double x = phit[(long long)i * nc + cl[k]];
-                const double g = guess[(long long)cl[k] * NN + i];
-                d[0] += x * y[k][i];
-                d[1] += x * Ag[i];
-                d[2] += g * Ag[i];
-                d[3] += x * r[k][i];
-                d[4] += g * r[k][i];
-                d[5] += (r[k][i] - y[k][i]) * (r[k][i] - y[k][i]);
-              }
-          }
-      d[0] = block_sum<W, 1>(d[0], red);
-      d[1] = block_sum<W, 0>(d[1], red);
-      d[2] = block_sum<W, 1>(d[2], red);
-      d[3] = block_sum<W, 0>(d[3], red);
-      d[4] = block_sum<W, 1>(d[4], red);
-      d[5] = block_sum<W, 0>(d[5], red);
-      rr_warm = d[5];
-      const double det = d[0] * d[2] - d[1] * d[1];
-      if (d[0] > 0.0 && d[2] > 0.0 && det > 1e-12 * d[0] * d[2])
-        {
-          alpha0 = (d[3] * d[2] - d[4] * d[1]) / det;
-          beta0 = (d[0] * d[4] - d[1] * d[3]) / det;
-        }
-      else if (!(d[0] > 0.0) && d[2] > 0.0)
-        {
+              d[5] += (r[k][i] - y[k][i]) * (r[k][i] - y[k][i]);
+          rr_warm = block_sum<W, 0>(d[5], red);
           alpha0 = 0.0;
-          beta0 = d[4] / d[2];
+          beta0 = 1.0;
         }
-      if (!isfinite(alpha0) || !isfinite(beta0))
+      else
         {
-          alpha0 = 1.0;
-          beta0 = 0.0;
+#pragma unroll
+          for (int k = 0; k < CPT; ++k)
+            if (own[k])
+              {
+                double Ag[NN];
+                sipg2d<N>(Pt, n0, n1, cx[k], cy[k], tab, Ag);
+#pragma unroll
+                for (int i = 0; i < NN; ++i)
+                  {
+                    const double x = phit[(long long)i * nc + cl[k]];
+                    const double g =
+                      guess[(long long)cl[k] * NN + i] - mean_g;
+                    d[0] += x * y[k][i];
+                    d[1] += x * Ag[i];
+                    d[2] += g * Ag[i];
+                    d[3] += x * r[k][i];
+                    d[4] += g * r[k][i];
+                    d[5] += (r[k][i] - y[k][i]) * (r[k][i] - y[k][i]);
+                  }
+              }
+          d[0] = block_sum<W, 1>(d[0], red);
+          d[1] = block_sum<W, 0>(d[1], red);
+          d[2] = block_sum<W, 1>(d[2], red);
+          d[3] = block_sum<W, 0>(d[3], red);
+          d[4] = block_sum<W, 1>(d[4], red);
+          d[5] = block_sum<W, 0>(d[5], red);
+          rr_warm = d[5];
+          const double det = d[0] * d[2] - d[1] * d[1];
+          if (d[0] > 0.0 && d[2] > 0.0 && det > 1e-12 * d[0] * d[2])
+            {
+              alpha0 = (d[3] * d[2] - d[4] * d[1]) / det;
+              beta0 = (d[0] * d[4] - d[1] * d[3]) / det;
+            }
+          else if (!(d[0] > 0.0) && d[2] > 0.0)
+            {
+              alpha0 = 0.0;
+              beta0 = d[4] / d[2];
+            }
+          if (!isfinite(alpha0) || !isfinite(beta0))
+            {
+              alpha0 = 1.0;
+              beta0 = 0.0;
+            }
         }
       // x = alpha phi + beta g, A x = alpha A phi + beta A g (A g again:
       // the direction buffer still holds g)
@@ -753,8 +784,9 @@ __global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
             for (int i = 0; i < NN; ++i)
               {
                 const long long gi = (long long)i * nc + cl[k];
-                phit[gi] = alpha0 * phit[gi] +
-                           beta0 * guess[(long long)cl[k] * NN + i];
+                phit[gi] =
+                  alpha0 * phit[gi] +
+                  beta0 * (guess[(long long)cl[k] * NN + i] - mean_g);
                 y[k][i] = alpha0 * y[k][i] + beta0 * Ag[i];
               }
           }
@@ -1546,7 +1578,7 @@ hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
                       const double *guess, double *efield, double *work,
                       double rel_tol, int max_it, long long *stats,
                       hipStream_t stream, const double *tab2d,
-                      double *residual_out)
+                      double *residual_out, bool guess_alone)
 {
   if (residual_out &&
       !(tab2d && vp_poisson2d_supported(xd.dim, N, xd.n_cells)))
@@ -1568,7 +1600,7 @@ hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
       hipLaunchKernelGGL((vlasov::poisson2d_kernel<NN_, CPT>), dim3(1),        \
                          dim3(vlasov::kSolverThreads / CPT), lds, stream, xd,  \
                          tab2d, rho, phi, guess, efield, work, rel_tol,        \
-                         max_it, stats, residual_out);                         \
+                         max_it, stats, residual_out, guess_alone ? 1 : 0);    \
       return hipGetLastError();                                                \
     }
       HD_VP2(2) HD_VP2(3) HD_VP2(4)

@@ -237,6 +237,17 @@ int hd_vp_solver_stats(hd_advection *op, long long *iterations,
 int hd_vp_solver_start(hd_advection *op, double *start_rel_residual,
                        double *previous_rel_residual, double *weights,
                        long long *guessed_solves);
+/* Exact start of the potential solve (enable != 0; off by default, when CG
+ * starts from the previous phi as in the reference, vlasov_poisson/include/
+ * application.h:575-576).  No counterpart in the reference.  On a periodic
+ * 2D x mesh the SIPG system separates into 1D operators per direction; their
+ * generalised eigenvectors (computed once, on the host, at the first enable)
+ * solve it with four dense products of (cells x nodes per direction)-sized
+ * matrices, and CG starts from that solution and checks it to the same
+ * tolerance (typically zero or one iteration).  dim_x == 2, the 2D
+ * tensor-product solver (k <= 3, at most 1024 x cells), at most 256 x nodes
+ * per direction.  Enabling it disables a learned guess. */
+int hd_vp_exact_start(hd_advection *op, int enable);
 /* x-space DoFs of phi / rho: (k+1)^dim_x per x cell */
 long long hd_vp_x_n_dofs(const hd_advection *op);
 /* copy phi device -> host (device_to_host != 0) or host -> device */
