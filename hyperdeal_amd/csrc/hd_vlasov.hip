@@ -1644,18 +1644,37 @@ hipError_t vp_moments(const double *f, const MeshDev &mesh, const NormDev &nt,
 
 namespace vlasov
 {
-// rho[i] = sum over v cells of part[vc][i], in v-cell order
-__global__ void density_from_partials(const double *__restrict__ part,
-                                      long long n_v_cells, long long n_x_dofs,
-                                      double *__restrict__ rho)
+// rho[i] = sum over v cells of part[vc][i].  A block takes kDensX x nodes;
+// its thread (x, g) sums the v cells g, g + kDensG, ... in order, then the
+// kDensG partial sums are added in group order (a fixed order: the result
+// does not depend on the launch).  Many blocks and independent loads per
+// thread keep enough reads in flight (one thread per x node, walking all v
+// cells, left the GPU idle).
+constexpr int kDensX = 32, kDensG = 8;
+__global__ __launch_bounds__(kDensX *kDensG) void density_from_partials(
+  const double *__restrict__ part, long long n_v_cells, long long n_x_dofs,
+  double *__restrict__ rho)
 {
-  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= n_x_dofs)
-    return;
+  __shared__ double sum[kDensG][kDensX];
+  const int x = threadIdx.x % kDensX, g = threadIdx.x / kDensX;
+  const long long i = (long long)blockIdx.x * kDensX + x;
   double s = 0.0;
-  for (long long vc = 0; vc < n_v_cells; ++vc)
-    s += part[vc * n_x_dofs + i];
-  rho[i] = s;
+  if (i < n_x_dofs)
+    {
+#pragma unroll 8
+      for (long long vc = g; vc < n_v_cells; vc += kDensG)
+        s += part[vc * n_x_dofs + i];
+    }
+  sum[g][x] = s;
+  __syncthreads();
+  if (g == 0 && i < n_x_dofs)
+    {
+      double r = sum[0][x];
+#pragma unroll
+      for (int k = 1; k < kDensG; ++k)
+        r += sum[k][x];
+      rho[i] = r;
+    }
 }
 } // namespace vlasov
 
@@ -1663,9 +1682,11 @@ hipError_t vp_density_from_partials(const double *part, long long n_v_cells,
                                     long long n_x_dofs, double *rho,
                                     hipStream_t stream)
 {
-  hipLaunchKernelGGL(vlasov::density_from_partials,
-                     dim3((unsigned)((n_x_dofs + 255) / 256)), dim3(256), 0,
-                     stream, part, n_v_cells, n_x_dofs, rho);
+  hipLaunchKernelGGL(
+    vlasov::density_from_partials,
+    dim3((unsigned)((n_x_dofs + vlasov::kDensX - 1) / vlasov::kDensX)),
+    dim3(vlasov::kDensX * vlasov::kDensG), 0, stream, part, n_v_cells,
+    n_x_dofs, rho);
   return hipGetLastError();
 }
 

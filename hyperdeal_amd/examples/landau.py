@@ -20,6 +20,9 @@ Runs (any subset, in this order):
                 phi and the network's correction for its residual; the
                 network trains online on the run's own solves
   fp64-learned  the same with the FP64 apply
+  fp64-exact    FP64, CG from the exact solution of the periodic SIPG system
+                (fast diagonalisation, hd_vp_exact_start)
+  fp32-exact    the same with the FP32-mixed apply
 
 Reported per run: wall time per step, CG iterations per solve, the share
 of the step spent on the field update (density, network, CG, training:
@@ -75,6 +78,8 @@ def run(H, torch, name, args, ref_f=None):
     mf = H.MatrixFree(2, 2, k, [n] * 4, [0, 0, -6, -6], [L, L, 6, 6])
     op = H.VlasovPoissonOperation(mf, poisson_tol=1e-10)
     prec = H.FP64 if name.startswith("fp64") else H.FP32
+    if name.endswith("-exact"):
+        op.exact_start()
     if name.endswith("-learned"):
         op.enable_learned_guess(seed=20261205, capacity=64, train_every=1,
                                 train_batch=8, train_steps=1,
