@@ -167,6 +167,61 @@ def batch_workload(H, torch, stream, steps):
             "cells_per_task": tuned}
 
 
+def landau_workload(H, torch, stream, steps):
+    """2+2 Vlasov-Poisson Landau damping, x in [0, 4 pi]^2, v in [-6, 6]^2,
+    32^4 cells, k=3 (268M DoF), rk45 at CFL 0.3: the time per step, the CG
+    iterations per Poisson solve (relative residual 1e-10) and the share of
+    the step spent on the field update (one minus frozen-field step time over
+    step time), for the Poisson start from the previous phi (FP64 and
+    FP32-mixed), the learned start (FP32-mixed; trained online from the
+    run's first solves) and the exact start (FP32-mixed).  The damping rate
+    needs the run to t = 10: hyperdeal_amd/examples/landau.py."""
+    import numpy as np
+    L, n, k = 4 * np.pi, 32, 3
+    dt = 0.3 * (L / n / 6.0) / k ** 1.5
+    integ = H.LowStorageRungeKuttaIntegrator("rk45")
+    out = {"workload": "Landau 2D+2D, 32^4 cells, k=3 (268435456 DoF), "
+                       "rk45, dt %.5f, Poisson CG to 1e-10" % dt,
+           "runs": {}}
+    for name in ("fp64", "fp32", "fp32-learned", "fp32-exact"):
+        mf = H.MatrixFree(2, 2, k, [n] * 4, [0, 0, -6, -6], [L, L, 6, 6])
+        op = H.VlasovPoissonOperation(mf, poisson_tol=1e-10)
+        if name.endswith("-exact"):
+            op.exact_start()
+        if name.endswith("-learned"):
+            op.enable_learned_guess(seed=20261205, capacity=64,
+                                    train_every=1, train_batch=8,
+                                    train_steps=1, learning_rate=1e-3,
+                                    min_pairs=8)
+        prec = H.FP64 if name == "fp64" else H.FP32
+        dtype = torch.float64 if prec == H.FP64 else torch.float32
+        f = mf.initialize_dof_vector()
+        H.VectorTools.interpolate(mf, f, H.FUNC_LANDAU, [0.01, 0.5, 2.0])
+        ta = torch.empty(f.numel(), dtype=dtype, device="cuda")
+        tb = torch.empty_like(ta)
+
+        def step():
+            integ.perform_time_step(op, f, ta, tb, 0.0, dt, precision=prec,
+                                    stream=stream)
+        for _ in range(3):
+            step()
+        torch.cuda.synchronize()
+        it0, s0, _ = op.solver_stats()
+        t = timed(step, steps, 0, stream)
+        it1, s1, _ = op.solver_stats()
+        op.set_auto_update(False)
+        t_frozen = timed(step, steps, 0, stream)
+        out["runs"][name] = {
+            "ms_per_step": round(t * 1e3, 3),
+            "ms_per_step_frozen_field": round(t_frozen * 1e3, 3),
+            "field_update_share": round(1.0 - t_frozen / t, 4),
+            "cg_iterations_per_solve": round((it1 - it0) / max(s1 - s0, 1),
+                                             2)}
+        del op, f, ta, tb
+        torch.cuda.empty_cache()
+    return out
+
+
 def gl_weights(N):
     """Gauss-Lobatto quadrature weights of N points on [0, 1]."""
     import numpy as np
@@ -275,7 +330,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--only", choices=["all", "fp64", "fp32", "stage",
-                                       "batch", "c3"],
+                                       "batch", "c3", "landau"],
                     default="all", help="restrict the timed kernels "
                     "(profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -341,12 +396,15 @@ def main():
     torch.cuda.synchronize()
     del u
     torch.cuda.empty_cache()
-    batch_result = c3_result = None
+    batch_result = c3_result = landau_result = None
     if args.only in ("all", "c3"):
         c3_result = c3_workload(H, torch, stream, max(3, args.steps // 10))
     if args.only in ("all", "batch"):
         batch_result = batch_workload(H, torch, stream, max(3, args.steps // 5))
         torch.cuda.empty_cache()
+    if args.only in ("all", "landau"):
+        landau_result = landau_workload(H, torch, stream,
+                                        max(5, args.steps // 5))
 
     def gdofs(t):
         return n / t / 1e9
@@ -402,6 +460,8 @@ def main():
         line["batch_2d2d"] = batch_result
     if c3_result is not None:
         line["c3_3d3d_k5"] = c3_result
+    if landau_result is not None:
+        line["landau_2d2d"] = landau_result
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(velocity)
     print(json.dumps(line), flush=True)
