@@ -324,6 +324,7 @@ int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
       return hip_fail(e, "hd_vp_learned_guess_enable");
     }
   op->learned = lg;
+  op->fdm_on = false; // the two starts exclude each other
   return ok();
 }
 
