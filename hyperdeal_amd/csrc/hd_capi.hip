@@ -582,6 +582,13 @@ int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
       std::copy(t.qpoints.begin(), t.qpoints.begin() + nd, o + 2 * nd * nd);
       std::copy(op->vpdev.wv, op->vpdev.wv + nd * nd,
                 o + 2 * nd * nd + nd);
+      // followed by its FP32 copy (the FP32-mixed kernel's coefficients)
+      const size_t n_tab = tab.size();
+      std::vector<float> tf(n_tab);
+      for (size_t i = 0; i < n_tab; ++i)
+        tf[i] = (float)tab[i];
+      tab.resize(n_tab + (n_tab + 1) / 2, 0.0);
+      std::memcpy(tab.data() + n_tab, tf.data(), n_tab * sizeof(float));
       e = hipMalloc(&op->vtab, tab.size() * sizeof(double));
       if (e == hipSuccess)
         e = hipMemcpy(op->vtab, tab.data(), tab.size() * sizeof(double),

@@ -260,7 +260,8 @@ hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
                       bool guess_alone = false);
 bool vp_poisson2d_supported(int dim_x, int N, long long n_x_cells);
 // the 2+2 Vlasov-Poisson apply with the velocity (v, grad phi): a table of
-// vp_apply_table_size(N) doubles (the unit-speed 1D operators of the four
+// vp_apply_table_size(N) doubles, then the same values as floats (the
+// FP32-mixed kernel's coefficients) (the unit-speed 1D operators of the four
 // directions for both signs, S, S^-1, Gauss points), precision 0 FP64, 1
 // FP32-mixed; mode 0 apply, 1 LSRK stage
 bool vp_apply_supported(int dim_x, int dim_v, int N);

@@ -1034,6 +1034,15 @@ __global__ __launch_bounds__(256) void apply_kernel(
   constexpr int NN = N * N, C = NN * NN, F = MP::F, FP = MP::FP;
   constexpr int s1 = N, s2 = NN, s3 = N * NN; // digit strides
   const cptr<double> tab = (cptr<double>)tab_in;
+  // the 1D coefficients in the arithmetic's precision: FP32 reads the float
+  // copy that follows the table (no conversion per use)
+  const cptr<float> tabf = (cptr<float>)(tab_in + TB::size);
+  auto tc = [&](int i) -> T {
+    if constexpr (sizeof(T) == 4)
+      return tabf[i];
+    else
+      return tab[i];
+  };
   __shared__ T U[C], T0p[C], T0n[C], T1p[C], T1n[C], Q1[C], Q2[C];
   __shared__ T FL[4][F], FR[4][F], G1[2][2][F], G2[2][2][F];
   __shared__ T V[2][2][NN]; // [d][+/-][i][l]
@@ -1179,23 +1188,23 @@ __global__ __launch_bounds__(256) void apply_kernel(
           for (int j = 0; j < N; ++j)
             u[j] = ld(U, b + j);
           {
-              T a = (T)tab[TB::bp(0) + i] * FL[0][l];
+              T a = tc(TB::bp(0) + i) * FL[0][l];
 #pragma unroll
               for (int j = 0; j < N; ++j)
-                a += (T)tab[TB::Ap(0) + i * N + j] * u[j];
+                a += tc(TB::Ap(0) + i * N + j) * u[j];
               T0p[b + i] = a;
             }
           {
-              T a = (T)tab[TB::bn(0) + i] * FR[0][l];
+              T a = tc(TB::bn(0) + i) * FR[0][l];
 #pragma unroll
               for (int j = 0; j < N; ++j)
-                a += (T)tab[TB::An(0) + i * N + j] * u[j];
+                a += tc(TB::An(0) + i * N + j) * u[j];
               T0n[b + i] = a;
             }
           T a = 0;
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::S + i * N + j] * u[j];
+            a += tc(TB::S + i * N + j) * u[j];
           Q1[b + i] = a;
         }
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
@@ -1209,17 +1218,17 @@ __global__ __launch_bounds__(256) void apply_kernel(
           for (int j = 0; j < N; ++j)
             u[j] = ld(U, b + j * s1);
           {
-              T a = (T)tab[TB::bp(1) + i] * FL[1][l];
+              T a = tc(TB::bp(1) + i) * FL[1][l];
 #pragma unroll
               for (int j = 0; j < N; ++j)
-                a += (T)tab[TB::Ap(1) + i * N + j] * u[j];
+                a += tc(TB::Ap(1) + i * N + j) * u[j];
               T1p[b + i * s1] = a;
             }
           {
-              T a = (T)tab[TB::bn(1) + i] * FR[1][l];
+              T a = tc(TB::bn(1) + i) * FR[1][l];
 #pragma unroll
               for (int j = 0; j < N; ++j)
-                a += (T)tab[TB::An(1) + i * N + j] * u[j];
+                a += tc(TB::An(1) + i * N + j) * u[j];
               T1n[b + i * s1] = a;
             }
         }
@@ -1234,7 +1243,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = 0;
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::S + i * N + j] * ld(fsrc, j + N * rest);
+            a += tc(TB::S + i * N + j) * ld(fsrc, j + N * rest);
           G1[ee][side][i + N * rest] = a;
         }
       lds_sync();
@@ -1249,7 +1258,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = 0;
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::S + i * N + j] * ld(Q1, b + j * s1);
+            a += tc(TB::S + i * N + j) * ld(Q1, b + j * s1);
           Q2[b + i * s1] = a;
         }
       for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
@@ -1261,7 +1270,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = 0;
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::S + i * N + j] *
+            a += tc(TB::S + i * N + j) *
                  ld(G1[ee][side], x0 + N * (j + N * vo));
           G2[ee][side][x0 + N * (i + N * vo)] = a;
         }
@@ -1319,13 +1328,13 @@ __global__ __launch_bounds__(256) void apply_kernel(
                 q[j] = ld(Q2, b + j * sd);
               // face index of the line: (x0, x1, other v) = (qx, v_other)
               const int f = qx + NN * (b / (ee == 0 ? s3 : s2) % N);
-              T yp = (T)tab[TB::bp(d) + i] * G2[ee][0][f];
-              T yn = (T)tab[TB::bn(d) + i] * G2[ee][1][f];
+              T yp = tc(TB::bp(d) + i) * G2[ee][0][f];
+              T yn = tc(TB::bn(d) + i) * G2[ee][1][f];
 #pragma unroll
               for (int j = 0; j < N; ++j)
                 {
-                  yp += (T)tab[TB::Ap(d) + i * N + j] * q[j];
-                  yn += (T)tab[TB::An(d) + i * N + j] * q[j];
+                  yp += tc(TB::Ap(d) + i * N + j) * q[j];
+                  yn += tc(TB::An(d) + i * N + j) * q[j];
                 }
               Yd[b + i * sd] = c * (c >= (T)0 ? yp : yn);
             }
@@ -1341,7 +1350,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = 0;
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::Si + i * N + j] * (ld(Y0, b + j) + ld(Y1, b + j));
+            a += tc(TB::Si + i * N + j) * (ld(Y0, b + j) + ld(Y1, b + j));
           Z[b + i] = a;
         }
       lds_sync();
@@ -1355,7 +1364,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           T a = ld(R0, b + i * s1) + ld(R1, b + i * s1);
 #pragma unroll
           for (int j = 0; j < N; ++j)
-            a += (T)tab[TB::Si + i * N + j] * ld(Z, b + j * s1);
+            a += tc(TB::Si + i * N + j) * ld(Z, b + j * s1);
           O[b + i * s1] = a;
         }
       lds_sync();
