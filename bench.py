@@ -28,6 +28,11 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
+# MI355X dense vector peaks (TFLOP/s)
+FP64_PEAK_TFLOPS = 78.6
+FP32_PEAK_TFLOPS = 157.3
+# the collapsed apply's arithmetic per DoF at 3+3, k=3: D (N + 1) FMA
+FLOP_PER_DOF = 2 * 6 * (3 + 2)
 SEED_VELOCITY = 20261202
 SEED_NOISE = 20261203
 # HBM bytes per FP64 apply launch, measured with rocprofv3 --pmc FETCH_SIZE /
@@ -439,6 +444,16 @@ def main():
             "GBs": round(16.0 * n / results["copy"] / 1e9, 1),
             "note": "torch copy_ of one FP64 vector (8.6 GB read + 8.6 GB "
                     "written), same stream and events"}
+    flop = {}
+    for key, peak in (("fp64", FP64_PEAK_TFLOPS), ("fp32", FP32_PEAK_TFLOPS)):
+        if key in results:
+            tf = FLOP_PER_DOF * n / results[key] / 1e12
+            flop[key] = {"TFLOPs": round(tf, 2), "peak": peak,
+                         "frac": round(tf / peak, 4)}
+    if flop:
+        line["flop_rate"] = dict(flop, flop_per_dof=FLOP_PER_DOF,
+                                 note="the collapsed apply's FMAs, D (N+1) "
+                                      "per DoF; vector peaks")
     if "fp32" in results:
         t = results["fp32"]
         line["fp32_mixed"] = {
