@@ -2,8 +2,8 @@
 
 TEST INFRASTRUCTURE ONLY: the oracle restates hyper.deal's phase-space DG
 advection operator on the CPU.  Only tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py may import this module, as the checker or as the
-timed CPU baseline.  The product (hyperdeal_amd) never imports it.
+cpu_baseline leg of bench.py may import this module, to check results or as
+the timed CPU baseline.  The product (hyperdeal_amd) never imports it.
 
 Every restated function cites its hyper.deal source in oracle/hd_oracle.cpp.
 """

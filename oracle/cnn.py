@@ -1,7 +1,7 @@
 """CPU oracle of the Poisson-guess network (numpy).
 
 TEST INFRASTRUCTURE ONLY: only tests/, __graft_entry__.smoke() and bench.py's
-cpu_baseline leg may import this module, as the checker.  The network has no
+cpu_baseline leg may import this module, to check results.  The network has no
 counterpart in hyper.deal (whose Poisson CG starts from the previous
 potential, examples/vlasov_poisson/include/application.h:575-576), so this
 module restates the network's own definition:

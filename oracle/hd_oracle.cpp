@@ -1,8 +1,8 @@
 // hd_oracle.cpp -- CPU oracle of the phase-space DG advection operator.
 //
 // TEST INFRASTRUCTURE ONLY.  Only tests/, __graft_entry__.smoke() and
-// bench.py's cpu_baseline leg may load this library, as the checker or as the
-// timed CPU baseline; the product path never does.
+// bench.py's cpu_baseline leg may load this library, to check results or as
+// the timed CPU baseline; the product path never does.
 //
 // What is restated here, and from where (paths relative to the hyper.deal
 // source tree):
