@@ -70,6 +70,8 @@ struct Shape1D
 
 namespace detail
 {
+constexpr long double kPi = 3.141592653589793238462643383279502884L;
+
 // Legendre P_n and P_n' at z in [-1, 1]
 inline void legendre(int n, long double z, long double &p, long double &dp)
 {
@@ -166,7 +168,8 @@ inline Shape1D make_shape(int degree)
   s.weights.resize(n);
   for (int i = 0; i < n; ++i)
     {
-      long double z = std::cos(M_PI * (i + 0.75) / (n + 0.5)), p, dp;
+      long double z = std::cos(detail::kPi * (i + 0.75L) / (n + 0.5L)), p,
+                  dp;
       for (int it = 0; it < 100; ++it)
         {
           detail::legendre(n, z, p, dp);
@@ -184,7 +187,7 @@ inline Shape1D make_shape(int degree)
   s.nodes[n - 1] = 1.0;
   for (int i = 1; i < n - 1; ++i)
     {
-      long double z = -std::cos(M_PI * i / (n - 1));
+      long double z = -std::cos(detail::kPi * i / (n - 1));
       for (int it = 0; it < 100; ++it)
         {
           // f = P_{n-1}'(z), f' = P_{n-1}''(z) from the Legendre ODE
@@ -742,7 +745,8 @@ public:
 
   // element-centric: face face_no = 2 d + side (side 1: the upper end) of
   // the cell; the minus evaluator takes the cell, the plus one the
-  // neighbour across the face
+  // neighbour across the face (a boundary face has none: check
+  // at_boundary() of the minus evaluator first, the plus one throws)
   void reinit(const TensorID cell, unsigned int face_no)
   {
     dir_ = (int)face_no / 2;
