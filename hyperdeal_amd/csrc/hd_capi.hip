@@ -31,8 +31,7 @@ int choose_algorithm(const hd_advection *op, int precision)
 {
   const hd_matrix_free *mf = op->mf;
   (void)precision; // both kernels have FP64 and FP32-mixed forms
-  const bool quad_ok =
-    mf->nq == mf->nd && hd::quadrature_supported(mf->D, mf->nd);
+  const bool quad_ok = hd::quadrature_supported(mf->D, mf->nd, mf->nq);
   const bool coll_ok = op->collapsed_ok;
   switch (op->algorithm)
     {
@@ -176,13 +175,13 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
                                     mf->mesh, op->inflow, stage, stream);
     }
   else if (precision == HD_FP64)
-    e = hd::quadrature_apply(mf->D, mf->nd, mode,
+    e = hd::quadrature_apply(mf->D, mf->nd, mf->nq, mode,
                              static_cast<const double *>(src),
                              static_cast<double *>(dst), op->quad, mf->mesh,
                              stage, time, mf->dim_x, mf->collocation,
                              op->vp ? op->efield : nullptr, stream);
   else
-    e = hd::quadrature_apply_f32(mf->D, mf->nd, mode,
+    e = hd::quadrature_apply_f32(mf->D, mf->nd, mf->nq, mode,
                                  static_cast<const float *>(src),
                                  static_cast<float *>(dst), op->quad,
                                  mf->mesh, stage, time, mf->dim_x,
@@ -400,7 +399,7 @@ int hd_advection_create(hd_matrix_free *mf, const double *velocity,
     }
 
   // quadrature form: the 1D tables by value
-  if (nq == nd && nd <= 8)
+  if (nq <= 8)
     {
       auto &q = op->quad;
       const auto &t = mf->tables;
@@ -458,7 +457,7 @@ int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
   for (int d = 0; d < mf->D; ++d)
     if (!mf->periodic[d])
       return fail(HD_ERR_UNSUPPORTED, "Vlasov-Poisson needs a periodic box");
-  if (mf->nq != mf->nd || !hd::quadrature_supported(mf->D, mf->nd))
+  if (mf->nq != mf->nd || !hd::quadrature_supported(mf->D, mf->nd, mf->nq))
     return fail(HD_ERR_UNSUPPORTED,
                 "Vlasov-Poisson runs the quadrature kernel: n_points must "
                 "be degree + 1");

@@ -218,17 +218,19 @@ hipError_t collapsed_split_f32(int D, int N, int mode, const float *src,
                                float *dst, float *part, const float *coef,
                                const MeshDev &mesh, const InflowDev &inflow,
                                const StageDev &stage, hipStream_t stream);
-bool quadrature_supported(int D, int N);
-hipError_t quadrature_apply(int D, int N, int mode, const double *src,
-                            double *dst, const QuadDev &q,
+// ND = k+1 nodes, NQ quadrature points per direction (NQ = ND or ND + 1)
+bool quadrature_supported(int D, int ND, int NQ);
+hipError_t quadrature_apply(int D, int ND, int NQ, int mode,
+                            const double *src, double *dst, const QuadDev &q,
                             const MeshDev &mesh, const StageDev &stage,
                             double time, int dim_x, int collocation,
                             const double *efield, hipStream_t stream);
 // the FP32-mixed form: FP32 vectors and contractions, FP64 LSRK update
-hipError_t quadrature_apply_f32(int D, int N, int mode, const float *src,
-                                float *dst, const QuadDev &q,
-                                const MeshDev &mesh, const StageDev &stage,
-                                double time, int dim_x, int collocation,
+hipError_t quadrature_apply_f32(int D, int ND, int NQ, int mode,
+                                const float *src, float *dst,
+                                const QuadDev &q, const MeshDev &mesh,
+                                const StageDev &stage, double time,
+                                int dim_x, int collocation,
                                 const double *efield, hipStream_t stream);
 hipError_t launch_interpolate(double *dst, const MeshDev &mesh,
                               const FuncDev &f, double t, long long n_dofs,

@@ -2,17 +2,22 @@
 // operator on gfx950: the reference's quadrature algorithm step by step
 // (operators/advection/advection_operation.h:221-566), for Cartesian meshes
 // with periodic or Dirichlet boundaries, any skew factor, Gauss quadrature
-// with k+1 points or Gauss-Lobatto collocation.
+// with k+1 or k+2 points, or Gauss-Lobatto collocation.
 //
-// One block works on CPB cells held in LDS:
+// ND = k+1 nodes and NQ >= ND quadrature points per direction.  One block
+// works on CPB cells held in LDS as NQ^D arrays (the nodal values sit at the
+// positions whose digits are all < ND):
 //   U  <- src cell (nodal), then U <- S^D U             (quadrature values)
 //   R  <- cell integral: -skew (a.grad u) JxW + sum_d D_d^T[(1-skew) u a_d/h_d JxW]
 //   per face f (2D of them): u- = F_s . U along d; u+ = S^{D-1} (neighbour's
 //       face nodes) or the Dirichlet ghost -u- + 2g; R += F_s^T (-flux JxW_f)
 //   R <- Sinv^D (R / JxW)                                 (inverse mass)
-// Every 1D contraction is done line by line: a thread owns whole lines of N
-// values along the current direction, so updates are in place and only a
-// barrier separates consecutive directions.
+// S is NQ x ND, Sinv = (S^T W S)^-1 S^T W is ND x NQ.  Every 1D contraction
+// is done line by line: a thread owns whole lines along the current
+// direction (reading ND or NQ values, writing NQ or ND), so updates are in
+// place and only a barrier separates consecutive directions.  Lines through
+// positions that hold no value yet (digits >= ND of a direction not yet
+// expanded, or already reduced) compute values that are never read.
 #include "hd_device.h"
 #include "hd_functions.h"
 
@@ -24,13 +29,34 @@ namespace quad
 {
 constexpr int kBlock = 256;
 
-template <int D, int N>
+template <int D, int ND, int NQ>
 struct Shape
 {
-  static constexpr int CELL = ipow(N, D);
-  static constexpr int FACE = ipow(N, D - 1);
+  static constexpr int CELL = ipow(NQ, D);  // quadrature-point array
+  static constexpr int CELLD = ipow(ND, D); // nodal values of a cell
+  static constexpr int FACE = ipow(NQ, D - 1);
   static constexpr int CPB = CELL >= 1024 ? 1 : 1024 / CELL;
 };
+
+// position in the NQ^DD array of the point with base-ND digits i
+template <int DD, int ND, int NQ>
+__device__ inline int qpos(int i)
+{
+  if constexpr (ND == NQ)
+    return i;
+  else
+    {
+      int r = 0, pw = 1;
+#pragma unroll
+      for (int j = 0; j < DD; ++j)
+        {
+          r += (i % ND) * pw;
+          i /= ND;
+          pw *= NQ;
+        }
+      return r;
+    }
+}
 
 // base index of line f (0 <= f < N^{D-1}) along direction d
 template <int D, int N>
@@ -50,28 +76,28 @@ __device__ inline int line_base(int f, int d)
   return base;
 }
 
-// in-place 1D contraction of all lines along d of the CPB cells in buf:
-// line <- M line, M(r, c) = M[r * sr + c * sc]
-template <int D, int N, class T>
+// in-place 1D contraction of all lines along d of the CPB cells in buf (NQ^D
+// arrays): line[0, NO) <- M line[0, NI), M(r, c) = M[r * sr + c * sc]
+template <int D, int NQ, int NI, int NO, class T>
 __device__ inline void contract_lines(T *buf, int cellp, int d, const T *M,
                                       int sr, int sc, int ncell)
 {
-  constexpr int FACE = ipow(N, D - 1);
-  const int sd = ipow(N, d);
+  constexpr int FACE = ipow(NQ, D - 1);
+  const int sd = ipow(NQ, d);
   for (int l = threadIdx.x; l < ncell * FACE; l += kBlock)
     {
       const int lc = l / FACE;
-      T *p = buf + lc * cellp + line_base<D, N>(l - lc * FACE, d);
-      T in[N];
+      T *p = buf + lc * cellp + line_base<D, NQ>(l - lc * FACE, d);
+      T in[NI];
 #pragma unroll
-      for (int j = 0; j < N; ++j)
+      for (int j = 0; j < NI; ++j)
         in[j] = p[j * sd];
 #pragma unroll
-      for (int i = 0; i < N; ++i)
+      for (int i = 0; i < NO; ++i)
         {
           T acc = 0;
 #pragma unroll
-          for (int j = 0; j < N; ++j)
+          for (int j = 0; j < NI; ++j)
             acc += M[i * sr + j * sc] * in[j];
           p[i * sd] = acc;
         }
@@ -129,14 +155,15 @@ __device__ inline double jxw_point(const QuadDev &q, const MeshDev &m, int i)
 // MODE 0: dst = apply(src); MODE 1: LSRK stage epilogue
 // T: vector, LDS and contraction type (double, or float for the FP32-mixed
 // form: the LSRK update of the FP64 solution stays in FP64)
-template <int D, int N, int MODE, class T>
+template <int D, int ND, int NQ, int MODE, class T>
 __global__ __launch_bounds__(kBlock) void apply_kernel(
   const T *__restrict__ src, T *__restrict__ dst, QuadDev q, MeshDev mesh,
   StageDev stage, double time, int dim_x, int collocation,
   const double *__restrict__ efield)
 {
-  using Sh = Shape<D, N>;
-  constexpr int CELL = Sh::CELL, FACE = Sh::FACE, CPB = Sh::CPB;
+  using Sh = Shape<D, ND, NQ>;
+  constexpr int CELL = Sh::CELL, CELLD = Sh::CELLD, FACE = Sh::FACE;
+  constexpr int CPB = Sh::CPB;
   constexpr int CELLP = CELL + 1;
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *U = reinterpret_cast<T *>(smem_raw); // CPB x CELLP
@@ -164,18 +191,25 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   const int ncell = (int)min((long long)CPB, mesh.n_cells - cell0);
   const int tid = threadIdx.x;
 
-  for (int i = tid; i < ncell * CELL; i += kBlock)
+  // the positions that hold no nodal value start at zero
+  if constexpr (ND < NQ)
     {
-      const int lc = i / CELL;
-      U[lc * CELLP + (i - lc * CELL)] = src[cell0 * CELL + i];
+      for (int i = tid; i < ncell * CELLP; i += kBlock)
+        U[i] = T(0);
+      __syncthreads();
+    }
+  for (int i = tid; i < ncell * CELLD; i += kBlock)
+    {
+      const int lc = i / CELLD;
+      U[lc * CELLP + qpos<D, ND, NQ>(i - lc * CELLD)] = src[cell0 * CELLD + i];
     }
   __syncthreads();
 
-  // 1) nodal -> quadrature values
+  // 1) nodal -> quadrature values (S: NQ x ND)
   if (!collocation)
     for (int d = 0; d < D; ++d)
       {
-        contract_lines<D, N, T>(U, CELLP, d, tb.S, N, 1, ncell);
+        contract_lines<D, NQ, ND, NQ, T>(U, CELLP, d, tb.S, ND, 1, ncell);
         __syncthreads();
       }
 
@@ -188,48 +222,48 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   __syncthreads();
   for (int d = 0; d < D; ++d)
     {
-      const int sd = ipow(N, d);
+      const int sd = ipow(NQ, d);
       for (int l = tid; l < ncell * FACE; l += kBlock)
         {
           const int lc = l / FACE;
-          const int base = line_base<D, N>(l - lc * FACE, d);
+          const int base = line_base<D, NQ>(l - lc * FACE, d);
           const T *u = U + lc * CELLP + base;
           T *r = R + lc * CELLP + base;
-          T uq[N], jw[N], acc[N];
+          T uq[NQ], jw[NQ], acc[NQ];
           const T skew = (T)q.skew;
 #pragma unroll
-          for (int j = 0; j < N; ++j)
+          for (int j = 0; j < NQ; ++j)
             {
               uq[j] = u[j * sd];
               // velocity / h folded into the quadrature weight
-              jw[j] = (T)(jxw_point<D, N>(q, mesh, base + j * sd) *
-                          point_velocity<D, N>(q, mesh, efield, dim_x, d,
-                                               cell0 + lc, base + j * sd) /
+              jw[j] = (T)(jxw_point<D, NQ>(q, mesh, base + j * sd) *
+                          point_velocity<D, NQ>(q, mesh, efield, dim_x, d,
+                                                cell0 + lc, base + j * sd) /
                           mesh.h[d]);
               acc[j] = 0;
             }
           if (q.skew != 0.0)
 #pragma unroll
-            for (int i = 0; i < N; ++i)
+            for (int i = 0; i < NQ; ++i)
               {
                 T g = 0;
 #pragma unroll
-                for (int j = 0; j < N; ++j)
-                  g += tb.Dq[i * N + j] * uq[j];
+                for (int j = 0; j < NQ; ++j)
+                  g += tb.Dq[i * NQ + j] * uq[j];
                 acc[i] += -skew * g * jw[i];
               }
           if (q.skew != 1.0)
 #pragma unroll
-            for (int i = 0; i < N; ++i)
+            for (int i = 0; i < NQ; ++i)
               {
                 T s = 0;
 #pragma unroll
-                for (int j = 0; j < N; ++j)
-                  s += tb.Dq[j * N + i] * ((1 - skew) * uq[j] * jw[j]);
+                for (int j = 0; j < NQ; ++j)
+                  s += tb.Dq[j * NQ + i] * ((1 - skew) * uq[j] * jw[j]);
                 acc[i] += s;
               }
 #pragma unroll
-          for (int i = 0; i < N; ++i)
+          for (int i = 0; i < NQ; ++i)
             r[i * sd] += acc[i];
         }
       __syncthreads();
@@ -239,37 +273,59 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   for (int f = 0; f < 2 * D; ++f)
     {
       const int d = f / 2, side = f % 2;
-      const int sd = ipow(N, d);
+      const int sd = ipow(NQ, d);
+      const int sdn = ipow(ND, d); // stride of direction d in the src cell
       const double nrm = side ? 1.0 : -1.0;
       // u- (trace) and the neighbour's face nodes
       for (int l = tid; l < ncell * FACE; l += kBlock)
         {
           const int lc = l / FACE, fi = l - lc * FACE;
-          const int base = line_base<D, N>(fi, d);
+          const int base = line_base<D, NQ>(fi, d);
           const T *u = U + lc * CELLP + base;
           T um = 0;
 #pragma unroll
-          for (int p = 0; p < N; ++p)
-            um += tb.F[side * N + p] * u[p * sd];
+          for (int p = 0; p < NQ; ++p)
+            um += tb.F[side * NQ + p] * u[p * sd];
           Fm[lc * FACE + fi] = um;
-          // neighbour
+          // neighbour: its face nodes sit at the face positions whose
+          // digits are all < ND; the others start at zero
+          int nbase = 0;
+          bool node = true;
+          {
+            int rem = fi, pw = 1;
+#pragma unroll
+            for (int j = 0; j < D; ++j)
+              {
+                if (j != d)
+                  {
+                    const int dg = rem % NQ;
+                    rem /= NQ;
+                    node = node && dg < ND;
+                    nbase += dg * pw;
+                  }
+                pw *= ND;
+              }
+          }
           const long long cell = cell0 + lc;
           const long long s = mesh.stride[d];
           const int c = (int)((cell / s) % mesh.n[d]);
           const int cn = c + (side ? 1 : -1);
+          const int off = nbase + (side ? 0 : (ND - 1)) * sdn;
           T up = 0;
-          if (cn >= 0 && cn < mesh.n[d])
-            up = src[(cell + (side ? s : -s)) * CELL + base +
-                     (side ? 0 : (N - 1)) * sd];
-          else if (q.periodic[d])
-            up = src[(cell + (side ? -(long long)(mesh.n[d] - 1) * s
-                                   : (long long)(mesh.n[d] - 1) * s)) *
-                       CELL +
-                     base + (side ? 0 : (N - 1)) * sd];
+          if (node)
+            {
+              if (cn >= 0 && cn < mesh.n[d])
+                up = src[(cell + (side ? s : -s)) * CELLD + off];
+              else if (q.periodic[d])
+                up = src[(cell + (side ? -(long long)(mesh.n[d] - 1) * s
+                                       : (long long)(mesh.n[d] - 1) * s)) *
+                           CELLD +
+                         off];
+            }
           Fp[lc * FACE + fi] = up;
         }
       __syncthreads();
-      // interpolate u+ along the face directions
+      // interpolate u+ along the face directions (S: NQ x ND)
       if (!collocation)
         for (int j = 0; j < D; ++j)
           {
@@ -277,36 +333,36 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
               continue;
             // lines of the face array along j (the face array is the cell
             // array with direction d of extent 1)
-            const int sj = ipow(N, j < d ? j : j - 1);
-            const int nl = FACE / N;
+            const int sj = ipow(NQ, j < d ? j : j - 1);
+            const int nl = FACE / NQ;
             for (int l = tid; l < ncell * nl; l += kBlock)
               {
                 const int lc = l / nl;
                 int li = l - lc * nl;
-                // base of line li along face-direction jj in N^{D-1}
+                // base of line li along face-direction jj in NQ^{D-1}
                 const int jj = j < d ? j : j - 1;
                 int base = 0, pw = 1;
                 for (int k = 0; k < D - 1; ++k)
                   {
                     if (k != jj)
                       {
-                        base += (li % N) * pw;
-                        li /= N;
+                        base += (li % NQ) * pw;
+                        li /= NQ;
                       }
-                    pw *= N;
+                    pw *= NQ;
                   }
                 T *p = Fp + lc * FACE + base;
-                T in[N];
+                T in[ND];
 #pragma unroll
-                for (int t = 0; t < N; ++t)
+                for (int t = 0; t < ND; ++t)
                   in[t] = p[t * sj];
 #pragma unroll
-                for (int i = 0; i < N; ++i)
+                for (int i = 0; i < NQ; ++i)
                   {
                     T acc = 0;
 #pragma unroll
-                    for (int t = 0; t < N; ++t)
-                      acc += tb.S[i * N + t] * in[t];
+                    for (int t = 0; t < ND; ++t)
+                      acc += tb.S[i * ND + t] * in[t];
                     p[i * sj] = acc;
                   }
               }
@@ -316,7 +372,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
       for (int l = tid; l < ncell * FACE; l += kBlock)
         {
           const int lc = l / FACE, fi = l - lc * FACE;
-          const int base = line_base<D, N>(fi, d);
+          const int base = line_base<D, NQ>(fi, d);
           const long long cell = cell0 + lc;
           const long long s = mesh.stride[d];
           const int c = (int)((cell / s) % mesh.n[d]);
@@ -332,8 +388,8 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
             for (int j = 0; j < D; ++j)
               {
                 if (j != d)
-                  jf *= mesh.h[j] * q.w[rem % N];
-                rem /= N;
+                  jf *= mesh.h[j] * q.w[rem % NQ];
+                rem /= NQ;
               }
           }
           if (boundary)
@@ -348,8 +404,8 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
                     {
                       const int cj = (int)(cc % mesh.n[j]);
                       cc /= mesh.n[j];
-                      const int ij = rem % N;
-                      rem /= N;
+                      const int ij = rem % NQ;
+                      rem /= NQ;
                       x[j] = mesh.lo[j] +
                              (cj + (j == d ? (double)side : q.qp[ij])) *
                                mesh.h[j];
@@ -360,38 +416,39 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
               up = -um + 2.0 * g;
             }
           const double an =
-            point_velocity<D, N>(q, mesh, efield, dim_x, d, cell, base) * nrm;
+            point_velocity<D, NQ>(q, mesh, efield, dim_x, d, cell, base) *
+            nrm;
           const double flux =
             0.5 * ((um + up) * an + fabs(an) * (um - up));
           const T fd = (T)(-(flux - q.skew * um * an) * jf);
           T *r = R + lc * CELLP + base;
 #pragma unroll
-          for (int p = 0; p < N; ++p)
-            r[p * sd] += tb.F[side * N + p] * fd;
+          for (int p = 0; p < NQ; ++p)
+            r[p * sd] += tb.F[side * NQ + p] * fd;
         }
       __syncthreads();
     }
 
-  // 4) inverse mass matrix
+  // 4) inverse mass matrix (Sinv: ND x NQ)
   for (int i = tid; i < ncell * CELL; i += kBlock)
     {
       const int lc = i / CELL, li = i - lc * CELL;
-      R[lc * CELLP + li] /= (T)jxw_point<D, N>(q, mesh, li);
+      R[lc * CELLP + li] /= (T)jxw_point<D, NQ>(q, mesh, li);
     }
   __syncthreads();
   if (!collocation)
     for (int d = D - 1; d >= 0; --d)
       {
-        contract_lines<D, N, T>(R, CELLP, d, tb.Sinv, N, 1, ncell);
+        contract_lines<D, NQ, NQ, ND, T>(R, CELLP, d, tb.Sinv, NQ, 1, ncell);
         __syncthreads();
       }
 
   // 5) write
-  for (int i = tid; i < ncell * CELL; i += kBlock)
+  for (int i = tid; i < ncell * CELLD; i += kBlock)
     {
-      const int lc = i / CELL;
-      const T k = R[lc * CELLP + (i - lc * CELL)];
-      const long long g = cell0 * CELL + i;
+      const int lc = i / CELLD;
+      const T k = R[lc * CELLP + qpos<D, ND, NQ>(i - lc * CELLD)];
+      const long long g = cell0 * CELLD + i;
       if (MODE == 0)
         dst[g] = k;
       else
@@ -404,83 +461,95 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
     }
 }
 
-template <int D, int N, class T>
+template <int D, int ND, int NQ, class T>
 hipError_t launch(int mode, const T *src, T *dst, const QuadDev &q,
                   const MeshDev &mesh, const StageDev &stage, double time,
                   int dim_x, int collocation, const double *efield,
                   hipStream_t stream)
 {
-  using Sh = Shape<D, N>;
+  using Sh = Shape<D, ND, NQ>;
   const size_t lds = sizeof(T) *
                      (2 * Sh::CPB * (Sh::CELL + 1) + 2 * Sh::CPB * Sh::FACE);
   const long long grid = (mesh.n_cells + Sh::CPB - 1) / Sh::CPB;
   // gfx950 has 160 KB of LDS per CU; opt in above the 64 KB default
   hipError_t e = hipFuncSetAttribute(
-    reinterpret_cast<const void *>(mode == 0 ? &apply_kernel<D, N, 0, T>
-                                             : &apply_kernel<D, N, 1, T>),
+    reinterpret_cast<const void *>(mode == 0
+                                     ? &apply_kernel<D, ND, NQ, 0, T>
+                                     : &apply_kernel<D, ND, NQ, 1, T>),
     hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
   if (e != hipSuccess)
     return e;
   if (mode == 0)
-    hipLaunchKernelGGL((apply_kernel<D, N, 0, T>), dim3((unsigned)grid),
+    hipLaunchKernelGGL((apply_kernel<D, ND, NQ, 0, T>), dim3((unsigned)grid),
                        dim3(kBlock), lds, stream, src, dst, q, mesh, stage,
                        time, dim_x, collocation, efield);
   else
-    hipLaunchKernelGGL((apply_kernel<D, N, 1, T>), dim3((unsigned)grid),
+    hipLaunchKernelGGL((apply_kernel<D, ND, NQ, 1, T>), dim3((unsigned)grid),
                        dim3(kBlock), lds, stream, src, dst, q, mesh, stage,
                        time, dim_x, collocation, efield);
   return hipGetLastError();
 }
 } // namespace quad
 
-bool quadrature_supported(int D, int N)
-{
-  if (D < 2 || D > 6 || N < 2 || N > 8)
-    return false;
-  // both LDS cell buffers must fit in 160 KB
-  const long long cell = (long long)std::pow(N, D);
-  return 2 * (cell + 1) * 8 + 2 * (cell / N) * 8 <= 160 * 1024;
-}
-
 namespace
 {
+// the instantiated (D, ND, NQ): NQ = ND (k+1 points) or ND + 1 (k+2)
 template <class T>
-hipError_t dispatch(int D, int N, int mode, const T *src, T *dst,
+hipError_t dispatch(int D, int ND, int NQ, int mode, const T *src, T *dst,
                     const QuadDev &q, const MeshDev &mesh,
                     const StageDev &stage, double time, int dim_x,
                     int collocation, const double *efield, hipStream_t stream)
 {
-#define HD_CASE(DD, NN_)                                                     \
-  if (D == DD && N == NN_)                                                   \
-    return quad::launch<DD, NN_, T>(mode, src, dst, q, mesh, stage, time,    \
-                                    dim_x, collocation, efield, stream);
-  HD_CASE(2, 2) HD_CASE(2, 3) HD_CASE(2, 4) HD_CASE(2, 5) HD_CASE(2, 6)
-  HD_CASE(3, 2) HD_CASE(3, 3) HD_CASE(3, 4) HD_CASE(3, 5) HD_CASE(3, 6)
-  HD_CASE(4, 2) HD_CASE(4, 3) HD_CASE(4, 4) HD_CASE(4, 5) HD_CASE(4, 6)
-  HD_CASE(5, 2) HD_CASE(5, 3) HD_CASE(5, 4)
-  HD_CASE(6, 2) HD_CASE(6, 3) HD_CASE(6, 4)
+#define HD_CASE(DD, ND_, NQ_)                                                \
+  if (D == DD && ND == ND_ && NQ == NQ_)                                     \
+    return quad::launch<DD, ND_, NQ_, T>(mode, src, dst, q, mesh, stage,     \
+                                         time, dim_x, collocation, efield,   \
+                                         stream);
+#define HD_CASE2(DD, ND_) HD_CASE(DD, ND_, ND_) HD_CASE(DD, ND_, ND_ + 1)
+  HD_CASE2(2, 2) HD_CASE2(2, 3) HD_CASE2(2, 4) HD_CASE2(2, 5) HD_CASE(2, 6, 6)
+  HD_CASE2(3, 2) HD_CASE2(3, 3) HD_CASE2(3, 4) HD_CASE2(3, 5) HD_CASE(3, 6, 6)
+  HD_CASE2(4, 2) HD_CASE2(4, 3) HD_CASE2(4, 4) HD_CASE2(4, 5) HD_CASE(4, 6, 6)
+  HD_CASE2(5, 2) HD_CASE2(5, 3) HD_CASE2(5, 4)
+  HD_CASE2(6, 2) HD_CASE2(6, 3) HD_CASE(6, 4, 4)
+#undef HD_CASE2
 #undef HD_CASE
   return hipErrorInvalidValue;
 }
 } // namespace
 
-hipError_t quadrature_apply(int D, int N, int mode, const double *src,
-                            double *dst, const QuadDev &q,
+bool quadrature_supported(int D, int ND, int NQ)
+{
+  if (D < 2 || D > 6 || ND < 2 || (NQ != ND && NQ != ND + 1))
+    return false;
+  // the instantiated node counts: k+1 points up to ND = 6 (D <= 4) or 4;
+  // k+2 points up to ND = 5 (D <= 4), 4 (D = 5) or 3 (D = 6)
+  const int max_nd =
+    NQ == ND ? (D <= 4 ? 6 : 4) : (D <= 4 ? 5 : D == 5 ? 4 : 3);
+  if (ND > max_nd)
+    return false;
+  // both LDS cell buffers must fit in 160 KB
+  const long long cell = (long long)std::pow(NQ, D);
+  return 2 * (cell + 1) * 8 + 2 * (cell / NQ) * 8 <= 160 * 1024;
+}
+
+hipError_t quadrature_apply(int D, int ND, int NQ, int mode,
+                            const double *src, double *dst, const QuadDev &q,
                             const MeshDev &mesh, const StageDev &stage,
                             double time, int dim_x, int collocation,
                             const double *efield, hipStream_t stream)
 {
-  return dispatch<double>(D, N, mode, src, dst, q, mesh, stage, time, dim_x,
-                          collocation, efield, stream);
+  return dispatch<double>(D, ND, NQ, mode, src, dst, q, mesh, stage, time,
+                          dim_x, collocation, efield, stream);
 }
 
-hipError_t quadrature_apply_f32(int D, int N, int mode, const float *src,
-                                float *dst, const QuadDev &q,
-                                const MeshDev &mesh, const StageDev &stage,
-                                double time, int dim_x, int collocation,
+hipError_t quadrature_apply_f32(int D, int ND, int NQ, int mode,
+                                const float *src, float *dst,
+                                const QuadDev &q, const MeshDev &mesh,
+                                const StageDev &stage, double time,
+                                int dim_x, int collocation,
                                 const double *efield, hipStream_t stream)
 {
-  return dispatch<float>(D, N, mode, src, dst, q, mesh, stage, time, dim_x,
-                         collocation, efield, stream);
+  return dispatch<float>(D, ND, NQ, mode, src, dst, q, mesh, stage, time,
+                         dim_x, collocation, efield, stream);
 }
 } // namespace hd

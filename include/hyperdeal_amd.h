@@ -119,7 +119,9 @@ int hd_advection_create(hd_matrix_free *mf, const double *velocity,
                         double skew_factor, int bc_kind, int bc_function,
                         const double *bc_params, hd_advection **out);
 int hd_advection_destroy(hd_advection *op);
-/* Force an algorithm (HD_ALGO_*); HD_ERR_UNSUPPORTED if not applicable. */
+/* Force an algorithm (HD_ALGO_*); HD_ERR_UNSUPPORTED if not applicable.
+ * HD_ALGO_COLLAPSED: periodic boxes, any n_points >= degree + 1.
+ * HD_ALGO_QUADRATURE: any boundary, n_points = degree + 1 or degree + 2. */
 int hd_advection_set_algorithm(hd_advection *op, int algorithm);
 /* The algorithm apply() runs with the current settings and precision. */
 int hd_advection_algorithm(const hd_advection *op, int precision);

@@ -170,6 +170,61 @@ def test_homogeneous_dirichlet_3d3d():
     assert _rel_err(dst.cpu().numpy(), ref) <= 1e-12
 
 
+OVERINT_CASES = [
+    # dims, degree, n_cells, periodic, skew
+    ((1, 1), 3, [4, 3], False, 0.0),
+    ((1, 1), 1, [5, 4], True, 0.5),
+    ((2, 2), 3, [3, 2, 2, 3], False, 0.0),
+    ((2, 2), 3, [2, 3, 2, 2], True, 1.0),
+    ((2, 2), 4, [2, 2, 2, 2], False, 0.3),
+    ((1, 2), 2, [3, 2, 3], False, 0.0),
+    ((2, 3), 2, [2, 2, 2, 2, 2], False, 0.5),
+    ((3, 3), 2, [2, 2, 2, 2, 2, 2], False, 0.0),
+]
+
+
+@pytest.mark.parametrize("dims,degree,n_cells,periodic,skew", OVERINT_CASES,
+                         ids=[f"{d[0]}+{d[1]}_k{k}_{'per' if p else 'dir'}"
+                              for d, k, _, p, _s in OVERINT_CASES])
+def test_quadrature_overintegration_matches_oracle(dims, degree, n_cells,
+                                                   periodic, skew):
+    """The quadrature kernel with k+2 Gauss points per direction (the
+    reference's over-integrated runs, e.g. tests/advection/adv_2D_2D_k3_q5):
+    FP64 apply at 1e-12 and FP32 at 2e-6 against the oracle, inhomogeneous
+    Dirichlet or periodic, and one fused LSRK stage."""
+    print("case: overintegration %s k=%d %s skew=%g" %
+          (dims, degree, "periodic" if periodic else "dirichlet", skew),
+          flush=True)
+    P, mf, op, a = _pair(dims, degree, n_cells, seed=31 + degree,
+                         n_points=degree + 2, periodic=periodic, skew=skew,
+                         lo=[-1.0] * sum(dims), hi=[1.0] * sum(dims))
+    op.set_algorithm(H.ALGO_QUADRATURE)
+    u = _rng(5).standard_normal(P.n_dofs)
+    t = 0.21
+    ref = P.apply(u, t=t)
+    src = torch.from_numpy(u).cuda()
+    dst = torch.empty_like(src)
+    op.apply(dst, src, time=t)
+    u32 = src.float()
+    ref32 = P.apply(u32.double().cpu().numpy(), t=t)
+    dst32 = torch.empty_like(u32)
+    op.apply(dst32, u32, time=t, precision=H.FP32)
+    torch.cuda.synchronize()
+    assert _rel_err(dst.cpu().numpy(), ref) <= 1e-12
+    assert _rel_err(dst32.double().cpu().numpy(), ref32) <= 2e-6
+    # fused stage: sol += b dt K, ti = sol_old + a dt K with K = apply(src)
+    sol0 = _rng(6).standard_normal(P.n_dofs)
+    sol = torch.from_numpy(sol0).cuda()
+    ti = torch.empty_like(sol)
+    op.stage(src, ti, sol, t, 0.3, 0.7)
+    torch.cuda.synchronize()
+    k = ref
+    np.testing.assert_allclose(sol.cpu().numpy(), sol0 + 0.7 * k,
+                               rtol=0, atol=1e-12 * np.max(np.abs(k)) + 1e-14)
+    np.testing.assert_allclose(ti.cpu().numpy(), sol0 + 0.3 * k,
+                               rtol=0, atol=1e-12 * np.max(np.abs(k)) + 1e-14)
+
+
 @pytest.mark.parametrize("n_points,collocation", [(5, False), (6, False),
                                                   (4, True)])
 def test_collapsed_overintegration_and_collocation(n_points, collocation):
@@ -301,21 +356,12 @@ def _gpu_run_advection(c):
     return np.array(rows)
 
 
-def _supported_on_gpu(c):
-    # the quadrature kernel (needed off the periodic path) runs nq == k+1
-    periodic = c["periodic_x"] and c["periodic_v"]
-    return periodic or c["n_points"] == c["degree"] + 1
-
-
 GOLD = AC.cases()
 
 
 @pytest.mark.parametrize("c", GOLD, ids=[c["name"] for c in GOLD])
 def test_reference_golden_end_to_end_on_gpu(c):
-    if not _supported_on_gpu(c):
-        pytest.skip("Dirichlet boundaries with over-integration (N_POINTS=5) "
-                    "need the quadrature kernel with n_points != degree+1, "
-                    "not implemented")
+    print("case: golden %s" % c["name"], flush=True)
     rows = _gpu_run_advection(c)
     ours = AC.parse_rows(AC.format_rows(rows))
     assert ours.shape == c["golden"].shape
