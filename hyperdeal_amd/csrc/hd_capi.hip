@@ -588,6 +588,26 @@ int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
         tf[i] = (float)tab[i];
       tab.resize(n_tab + (n_tab + 1) / 2, 0.0);
       std::memcpy(tab.data() + n_tab, tf.data(), n_tab * sizeof(float));
+      // then the v-direction factors of the x terms, S^-1 diag(v_d(q)+/-) S
+      // along v_d, per v_d cell: [d][cell][sign][i][l] (they depend on the
+      // v cell only, so the kernel loads them instead of forming them)
+      for (int d = 0; d < 2; ++d)
+        for (int c = 0; c < mf->mesh.n[2 + d]; ++c)
+          for (int sg = 0; sg < 2; ++sg)
+            for (int r = 0; r < nd * nd; ++r)
+              {
+                const int i = r / nd, l = r % nd;
+                const double lo = mf->mesh.lo[2 + d] + c * mf->mesh.h[2 + d];
+                double acc = 0.0;
+                for (int q = 0; q < nd; ++q)
+                  {
+                    const double v = lo + t.qpoints[q] * mf->mesh.h[2 + d];
+                    const double w = sg == 0 ? std::fmax(v, 0.0)
+                                             : std::fmin(v, 0.0);
+                    acc += t.Sinv[i * nd + q] * w * t.S[q * nd + l];
+                  }
+                tab.push_back(acc);
+              }
       e = hipMalloc(&op->vtab, tab.size() * sizeof(double));
       if (e == hipSuccess)
         e = hipMemcpy(op->vtab, tab.data(), tab.size() * sizeof(double),

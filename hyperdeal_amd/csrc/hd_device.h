@@ -264,8 +264,9 @@ bool vp_poisson2d_supported(int dim_x, int N, long long n_x_cells);
 // the 2+2 Vlasov-Poisson apply with the velocity (v, grad phi): a table of
 // vp_apply_table_size(N) doubles, then the same values as floats (the
 // FP32-mixed kernel's coefficients) (the unit-speed 1D operators of the four
-// directions for both signs, S, S^-1, Gauss points), precision 0 FP64, 1
-// FP32-mixed; mode 0 apply, 1 LSRK stage
+// directions for both signs, S, S^-1, Gauss points), then, in doubles, the
+// v factors S^-1 diag(v_d(q)+/-) S per v_d cell ([d][cell][sign][N][N]);
+// precision 0 FP64, 1 FP32-mixed; mode 0 apply, 1 LSRK stage
 bool vp_apply_supported(int dim_x, int dim_v, int N);
 // rho = the sum over v cells of the partials a phase-space stage wrote
 hipError_t vp_density_from_partials(const double *part, long long n_v_cells,

@@ -1044,15 +1044,22 @@ __global__ __launch_bounds__(256) void apply_kernel(
       return tab[i];
   };
   __shared__ T U[C], T0p[C], T0n[C], T1p[C], T1n[C], Q1[C], Q2[C];
-  __shared__ T FL[4][F], FR[4][F], G1[2][2][F], G2[2][2][F];
+  // the inflow faces, left (FL) and right (FR) of each direction; after S1
+  // the 8 N^3 >= N^4 values hold R1
+  __shared__ T FLR[8][F], G1[2][2][F], G2[2][2][F];
+  T(*FL)[F] = FLR;
+  T(*FR)[F] = FLR + 4;
   __shared__ T V[2][2][NN]; // [d][+/-][i][l]
+  // the v factors of the x terms per v cell follow the two coefficient
+  // tables: [d][v_d cell][sign][i][l]
+  const double *vfac = tab_in + TB::size + (TB::size + 1) / 2;
   __shared__ T E[2][NN];    // E_e at the x Gauss points
   __shared__ double RD[MODE == 1 ? C : 1]; // the stage value, for rho
   // the next cell's values, field and eight inflow faces (both sides of
   // every direction) travel into registers while this cell computes
   constexpr int PU = (C + 255) / 256, PE = (2 * NN + 255) / 256;
-  constexpr int PF = (8 * F + 255) / 256;
-  T pu[PU], pe[PE], pf[PF];
+  constexpr int PF = (8 * F + 255) / 256, PV = (4 * NN + 255) / 256;
+  T pu[PU], pe[PE], pf[PF], pv[PV];
   auto prefetch = [&](long long cell) {
 #pragma unroll
     for (int k = 0; k < PU; ++k)
@@ -1070,6 +1077,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
           pe[k] = (T)efield[(xc * NN + t % NN) * 2 + t / NN];
       }
     long long nbl[4], nbr[4];
+    int cv[2];
     {
       unsigned rem = (unsigned)cell;
 #pragma unroll
@@ -1079,11 +1087,24 @@ __global__ __launch_bounds__(256) void apply_kernel(
           const unsigned q = rem / n;
           const unsigned c = rem - q * n;
           rem = q;
+          if (d >= 2)
+            cv[d - 2] = (int)c;
           const long long st = mesh.stride[d];
           nbl[d] = c == 0 ? cell + (long long)(n - 1) * st : cell - st;
           nbr[d] = c == n - 1 ? cell - (long long)(n - 1) * st : cell + st;
         }
     }
+#pragma unroll
+    for (int k = 0; k < PV; ++k)
+      {
+        const int t = threadIdx.x + 256 * k;
+        if (t < 4 * NN)
+          {
+            const int d = t / (2 * NN);
+            pv[k] = (T)vfac[(d == 0 ? 0 : mesh.n[2] * 2 * NN) +
+                            (d == 0 ? cv[0] : cv[1]) * 2 * NN + t % (2 * NN)];
+          }
+      }
 #pragma unroll
     for (int k = 0; k < PF; ++k)
       {
@@ -1105,27 +1126,15 @@ __global__ __launch_bounds__(256) void apply_kernel(
           }
       }
   };
-  // reuse: R0 = U (after S2), R1 = Q1 (after S2), Y0 = T0p, Y1 = T0n,
+  // reuse: R0 = U and R1 = FLR (both consumed by S1), Y0 = T0p, Y1 = T0n,
   // Z = T1p, the result = T1n
-  T *R0 = U, *R1 = Q1, *Y0 = T0p, *Y1 = T0n, *Z = T1p, *O = T1n;
+  T *R0 = U, *R1 = &FLR[0][0], *Y0 = T0p, *Y1 = T0n, *Z = T1p, *O = T1n;
 
   if ((long long)blockIdx.x < mesh.n_cells)
     prefetch(blockIdx.x);
   for (long long cell = blockIdx.x; cell < mesh.n_cells; cell += gridDim.x)
     {
-      int cc[4];
-      {
-        unsigned rem = (unsigned)cell;
-#pragma unroll
-        for (int d = 0; d < 4; ++d)
-          {
-            const unsigned n = (unsigned)mesh.n[d];
-            const unsigned q = rem / n;
-            cc[d] = (int)(rem - q * n);
-            rem = q;
-          }
-      }
-      // S0: the prefetched cell, field and faces into LDS; the v_d factors
+      // S0: the prefetched cell, field, faces and v_d factors into LDS
 #pragma unroll
       for (int k = 0; k < PU; ++k)
         {
@@ -1153,20 +1162,12 @@ __global__ __launch_bounds__(256) void apply_kernel(
                 FR[ds / 2][f] = pf[k];
             }
         }
-      for (int t = threadIdx.x; t < 4 * NN; t += blockDim.x)
-        {
-          const int d = t / (2 * NN), sg = (t / NN) % 2, r = t % NN;
-          const int i = r / N, l = r % N;
-          const double lo = mesh.lo[2 + d] + cc[2 + d] * mesh.h[2 + d];
-          double acc = 0.0;
 #pragma unroll
-          for (int q = 0; q < N; ++q)
-            {
-              const double v = lo + tab[TB::qp + q] * mesh.h[2 + d];
-              const double w = sg == 0 ? fmax(v, 0.0) : fmin(v, 0.0);
-              acc += tab[TB::Si + i * N + q] * w * tab[TB::S + q * N + l];
-            }
-          V[d][sg][r] = (T)acc;
+      for (int k = 0; k < PV; ++k)
+        {
+          const int t = threadIdx.x + 256 * k;
+          if (t < 4 * NN)
+            (&V[0][0][0])[t] = pv[k];
         }
       lds_sync();
       // the next cell's loads go out now and land during the arithmetic
@@ -1247,8 +1248,8 @@ __global__ __launch_bounds__(256) void apply_kernel(
           G1[ee][side][i + N * rest] = a;
         }
       lds_sync();
-      // S2: the v_d factors of the x part (i2, i3 uniform), the x1
-      // transform of Q1 and of the v faces (i1 uniform)
+      // S2: the x1 transform of Q1 and of the v faces (i1 uniform), the
+      // v_d factors of the x part
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
         {
           const int i = uniform(e / FP), l = e % FP;
@@ -1274,8 +1275,8 @@ __global__ __launch_bounds__(256) void apply_kernel(
                  ld(G1[ee][side], x0 + N * (j + N * vo));
           G2[ee][side][x0 + N * (i + N * vo)] = a;
         }
-      lds_sync();
-      // (R0, R1 overwrite U and Q1, which S2 has consumed)
+      // the v_d factors of the x part (i2, i3 uniform) into R0, R1 (U and
+      // FLR: S1 has consumed them)
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
         {
           const int i = uniform(e / FP), l = e % FP;
