@@ -1127,8 +1127,8 @@ __global__ __launch_bounds__(256) void apply_kernel(
       }
   };
   // reuse: R0 = U and R1 = FLR (both consumed by S1), Y0 = T0p, Y1 = T0n,
-  // Z = T1p, the result = T1n
-  T *R0 = U, *R1 = &FLR[0][0], *Y0 = T0p, *Y1 = T0n, *Z = T1p, *O = T1n;
+  // Z = T1p
+  T *R0 = U, *R1 = &FLR[0][0], *Y0 = T0p, *Y1 = T0n, *Z = T1p;
 
   if ((long long)blockIdx.x < mesh.n_cells)
     prefetch(blockIdx.x);
@@ -1355,7 +1355,10 @@ __global__ __launch_bounds__(256) void apply_kernel(
           Z[b + i] = a;
         }
       lds_sync();
-      // S5: back along x1 (i1 uniform), plus the x part, into O
+      // S5: back along x1 (i1 uniform), plus the x part, straight to HBM
+      // (a wave writes runs of N values along x0); with rho_part set, the
+      // next stage input also goes to LDS unrounded, for its density
+      const bool dens = MODE == 1 && stage.rho_part && stage.ti_out;
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
         {
           const int i = uniform(e / FP), l = e % FP;
@@ -1366,15 +1369,7 @@ __global__ __launch_bounds__(256) void apply_kernel(
 #pragma unroll
           for (int j = 0; j < N; ++j)
             a += tc(TB::Si + i * N + j) * ld(Z, b + j * s1);
-          O[b + i * s1] = a;
-        }
-      lds_sync();
-      // coalesced write; with rho_part set, the next stage input also goes
-      // to LDS unrounded, for its density
-      const bool dens = MODE == 1 && stage.rho_part && stage.ti_out;
-      for (int t = threadIdx.x; t < C; t += blockDim.x)
-        {
-          const T a = O[t];
+          const int t = b + i * s1;
           const long long g = cell * C + t;
           if constexpr (MODE == 0)
             dst[g] = a;
