@@ -225,6 +225,23 @@ def test_quadrature_overintegration_matches_oracle(dims, degree, n_cells,
                                rtol=0, atol=1e-12 * np.max(np.abs(k)) + 1e-14)
 
 
+@pytest.mark.parametrize("dims,degree,n_points", [((2, 2), 2, 5),
+                                                 ((3, 3), 3, 5)])
+def test_quadrature_unsupported_points_rejected(dims, degree, n_points):
+    """Dirichlet boundaries need the quadrature kernel, which runs k+1 and
+    k+2 points (k+2 in 3+3 only up to k=2): other sizes are an error before
+    any launch, not a silent fallback."""
+    D = sum(dims)
+    mf = H.MatrixFree(dims[0], dims[1], degree, [2] * D, [0.0] * D,
+                      [1.0] * D, periodic_x=False, periodic_v=False,
+                      n_points=n_points)
+    op = H.AdvectionOperation(mf, [0.3] * D)
+    src = torch.zeros(mf.n_dofs(), dtype=torch.float64, device="cuda")
+    dst = torch.empty_like(src)
+    with pytest.raises(H.HyperdealError, match="no kernel"):
+        op.apply(dst, src)
+
+
 @pytest.mark.parametrize("n_points,collocation", [(5, False), (6, False),
                                                   (4, True)])
 def test_collapsed_overintegration_and_collocation(n_points, collocation):
