@@ -1046,7 +1046,14 @@ __global__ __launch_bounds__(256) void apply_kernel(
   __shared__ T U[C], T0p[C], T0n[C], T1p[C], T1n[C], Q1[C], Q2[C];
   // the inflow faces, left (FL) and right (FR) of each direction; after S1
   // the 8 N^3 >= N^4 values hold R1
-  __shared__ T FLR[8][F], G1[2][2][F], G2[2][2][F];
+  // FP32: the cell and the v faces go to the x Gauss points in one pass
+  // each (S x S along x0, x1), so the x-part products and the v operators
+  // share a phase (five barriers per cell).  FP64: one direction per pass
+  // (Q1, G1), six barriers -- the one-pass form needs more registers and
+  // 64-bit LDS reads, which cost FP64 a resident block (measured slower).
+  constexpr bool kDirect = sizeof(T) == 4;
+  // G1Y: the x0 transform of the v faces (FP64, [2][2][F]) or Y1 (FP32)
+  __shared__ T FLR[8][F], G2[2][2][F], G1Y[kDirect ? C : 4 * F];
   T(*FL)[F] = FLR;
   T(*FR)[F] = FLR + 4;
   __shared__ T V[2][2][NN]; // [d][+/-][i][l]
@@ -1126,9 +1133,10 @@ __global__ __launch_bounds__(256) void apply_kernel(
           }
       }
   };
-  // reuse: R0 = U and R1 = FLR (both consumed by S1), Y0 = T0p, Y1 = T0n,
-  // Z = T1p
-  T *R0 = U, *R1 = &FLR[0][0], *Y0 = T0p, *Y1 = T0n, *Z = T1p;
+  // reuse: R0 = U and R1 = FLR (both consumed by S1), Z = T1p; Y0, Y1 =
+  // Q1, G1Y (FP32) or T0p, T0n (FP64: consumed by the x-part products)
+  T *R0 = U, *R1 = &FLR[0][0], *Z = T1p;
+  T *Y0 = kDirect ? Q1 : T0p, *Y1 = kDirect ? G1Y : T0n;
 
   if ((long long)blockIdx.x < mesh.n_cells)
     prefetch(blockIdx.x);
@@ -1176,8 +1184,8 @@ __global__ __launch_bounds__(256) void apply_kernel(
         if (next < mesh.n_cells)
           prefetch(next);
       }
-      // S1: x0 operators and the x0 transform (i0 uniform), x1 operators
-      // (i1 uniform), the x0 transform of the v faces
+      // S1: x0 operators (i0 uniform), x1 operators (i1 uniform); the cell
+      // and the v faces at the x Gauss points (FP32) or along x0 (FP64)
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
         {
           const int i = uniform(e / FP), l = e % FP;
@@ -1202,11 +1210,14 @@ __global__ __launch_bounds__(256) void apply_kernel(
                 a += tc(TB::An(0) + i * N + j) * u[j];
               T0n[b + i] = a;
             }
-          T a = 0;
+          if constexpr (!kDirect)
+            {
+              T a = 0;
 #pragma unroll
-          for (int j = 0; j < N; ++j)
-            a += tc(TB::S + i * N + j) * u[j];
-          Q1[b + i] = a;
+              for (int j = 0; j < N; ++j)
+                a += tc(TB::S + i * N + j) * u[j];
+              Q1[b + i] = a;
+            }
         }
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
         {
@@ -1233,50 +1244,97 @@ __global__ __launch_bounds__(256) void apply_kernel(
               T1n[b + i * s1] = a;
             }
         }
-      // v faces (digits x0, x1, other v): along x0, i0 uniform; lines are
-      // (face, side, x1, other v)
-      for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
+      if constexpr (kDirect)
         {
-          const int i = e / (4 * NN), r = e % (4 * NN);
-          const int fs = r / NN, rest = r % NN; // rest = x1 + N * v_other
-          const int ee = fs / 2, side = fs % 2;
-          const T *fsrc = side == 0 ? FL[2 + ee] : FR[2 + ee];
-          T a = 0;
+          // the cell and the v faces at the x Gauss points, (S x S) along
+          // x0, x1 in one pass (i1 uniform): Q2 = (S x S) u, G2 likewise
+          for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+            {
+              const int i1 = uniform(e / FP), l = e % FP;
+              if (l >= F)
+                continue;
+              const int i0 = l % N, v = l / N; // v = v0 + N v1
+              const T *ub = U + NN * v;
+              T a = 0;
 #pragma unroll
-          for (int j = 0; j < N; ++j)
-            a += tc(TB::S + i * N + j) * ld(fsrc, j + N * rest);
-          G1[ee][side][i + N * rest] = a;
+              for (int j1 = 0; j1 < N; ++j1)
+                {
+                  T t = 0;
+#pragma unroll
+                  for (int j0 = 0; j0 < N; ++j0)
+                    t += tc(TB::S + i0 * N + j0) * ld(ub, j0 + N * j1);
+                  a += tc(TB::S + i1 * N + j1) * t;
+                }
+              Q2[i0 + N * i1 + NN * v] = a;
+            }
+          for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
+            {
+              const int i1 = e / (4 * NN), r = e % (4 * NN);
+              const int fs = r / NN, rest = r % NN; // rest = x0 + N * v_other
+              const int ee = fs / 2, side = fs % 2;
+              const int i0 = rest % N, vo = rest / N;
+              const T *fsrc = (side == 0 ? FL[2 + ee] : FR[2 + ee]) + NN * vo;
+              T a = 0;
+#pragma unroll
+              for (int j1 = 0; j1 < N; ++j1)
+                {
+                  T t = 0;
+#pragma unroll
+                  for (int j0 = 0; j0 < N; ++j0)
+                    t += tc(TB::S + i0 * N + j0) * ld(fsrc, j0 + N * j1);
+                  a += tc(TB::S + i1 * N + j1) * t;
+                }
+              G2[ee][side][i0 + N * (i1 + N * vo)] = a;
+            }
         }
+      else
+        // v faces (digits x0, x1, other v): along x0, i0 uniform; lines are
+        // (face, side, x1, other v)
+        for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
+          {
+            const int i = e / (4 * NN), r = e % (4 * NN);
+            const int fs = r / NN, rest = r % NN; // rest = x1 + N * v_other
+            const int ee = fs / 2, side = fs % 2;
+            const T *fsrc = side == 0 ? FL[2 + ee] : FR[2 + ee];
+            T a = 0;
+#pragma unroll
+            for (int j = 0; j < N; ++j)
+              a += tc(TB::S + i * N + j) * ld(fsrc, j + N * rest);
+            G1Y[fs * F + i + N * rest] = a;
+          }
       lds_sync();
-      // S2: the x1 transform of Q1 and of the v faces (i1 uniform), the
-      // v_d factors of the x part
-      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+      // S2 (FP64): the x1 transform of Q1 and of the v faces (i1 uniform)
+      if constexpr (!kDirect)
         {
-          const int i = uniform(e / FP), l = e % FP;
-          if (l >= F)
-            continue;
-          const int b = line_base<N>(l, 1);
-          T a = 0;
+          for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
+            {
+              const int i = uniform(e / FP), l = e % FP;
+              if (l >= F)
+                continue;
+              const int b = line_base<N>(l, 1);
+              T a = 0;
 #pragma unroll
-          for (int j = 0; j < N; ++j)
-            a += tc(TB::S + i * N + j) * ld(Q1, b + j * s1);
-          Q2[b + i * s1] = a;
-        }
-      for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
-        {
-          const int i = e / (4 * NN), r = e % (4 * NN);
-          const int fs = r / NN, rest = r % NN; // rest = x0 + N * v_other
-          const int ee = fs / 2, side = fs % 2;
-          const int x0 = rest % N, vo = rest / N;
-          T a = 0;
+              for (int j = 0; j < N; ++j)
+                a += tc(TB::S + i * N + j) * ld(Q1, b + j * s1);
+              Q2[b + i * s1] = a;
+            }
+          for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
+            {
+              const int i = e / (4 * NN), r = e % (4 * NN);
+              const int fs = r / NN, rest = r % NN; // x0 + N * v_other
+              const int ee = fs / 2, side = fs % 2;
+              const int x0 = rest % N, vo = rest / N;
+              T a = 0;
 #pragma unroll
-          for (int j = 0; j < N; ++j)
-            a += tc(TB::S + i * N + j) *
-                 ld(G1[ee][side], x0 + N * (j + N * vo));
-          G2[ee][side][x0 + N * (i + N * vo)] = a;
+              for (int j = 0; j < N; ++j)
+                a += tc(TB::S + i * N + j) *
+                     ld(G1Y + fs * F, x0 + N * (j + N * vo));
+              G2[ee][side][x0 + N * (i + N * vo)] = a;
+            }
         }
       // the v_d factors of the x part (i2, i3 uniform) into R0, R1 (U and
-      // FLR: S1 has consumed them)
+      // FLR: S1 has consumed them); FP32: the v_e operators follow in the
+      // same phase
       for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
         {
           const int i = uniform(e / FP), l = e % FP;
@@ -1307,9 +1365,11 @@ __global__ __launch_bounds__(256) void apply_kernel(
               a += V[1][1][i * N + j] * ld(T1n, b + j * s3);
           R1[b + i * s3] = a;
         }
-      lds_sync();
-      // S3: the v_e operators at the x Gauss points (i2 / i3 uniform), into
-      // Y0, Y1 (T0p, T0n: consumed)
+      // FP64: R0, R1 have read T0p, T0n, which become Y0, Y1
+      if constexpr (!kDirect)
+        lds_sync();
+      // the v_e operators at the x Gauss points (i2 / i3 uniform), into Y0,
+      // Y1
 #pragma unroll
       for (int ee = 0; ee < 2; ++ee)
         {
