@@ -237,35 +237,17 @@ def gl_weights(N):
     return 1.0 / ((N - 1) * N * L.legval(x, c) ** 2)
 
 
-def c3_workload(H, torch, stream, steps):
-    """3+3, k=5, 6^3 x 6^3 cells (46,656 DoF per cell, 2,176,782,336 DoF):
-    the split collapsed apply in FP64 and FP32-mixed, then 20 rk45 steps in
-    each precision from the same f0 (product of sines + 1e-3 noise, the C2
-    seeds), dt = 0.1 * dt_crit / k^1.5.  Reported: apply GDoF/s and their
-    ratio, the relative L2 difference FP32-mixed vs FP64 after 20 steps and
-    the mass drift of each run (both with the Gauss-Lobatto rule on the
-    nodes)."""
+def lsrk_precision_run(H, torch, stream, mf, op, u, vel, k, h):
+    """20 rk45 steps in FP64 and in FP32-mixed from the same f0 = u on a
+    uniform 3+3 mesh of cell size h, dt = 0.1 * dt_crit / k^1.5: the
+    relative L2 difference FP32-mixed vs FP64 after 20 steps and the mass
+    drift of each run (both with the Gauss-Lobatto rule on the nodes), and
+    the time per step."""
     import numpy as np
-    k, N, nc = 5, 6, [6] * 6
-    vel = [splitmix_uniform(SEED_VELOCITY, i) for i in range(6)]
-    mf = H.MatrixFree(3, 3, k, nc, [0.0] * 6, [1.0] * 6)
-    op = H.AdvectionOperation(mf, vel)
+    N = k + 1
     n = mf.n_dofs()
     cell = N ** 6
-    u = torch.empty(n, dtype=torch.float64, device="cuda")
-    H.VectorTools.interpolate(mf, u, H.FUNC_PRODUCT_SINES, [0.0] * 6)
-    H.VectorTools.add_noise(mf, u, SEED_NOISE, 1e-3)
-    dst = torch.empty_like(u)
-    t64 = timed(lambda: op.apply(dst, u, 0.0, H.FP64, stream), steps, 2,
-                stream)
-    del dst
-    u32 = u.float()
-    d32 = torch.empty_like(u32)
-    t32 = timed(lambda: op.apply(d32, u32, 0.0, H.FP32, stream), steps, 2,
-                stream)
-    del u32, d32
-    torch.cuda.empty_cache()
-    w1 = gl_weights(N) / 6.0  # h = 1/6
+    w1 = gl_weights(N) * h
     W = w1
     for _ in range(5):
         W = np.multiply.outer(W, w1)
@@ -303,6 +285,48 @@ def c3_workload(H, torch, stream, steps):
         den += float(((a ** 2) @ W).sum())
     l1 = float((u.abs().view(-1, cell) @ W).sum())
     out = {
+        "dt": dt,
+        "rel_l2_fp32_mixed_vs_fp64": (num / den) ** 0.5,
+        "mass_drift_fp64": runs["fp64"][2] - runs["fp64"][1],
+        "mass_drift_fp32_mixed":
+            runs["fp32_mixed"][2] - runs["fp32_mixed"][1],
+        "mass_scale": l1,
+        "note": "mass drifts are absolute; mass_scale = int |f0|",
+        "s_per_step_fp64": round(runs["fp64"][3] / 20, 4),
+        "s_per_step_fp32_mixed": round(runs["fp32_mixed"][3] / 20, 4)}
+    del runs, s64, s32
+    torch.cuda.empty_cache()
+    return out
+
+
+def c3_workload(H, torch, stream, steps):
+    """3+3, k=5, 6^3 x 6^3 cells (46,656 DoF per cell, 2,176,782,336 DoF):
+    the split collapsed apply in FP64 and FP32-mixed, then 20 rk45 steps in
+    each precision from the same f0 (product of sines + 1e-3 noise, the C2
+    seeds), dt = 0.1 * dt_crit / k^1.5.  Reported: apply GDoF/s and their
+    ratio, the relative L2 difference FP32-mixed vs FP64 after 20 steps and
+    the mass drift of each run (both with the Gauss-Lobatto rule on the
+    nodes)."""
+    k, nc = 5, [6] * 6
+    vel = [splitmix_uniform(SEED_VELOCITY, i) for i in range(6)]
+    mf = H.MatrixFree(3, 3, k, nc, [0.0] * 6, [1.0] * 6)
+    op = H.AdvectionOperation(mf, vel)
+    n = mf.n_dofs()
+    u = torch.empty(n, dtype=torch.float64, device="cuda")
+    H.VectorTools.interpolate(mf, u, H.FUNC_PRODUCT_SINES, [0.0] * 6)
+    H.VectorTools.add_noise(mf, u, SEED_NOISE, 1e-3)
+    dst = torch.empty_like(u)
+    t64 = timed(lambda: op.apply(dst, u, 0.0, H.FP64, stream), steps, 2,
+                stream)
+    del dst
+    u32 = u.float()
+    d32 = torch.empty_like(u32)
+    t32 = timed(lambda: op.apply(d32, u32, 0.0, H.FP32, stream), steps, 2,
+                stream)
+    del u32, d32
+    torch.cuda.empty_cache()
+    lsrk = lsrk_precision_run(H, torch, stream, mf, op, u, vel, k, 1.0 / 6)
+    out = {
         "workload": "3D+3D apply, periodic [0,1]^6, 6^3 x 6^3 cells, k=5 "
                     "(46656 DoF/cell, %d DoF), split collapsed form" % n,
         "fp64": {"value": round(n / t64 / 1e9, 3), "unit": "GDoF/s",
@@ -312,18 +336,9 @@ def c3_workload(H, torch, stream, steps):
                        "ms_per_apply": round(t32 * 1e3, 3),
                        "achieved_GBs": round(8.0 * n / t32 / 1e9, 1)},
         "ratio_fp32_vs_fp64": round(t64 / t32, 3),
-        "lsrk_20_steps": {
-            "dt": dt,
-            "rel_l2_fp32_mixed_vs_fp64": (num / den) ** 0.5,
-            "mass_drift_fp64": runs["fp64"][2] - runs["fp64"][1],
-            "mass_drift_fp32_mixed":
-                runs["fp32_mixed"][2] - runs["fp32_mixed"][1],
-            "mass_scale": l1,
-            "note": "mass drifts are absolute; mass_scale = int |f0|",
-            "s_per_step_fp64": round(runs["fp64"][3] / 20, 4),
-            "s_per_step_fp32_mixed": round(runs["fp32_mixed"][3] / 20, 4)},
+        "lsrk_20_steps": lsrk,
     }
-    del runs, s64, s32, u
+    del u
     torch.cuda.empty_cache()
     return out
 
@@ -387,6 +402,11 @@ def main():
                   args.warmup, stream)
         results["fp32"] = t
         del u32, d32
+    c2_lsrk = None
+    if args.only in ("all", "fp32"):
+        # the precision run on the same mesh: 20 rk45 steps per precision
+        c2_lsrk = lsrk_precision_run(H, torch, stream, mf, op, u, velocity,
+                                     3, 1.0 / 8)
     if args.only in ("all", "stage"):
         # fused LSRK stage, FP64 (Ti, sol in FP64) and FP32-mixed (Ti FP32)
         for prec, dtype in ((H.FP64, torch.float64), (H.FP32, torch.float32)):
@@ -464,6 +484,8 @@ def main():
         if "fp64" in results:
             line["fp32_mixed"]["ratio_vs_fp64"] = round(
                 results["fp64"] / t, 3)
+        if c2_lsrk is not None:
+            line["fp32_mixed"]["lsrk_20_steps"] = c2_lsrk
     for key, bpd in (("stage_fp64", 32.0), ("stage_fp32", 24.0)):
         if key in results:
             t = results[key]
