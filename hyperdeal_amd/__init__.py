@@ -390,8 +390,15 @@ class VlasovPoissonOperation(AdvectionOperation):
         """Start the potential solves from the exact solution of the
         periodic 2D SIPG system (fast diagonalisation); CG still runs to the
         tolerance from there.  Disables a learned guess."""
+        if enable:
+            self._release_learned_net()
         _check(self._lib.hd_vp_exact_start(self.handle, 1 if enable else 0),
                "hd_vp_exact_start")
+
+    def _release_learned_net(self):
+        # a network borrowed from the learned guess (learned_guess_net) dies
+        # with it: a new generation invalidates the borrowed handles
+        self._learned_gen = getattr(self, "_learned_gen", 0) + 1
 
     def enable_learned_guess(self, seed=20261205, capacity=64, train_every=1,
                              train_batch=8, train_steps=1,
@@ -399,12 +406,14 @@ class VlasovPoissonOperation(AdvectionOperation):
         """Start the potential solves from the previous phi combined with
         the CNN's correction for its residual; the network trains on this
         operator's own solves (residual -> correction pairs)."""
+        self._release_learned_net()
         _check(self._lib.hd_vp_learned_guess_enable(
             self.handle, int(seed), int(capacity), int(train_every),
             int(train_batch), int(train_steps), float(learning_rate),
             int(min_pairs)), "hd_vp_learned_guess_enable")
 
     def disable_learned_guess(self):
+        self._release_learned_net()
         _check(self._lib.hd_vp_learned_guess_disable(self.handle),
                "hd_vp_learned_guess_disable")
 
@@ -419,7 +428,10 @@ class VlasovPoissonOperation(AdvectionOperation):
         return st.value, gs.value, pr.value, loss.value
 
     def learned_guess_net(self):
-        """The network of the learned guess (borrowed from this operator)."""
+        """The network of the learned guess, borrowed from this operator:
+        usable until the learned guess is disabled, enabled again or
+        replaced by the exact start (calls raise HyperdealError after
+        that)."""
         h = self._lib.hd_vp_learned_guess_net(self.handle)
         if not h:
             raise HyperdealError("no learned guess on this operator")
@@ -438,6 +450,9 @@ class VlasovPoissonOperation(AdvectionOperation):
     def set_potential(self, phi):
         import numpy as np
         phi = np.ascontiguousarray(phi, dtype=np.float64)
+        if phi.ndim != 1 or phi.size != self.x_n_dofs():
+            raise HyperdealError("set_potential: %d values, expected %d"
+                                 % (phi.size, self.x_n_dofs()))
         _check(self._lib.hd_vp_potential(
             self.handle, phi.ctypes.data_as(_dp), 0), "hd_vp_potential")
 
@@ -459,14 +474,25 @@ class Cnn:
         self._lib = lib or _lib or load_library()
         self._owner = _owner
         if _handle is not None:
-            self.handle = _vp(_handle)
+            self._handle = _vp(_handle)
             self._owned = False
+            self._gen = getattr(_owner, "_learned_gen", 0)
             return
         h = _vp()
         _check(self._lib.hd_cnn_create(int(height), int(width), int(seed),
                                        ctypes.byref(h)), "hd_cnn_create")
-        self.handle = h
+        self._handle = h
         self._owned = True
+
+    @property
+    def handle(self):
+        """the C handle; for a borrowed network, only while its operator's
+        learned guess lives"""
+        if not self._owned and \
+                getattr(self._owner, "_learned_gen", 0) != self._gen:
+            raise HyperdealError("the learned guess this network belonged "
+                                 "to was released")
+        return self._handle
 
     @staticmethod
     def n_params(lib=None):
@@ -530,9 +556,9 @@ class Cnn:
         return loss.value
 
     def __del__(self):
-        if getattr(self, "_owned", False) and self.handle:
-            self._lib.hd_cnn_destroy(self.handle)
-            self.handle = None
+        if getattr(self, "_owned", False) and self._handle:
+            self._lib.hd_cnn_destroy(self._handle)
+            self._handle = None
 
 
 def vp_moments(mf, f, stream=None):

@@ -667,7 +667,8 @@ int hd_vp_solver_stats(hd_advection *op, long long *iterations,
     return fail(HD_ERR_INVALID, "not a Vlasov-Poisson operator");
   long long s[3] = {0, 0, 0};
   const hipError_t e =
-    hipMemcpy(s, op->stats, 3 * sizeof(long long), hipMemcpyDeviceToHost);
+    hd_abi::device_sync_copy(s, op->stats, 3 * sizeof(long long),
+                             hipMemcpyDeviceToHost);
   if (e != hipSuccess)
     return hip_fail(e, "hd_vp_solver_stats");
   if (iterations)
@@ -687,7 +688,8 @@ int hd_vp_solver_start(hd_advection *op, double *start_rel_residual,
     return fail(HD_ERR_INVALID, "not a Vlasov-Poisson operator");
   long long s[8];
   const hipError_t e =
-    hipMemcpy(s, op->stats, 8 * sizeof(long long), hipMemcpyDeviceToHost);
+    hd_abi::device_sync_copy(s, op->stats, 8 * sizeof(long long),
+                             hipMemcpyDeviceToHost);
   if (e != hipSuccess)
     return hip_fail(e, "hd_vp_solver_start");
   if (start_rel_residual)
@@ -755,7 +757,7 @@ int hd_vp_potential(hd_advection *op, double *phi, int device_to_host)
   if (!op || !op->vp || !phi)
     return fail(HD_ERR_INVALID, "hd_vp_potential: invalid argument");
   const hipError_t e =
-    hipMemcpy(device_to_host ? (void *)phi : (void *)op->phi,
+    hd_abi::device_sync_copy(device_to_host ? (void *)phi : (void *)op->phi,
               device_to_host ? (void *)op->phi : (void *)phi,
               op->n_x_dofs * sizeof(double),
               device_to_host ? hipMemcpyDeviceToHost : hipMemcpyHostToDevice);
@@ -771,9 +773,8 @@ int hd_vp_field_energy(hd_advection *op, double *out)
   hd_matrix_free *mf = op->mf;
   const int dx = mf->dim_x, nq = mf->nq, nqx = hd::ipow(nq, dx);
   std::vector<double> ef((size_t)op->n_x_cells * nqx * dx);
-  const hipError_t e = hipMemcpy(ef.data(), op->efield,
-                                 ef.size() * sizeof(double),
-                                 hipMemcpyDeviceToHost);
+  const hipError_t e = hd_abi::device_sync_copy(
+    ef.data(), op->efield, ef.size() * sizeof(double), hipMemcpyDeviceToHost);
   if (e != hipSuccess)
     return hip_fail(e, "hd_vp_field_energy");
   for (int k = 0; k < dx; ++k)
@@ -1094,13 +1095,15 @@ int hd_device_free(void *ptr)
 
 int hd_memcpy_htod(void *dst, const void *src, uint64_t bytes)
 {
-  const hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice);
+  const hipError_t e =
+    hd_abi::device_sync_copy(dst, src, bytes, hipMemcpyHostToDevice);
   return e == hipSuccess ? ok() : hip_fail(e, "hd_memcpy_htod");
 }
 
 int hd_memcpy_dtoh(void *dst, const void *src, uint64_t bytes)
 {
-  const hipError_t e = hipMemcpy(dst, src, bytes, hipMemcpyDeviceToHost);
+  const hipError_t e =
+    hd_abi::device_sync_copy(dst, src, bytes, hipMemcpyDeviceToHost);
   return e == hipSuccess ? ok() : hip_fail(e, "hd_memcpy_dtoh");
 }
 

@@ -34,6 +34,23 @@ inline hipStream_t as_stream(void *s)
 {
   return reinterpret_cast<hipStream_t>(s);
 }
+
+// A copy of operator state between host and device for the entry points
+// that take no stream (statistics, the potential, the field energy, network
+// parameters).  The state is written by kernels on the caller's streams,
+// which may be non-blocking (they do not wait for the null stream, nor it for
+// them), so the whole device is synchronised before the copy; the copy itself
+// is blocking, so work issued afterwards on any stream sees its result.
+inline hipError_t device_sync_copy(void *dst, const void *src, size_t bytes,
+                                   hipMemcpyKind kind)
+{
+  hipError_t e = hipDeviceSynchronize();
+  if (e == hipSuccess)
+    e = hipMemcpy(dst, src, bytes, kind);
+  if (e == hipSuccess && kind != hipMemcpyDeviceToHost)
+    e = hipDeviceSynchronize();
+  return e;
+}
 } // namespace hd_abi
 
 struct hd_matrix_free

@@ -230,10 +230,16 @@ int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
 int hd_vp_update_field(hd_advection *op, const double *f, void *stream);
 int hd_vp_set_auto_update(hd_advection *op, int enable);
 int hd_vp_set_poisson_tolerance(hd_advection *op, double rel_tol, int max_it);
-/* accumulated CG iterations and solves, last relative residual (sync) */
+/* accumulated CG iterations and solves, last relative residual.  This and
+ * the other entry points below that take no stream (hd_vp_solver_start,
+ * hd_vp_potential, hd_vp_field_energy, hd_cnn_params, hd_cnn_last_loss,
+ * hd_vp_learned_guess_stats, hd_memcpy_htod / dtoh) synchronise the whole
+ * device before they read or write operator state, so they see the work of
+ * every stream, non-blocking ones included, and work issued after them sees
+ * what they wrote. */
 int hd_vp_solver_stats(hd_advection *op, long long *iterations,
                        long long *solves, double *last_rel_residual);
-/* the start of the last solve (sync): its relative residual, that of the
+/* the start of the last solve (device-synchronising): its relative residual, that of the
  * previous phi alone, the weights (previous phi, guess) of the start, and
  * the number of solves that started from a learned guess */
 int hd_vp_solver_start(hd_advection *op, double *start_rel_residual,
@@ -252,9 +258,10 @@ int hd_vp_solver_start(hd_advection *op, double *start_rel_residual,
 int hd_vp_exact_start(hd_advection *op, int enable);
 /* x-space DoFs of phi / rho: (k+1)^dim_x per x cell */
 long long hd_vp_x_n_dofs(const hd_advection *op);
-/* copy phi device -> host (device_to_host != 0) or host -> device */
+/* copy phi device -> host (device_to_host != 0) or host -> device
+ * (device-synchronising); the length is hd_vp_x_n_dofs */
 int hd_vp_potential(hd_advection *op, double *phi, int device_to_host);
-/* sum (d_e phi)^2 JxW per x component (sync) */
+/* sum (d_e phi)^2 JxW per x component (device-synchronising) */
 int hd_vp_field_energy(hd_advection *op, double *out);
 /* mass, ||f||_L2, kinetic energy, momentum[3] of f (sync) */
 int hd_vp_moments(hd_matrix_free *mf, const double *f, double *out,
@@ -310,7 +317,10 @@ int hd_cnn_loss_grad(hd_cnn *net, const float *in, const float *target,
 int hd_cnn_train_step(hd_cnn *net, const float *in, const float *target,
                       int batch, float learning_rate, void *stream);
 int hd_cnn_last_loss(hd_cnn *net, double *loss);
-/* the network of an operator's learned guess (borrowed; NULL if off) */
+/* the network of an operator's learned guess (borrowed; NULL if off).  It
+ * is freed by hd_vp_learned_guess_disable, by a second
+ * hd_vp_learned_guess_enable and by hd_vp_exact_start(op, 1): the pointer
+ * must not be used after any of them. */
 hd_cnn *hd_vp_learned_guess_net(hd_advection *op);
 
 /* dst (FP64) = function at the Gauss-Lobatto nodes (numerics/
@@ -336,7 +346,8 @@ int hd_vector_convert(hd_matrix_free *mf, const double *src, void *dst,
 /* CFL time step (operators/advection/cfl.h:94-124). */
 double hd_critical_time_step(const hd_matrix_free *mf, const double *velocity);
 
-/* Device memory helpers for hosts without another allocator. */
+/* Device memory helpers for hosts without another allocator; the copies are
+ * device-synchronising. */
 int hd_device_malloc(void **ptr, uint64_t bytes);
 int hd_device_free(void *ptr);
 int hd_memcpy_htod(void *dst, const void *src, uint64_t bytes);

@@ -184,6 +184,8 @@ public:
   void apply(DeviceVector<Number> &dst, const DeviceVector<Number> &src,
              const double time, void *stream = nullptr) const
   {
+    check_size(src, "AdvectionOperation::apply: src");
+    check_size(dst, "AdvectionOperation::apply: dst");
     check(hd_advection_apply(op_, src.data(), dst.data(),
                              precision_of<Number>(), time, stream),
           "AdvectionOperation::apply");
@@ -192,6 +194,15 @@ public:
   const std::array<double, dim> &velocity() const { return velocity_; }
   hd_advection *handle() const { return op_; }
   const MatrixFree<dim_x, dim_v> &get_matrix_free() const { return data_; }
+
+  // a vector of the operator's size (the kernels index it up to n_dofs)
+  template <typename V>
+  void check_size(const DeviceVector<V> &v, const char *what) const
+  {
+    if (v.size() != data_.n_dofs())
+      throw Error(std::string(what) + ": size " + std::to_string(v.size()) +
+                  " != n_dofs " + std::to_string(data_.n_dofs()));
+  }
 
 private:
   const MatrixFree<dim_x, dim_v> &data_;
@@ -220,6 +231,9 @@ public:
                          const double current_time, const double time_step,
                          void *stream = nullptr)
   {
+    op.check_size(solution, "perform_time_step: solution");
+    op.check_size(ti_, "perform_time_step: Ti");
+    op.check_size(ki_, "perform_time_step: Ki");
     check(hd_lsrk_step(op.handle(), type_.c_str(), precision_of<Number>(),
                        solution.data(), ti_.data(), ki_.data(), current_time,
                        time_step, stream),
@@ -263,6 +277,8 @@ struct VectorTools
                           const double *params, double time,
                           DeviceVector<double> &dst, void *stream = nullptr)
   {
+    if (dst.size() != mf.n_dofs())
+      throw Error("VectorTools::interpolate: size mismatch");
     check(hd_interpolate(mf.handle(), function, params, time, dst.data(),
                          stream),
           "VectorTools::interpolate");
@@ -274,6 +290,8 @@ struct VectorTools
                  int function, const double *params, double time,
                  void *stream = nullptr)
   {
+    if (src.size() != mf.n_dofs())
+      throw Error("VectorTools::norm_and_error: size mismatch");
     std::array<double, 2> out{};
     check(hd_norm_and_error(mf.handle(), src.data(), function, params, time,
                             out.data(), stream),

@@ -8,41 +8,73 @@
 // evaluators FEEvaluation, FEFaceEvaluation and FEEvaluationInverse read
 // DoFs, hold per-point data and submit integrands
 // (matrix_free/fe_evaluation_cell.h:41-404, fe_evaluation_face.h:41-568,
-// fe_evaluation_cell_inverse.h:36-134); the 1D sum-factorisation kernels come
-// from deal.II's EvaluatorTensorProduct.  This header restates that
-// interface with the same names, callback signatures and argument meanings
-// for Cartesian phase-space meshes (affine cells, periodic or Dirichlet
-// directions), running the callbacks on the host.  The GPU runs the built-in
-// operators (AdvectionOperation, the Vlasov-Poisson operator) through the C
-// ABI; device code cannot call host lambdas, so operators written against
-// this interface run where it runs.
+// fe_evaluation_cell_inverse.h:36-134); the 1D sum-factorisation kernels and
+// the low-dimensional matrix-free objects come from deal.II.  This header
+// restates that interface with the same class templates, constructor and
+// loop signatures, callback types and argument meanings for Cartesian
+// phase-space meshes (affine cells, periodic or Dirichlet directions):
+//
+//   MatrixFree<dim_x, dim_v, Number, VectorizedArrayType>
+//     (x-space matrix-free, v-space matrix-free); reinit(AdditionalData);
+//     loop(cell, face, boundary, dst, src, DataAccessOnFaces dst_access,
+//          DataAccessOnFaces src_access, Timers *);
+//     loop_cell_centric(cell, dst, src, DataAccessOnFaces src_access,
+//                       Timers *);
+//     cell_loop(cell, dst, src)  -- each with std::function or member
+//     pointer + owner;
+//   FEEvaluation / FEEvaluationInverse<dim_x, dim_v, degree, n_points,
+//     Number, VectorizedArrayType>(mf, dof_no_x, dof_no_v, quad_no_x,
+//     quad_no_v);
+//   FEFaceEvaluation<...>(mf, is_minus_face, dof_no_x, dof_no_v, quad_no_x,
+//     quad_no_v);
+//   EvaluatorTensorProduct<dim, n_rows, n_columns, Number>::values /
+//     gradients / hessians<direction, contract_over_rows, add>,
+//   FEEvaluationImplBasisChange<dim, n_rows, n_columns, Number>::do_forward /
+//     do_backward (n_points != degree + 1),
+//   FEFaceNormalEvaluationImpl<dim_x, dim_v, n_points - 1, Number>::
+//     interpolate_quadrature<evaluate, add> (cell points <-> face points).
+//
+// The low-dimensional spaces (deal.II's MatrixFree<dim> in the reference) are
+// CartesianMatrixFree<dim>: a subdivided box with one FE_DGQ(degree) on
+// Gauss-Lobatto support points and a list of 1D quadratures, numbered as the
+// reference's advection application numbers them (examples/advection/
+// include/application.h:217-223): 0 = QGaussLobatto(n_points) with
+// collocation, else QGauss(n_points); 1 = QGauss(degree + 1);
+// 2 = QGaussLobatto(degree + 1).
+//
+// The callbacks run on the host: device code cannot call host lambdas.  The
+// GPU runs the built-in operators (AdvectionOperation, the Vlasov-Poisson
+// operator) through the C ABI.
 //
 // Conventions (those of the reference):
 //   - a cell's DoFs are its (degree+1)^D values at the Gauss-Lobatto nodes,
 //     lexicographic with direction 0 fastest, x directions before v; cells
-//     are numbered x first (cell = x + n_x_cells v);
-//   - evaluators hold one buffer of n^D values (n = degree + 1 = n_points);
-//     TensorProduct::values / gradients / hessians<dir, contract_over_rows,
-//     add> apply the 1D shape values S (nodes -> Gauss points), the
-//     collocation derivative at the Gauss points, and S^-1 (the inverse
-//     mass matrix's factor) along one direction, contract_over_rows = true
-//     in the forward sense (nodes -> points, or the derivative at points);
+//     are numbered x first (cell = x + n_x_cells v); a cell evaluator's
+//     buffer holds max(degree + 1, n_points)^D values;
+//   - quadrature point q = qx + n_points^dim_x qv; a face's points are its
+//     directions in increasing order, q = qx + (x points of the face) qv;
 //   - get_gradient_x / _v read the reference-cell derivatives of direction e
 //     at grad_in[q + e n_q^D]; submit_gradient_x / _v write them there,
 //     multiplied by JxW and the inverse Jacobian's transpose;
 //   - FEFaceEvaluation::submit_value stores -value JxW_face; the normal is
-//     the outward normal of the evaluator's own cell; a face buffer holds
-//     n^(D-1) values, the face's directions in increasing order.
-// Only degree + 1 == n_points (Gauss points) is supported.
+//     the outward normal of the evaluator's own cell.
+// VectorizedArrayType has one lane here (it is the scalar Number): a
+// "macro cell" is one cell, and n_vectorization_lanes_filled() is 1.
 #pragma once
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cmath>
 #include <cstddef>
+#include <deque>
 #include <functional>
+#include <limits>
+#include <map>
+#include <ostream>
 #include <stdexcept>
 #include <string>
+#include <type_traits>
 #include <vector>
 
 namespace hyperdeal_amd
@@ -50,22 +82,159 @@ namespace hyperdeal_amd
 namespace host
 {
 // ---------------------------------------------------------------------------
-// 1D shape data
+// timers (base/timers.h: Timer, Timers, ScopedTimerWrapper)
 // ---------------------------------------------------------------------------
-struct Shape1D
+class Timer
 {
-  int n = 0;                   // nodes = Gauss points = degree + 1
-  std::vector<double> nodes;   // Gauss-Lobatto nodes on [0, 1]
-  std::vector<double> points;  // Gauss points on [0, 1]
-  std::vector<double> weights; // Gauss weights on [0, 1]
-  // [dof][q] = l_dof(x_q): values<dir, true> maps nodal values to points
-  std::vector<double> values;
-  // [q][q'] = derivative at x_q' of the Lagrange polynomial on the Gauss
-  // points that is 1 at x_q: gradients<dir, true> differentiates point values
-  std::vector<double> gradients;
-  // [dof][q] = (S^-1)[dof][q] with S[q][dof] = l_dof(x_q): hessians<dir,
-  // true> applies S^-T, hessians<dir, false> S^-1
-  std::vector<double> inverse;
+public:
+  void reserve(unsigned int max_size) { times_.reserve(max_size); }
+  void reset()
+  {
+    counter_ = 0;
+    accumulated_ = 0.0;
+  }
+  void start() { t0_ = std::chrono::steady_clock::now(); }
+  void stop()
+  {
+    const double dt = std::chrono::duration<double, std::micro>(
+                        std::chrono::steady_clock::now() - t0_)
+                        .count();
+    accumulated_ += dt;
+    if (times_.capacity() > 0)
+      times_.push_back(dt);
+    ++counter_;
+  }
+  unsigned int get_counter() const { return counter_; }
+  // microseconds
+  double get_accumulated_time() const { return accumulated_; }
+  const std::vector<double> &get_log() const { return times_; }
+
+private:
+  unsigned int counter_ = 0;
+  double accumulated_ = 0.0;
+  std::chrono::steady_clock::time_point t0_{};
+  std::vector<double> times_;
+};
+
+// timers addressed by label, nested by enter() / leave() ("a:b:label");
+// references returned by operator[] stay valid (deque storage)
+class Timers
+{
+public:
+  explicit Timers(bool log_all_calls = false) : log_all_(log_all_calls)
+  {
+    path_.emplace_back("");
+  }
+  Timer &operator[](const std::string &label)
+  {
+    const std::string key = path_.back() + label;
+    auto it = map_.find(key);
+    if (it != map_.end())
+      return timers_[it->second];
+    map_[key] = timers_.size();
+    timers_.emplace_back();
+    if (log_all_)
+      timers_.back().reserve(1000);
+    return timers_.back();
+  }
+  void reset()
+  {
+    for (auto &t : timers_)
+      t.reset();
+  }
+  void enter(const std::string &label)
+  {
+    path_.emplace_back(path_.back() + label + ":");
+  }
+  void leave()
+  {
+    if (path_.size() > 1)
+      path_.pop_back();
+  }
+  // one line per timer: label, seconds, calls
+  void print(std::ostream &os) const
+  {
+    for (const auto &kv : map_)
+      os << kv.first << " " << timers_[kv.second].get_accumulated_time() * 1e-6
+         << " " << timers_[kv.second].get_counter() << "\n";
+  }
+  // the timer of a full label ("a:b:label"), or null
+  const Timer *find(const std::string &full_label) const
+  {
+    auto it = map_.find(full_label);
+    return it == map_.end() ? nullptr : &timers_[it->second];
+  }
+
+private:
+  bool log_all_;
+  std::vector<std::string> path_;
+  std::map<std::string, std::size_t> map_;
+  std::deque<Timer> timers_;
+};
+
+// times its scope into (*timers)[label]; does nothing for timers == nullptr
+class ScopedTimerWrapper
+{
+public:
+  ScopedTimerWrapper(Timers *timers, const std::string &label)
+    : timer_(timers ? &(*timers)[label] : nullptr)
+  {
+    if (timer_)
+      timer_->start();
+  }
+  ~ScopedTimerWrapper()
+  {
+    if (timer_)
+      timer_->stop();
+  }
+  ScopedTimerWrapper(const ScopedTimerWrapper &) = delete;
+  ScopedTimerWrapper &operator=(const ScopedTimerWrapper &) = delete;
+
+private:
+  Timer *timer_;
+};
+
+// one lane: deal.II's VectorizedArray<Number, width> is Number itself here
+template <typename Number, std::size_t width = 1>
+using VectorizedArray = Number;
+
+// ---------------------------------------------------------------------------
+// IDs (matrix_free/id.h:26-70)
+// ---------------------------------------------------------------------------
+struct TensorID
+{
+  // a cell (XV), a face in x (X: x = face index, v = cell) or in v (V: x =
+  // cell, v = face index); inner faces come first, boundary faces after
+  enum class SpaceType
+  {
+    XV,
+    X,
+    V
+  };
+  TensorID(unsigned int x, unsigned int v, unsigned int macro,
+           SpaceType type = SpaceType::XV)
+    : x(x), v(v), macro(macro), type(type)
+  {
+  }
+  const unsigned int x, v;
+  // running number of the cell or face within its loop
+  const unsigned int macro;
+  const SpaceType type;
+};
+
+// boundary id of an inner face (deal.II's numbers::internal_face_boundary_id)
+constexpr unsigned int internal_face_boundary_id =
+  std::numeric_limits<unsigned char>::max();
+
+// ---------------------------------------------------------------------------
+// 1D quadratures and shape data
+// ---------------------------------------------------------------------------
+struct Quadrature1D
+{
+  bool lobatto = false;
+  int n = 0;
+  std::vector<double> points;  // on [0, 1], increasing
+  std::vector<double> weights; // sum 1
 };
 
 namespace detail
@@ -120,9 +289,9 @@ inline long double lagrange_derivative(const std::vector<double> &xs, int j,
 }
 
 // n x n dense inverse (Gauss-Jordan, partial pivoting, long double)
-inline std::vector<double> invert(const std::vector<double> &A, int n)
+inline std::vector<long double> invert(std::vector<long double> a, int n)
 {
-  std::vector<long double> a(A.begin(), A.end()), b((size_t)n * n, 0.0L);
+  std::vector<long double> b((size_t)n * n, 0.0L);
   for (int i = 0; i < n; ++i)
     b[(size_t)i * n + i] = 1.0L;
   for (int c = 0; c < n; ++c)
@@ -138,6 +307,8 @@ inline std::vector<double> invert(const std::vector<double> &A, int n)
           std::swap(b[(size_t)c * n + k], b[(size_t)piv * n + k]);
         }
       const long double d = a[(size_t)c * n + c];
+      if (d == 0.0L)
+        throw std::runtime_error("singular matrix");
       for (int k = 0; k < n; ++k)
         {
           a[(size_t)c * n + k] /= d;
@@ -154,22 +325,38 @@ inline std::vector<double> invert(const std::vector<double> &A, int n)
               }
           }
     }
-  return std::vector<double>(b.begin(), b.end());
+  return b;
+}
+
+inline int ipow(int b, int e)
+{
+  int r = 1;
+  while (e-- > 0)
+    r *= b;
+  return r;
+}
+
+constexpr int cpow(int b, int e)
+{
+  return e <= 0 ? 1 : b * cpow(b, e - 1);
 }
 } // namespace detail
 
-inline Shape1D make_shape(int degree)
+// Gauss-Legendre rule with n points on [0, 1]: the roots of P_n (Newton from
+// Chebyshev guesses), weights 1 / ((1 - z^2) P_n'(z)^2)
+inline Quadrature1D QGauss(int n)
 {
-  Shape1D s;
-  const int n = degree + 1;
-  s.n = n;
-  // Gauss points: roots of P_n (Newton from Chebyshev guesses)
-  s.points.resize(n);
-  s.weights.resize(n);
+  if (n < 1)
+    throw std::invalid_argument("QGauss: n < 1");
+  Quadrature1D q;
+  q.n = n;
+  q.points.resize(n);
+  q.weights.resize(n);
   for (int i = 0; i < n; ++i)
     {
-      long double z = std::cos(detail::kPi * (i + 0.75L) / (n + 0.5L)), p,
-                  dp;
+      long double z =
+                    std::cos(detail::kPi * (i + 0.75L) / (n + 0.5L)),
+                  p, dp;
       for (int it = 0; it < 100; ++it)
         {
           detail::legendre(n, z, p, dp);
@@ -179,689 +366,1299 @@ inline Shape1D make_shape(int degree)
             break;
         }
       detail::legendre(n, z, p, dp);
-      s.points[n - 1 - i] = (double)(0.5L * (1.0L + z));
-      s.weights[n - 1 - i] = (double)(1.0L / ((1.0L - z * z) * dp * dp));
+      q.points[n - 1 - i] = (double)(0.5L * (1.0L + z));
+      q.weights[n - 1 - i] = (double)(1.0L / ((1.0L - z * z) * dp * dp));
     }
-  // Gauss-Lobatto nodes: 0, 1 and the roots of P_{n-1}'
-  s.nodes.assign(n, 0.0);
-  s.nodes[n - 1] = 1.0;
+  return q;
+}
+
+// Gauss-Lobatto rule with n >= 2 points on [0, 1]: 0, 1 and the roots of
+// P_{n-1}'; weights 2 / (n (n-1) P_{n-1}(z)^2) on [-1, 1]
+inline Quadrature1D QGaussLobatto(int n)
+{
+  if (n < 2)
+    throw std::invalid_argument("QGaussLobatto: n < 2");
+  Quadrature1D q;
+  q.lobatto = true;
+  q.n = n;
+  q.points.assign(n, 0.0);
+  q.weights.assign(n, 0.0);
+  std::vector<long double> z(n);
+  z[0] = -1.0L;
+  z[n - 1] = 1.0L;
   for (int i = 1; i < n - 1; ++i)
     {
-      long double z = -std::cos(detail::kPi * i / (n - 1));
+      long double x = -std::cos(detail::kPi * i / (n - 1));
       for (int it = 0; it < 100; ++it)
         {
-          // f = P_{n-1}'(z), f' = P_{n-1}''(z) from the Legendre ODE
+          // f = P_{n-1}'(x), f' = P_{n-1}''(x) from the Legendre ODE
           long double p, dp;
-          detail::legendre(n - 1, z, p, dp);
+          detail::legendre(n - 1, x, p, dp);
           const long double d2 =
-            (2.0L * z * dp - (n - 1) * n * p) / (1.0L - z * z);
-          const long double dz = dp / d2;
-          z -= dz;
-          if (std::fabs((double)dz) < 1e-19)
+            (2.0L * x * dp - (n - 1) * n * p) / (1.0L - x * x);
+          const long double dx = dp / d2;
+          x -= dx;
+          if (std::fabs((double)dx) < 1e-19)
             break;
         }
-      s.nodes[i] = (double)(0.5L * (1.0L + z));
+      z[i] = x;
     }
-  s.values.resize((size_t)n * n);
-  s.gradients.resize((size_t)n * n);
-  std::vector<double> S((size_t)n * n);
+  for (int i = 0; i < n; ++i)
+    {
+      long double p, dp;
+      detail::legendre(n - 1, z[i], p, dp);
+      q.points[i] = (double)(0.5L * (1.0L + z[i]));
+      q.weights[i] = (double)(1.0L / (n * (n - 1) * p * p));
+    }
+  return q;
+}
+
+// deal.II's element types of the shape data (shape_info.h)
+enum class ElementType
+{
+  tensor_symmetric_collocation, // nodes == quadrature points
+  tensor_symmetric
+};
+
+// The 1D shape data of FE_DGQ(degree) (Lagrange basis on the degree + 1
+// Gauss-Lobatto nodes) on one quadrature -- the role of deal.II's
+// UnivariateShapeData.  Matrices are row-major with the basis (dof) side as
+// rows, the point side as columns:
+//   shape_values [dof][q] = l_dof(x_q)                   (n_dofs x n_q)
+//   shape_gradients_collocation [q][q'] = L_q'(x_q')     (n_q x n_q),
+//     L_q the Lagrange polynomial on the quadrature points (the derivative
+//     of point values, taken at the points)
+//   inverse_shape_values [dof][q]: (S^T W S)^-1 S^T W with S[q][dof] =
+//     l_dof(x_q), W = diag(weights) -- the L2 projection from point values to
+//     nodal values; S^-1 when n_q = n_dofs
+//   quadrature_data_on_face[side][q] = L_q(side)  (the trace of point values)
+struct UnivariateShapeData
+{
+  ElementType element_type = ElementType::tensor_symmetric;
+  int fe_degree = 0;
+  int n_q_points_1d = 0;
+  std::vector<double> nodes;
+  Quadrature1D quadrature;
+  std::vector<double> shape_values;
+  std::vector<double> shape_gradients_collocation;
+  std::vector<double> inverse_shape_values;
+  std::array<std::vector<double>, 2> quadrature_data_on_face;
+};
+
+inline UnivariateShapeData make_shape_data(int degree,
+                                           const Quadrature1D &quad)
+{
+  UnivariateShapeData s;
+  const int n = degree + 1, nq = quad.n;
+  s.fe_degree = degree;
+  s.n_q_points_1d = nq;
+  s.quadrature = quad;
+  s.nodes = degree == 0 ? std::vector<double>{0.5}
+                        : QGaussLobatto(degree + 1).points;
+  bool coll = quad.lobatto && nq == n;
+  s.element_type = coll ? ElementType::tensor_symmetric_collocation
+                        : ElementType::tensor_symmetric;
+  s.shape_values.resize((size_t)n * nq);
   for (int j = 0; j < n; ++j)
-    for (int q = 0; q < n; ++q)
+    for (int q = 0; q < nq; ++q)
+      s.shape_values[(size_t)j * nq + q] =
+        (double)detail::lagrange(s.nodes, j, quad.points[q]);
+  s.shape_gradients_collocation.resize((size_t)nq * nq);
+  for (int j = 0; j < nq; ++j)
+    for (int q = 0; q < nq; ++q)
+      s.shape_gradients_collocation[(size_t)j * nq + q] =
+        (double)detail::lagrange_derivative(quad.points, j, quad.points[q]);
+  // (S^T W S)^-1 S^T W, in long double
+  std::vector<long double> M((size_t)n * n, 0.0L);
+  for (int i = 0; i < n; ++i)
+    for (int j = 0; j < n; ++j)
+      for (int q = 0; q < nq; ++q)
+        M[(size_t)i * n + j] += (long double)quad.weights[q] *
+                                detail::lagrange(s.nodes, i, quad.points[q]) *
+                                detail::lagrange(s.nodes, j, quad.points[q]);
+  const std::vector<long double> Mi = detail::invert(M, n);
+  s.inverse_shape_values.assign((size_t)n * nq, 0.0);
+  for (int i = 0; i < n; ++i)
+    for (int q = 0; q < nq; ++q)
       {
-        s.values[(size_t)j * n + q] =
-          (double)detail::lagrange(s.nodes, j, s.points[q]);
-        S[(size_t)q * n + j] = s.values[(size_t)j * n + q];
-        s.gradients[(size_t)j * n + q] =
-          (double)detail::lagrange_derivative(s.points, j, s.points[q]);
+        long double acc = 0.0L;
+        for (int j = 0; j < n; ++j)
+          acc += Mi[(size_t)i * n + j] *
+                 detail::lagrange(s.nodes, j, quad.points[q]);
+        s.inverse_shape_values[(size_t)i * nq + q] =
+          (double)(acc * quad.weights[q]);
       }
-  s.inverse = detail::invert(S, n);
+  for (int side = 0; side < 2; ++side)
+    {
+      s.quadrature_data_on_face[side].resize(nq);
+      for (int q = 0; q < nq; ++q)
+        s.quadrature_data_on_face[side][q] =
+          (double)detail::lagrange(quad.points, q, (long double)side);
+    }
   return s;
 }
 
+// the shape data of one (dof handler, quadrature) pair (deal.II's ShapeInfo:
+// .data.front() is the univariate data)
+struct ShapeInfo
+{
+  std::vector<UnivariateShapeData> data;
+};
+
 // ---------------------------------------------------------------------------
-// sum factorisation along one direction (EvaluatorTensorProduct's role)
+// sum factorisation (deal.II's EvaluatorTensorProduct and friends)
 // ---------------------------------------------------------------------------
-class TensorProduct
+// 1D contractions along one direction of a dim-dimensional tensor.
+// Matrices are n_rows x n_columns, row-major (rows: the basis side).
+// contract_over_rows = true: in has n_rows entries along `direction`, out
+// n_columns (out[c] = sum_r M[r][c] in[r]); false: the transpose.
+// Directions below `direction` have extent n_columns, those above n_rows
+// (a forward sweep 0, 1, ... or a backward sweep ..., 1, 0).  in and out may
+// alias.
+template <int dim, int n_rows, int n_columns, typename Number = double>
+class EvaluatorTensorProduct
 {
 public:
-  // dim directions of extent n; the three n x n matrices (rows: the nodal
-  // or basis side, columns: the point side), any of them may be null
-  TensorProduct(int dim, int n, const std::vector<double> *values,
-                const std::vector<double> *gradients,
-                const std::vector<double> *hessians)
-    : dim_(dim), n_(n), values_(values), gradients_(gradients),
-      hessians_(hessians)
+  EvaluatorTensorProduct(const std::vector<double> &shape_values,
+                         const std::vector<double> &shape_gradients,
+                         const std::vector<double> &shape_hessians)
+    : values_(shape_values.empty() ? nullptr : &shape_values),
+      gradients_(shape_gradients.empty() ? nullptr : &shape_gradients),
+      hessians_(shape_hessians.empty() ? nullptr : &shape_hessians)
   {
   }
 
-  template <int dir, bool contract_over_rows, bool add>
-  void values(const double *in, double *out) const
+  template <int direction, bool contract_over_rows, bool add>
+  void values(const Number *in, Number *out) const
   {
-    apply(values_, dir, contract_over_rows, add, in, out);
+    apply<direction, contract_over_rows, add>(values_, in, out);
   }
-  template <int dir, bool contract_over_rows, bool add>
-  void gradients(const double *in, double *out) const
+  template <int direction, bool contract_over_rows, bool add>
+  void gradients(const Number *in, Number *out) const
   {
-    apply(gradients_, dir, contract_over_rows, add, in, out);
+    apply<direction, contract_over_rows, add>(gradients_, in, out);
   }
-  template <int dir, bool contract_over_rows, bool add>
-  void hessians(const double *in, double *out) const
+  template <int direction, bool contract_over_rows, bool add>
+  void hessians(const Number *in, Number *out) const
   {
-    apply(hessians_, dir, contract_over_rows, add, in, out);
+    apply<direction, contract_over_rows, add>(hessians_, in, out);
   }
 
-  // the same with the direction at run time
-  void apply(const std::vector<double> *M, int dir, bool contract_over_rows,
-             bool add, const double *in, double *out) const
+  template <int direction, bool contract_over_rows, bool add>
+  static void apply(const std::vector<double> *M, const Number *in,
+                    Number *out)
   {
-    if (!M || dir < 0 || dir >= dim_)
-      throw std::invalid_argument("TensorProduct: no matrix / bad direction");
-    const int n = n_;
-    long long stride = 1, total = 1;
-    for (int d = 0; d < dim_; ++d)
-      {
-        if (d < dir)
-          stride *= n;
-        total *= n;
-      }
-    const long long outer = total / (stride * n);
-    std::vector<double> line(n), res(n);
-    for (long long o = 0; o < outer; ++o)
-      for (long long s = 0; s < stride; ++s)
+    static_assert(direction >= 0 && direction < dim, "direction");
+    if (!M || M->size() != (size_t)n_rows * n_columns)
+      throw std::invalid_argument(
+        "EvaluatorTensorProduct: no matrix of this size");
+    constexpr int mm = contract_over_rows ? n_rows : n_columns;
+    constexpr int nn = contract_over_rows ? n_columns : n_rows;
+    constexpr int stride = detail::cpow(n_columns, direction);
+    constexpr int blocks2 = detail::cpow(n_rows, dim - direction - 1);
+    constexpr int in_size = stride * mm * blocks2;
+    constexpr int out_size = stride * nn * blocks2;
+    std::vector<Number> tmp(in, in + in_size);
+    std::vector<Number> res((size_t)out_size);
+    for (int b2 = 0; b2 < blocks2; ++b2)
+      for (int s = 0; s < stride; ++s)
         {
-          const long long base = o * stride * n + s;
-          for (int i = 0; i < n; ++i)
-            line[i] = in[base + i * stride];
-          for (int i = 0; i < n; ++i)
+          const Number *i0 = tmp.data() + (size_t)b2 * stride * mm + s;
+          Number *o0 = res.data() + (size_t)b2 * stride * nn + s;
+          for (int c = 0; c < nn; ++c)
             {
-              double a = 0.0;
-              for (int j = 0; j < n; ++j)
-                a += contract_over_rows ? (*M)[(size_t)j * n + i] * line[j]
-                                        : (*M)[(size_t)i * n + j] * line[j];
-              res[i] = a;
+              Number acc = Number(0);
+              for (int r = 0; r < mm; ++r)
+                acc += (contract_over_rows ? (*M)[(size_t)r * n_columns + c]
+                                           : (*M)[(size_t)c * n_columns + r]) *
+                       i0[(size_t)r * stride];
+              o0[(size_t)c * stride] = acc;
             }
-          for (int i = 0; i < n; ++i)
-            out[base + i * stride] = add ? out[base + i * stride] + res[i]
-                                         : res[i];
         }
+    for (int i = 0; i < out_size; ++i)
+      out[i] = add ? out[i] + res[i] : res[i];
   }
 
 private:
-  int dim_, n_;
   const std::vector<double> *values_, *gradients_, *hessians_;
 };
 
-// ---------------------------------------------------------------------------
-// IDs and the matrix-free loops
-// ---------------------------------------------------------------------------
-struct TensorID
+// basis change between the n_rows^dim nodal values and the n_columns^dim
+// point values along every direction (deal.II's FEEvaluationImplBasisChange):
+// do_forward applies the matrix with contract_over_rows in directions 0, 1,
+// ...; do_backward its transpose in directions ..., 1, 0 (add_into_result:
+// out +=).  in and out may alias when out has room for both sizes.
+template <int dim, int n_rows, int n_columns, typename Number = double>
+struct FEEvaluationImplBasisChange
 {
-  // a cell (XV), a face in x (X: x = face index, v = cell) or in v (V: x =
-  // cell, v = face index); inner faces come first, boundary faces after
-  enum class SpaceType
+  static void do_forward(unsigned int n_components,
+                         const std::vector<double> &transformation_matrix,
+                         const Number *values_in, Number *values_out)
   {
-    XV,
-    X,
-    V
-  };
-  unsigned int x = 0, v = 0;
-  unsigned int id = 0; // running number within the loop
-  SpaceType type = SpaceType::XV;
+    constexpr int sin = detail::cpow(n_rows, dim);
+    constexpr int sout = detail::cpow(n_columns, dim);
+    for (unsigned int c = 0; c < n_components; ++c)
+      {
+        std::vector<Number> t(values_in + c * sin, values_in + (c + 1) * sin);
+        t.resize(std::max(sin, sout));
+        sweep<0, true>(&transformation_matrix, t.data());
+        std::copy(t.begin(), t.begin() + sout, values_out + c * sout);
+      }
+  }
+  static void do_backward(unsigned int n_components,
+                          const std::vector<double> &transformation_matrix,
+                          bool add_into_result, const Number *values_in,
+                          Number *values_out)
+  {
+    constexpr int sin = detail::cpow(n_columns, dim);
+    constexpr int sout = detail::cpow(n_rows, dim);
+    for (unsigned int c = 0; c < n_components; ++c)
+      {
+        std::vector<Number> t(values_in + c * sin, values_in + (c + 1) * sin);
+        t.resize(std::max(sin, sout));
+        sweep<dim - 1, false>(&transformation_matrix, t.data());
+        for (int i = 0; i < sout; ++i)
+          values_out[c * sout + i] =
+            add_into_result ? values_out[c * sout + i] + t[i] : t[i];
+      }
+  }
+
+private:
+  template <int d, bool forward>
+  static void sweep(const std::vector<double> *M, Number *t)
+  {
+    if constexpr (d >= 0 && d < dim)
+      {
+        EvaluatorTensorProduct<dim, n_rows, n_columns, Number>::
+          template apply<d, forward, false>(M, t, t);
+        sweep<forward ? d + 1 : d - 1, forward>(M, t);
+      }
+  }
 };
 
-template <int dim_x, int dim_v>
-class MatrixFree
+// evaluation flags (only values are interpolated to faces here)
+enum class EvaluationFlags
+{
+  nothing = 0,
+  values = 1
+};
+
+// Interpolation between a cell's point values (n^D) and a face's point
+// values (n^(D-1)) of phase-space face face_no = 2 d + side, with the 1D
+// trace vector quadrature_data_on_face[side] (hyper.deal's
+// matrix_free/evaluation_kernels.h:229-330).  do_evaluate: in = cell, out =
+// face; otherwise in = face, out = cell (transpose, the lift); add_into_output:
+// out +=.
+template <int dim_x, int dim_v, int degree_of_points, typename Number>
+struct FEFaceNormalEvaluationImpl
+{
+  static constexpr int dim = dim_x + dim_v;
+  static constexpr int n = degree_of_points + 1;
+
+  template <bool do_evaluate, bool add_into_output>
+  static void interpolate_quadrature(unsigned int n_components,
+                                     EvaluationFlags flags,
+                                     const ShapeInfo &shape_info,
+                                     const Number *input, Number *output,
+                                     unsigned int face_no)
+  {
+    if (flags != EvaluationFlags::values)
+      return;
+    const UnivariateShapeData &s = shape_info.data.front();
+    if (s.n_q_points_1d != n)
+      throw std::invalid_argument("interpolate_quadrature: point count");
+    const int d = (int)face_no / 2;
+    const std::vector<double> &tr = s.quadrature_data_on_face[face_no % 2];
+    const int stride = detail::ipow(n, d);
+    const int outer = detail::ipow(n, dim - d - 1);
+    const int cell = detail::cpow(n, dim), face = cell / n;
+    for (unsigned int c = 0; c < n_components; ++c)
+      {
+        const Number *in = input + c * (do_evaluate ? cell : face);
+        Number *out = output + c * (do_evaluate ? face : cell);
+        for (int o = 0; o < outer; ++o)
+          for (int i = 0; i < stride; ++i)
+            {
+              const int fi = o * stride + i;
+              const int ci = o * stride * n + i;
+              if (do_evaluate)
+                {
+                  Number acc = Number(0);
+                  for (int k = 0; k < n; ++k)
+                    acc += tr[k] * in[ci + k * stride];
+                  out[fi] = add_into_output ? out[fi] + acc : acc;
+                }
+              else
+                for (int k = 0; k < n; ++k)
+                  {
+                    const Number v = tr[k] * in[fi];
+                    out[ci + k * stride] =
+                      add_into_output ? out[ci + k * stride] + v : v;
+                  }
+            }
+      }
+  }
+};
+
+// ---------------------------------------------------------------------------
+// the low-dimensional spaces (deal.II's MatrixFree<dim> over a subdivided
+// box: cells, faces, geometry, shape data per quadrature)
+// ---------------------------------------------------------------------------
+template <int dim>
+class CartesianMatrixFree
 {
 public:
-  static constexpr int dim = dim_x + dim_v;
-  using ID = TensorID;
-  using SpaceType = TensorID::SpaceType;
-
-  // a face between the cells minus and plus of one space (x or v) across
-  // direction dir of that space; boundary faces have plus = -1 and side 0
-  // (at the lower end of the box) or 1 (upper end)
+  // a face between the cells `interior` and `exterior` across direction
+  // `dir`: the face is the upper end of the interior cell; a boundary face
+  // has exterior == -1 and `side` 0 (lower end of the box, the interior
+  // cell's lower end) or 1
   struct Face
   {
     int dir;
-    long long minus, plus;
+    long long interior, exterior;
     int side;
+    unsigned int boundary_id;
   };
 
-  // n_cells, lo, hi per phase-space direction (x first); periodic or
-  // Dirichlet per space (the reference's two flags)
+  // n_cells, lo, hi per direction; periodic or not (non-periodic faces get
+  // boundary id 0, as the reference's hyperrectangle cases set them);
+  // degree of FE_DGQ; the 1D quadratures, by quadrature number
   void reinit(const std::array<int, dim> &n_cells,
               const std::array<double, dim> &lo,
-              const std::array<double, dim> &hi, int degree,
-              bool periodic_x = true, bool periodic_v = true)
+              const std::array<double, dim> &hi, bool periodic, int degree,
+              const std::vector<Quadrature1D> &quadratures)
   {
+    if (degree < 0 || quadratures.empty())
+      throw std::invalid_argument("CartesianMatrixFree: degree / quadrature");
     for (int d = 0; d < dim; ++d)
       if (n_cells[d] < 1 || !(hi[d] > lo[d]))
-        throw std::invalid_argument("MatrixFree::reinit: invalid mesh");
+        throw std::invalid_argument("CartesianMatrixFree: invalid box");
     n_ = n_cells;
     lo_ = lo;
     for (int d = 0; d < dim; ++d)
       h_[d] = (hi[d] - lo[d]) / n_cells[d];
-    periodic_x_ = periodic_x;
-    periodic_v_ = periodic_v;
+    periodic_ = periodic;
     degree_ = degree;
-    shape_ = make_shape(degree);
-    n_x_cells_ = 1;
-    n_v_cells_ = 1;
-    for (int d = 0; d < dim_x; ++d)
-      n_x_cells_ *= n_[d];
-    for (int d = dim_x; d < dim; ++d)
-      n_v_cells_ *= n_[d];
-    build_faces(0, dim_x, periodic_x, faces_x_, n_inner_x_);
-    build_faces(dim_x, dim_v, periodic_v, faces_v_, n_inner_v_);
+    shape_.clear();
+    for (const auto &q : quadratures)
+      shape_.push_back(ShapeInfo{{make_shape_data(degree, q)}});
+    n_cells_ = 1;
+    for (int d = 0; d < dim; ++d)
+      n_cells_ *= n_[d];
+    build_faces();
   }
 
-  int get_degree() const { return degree_; }
-  int n_points() const { return shape_.n; }
-  long long n_cells_x() const { return n_x_cells_; }
-  long long n_cells_v() const { return n_v_cells_; }
-  long long n_cells() const { return n_x_cells_ * n_v_cells_; }
-  long long dofs_per_cell() const
+  unsigned int n_cell_batches() const { return (unsigned int)n_cells_; }
+  unsigned int n_active_entries_per_cell_batch(unsigned int) const
   {
-    long long r = 1;
-    for (int d = 0; d < dim; ++d)
-      r *= shape_.n;
-    return r;
+    return 1;
   }
-  std::size_t n_dofs() const { return (size_t)(n_cells() * dofs_per_cell()); }
-  const Shape1D &get_shape_info() const { return shape_; }
+  unsigned int n_active_entries_per_face_batch(unsigned int) const
+  {
+    return 1;
+  }
+  unsigned int n_inner_face_batches() const { return (unsigned int)n_inner_; }
+  unsigned int n_boundary_face_batches() const
+  {
+    return (unsigned int)(faces_.size() - n_inner_);
+  }
+  const Face &get_face(unsigned int f) const { return faces_.at(f); }
+  const ShapeInfo &get_shape_info(unsigned int dof_no = 0,
+                                  unsigned int quad_no = 0) const
+  {
+    if (dof_no != 0 || quad_no >= shape_.size())
+      throw std::invalid_argument("get_shape_info: no such dof / quadrature");
+    return shape_[quad_no];
+  }
+  unsigned int n_quadratures() const { return (unsigned int)shape_.size(); }
+  int get_degree() const { return degree_; }
+  bool periodic() const { return periodic_; }
+  int n_cells_dir(int d) const { return n_[d]; }
   double h(int d) const { return h_[d]; }
   double lo(int d) const { return lo_[d]; }
-  int n_cells_dir(int d) const { return n_[d]; }
-  const std::vector<Face> &faces_x() const { return faces_x_; }
-  const std::vector<Face> &faces_v() const { return faces_v_; }
-  long long n_inner_faces_x() const { return n_inner_x_; }
-  long long n_inner_faces_v() const { return n_inner_v_; }
-
-  void initialize_dof_vector(std::vector<double> &v) const
+  long long n_cells() const { return n_cells_; }
+  // index per direction of cell c
+  std::array<int, dim> cell_index(long long c) const
   {
-    v.assign(n_dofs(), 0.0);
-  }
-
-  // the index per direction of cell `c` within its space (x: directions
-  // 0..dim_x-1 of the x cell, v: of the v cell)
-  void cell_index(long long x_cell, long long v_cell,
-                  std::array<int, dim> &idx) const
-  {
-    for (int d = 0; d < dim_x; ++d)
+    std::array<int, dim> idx{};
+    for (int d = 0; d < dim; ++d)
       {
-        idx[d] = (int)(x_cell % n_[d]);
-        x_cell /= n_[d];
+        idx[d] = (int)(c % n_[d]);
+        c /= n_[d];
       }
-    for (int d = dim_x; d < dim; ++d)
+    return idx;
+  }
+  long long cell_number(const std::array<int, dim> &idx) const
+  {
+    long long c = 0;
+    for (int d = dim - 1; d >= 0; --d)
+      c = c * n_[d] + idx[d];
+    return c;
+  }
+  // neighbour of cell c across face face_no = 2 d + side, or -1 at a
+  // boundary
+  long long neighbor(long long c, unsigned int face_no) const
+  {
+    const int d = (int)face_no / 2;
+    auto idx = cell_index(c);
+    const int i = idx[d] + (face_no % 2 ? 1 : -1);
+    if (i < 0 || i >= n_[d])
       {
-        idx[d] = (int)(v_cell % n_[d]);
-        v_cell /= n_[d];
+        if (!periodic_)
+          return -1;
+        idx[d] = (i + n_[d]) % n_[d];
       }
-  }
-
-  template <typename Out, typename In>
-  using Op =
-    std::function<void(const MatrixFree &, Out &, const In &, const ID)>;
-
-  // every cell, x cells fastest (matrix_free.templates.h:1417-1440)
-  template <typename Out, typename In>
-  void cell_loop(const Op<Out, In> &cell_operation, Out &dst,
-                 const In &src) const
-  {
-    unsigned int i0 = 0;
-    for (long long v = 0; v < n_v_cells_; ++v)
-      for (long long x = 0; x < n_x_cells_; ++x)
-        cell_operation(*this, dst, src,
-                       ID{(unsigned)x, (unsigned)v, i0++, SpaceType::XV});
-  }
-
-  // element-centric: the cell operation visits its own faces
-  // (matrix_free.templates.h:1500-1600)
-  template <typename Out, typename In>
-  void loop_cell_centric(const Op<Out, In> &cell_operation, Out &dst,
-                         const In &src) const
-  {
-    cell_loop(cell_operation, dst, src);
-  }
-
-  // face-centric: cells, then inner faces in x and in v, then boundary
-  // faces (matrix_free.templates.h:1640-1746)
-  template <typename Out, typename In>
-  void loop(const Op<Out, In> &cell_operation,
-            const Op<Out, In> &face_operation,
-            const Op<Out, In> &boundary_operation, Out &dst,
-            const In &src) const
-  {
-    cell_loop(cell_operation, dst, src);
-    unsigned int i1 = 0, i2 = 0;
-    for (long long v = 0; v < n_v_cells_; ++v)
-      for (long long f = 0; f < n_inner_x_; ++f)
-        face_operation(*this, dst, src,
-                       ID{(unsigned)f, (unsigned)v, i1++, SpaceType::X});
-    for (long long f = 0; f < n_inner_v_; ++f)
-      for (long long x = 0; x < n_x_cells_; ++x)
-        face_operation(*this, dst, src,
-                       ID{(unsigned)x, (unsigned)f, i1++, SpaceType::V});
-    for (long long v = 0; v < n_v_cells_; ++v)
-      for (long long f = n_inner_x_; f < (long long)faces_x_.size(); ++f)
-        boundary_operation(*this, dst, src,
-                           ID{(unsigned)f, (unsigned)v, i2++, SpaceType::X});
-    for (long long f = n_inner_v_; f < (long long)faces_v_.size(); ++f)
-      for (long long x = 0; x < n_x_cells_; ++x)
-        boundary_operation(*this, dst, src,
-                           ID{(unsigned)x, (unsigned)f, i2++, SpaceType::V});
-  }
-
-  // the member-pointer forms (matrix_free.h:138-228)
-  template <typename CLASS, typename Out, typename In>
-  void cell_loop(void (CLASS::*cell_operation)(const MatrixFree &, Out &,
-                                               const In &, const ID),
-                 CLASS *owner, Out &dst, const In &src) const
-  {
-    cell_loop<Out, In>(bind(cell_operation, owner), dst, src);
-  }
-  template <typename CLASS, typename Out, typename In>
-  void loop_cell_centric(void (CLASS::*cell_operation)(const MatrixFree &,
-                                                       Out &, const In &,
-                                                       const ID),
-                         CLASS *owner, Out &dst, const In &src) const
-  {
-    loop_cell_centric<Out, In>(bind(cell_operation, owner), dst, src);
-  }
-  template <typename CLASS, typename Out, typename In>
-  void loop(void (CLASS::*cell_operation)(const MatrixFree &, Out &,
-                                          const In &, const ID),
-            void (CLASS::*face_operation)(const MatrixFree &, Out &,
-                                          const In &, const ID),
-            void (CLASS::*boundary_operation)(const MatrixFree &, Out &,
-                                              const In &, const ID),
-            CLASS *owner, Out &dst, const In &src) const
-  {
-    loop<Out, In>(bind(cell_operation, owner), bind(face_operation, owner),
-                  bind(boundary_operation, owner), dst, src);
+    else
+      idx[d] = i;
+    return cell_number(idx);
   }
 
 private:
-  template <typename CLASS, typename Out, typename In>
-  static Op<Out, In> bind(void (CLASS::*f)(const MatrixFree &, Out &,
-                                           const In &, const ID),
-                          CLASS *owner)
+  // inner faces (each once, interior = the lower cell) then boundary faces
+  void build_faces()
   {
-    return [f, owner](const MatrixFree &mf, Out &dst, const In &src,
-                      const ID id) { (owner->*f)(mf, dst, src, id); };
-  }
-
-  // faces of the space whose directions are [d0, d0 + nd): inner faces
-  // (each once, minus = the lower cell) then boundary faces
-  void build_faces(int d0, int nd, bool periodic, std::vector<Face> &faces,
-                   long long &n_inner)
-  {
-    faces.clear();
-    long long n_space = 1;
-    for (int d = d0; d < d0 + nd; ++d)
-      n_space *= n_[d];
+    faces_.clear();
     std::vector<Face> boundary;
-    for (int e = 0; e < nd; ++e)
+    for (int e = 0; e < dim; ++e)
       {
         long long stride = 1;
-        for (int d = d0; d < d0 + e; ++d)
+        for (int d = 0; d < e; ++d)
           stride *= n_[d];
-        const int ne = n_[d0 + e];
-        for (long long c = 0; c < n_space; ++c)
+        const int ne = n_[e];
+        for (long long c = 0; c < n_cells_; ++c)
           {
             const int i = (int)((c / stride) % ne);
             if (i + 1 < ne)
-              faces.push_back({e, c, c + stride, -1});
-            else if (periodic)
-              faces.push_back({e, c, c - (long long)(ne - 1) * stride, -1});
+              faces_.push_back({e, c, c + stride, 1, internal_face_boundary_id});
+            else if (periodic_)
+              faces_.push_back({e, c, c - (long long)(ne - 1) * stride, 1,
+                                internal_face_boundary_id});
             else
-              boundary.push_back({e, c, -1, 1});
-            if (!periodic && i == 0)
-              boundary.push_back({e, c, -1, 0});
+              boundary.push_back({e, c, -1, 1, 0});
+            if (!periodic_ && i == 0)
+              boundary.push_back({e, c, -1, 0, 0});
           }
       }
-    n_inner = (long long)faces.size();
-    faces.insert(faces.end(), boundary.begin(), boundary.end());
+    n_inner_ = (long long)faces_.size();
+    faces_.insert(faces_.end(), boundary.begin(), boundary.end());
   }
 
   std::array<int, dim> n_{};
   std::array<double, dim> lo_{}, h_{};
-  bool periodic_x_ = true, periodic_v_ = true;
+  bool periodic_ = true;
   int degree_ = 0;
-  Shape1D shape_;
-  long long n_x_cells_ = 0, n_v_cells_ = 0;
-  std::vector<Face> faces_x_, faces_v_;
-  long long n_inner_x_ = 0, n_inner_v_ = 0;
+  std::vector<ShapeInfo> shape_;
+  long long n_cells_ = 0;
+  std::vector<Face> faces_;
+  long long n_inner_ = 0;
+};
+
+// the reference's quadrature list of an advection run (examples/advection/
+// include/application.h:217-223)
+inline std::vector<Quadrature1D> advection_quadratures(int degree,
+                                                       int n_points,
+                                                       bool collocation)
+{
+  return {collocation ? QGaussLobatto(n_points) : QGauss(n_points),
+          QGauss(degree + 1), QGaussLobatto(degree + 1)};
+}
+
+// ---------------------------------------------------------------------------
+// the phase-space matrix-free object and its loops
+// ---------------------------------------------------------------------------
+template <int dim_x, int dim_v, typename Number = double,
+          typename VectorizedArrayType = Number>
+class MatrixFree
+{
+  static_assert(std::is_same<Number, VectorizedArrayType>::value,
+                "one lane: VectorizedArrayType is Number");
+
+public:
+  static constexpr int dim = dim_x + dim_v;
+  using ID = TensorID;
+  using SpaceType = TensorID::SpaceType;
+  using VectorizedArrayTypeX = VectorizedArrayType;
+  using VectorizedArrayTypeV = VectorizedArrayType;
+  using MatrixFreeX = CartesianMatrixFree<dim_x>;
+  using MatrixFreeV = CartesianMatrixFree<dim_v>;
+
+  // type of access on faces; only none and values are supported, as in the
+  // reference (matrix_free.templates.h:1506-1508, 1662-1663, 1744-1745)
+  enum class DataAccessOnFaces
+  {
+    none,
+    values,
+    gradients,
+    unspecified
+  };
+
+  struct AdditionalData
+  {
+    AdditionalData()
+      : do_ghost_faces(true), do_buffering(false), use_ecl(true),
+        overlapping_level(0)
+    {
+    }
+    bool do_ghost_faces;
+    bool do_buffering;
+    bool use_ecl;
+    unsigned int overlapping_level;
+  };
+
+  // the x-space and v-space matrix-free objects are held by reference
+  // (matrix_free.h:337-344)
+  MatrixFree(const MatrixFreeX &matrix_free_x, const MatrixFreeV &matrix_free_v)
+    : mf_x_(matrix_free_x), mf_v_(matrix_free_v)
+  {
+  }
+
+  void reinit(const AdditionalData &ad = AdditionalData())
+  {
+    if (mf_x_.get_degree() != mf_v_.get_degree())
+      throw std::invalid_argument("MatrixFree::reinit: degrees differ");
+    if (!ad.do_ghost_faces)
+      throw std::invalid_argument(
+        "MatrixFree::reinit: only ghost faces are supported");
+    ad_ = ad;
+    reinit_done_ = true;
+  }
+
+  const MatrixFreeX &get_matrix_free_x() const { return mf_x_; }
+  const MatrixFreeV &get_matrix_free_v() const { return mf_v_; }
+  bool is_ecl_supported() const { return ad_.use_ecl; }
+  bool are_ghost_faces_supported() const { return ad_.do_ghost_faces; }
+  int get_degree() const { return mf_x_.get_degree(); }
+  long long n_cells_x() const { return mf_x_.n_cells(); }
+  long long n_cells_v() const { return mf_v_.n_cells(); }
+  long long n_cells() const { return n_cells_x() * n_cells_v(); }
+  long long dofs_per_cell() const
+  {
+    return detail::ipow(get_degree() + 1, dim);
+  }
+  std::size_t n_dofs() const { return (size_t)(n_cells() * dofs_per_cell()); }
+
+  // (matrix_free.templates.h:1367-1413): caller-owned vector of n_dofs zeros
+  template <typename V>
+  void initialize_dof_vector(V &vec, unsigned int dof_handler_index = 0,
+                             bool do_ghosts = true,
+                             bool zero_out_values = true) const
+  {
+    (void)dof_handler_index;
+    (void)do_ghosts;
+    (void)zero_out_values;
+    vec.assign(n_dofs(), Number(0));
+  }
+
+  // boundary id of a face of the FCL loop (internal_face_boundary_id for an
+  // inner face)
+  unsigned int get_boundary_id(const ID macro_face) const
+  {
+    if (macro_face.type == SpaceType::X)
+      return mf_x_.get_face(macro_face.x).boundary_id;
+    if (macro_face.type == SpaceType::V)
+      return mf_v_.get_face(macro_face.v).boundary_id;
+    throw std::invalid_argument("get_boundary_id: not a face");
+  }
+
+  // boundary id of face face_no (2 d + side; v faces from 2 dim_x) of a cell:
+  // internal_face_boundary_id unless it lies on a non-periodic boundary,
+  // whose faces all carry id 0 (the reference's hyperrectangle cases,
+  // examples/advection/cases/hyperrectangle.h:190-197)
+  unsigned int get_faces_by_cells_boundary_id(const ID &macro_cell,
+                                              unsigned int face_number) const
+  {
+    const bool boundary =
+      face_number < 2 * dim_x
+        ? mf_x_.neighbor(macro_cell.x, face_number) < 0
+        : mf_v_.neighbor(macro_cell.v, face_number - 2 * dim_x) < 0;
+    return boundary ? 0u : internal_face_boundary_id;
+  }
+
+  template <typename OutVector, typename InVector>
+  using CellOp = std::function<void(const MatrixFree &, OutVector &,
+                                    const InVector &, const ID)>;
+
+  // every cell, x cells fastest (matrix_free.templates.h:1417-1440)
+  template <typename OutVector, typename InVector>
+  void cell_loop(const CellOp<OutVector, InVector> &cell_operation,
+                 OutVector &dst, const InVector &src) const
+  {
+    check_reinit();
+    unsigned int i0 = 0;
+    for (unsigned int j = 0; j < mf_v_.n_cell_batches(); ++j)
+      for (unsigned int i = 0; i < mf_x_.n_cell_batches(); ++i)
+        cell_operation(*this, dst, src, ID(i, j, i0++));
+  }
+
+  template <typename CLASS, typename OutVector, typename InVector>
+  void cell_loop(void (CLASS::*cell_operation)(const MatrixFree &,
+                                               OutVector &, const InVector &,
+                                               const ID),
+                 CLASS *owning_class, OutVector &dst, const InVector &src) const
+  {
+    cell_loop<OutVector, InVector>(bind(cell_operation, owning_class), dst,
+                                   src);
+  }
+
+  // element-centric: the cell operation visits its own faces, reading the
+  // neighbours' values from src (matrix_free.templates.h:1494-1581); the
+  // ghost update the reference times as "update_ghost_values_0..2" is not
+  // needed (src holds every cell), "loop" times the cell sweep
+  template <typename OutVector, typename InVector>
+  void loop_cell_centric(const CellOp<OutVector, InVector> &cell_operation,
+                         OutVector &dst, const InVector &src,
+                         const DataAccessOnFaces src_vector_face_access =
+                           DataAccessOnFaces::unspecified,
+                         Timers *timers = nullptr) const
+  {
+    if (src_vector_face_access != DataAccessOnFaces::values &&
+        src_vector_face_access != DataAccessOnFaces::none)
+      throw std::invalid_argument(
+        "loop_cell_centric: src face access must be values or none");
+    if (!ad_.use_ecl)
+      throw std::invalid_argument(
+        "loop_cell_centric: MatrixFree set up for face-centric loops");
+    ScopedTimerWrapper timer(timers, "loop");
+    cell_loop(cell_operation, dst, src);
+  }
+
+  template <typename CLASS, typename OutVector, typename InVector>
+  void loop_cell_centric(void (CLASS::*cell_operation)(const MatrixFree &,
+                                                       OutVector &,
+                                                       const InVector &,
+                                                       const ID),
+                         CLASS *owning_class, OutVector &dst,
+                         const InVector &src,
+                         const DataAccessOnFaces src_vector_face_access =
+                           DataAccessOnFaces::unspecified,
+                         Timers *timers = nullptr) const
+  {
+    loop_cell_centric<OutVector, InVector>(bind(cell_operation, owning_class),
+                                           dst, src, src_vector_face_access,
+                                           timers);
+  }
+
+  // face-centric: cells, inner faces in x then in v, boundary faces in x
+  // then in v (matrix_free.templates.h:1634-1746); both face accesses must
+  // be values (the reference throws otherwise)
+  template <typename OutVector, typename InVector>
+  void loop(const CellOp<OutVector, InVector> &cell_operation,
+            const CellOp<OutVector, InVector> &face_operation,
+            const CellOp<OutVector, InVector> &boundary_operation,
+            OutVector &dst, const InVector &src,
+            const DataAccessOnFaces dst_vector_face_access =
+              DataAccessOnFaces::unspecified,
+            const DataAccessOnFaces src_vector_face_access =
+              DataAccessOnFaces::unspecified,
+            Timers *timers = nullptr) const
+  {
+    check_reinit();
+    if (src_vector_face_access != DataAccessOnFaces::values ||
+        dst_vector_face_access != DataAccessOnFaces::values)
+      throw std::invalid_argument(
+        "loop: dst and src face access must be values");
+    const unsigned int ncx = mf_x_.n_cell_batches();
+    const unsigned int ncv = mf_v_.n_cell_batches();
+    const unsigned int nfx = mf_x_.n_inner_face_batches();
+    const unsigned int nbx = mf_x_.n_boundary_face_batches();
+    const unsigned int nfv = mf_v_.n_inner_face_batches();
+    const unsigned int nbv = mf_v_.n_boundary_face_batches();
+    unsigned int i0 = 0, i1 = 0;
+    {
+      ScopedTimerWrapper timer(timers, "cell_loop");
+      for (unsigned int j = 0; j < ncv; ++j)
+        for (unsigned int i = 0; i < ncx; ++i)
+          cell_operation(*this, dst, src, ID(i, j, i0++));
+    }
+    {
+      ScopedTimerWrapper timer(timers, "face_loop_x");
+      for (unsigned int j = 0; j < ncv; ++j)
+        for (unsigned int i = 0; i < nfx; ++i)
+          face_operation(*this, dst, src, ID(i, j, i1++, SpaceType::X));
+    }
+    {
+      ScopedTimerWrapper timer(timers, "face_loop_v");
+      for (unsigned int j = 0; j < nfv; ++j)
+        for (unsigned int i = 0; i < ncx; ++i)
+          face_operation(*this, dst, src, ID(i, j, i1++, SpaceType::V));
+    }
+    {
+      ScopedTimerWrapper timer(timers, "boundary_loop_x");
+      for (unsigned int j = 0; j < ncv; ++j)
+        for (unsigned int i = nfx; i < nfx + nbx; ++i)
+          boundary_operation(*this, dst, src, ID(i, j, i1++, SpaceType::X));
+    }
+    {
+      ScopedTimerWrapper timer(timers, "boundary_loop_v");
+      for (unsigned int j = nfv; j < nfv + nbv; ++j)
+        for (unsigned int i = 0; i < ncx; ++i)
+          boundary_operation(*this, dst, src, ID(i, j, i1++, SpaceType::V));
+    }
+  }
+
+  template <typename CLASS, typename OutVector, typename InVector>
+  void loop(void (CLASS::*cell_operation)(const MatrixFree &, OutVector &,
+                                          const InVector &, const ID),
+            void (CLASS::*face_operation)(const MatrixFree &, OutVector &,
+                                          const InVector &, const ID),
+            void (CLASS::*boundary_operation)(const MatrixFree &, OutVector &,
+                                              const InVector &, const ID),
+            CLASS *owning_class, OutVector &dst, const InVector &src,
+            const DataAccessOnFaces dst_vector_face_access =
+              DataAccessOnFaces::unspecified,
+            const DataAccessOnFaces src_vector_face_access =
+              DataAccessOnFaces::unspecified,
+            Timers *timers = nullptr) const
+  {
+    loop<OutVector, InVector>(bind(cell_operation, owning_class),
+                              bind(face_operation, owning_class),
+                              bind(boundary_operation, owning_class), dst, src,
+                              dst_vector_face_access, src_vector_face_access,
+                              timers);
+  }
+
+  // phase-space cell number of (x cell, v cell)
+  long long cell_number(long long x, long long v) const
+  {
+    return x + n_cells_x() * v;
+  }
+
+private:
+  void check_reinit() const
+  {
+    if (!reinit_done_)
+      throw std::logic_error("MatrixFree: reinit() was not called");
+  }
+
+  template <typename CLASS, typename OutVector, typename InVector>
+  static CellOp<OutVector, InVector>
+  bind(void (CLASS::*f)(const MatrixFree &, OutVector &, const InVector &,
+                        const ID),
+       CLASS *owner)
+  {
+    return [f, owner](const MatrixFree &mf, OutVector &dst,
+                      const InVector &src,
+                      const ID id) { (owner->*f)(mf, dst, src, id); };
+  }
+
+  const MatrixFreeX &mf_x_;
+  const MatrixFreeV &mf_v_;
+  AdditionalData ad_{};
+  bool reinit_done_ = false;
 };
 
 // ---------------------------------------------------------------------------
-// cell evaluators
+// cell evaluators (fe_evaluation_base.h, fe_evaluation_cell.h)
 // ---------------------------------------------------------------------------
-template <int dim_x, int dim_v>
-class FEEvaluation
+template <int dim_x, int dim_v, int degree, int n_points, typename Number,
+          typename VectorizedArrayType>
+class FEEvaluationBase
 {
 public:
   static constexpr int dim = dim_x + dim_v;
-  using MF = MatrixFree<dim_x, dim_v>;
+  using MF = MatrixFree<dim_x, dim_v, Number, VectorizedArrayType>;
+  static constexpr int static_dofs =
+    detail::cpow(degree + 1 > n_points ? degree + 1 : n_points, dim);
 
-  explicit FEEvaluation(const MF &mf)
-    : mf_(mf), n_(mf.n_points()), nq_((int)mf.dofs_per_cell()),
-      nqx_(ipow(mf.n_points(), dim_x)), data_((size_t)nq_ * (dim + 1))
+  explicit FEEvaluationBase(const MF &matrix_free)
+    : matrix_free(matrix_free), matrix_free_x(matrix_free.get_matrix_free_x()),
+      matrix_free_v(matrix_free.get_matrix_free_v()),
+      data((size_t)static_dofs, VectorizedArrayType(0))
   {
+    if (matrix_free.get_degree() != degree)
+      throw std::invalid_argument("evaluator degree != the space's degree");
   }
+  virtual ~FEEvaluationBase() = default;
 
-  void reinit(const TensorID cell)
+  virtual void reinit(typename MF::ID cell_index)
   {
-    x_ = cell.x;
-    v_ = cell.v;
-    cell_ = x_ + mf_.n_cells_x() * v_;
-    mf_.cell_index(x_, v_, idx_);
+    macro_cell_x = cell_index.x;
+    macro_cell_v = cell_index.v;
+    macro = cell_index.macro;
   }
-
-  void read_dof_values(const std::vector<double> &src, double *data) const
-  {
-    std::copy(src.begin() + cell_ * nq_, src.begin() + (cell_ + 1) * nq_,
-              data);
-  }
-  void read_dof_values(const std::vector<double> &src)
-  {
-    read_dof_values(src, data_.data());
-  }
-  void set_dof_values(std::vector<double> &dst) const
-  {
-    std::copy(data_.begin(), data_.begin() + nq_, dst.begin() + cell_ * nq_);
-  }
-  void distribute_local_to_global(std::vector<double> &dst) const
-  {
-    for (int i = 0; i < nq_; ++i)
-      dst[cell_ * nq_ + i] += data_[i];
-  }
-
-  // the cell's values (first n^D entries) and room for dim more (gradients)
-  double *get_data_ptr() { return data_.data(); }
-  const std::vector<double> *get_shape_values() const
-  {
-    return &mf_.get_shape_info().values;
-  }
+  VectorizedArrayType *get_data_ptr() { return data.data(); }
+  const std::vector<double> *get_shape_values() const { return shape_values; }
   const std::vector<double> *get_shape_gradients() const
   {
-    return &mf_.get_shape_info().gradients;
-  }
-  const std::vector<double> *get_inverse_shape() const
-  {
-    return &mf_.get_shape_info().inverse;
-  }
-  unsigned int n_q_points() const { return (unsigned)nq_; }
-  unsigned int n_q_points_x() const { return (unsigned)nqx_; }
-
-  double JxW(unsigned int qx, unsigned int qv) const
-  {
-    return jxw_space(qx, 0, dim_x) * jxw_space(qv, dim_x, dim_v);
+    return shape_gradients;
   }
 
-  // the x (v) gradient at point q from reference derivatives at
-  // grad_in[q + e n_q^D] (fe_evaluation_cell.h:219-271)
-  std::array<double, dim_x> get_gradient_x(const double *grad_in,
-                                           unsigned int q, unsigned int qx,
-                                           unsigned int qv) const
+protected:
+  // the shape data of quadrature quad_no of the space, checked against the
+  // evaluator's n_points
+  static const UnivariateShapeData &shape_of(const ShapeInfo &si)
+  {
+    const UnivariateShapeData &s = si.data.front();
+    if (s.n_q_points_1d != n_points)
+      throw std::invalid_argument(
+        "evaluator n_points != the quadrature's point count");
+    return s;
+  }
+
+  const MF &matrix_free;
+  const typename MF::MatrixFreeX &matrix_free_x;
+  const typename MF::MatrixFreeV &matrix_free_v;
+  std::vector<VectorizedArrayType> data;
+  unsigned int macro_cell_x = 0, macro_cell_v = 0, macro = 0;
+  const std::vector<double> *shape_values = nullptr;
+  const std::vector<double> *shape_gradients = nullptr;
+};
+
+template <int dim_x, int dim_v, int degree, int n_points, typename Number,
+          typename VectorizedArrayType>
+class FEEvaluation : public FEEvaluationBase<dim_x, dim_v, degree, n_points,
+                                             Number, VectorizedArrayType>
+{
+public:
+  using PARENT = FEEvaluationBase<dim_x, dim_v, degree, n_points, Number,
+                                  VectorizedArrayType>;
+  using MF = typename PARENT::MF;
+  static constexpr int dim = dim_x + dim_v;
+  static constexpr unsigned int static_dofs_per_cell_x =
+    detail::cpow(degree + 1, dim_x);
+  static constexpr unsigned int static_dofs_per_cell_v =
+    detail::cpow(degree + 1, dim_v);
+  static constexpr unsigned int static_dofs_per_cell =
+    static_dofs_per_cell_x * static_dofs_per_cell_v;
+  static constexpr unsigned int n_q_points_x = detail::cpow(n_points, dim_x);
+  static constexpr unsigned int n_q_points_v = detail::cpow(n_points, dim_v);
+  static constexpr unsigned int n_q_points = n_q_points_x * n_q_points_v;
+
+  FEEvaluation(const MF &matrix_free, const unsigned int dof_no_x,
+               const unsigned int dof_no_v, const unsigned int quad_no_x,
+               const unsigned int quad_no_v)
+    : PARENT(matrix_free),
+      sx_(&PARENT::shape_of(
+        matrix_free.get_matrix_free_x().get_shape_info(dof_no_x, quad_no_x))),
+      sv_(&PARENT::shape_of(
+        matrix_free.get_matrix_free_v().get_shape_info(dof_no_v, quad_no_v)))
+  {
+    this->shape_values = &sx_->shape_values;
+    this->shape_gradients = &sx_->shape_gradients_collocation;
+  }
+
+  void reinit(typename MF::ID cell_index) override
+  {
+    PARENT::reinit(cell_index);
+    idx_x_ = this->matrix_free_x.cell_index(cell_index.x);
+    idx_v_ = this->matrix_free_v.cell_index(cell_index.v);
+    cell_ = this->matrix_free.cell_number(cell_index.x, cell_index.v);
+  }
+
+  // the cell's (degree+1)^D nodal values into data (fe_evaluation_cell.h:
+  // 115-142)
+  template <typename V>
+  void read_dof_values(const V &src, VectorizedArrayType *data) const
+  {
+    const long long n = static_dofs_per_cell;
+    for (long long i = 0; i < n; ++i)
+      data[i] = src[cell_ * n + i];
+  }
+  template <typename V>
+  void read_dof_values(const V &src)
+  {
+    read_dof_values(src, this->data.data());
+  }
+  // fe_evaluation_cell.h:149-163
+  template <typename V>
+  void set_dof_values(V &dst) const
+  {
+    const long long n = static_dofs_per_cell;
+    for (long long i = 0; i < n; ++i)
+      dst[cell_ * n + i] = this->data[i];
+  }
+  template <typename V>
+  void distribute_local_to_global(V &dst) const
+  {
+    const long long n = static_dofs_per_cell;
+    for (long long i = 0; i < n; ++i)
+      dst[cell_ * n + i] += this->data[i];
+  }
+
+  unsigned int n_vectorization_lanes_filled() const { return 1; }
+
+  // fe_evaluation_cell.h:207-212
+  VectorizedArrayType JxW(unsigned int qx, unsigned int qv) const
+  {
+    return jxw_x(qx) * jxw_v(qv);
+  }
+  VectorizedArrayType submit_inplace(const VectorizedArrayType value,
+                                     unsigned int q) const
+  {
+    return value * JxW(q % n_q_points_x, q / n_q_points_x);
+  }
+
+  // the x gradient at point q from the reference-cell derivatives at
+  // grad_in[q + e n_q_points] (fe_evaluation_cell.h:219-243)
+  std::array<VectorizedArrayType, dim_x>
+  get_gradient_x(const VectorizedArrayType *grad_in, const unsigned int q,
+                 const unsigned int qx, const unsigned int qv) const
   {
     (void)qx;
     (void)qv;
-    std::array<double, dim_x> g{};
+    std::array<VectorizedArrayType, dim_x> r{};
     for (int d = 0; d < dim_x; ++d)
-      g[d] = grad_in[q + (size_t)d * nq_] / mf_.h(d);
-    return g;
+      r[d] = grad_in[q + d * n_q_points] / this->matrix_free_x.h(d);
+    return r;
   }
-  std::array<double, dim_v> get_gradient_v(const double *grad_in,
-                                           unsigned int q, unsigned int qx,
-                                           unsigned int qv) const
+  // fe_evaluation_cell.h:246-271
+  std::array<VectorizedArrayType, dim_v>
+  get_gradient_v(const VectorizedArrayType *grad_in, const unsigned int q,
+                 const unsigned int qx, const unsigned int qv) const
   {
     (void)qx;
     (void)qv;
-    std::array<double, dim_v> g{};
+    std::array<VectorizedArrayType, dim_v> r{};
     for (int d = 0; d < dim_v; ++d)
-      g[d] = grad_in[q + (size_t)d * nq_] / mf_.h(dim_x + d);
-    return g;
+      r[d] = grad_in[q + d * n_q_points] / this->matrix_free_v.h(d);
+    return r;
   }
 
   // data[q] (+)= value JxW (fe_evaluation_cell.h:278-294)
   template <bool do_add>
-  void submit_value(double *data, double value, unsigned int q,
-                    unsigned int qx, unsigned int qv) const
+  void submit_value(VectorizedArrayType *data_ptr_out,
+                    const VectorizedArrayType values_in, const unsigned int q,
+                    const unsigned int qx, const unsigned int qv) const
   {
-    const double v = value * JxW(qx, qv);
-    data[q] = do_add ? data[q] + v : v;
+    const VectorizedArrayType v = values_in * JxW(qx, qv);
+    data_ptr_out[q] = do_add ? data_ptr_out[q] + v : v;
   }
-  // data[q + e n_q^D] = (J^-T grad)_e JxW (fe_evaluation_cell.h:301-348)
-  void submit_gradient_x(double *data, const std::array<double, dim_x> &grad,
-                         unsigned int q, unsigned int qx,
-                         unsigned int qv) const
+  // data[q + e n_q_points] = (J^-T grad)_e JxW (fe_evaluation_cell.h:
+  // 301-348)
+  void submit_gradient_x(VectorizedArrayType *data_ptr_out,
+                         const VectorizedArrayType *grad_in,
+                         const unsigned int q, const unsigned int qx,
+                         const unsigned int qv) const
   {
-    const double w = JxW(qx, qv);
+    const VectorizedArrayType w = JxW(qx, qv);
     for (int d = 0; d < dim_x; ++d)
-      data[q + (size_t)d * nq_] = grad[d] / mf_.h(d) * w;
+      data_ptr_out[q + d * n_q_points] =
+        grad_in[d] / this->matrix_free_x.h(d) * w;
   }
-  void submit_gradient_v(double *data, const std::array<double, dim_v> &grad,
-                         unsigned int q, unsigned int qx,
-                         unsigned int qv) const
+  void submit_gradient_v(VectorizedArrayType *data_ptr_out,
+                         const VectorizedArrayType *grad_in,
+                         const unsigned int q, const unsigned int qx,
+                         const unsigned int qv) const
   {
-    const double w = JxW(qx, qv);
+    const VectorizedArrayType w = JxW(qx, qv);
     for (int d = 0; d < dim_v; ++d)
-      data[q + (size_t)d * nq_] = grad[d] / mf_.h(dim_x + d) * w;
+      data_ptr_out[q + d * n_q_points] =
+        grad_in[d] / this->matrix_free_v.h(d) * w;
   }
 
-  // (x, v) of point q (fe_evaluation_cell.h:357-397)
-  std::array<double, dim> get_quadrature_point(unsigned int q) const
+  // (x, v) of point q, or of (qx, qv) (fe_evaluation_cell.h:357-397)
+  std::array<VectorizedArrayType, dim> get_quadrature_point(unsigned int q) const
   {
-    std::array<double, dim> p{};
-    const Shape1D &s = mf_.get_shape_info();
-    for (int d = 0; d < dim; ++d)
+    return get_quadrature_point(q % n_q_points_x, q / n_q_points_x);
+  }
+  std::array<VectorizedArrayType, dim>
+  get_quadrature_point(unsigned int qx, unsigned int qv) const
+  {
+    std::array<VectorizedArrayType, dim> p{};
+    for (int d = 0; d < dim_x; ++d)
       {
-        p[d] = mf_.lo(d) + (idx_[d] + s.points[q % n_]) * mf_.h(d);
-        q /= n_;
+        p[d] = this->matrix_free_x.lo(d) +
+               (idx_x_[d] + sx_->quadrature.points[qx % n_points]) *
+                 this->matrix_free_x.h(d);
+        qx /= n_points;
+      }
+    for (int d = 0; d < dim_v; ++d)
+      {
+        p[dim_x + d] = this->matrix_free_v.lo(d) +
+                       (idx_v_[d] + sv_->quadrature.points[qv % n_points]) *
+                         this->matrix_free_v.h(d);
+        qv /= n_points;
       }
     return p;
   }
-
-protected:
-  static int ipow(int b, int e)
+  std::array<VectorizedArrayType, dim_v>
+  get_quadrature_point_v(unsigned int qv) const
   {
-    int r = 1;
-    while (e-- > 0)
-      r *= b;
+    const auto p = get_quadrature_point(0, qv);
+    std::array<VectorizedArrayType, dim_v> r{};
+    for (int d = 0; d < dim_v; ++d)
+      r[d] = p[dim_x + d];
     return r;
   }
-  // product of w h over the nd directions from d0, point index q of them
-  double jxw_space(unsigned int q, int d0, int nd) const
+
+protected:
+  VectorizedArrayType jxw_x(unsigned int qx) const
   {
-    const Shape1D &s = mf_.get_shape_info();
-    double w = 1.0;
-    for (int d = d0; d < d0 + nd; ++d)
+    VectorizedArrayType w(1);
+    for (int d = 0; d < dim_x; ++d)
       {
-        w *= s.weights[q % n_] * mf_.h(d);
-        q /= n_;
+        w *= sx_->quadrature.weights[qx % n_points] * this->matrix_free_x.h(d);
+        qx /= n_points;
+      }
+    return w;
+  }
+  VectorizedArrayType jxw_v(unsigned int qv) const
+  {
+    VectorizedArrayType w(1);
+    for (int d = 0; d < dim_v; ++d)
+      {
+        w *= sv_->quadrature.weights[qv % n_points] * this->matrix_free_v.h(d);
+        qv /= n_points;
       }
     return w;
   }
 
-  const MF &mf_;
-  int n_, nq_, nqx_;
-  std::vector<double> data_;
-  long long x_ = 0, v_ = 0, cell_ = 0;
-  std::array<int, dim> idx_{};
+  const UnivariateShapeData *sx_, *sv_;
+  std::array<int, dim_x> idx_x_{};
+  std::array<int, dim_v> idx_v_{};
+  long long cell_ = 0;
 };
 
 // the inverse mass matrix's point step (fe_evaluation_cell_inverse.h:
-// 123-134): data[q] /= JxW
-template <int dim_x, int dim_v>
-class FEEvaluationInverse : public FEEvaluation<dim_x, dim_v>
+// 36-134): data[q] /= JxW, with the quadrature's inverse shape values
+template <int dim_x, int dim_v, int degree, int n_points, typename Number,
+          typename VectorizedArrayType>
+class FEEvaluationInverse
+  : public FEEvaluation<dim_x, dim_v, degree, n_points, Number,
+                        VectorizedArrayType>
 {
 public:
-  using FEEvaluation<dim_x, dim_v>::FEEvaluation;
-  void submit_inv(double *data, unsigned int q, unsigned int qx,
-                  unsigned int qv) const
+  using PARENT =
+    FEEvaluation<dim_x, dim_v, degree, n_points, Number, VectorizedArrayType>;
+
+  FEEvaluationInverse(const typename PARENT::MF &matrix_free,
+                      const unsigned int dof_no_x, const unsigned int dof_no_v,
+                      const unsigned int quad_no_x,
+                      const unsigned int quad_no_v)
+    : PARENT(matrix_free, dof_no_x, dof_no_v, quad_no_x, quad_no_v)
   {
-    data[q] /= this->JxW(qx, qv);
+  }
+
+  const std::vector<double> *get_inverse_shape() const
+  {
+    return &this->sx_->inverse_shape_values;
+  }
+
+  void submit_inv(VectorizedArrayType *data_ptr, const unsigned int q,
+                  const unsigned int q1, const unsigned int q2) const
+  {
+    data_ptr[q] /= this->JxW(q1, q2);
   }
 };
 
 // ---------------------------------------------------------------------------
 // face evaluators (fe_evaluation_face.h:41-568)
 // ---------------------------------------------------------------------------
-template <int dim_x, int dim_v>
-class FEFaceEvaluation
+template <int dim_x, int dim_v, int degree, int n_points, typename Number,
+          typename VectorizedArrayType>
+class FEFaceEvaluation : public FEEvaluationBase<dim_x, dim_v, degree,
+                                                 n_points, Number,
+                                                 VectorizedArrayType>
 {
 public:
-  static constexpr int dim = dim_x + dim_v;
-  using MF = MatrixFree<dim_x, dim_v>;
+  using PARENT = FEEvaluationBase<dim_x, dim_v, degree, n_points, Number,
+                                  VectorizedArrayType>;
+  using MF = typename PARENT::MF;
   using SpaceType = TensorID::SpaceType;
+  static constexpr int dim = dim_x + dim_v;
+  static constexpr unsigned int static_dofs_per_face =
+    detail::cpow(degree + 1, dim - 1);
 
-  FEFaceEvaluation(const MF &mf, bool is_minus)
-    : mf_(mf), is_minus_(is_minus), n_(mf.n_points()),
-      nf_((int)(mf.dofs_per_cell() / mf.n_points())), data_((size_t)nf_)
+  FEFaceEvaluation(const MF &matrix_free, const bool is_minus_face,
+                   const unsigned int dof_no_x, const unsigned int dof_no_v,
+                   const unsigned int quad_no_x, const unsigned int quad_no_v)
+    : PARENT(matrix_free), is_minus_face(is_minus_face),
+      sx_(&PARENT::shape_of(
+        matrix_free.get_matrix_free_x().get_shape_info(dof_no_x, quad_no_x))),
+      sv_(&PARENT::shape_of(
+        matrix_free.get_matrix_free_v().get_shape_info(dof_no_v, quad_no_v)))
   {
+    this->shape_values = &sx_->shape_values;
+    this->shape_gradients = &sx_->shape_gradients_collocation;
   }
 
-  // face-centric: the face of the loop (inner or boundary); this evaluator
-  // takes the minus or the plus cell
-  void reinit(const TensorID face)
+  // face-centric: the face of the loop; the minus evaluator takes the
+  // interior cell, the plus one the exterior cell (fe_evaluation_face.h:
+  // 133-183)
+  void reinit(typename MF::ID face_index) override
   {
-    type_ = face.type;
+    PARENT::reinit(face_index);
+    type_ = face_index.type;
+    is_ecl_ = false;
     if (type_ == SpaceType::XV)
       throw std::invalid_argument("FEFaceEvaluation::reinit: not a face");
     const bool in_x = type_ == SpaceType::X;
-    const auto &f = in_x ? mf_.faces_x()[face.x] : mf_.faces_v()[face.v];
-    boundary_ = f.plus < 0;
-    if (boundary_ && !is_minus_)
-      throw std::invalid_argument("FEFaceEvaluation: a boundary face has "
-                                  "no plus side");
-    dir_ = (in_x ? 0 : dim_x) + f.dir;
-    const long long c = is_minus_ ? f.minus : f.plus;
-    // the face is the upper end of the minus cell and the lower end of the
-    // plus cell; a boundary face is the end given by its side
-    upper_ = boundary_ ? f.side == 1 : is_minus_;
-    set_cell(in_x ? c : face.x, in_x ? face.v : c);
-  }
-
-  // element-centric: face face_no = 2 d + side (side 1: the upper end) of
-  // the cell; the minus evaluator takes the cell, the plus one the
-  // neighbour across the face (a boundary face has none: check
-  // at_boundary() of the minus evaluator first, the plus one throws)
-  void reinit(const TensorID cell, unsigned int face_no)
-  {
-    dir_ = (int)face_no / 2;
-    type_ = dir_ < dim_x ? SpaceType::X : SpaceType::V;
-    const bool up = face_no % 2 == 1;
-    std::array<int, dim> idx{};
-    mf_.cell_index(cell.x, cell.v, idx);
-    const int nd = mf_.n_cells_dir(dir_);
-    boundary_ = false;
-    long long x = cell.x, v = cell.v;
-    if (!is_minus_)
+    long long cx = face_index.x, cv = face_index.v;
+    int dir, side;
+    if (in_x)
       {
-        int i = idx[dir_] + (up ? 1 : -1);
-        if (i < 0 || i >= nd)
-          {
-            if (!periodic_dir(dir_))
-              throw std::invalid_argument("FEFaceEvaluation: no neighbour "
-                                          "across a boundary face");
-            i = (i + nd) % nd;
-          }
-        idx[dir_] = i;
-        x = 0;
-        v = 0;
-        for (int d = dim_x - 1; d >= 0; --d)
-          x = x * mf_.n_cells_dir(d) + idx[d];
-        for (int d = dim - 1; d >= dim_x; --d)
-          v = v * mf_.n_cells_dir(d) + idx[d];
+        const auto &f = this->matrix_free_x.get_face(face_index.x);
+        boundary_ = f.exterior < 0;
+        if (boundary_ && !is_minus_face)
+          throw std::invalid_argument(
+            "FEFaceEvaluation: a boundary face has no exterior cell");
+        cx = is_minus_face ? f.interior : f.exterior;
+        dir = f.dir;
+        side = boundary_ ? f.side : (is_minus_face ? 1 : 0);
       }
     else
-      boundary_ = !periodic_dir(dir_) &&
-                  ((up && idx[dir_] == nd - 1) || (!up && idx[dir_] == 0));
-    upper_ = is_minus_ ? up : !up;
-    set_cell(x, v);
+      {
+        const auto &f = this->matrix_free_v.get_face(face_index.v);
+        boundary_ = f.exterior < 0;
+        if (boundary_ && !is_minus_face)
+          throw std::invalid_argument(
+            "FEFaceEvaluation: a boundary face has no exterior cell");
+        cv = is_minus_face ? f.interior : f.exterior;
+        dir = dim_x + f.dir;
+        side = boundary_ ? f.side : (is_minus_face ? 1 : 0);
+      }
+    face_no_ = 2 * dir + side;
+    set_cell(cx, cv);
+  }
+
+  // element-centric: face face_no (2 d + side) of the cell; the minus
+  // evaluator takes the cell, the plus one the neighbour across the face
+  // (fe_evaluation_face.h:190-222)
+  void reinit(typename MF::ID cell_index, unsigned int face)
+  {
+    PARENT::reinit(cell_index);
+    if (face >= 2 * (unsigned)dim)
+      throw std::invalid_argument("FEFaceEvaluation::reinit: face number");
+    type_ = face >= 2 * (unsigned)dim_x ? SpaceType::V : SpaceType::X;
+    is_ecl_ = true;
+    long long cx = cell_index.x, cv = cell_index.v;
+    if (type_ == SpaceType::X)
+      {
+        const long long nb = this->matrix_free_x.neighbor(cx, face);
+        boundary_ = nb < 0;
+        if (!is_minus_face)
+          {
+            if (boundary_)
+              throw std::invalid_argument(
+                "FEFaceEvaluation: no neighbour across a boundary face");
+            cx = nb;
+          }
+      }
+    else
+      {
+        const long long nb =
+          this->matrix_free_v.neighbor(cv, face - 2 * dim_x);
+        boundary_ = nb < 0;
+        if (!is_minus_face)
+          {
+            if (boundary_)
+              throw std::invalid_argument(
+                "FEFaceEvaluation: no neighbour across a boundary face");
+            cv = nb;
+          }
+      }
+    // the plus evaluator sees the face from the neighbour: the other end
+    face_no_ = is_minus_face ? face : (face ^ 1u);
+    set_cell(cx, cv);
   }
 
   bool at_boundary() const { return boundary_; }
   SpaceType get_type() const { return type_; }
-  double *get_data_ptr() { return data_.data(); }
-  const std::vector<double> *get_shape_values() const
+  unsigned int get_face_no() const { return face_no_; }
+  unsigned int n_vectorization_lanes_filled() const { return 1; }
+
+  // the (degree+1)^(D-1) nodal values of the cell on the face
+  // (fe_evaluation_face.h:241-288)
+  template <typename V>
+  void read_dof_values(const V &src, VectorizedArrayType *data) const
   {
-    return &mf_.get_shape_info().values;
+    const long long n = detail::cpow(degree + 1, dim);
+    for (unsigned int i = 0; i < static_dofs_per_face; ++i)
+      data[i] = src[cell_ * n + cell_node(i)];
+  }
+  template <typename V>
+  void read_dof_values(const V &src)
+  {
+    read_dof_values(src, this->data.data());
+  }
+  // from / into a cell buffer of (degree+1)^D nodal values (the collocation
+  // path of the element-centric loop; fe_evaluation_face.h:295-318)
+  void read_dof_values_from_buffer(const VectorizedArrayType *src)
+  {
+    for (unsigned int i = 0; i < static_dofs_per_face; ++i)
+      this->data[i] = src[cell_node(i)];
+  }
+  void distribute_to_buffer(VectorizedArrayType *dst) const
+  {
+    for (unsigned int i = 0; i < static_dofs_per_face; ++i)
+      dst[cell_node(i)] += this->data[i];
+  }
+  // fe_evaluation_face.h:324-364 (face-centric only)
+  template <typename V>
+  void distribute_local_to_global(V &dst) const
+  {
+    if (is_ecl_)
+      throw std::logic_error(
+        "distribute_local_to_global: not in an element-centric loop");
+    const long long n = detail::cpow(degree + 1, dim);
+    for (unsigned int i = 0; i < static_dofs_per_face; ++i)
+      dst[cell_ * n + cell_node(i)] += this->data[i];
   }
 
-  // the n^(D-1) nodal values of the cell on the face
-  void read_dof_values(const std::vector<double> &src)
-  {
-    read_dof_values_from_buffer(src.data() + cell_ * mf_.dofs_per_cell());
-  }
-  void read_dof_values_from_buffer(const double *cell_data)
-  {
-    for (int i = 0; i < nf_; ++i)
-      data_[i] = cell_data[cell_node(i)];
-  }
-  void distribute_to_buffer(double *cell_data) const
-  {
-    for (int i = 0; i < nf_; ++i)
-      cell_data[cell_node(i)] += data_[i];
-  }
-  void distribute_local_to_global(std::vector<double> &dst) const
-  {
-    distribute_to_buffer(dst.data() + cell_ * mf_.dofs_per_cell());
-  }
-
-  // data[q] = -value JxW_face (fe_evaluation_face.h:381-399); q = qx + (x
-  // face points) qv
+  // data[q] = -value JxW_face (fe_evaluation_face.h:381-399)
   template <SpaceType stype>
-  void submit_value(double *data, double value, unsigned int q,
-                    unsigned int qx, unsigned int qv) const
+  void submit_value(VectorizedArrayType *data_ptr,
+                    const VectorizedArrayType &value, const unsigned int q,
+                    const unsigned int qx, const unsigned int qv) const
   {
     if (stype != type_)
       throw std::invalid_argument("FEFaceEvaluation: face types differ");
-    data[q] = -value * jxw(qx, qv);
+    (void)q;
+    data_ptr[q] = -value * jxw(qx, qv);
   }
 
   // the outward normal of this evaluator's cell (fe_evaluation_face.h:
   // 434-463): zero in the space the face does not cut
-  std::array<double, dim_x> get_normal_vector_x(unsigned int qx) const
+  std::array<VectorizedArrayType, dim_x>
+  get_normal_vector_x(const unsigned int qx) const
   {
     (void)qx;
-    std::array<double, dim_x> n{};
-    if (dir_ < dim_x)
-      n[dir_] = upper_ ? 1.0 : -1.0;
-    return n;
+    std::array<VectorizedArrayType, dim_x> nv{};
+    const int d = (int)face_no_ / 2;
+    if (d < dim_x)
+      nv[d] = face_no_ % 2 ? 1.0 : -1.0;
+    return nv;
   }
-  std::array<double, dim_v> get_normal_vector_v(unsigned int qv) const
+  std::array<VectorizedArrayType, dim_v>
+  get_normal_vector_v(const unsigned int qv) const
   {
     (void)qv;
-    std::array<double, dim_v> n{};
-    if (dir_ >= dim_x)
-      n[dir_ - dim_x] = upper_ ? 1.0 : -1.0;
-    return n;
+    std::array<VectorizedArrayType, dim_v> nv{};
+    const int d = (int)face_no_ / 2;
+    if (d >= dim_x)
+      nv[d - dim_x] = face_no_ % 2 ? 1.0 : -1.0;
+    return nv;
   }
 
-  // (x, v) of face point q
+  // (x, v) of face point (qx, qv)
   template <SpaceType stype>
-  std::array<double, dim> get_quadrature_point(unsigned int qx,
-                                               unsigned int qv) const
+  std::array<VectorizedArrayType, dim>
+  get_quadrature_point(unsigned int qx, unsigned int qv) const
   {
     if (stype != type_)
       throw std::invalid_argument("FEFaceEvaluation: face types differ");
-    const unsigned int nqx = face_points_x();
-    unsigned int q = qx + nqx * qv;
-    const Shape1D &s = mf_.get_shape_info();
-    std::array<double, dim> p{};
+    const int fd = (int)face_no_ / 2;
+    std::array<VectorizedArrayType, dim> p{};
     for (int d = 0; d < dim; ++d)
       {
+        const bool in_x = d < dim_x;
+        unsigned int &qq = in_x ? qx : qv;
+        const auto &s = in_x ? *sx_ : *sv_;
         double t;
-        if (d == dir_)
-          t = upper_ ? 1.0 : 0.0;
+        if (d == fd)
+          t = face_no_ % 2 ? 1.0 : 0.0;
         else
           {
-            t = s.points[q % n_];
-            q /= n_;
+            t = s.quadrature.points[qq % n_points];
+            qq /= n_points;
           }
-        p[d] = mf_.lo(d) + (idx_[d] + t) * mf_.h(d);
+        const int dd = in_x ? d : d - dim_x;
+        if (in_x)
+          p[d] = this->matrix_free_x.lo(dd) +
+                 (idx_x_[dd] + t) * this->matrix_free_x.h(dd);
+        else
+          p[d] = this->matrix_free_v.lo(dd) +
+                 (idx_v_[dd] + t) * this->matrix_free_v.h(dd);
       }
     return p;
   }
@@ -869,73 +1666,68 @@ public:
   // face points in x (the q = qx + face_points_x() qv split)
   unsigned int face_points_x() const
   {
-    int r = 1;
-    for (int d = 0; d < dim_x - (type_ == SpaceType::X ? 1 : 0); ++d)
-      r *= n_;
-    return (unsigned)r;
+    return detail::ipow(n_points,
+                        dim_x - (type_ == SpaceType::X ? 1 : 0));
   }
 
 private:
-  bool periodic_dir(int d) const
-  {
-    // a direction is periodic when its space has no boundary faces
-    const bool in_x = d < dim_x;
-    const auto &faces = in_x ? mf_.faces_x() : mf_.faces_v();
-    const long long inner = in_x ? mf_.n_inner_faces_x() : mf_.n_inner_faces_v();
-    return (long long)faces.size() == inner;
-  }
-
   void set_cell(long long x, long long v)
   {
-    cell_ = x + mf_.n_cells_x() * v;
-    mf_.cell_index(x, v, idx_);
+    cell_ = this->matrix_free.cell_number(x, v);
+    idx_x_ = this->matrix_free_x.cell_index(x);
+    idx_v_ = this->matrix_free_v.cell_index(v);
   }
 
   // cell-local node of face entry i (the face's directions in increasing
-  // order, the normal direction at its end)
-  int cell_node(int i) const
+  // order, the normal direction at the face's end)
+  int cell_node(unsigned int i) const
   {
+    const int fd = (int)face_no_ / 2, k = degree + 1;
     int node = 0, stride = 1;
     for (int d = 0; d < dim; ++d)
       {
-        int k;
-        if (d == dir_)
-          k = upper_ ? n_ - 1 : 0;
+        int c;
+        if (d == fd)
+          c = face_no_ % 2 ? k - 1 : 0;
         else
           {
-            k = i % n_;
-            i /= n_;
+            c = (int)(i % k);
+            i /= k;
           }
-        node += k * stride;
-        stride *= n_;
+        node += c * stride;
+        stride *= k;
       }
     return node;
   }
 
   // JxW of face point (qx, qv): w h over the face's directions
-  double jxw(unsigned int qx, unsigned int qv) const
+  VectorizedArrayType jxw(unsigned int qx, unsigned int qv) const
   {
-    unsigned int q = qx + face_points_x() * qv;
-    const Shape1D &s = mf_.get_shape_info();
-    double w = 1.0;
+    const int fd = (int)face_no_ / 2;
+    VectorizedArrayType w(1);
     for (int d = 0; d < dim; ++d)
-      if (d != dir_)
-        {
-          w *= s.weights[q % n_] * mf_.h(d);
-          q /= n_;
-        }
+      {
+        if (d == fd)
+          continue;
+        const bool in_x = d < dim_x;
+        unsigned int &qq = in_x ? qx : qv;
+        const auto &s = in_x ? *sx_ : *sv_;
+        w *= s.quadrature.weights[qq % n_points] *
+             (in_x ? this->matrix_free_x.h(d)
+                   : this->matrix_free_v.h(d - dim_x));
+        qq /= n_points;
+      }
     return w;
   }
 
-  const MF &mf_;
-  bool is_minus_;
-  int n_, nf_;
-  std::vector<double> data_;
+  const bool is_minus_face;
+  const UnivariateShapeData *sx_, *sv_;
   SpaceType type_ = SpaceType::X;
-  int dir_ = 0;
-  bool upper_ = true, boundary_ = false;
+  bool is_ecl_ = false, boundary_ = false;
+  unsigned int face_no_ = 0;
   long long cell_ = 0;
-  std::array<int, dim> idx_{};
+  std::array<int, dim_x> idx_x_{};
+  std::array<int, dim_v> idx_v_{};
 };
 } // namespace host
 } // namespace hyperdeal_amd

@@ -1,10 +1,12 @@
 // The advection operator written against the host loop / evaluator
-// interface against the GPU operator of the same problem through the C ABI
-// (include/hyperdeal_amd.hpp): the two sides of the drop-in boundary give
-// the same apply.
+// interface (advection_op.hpp: hyper.deal's AdvectionOperation shape,
+// face-centric and element-centric) against the GPU operator of the same
+// problem through the C++ wrapper of the C ABI (include/hyperdeal_amd.hpp):
+// the two sides of the drop-in boundary give the same apply.
 //
-// usage: host_vs_gpu <dim_x> <dim_v> <degree> <periodic 0|1> <skew>
-// prints: "gpu <rel err>"
+// usage: host_vs_gpu <dim_x> <dim_v> <degree> <n_points> <collocation 0|1>
+//                    <periodic 0|1> <skew>
+// prints: "fcl <rel err> ecl <rel err>" (each: GPU against the host loop)
 #include "advection_op.hpp"
 #include "hyperdeal_amd.hpp"
 
@@ -15,14 +17,23 @@
 #include <random>
 #include <vector>
 
-using advection_host::Advection;
+namespace hh = hyperdeal_amd::host;
 
-template <int dim_x, int dim_v>
-int run(int degree, bool periodic, double skew)
+template <int dim_x, int dim_v, int degree, int n_points>
+int run(bool collocation, bool periodic, double skew)
 {
   constexpr int dim = dim_x + dim_v;
+  using VectorType = std::vector<double>;
+  using MF = hh::MatrixFree<dim_x, dim_v, double, hh::VectorizedArray<double>>;
+  using Op = advection_host::AdvectionOperation<dim_x, dim_v, degree,
+                                                n_points, double, VectorType,
+                                                double>;
   std::array<int, dim> nc{};
   std::array<double, dim> lo{}, hi{}, a{};
+  std::array<int, dim_x> ncx{};
+  std::array<int, dim_v> ncv{};
+  std::array<double, dim_x> lox{}, hix{};
+  std::array<double, dim_v> lov{}, hiv{};
   const int cells[6] = {4, 3, 3, 2, 3, 2};
   const double vel[6] = {-0.6, 0.35, 0.8, -0.45, 0.25, -0.9};
   for (int d = 0; d < dim; ++d)
@@ -31,32 +42,57 @@ int run(int degree, bool periodic, double skew)
       lo[d] = 0.1 * d;
       hi[d] = lo[d] + 0.5 + 0.25 * d;
       a[d] = vel[d];
+      if (d < dim_x)
+        {
+          ncx[d] = nc[d];
+          lox[d] = lo[d];
+          hix[d] = hi[d];
+        }
+      else
+        {
+          ncv[d - dim_x] = nc[d];
+          lov[d - dim_x] = lo[d];
+          hiv[d - dim_x] = hi[d];
+        }
     }
-  hyperdeal_amd::host::MatrixFree<dim_x, dim_v> hmf;
-  hmf.reinit(nc, lo, hi, degree, periodic, periodic);
-  Advection<dim_x, dim_v> op(hmf, a, skew);
-  std::vector<double> src, dst;
-  hmf.initialize_dof_vector(src);
-  hmf.initialize_dof_vector(dst);
-  std::mt19937_64 rng(11);
-  std::uniform_real_distribution<double> U(-1.0, 1.0);
-  for (auto &x : src)
-    x = U(rng);
-  using A = Advection<dim_x, dim_v>;
-  hmf.loop(&A::local_apply_cell, &A::local_apply_face,
-           &A::local_apply_boundary, &op, dst, src);
-  hmf.cell_loop(&A::local_apply_inverse_mass_matrix, &op, dst, dst);
+  hh::CartesianMatrixFree<dim_x> mfx;
+  hh::CartesianMatrixFree<dim_v> mfv;
+  const auto quads = hh::advection_quadratures(degree, n_points, collocation);
+  mfx.reinit(ncx, lox, hix, periodic, degree, quads);
+  mfv.reinit(ncv, lov, hiv, periodic, degree, quads);
+  VectorType src, host_fcl, host_ecl;
+  std::array<VectorType *, 2> out = {&host_fcl, &host_ecl};
+  for (int ecl = 0; ecl < 2; ++ecl)
+    {
+      MF data(mfx, mfv);
+      typename MF::AdditionalData ad;
+      ad.use_ecl = ecl == 1;
+      data.reinit(ad);
+      if (src.empty())
+        {
+          data.initialize_dof_vector(src);
+          std::mt19937_64 rng(11);
+          std::uniform_real_distribution<double> U(-1.0, 1.0);
+          for (auto &x : src)
+            x = U(rng);
+        }
+      data.initialize_dof_vector(*out[ecl]);
+      Op op(data);
+      op.reinit(a, skew);
+      op.apply(*out[ecl], src, 0.0);
+    }
 
-  typename hyperdeal_amd::MatrixFree<dim_x, dim_v>::AdditionalData ad;
-  ad.n_cells = nc;
-  ad.lo = lo;
-  ad.hi = hi;
-  ad.periodic_x = periodic;
-  ad.periodic_v = periodic;
-  ad.degree = degree;
-  ad.n_points = degree + 1;
+  typename hyperdeal_amd::MatrixFree<dim_x, dim_v>::AdditionalData gad;
+  gad.n_cells = nc;
+  gad.lo = lo;
+  gad.hi = hi;
+  gad.periodic_x = periodic;
+  gad.periodic_v = periodic;
+  gad.degree = degree;
+  gad.n_points = n_points;
+  gad.do_collocation = collocation;
   hyperdeal_amd::MatrixFree<dim_x, dim_v> gmf;
-  gmf.reinit(ad);
+  gmf.reinit(gad);
   hyperdeal_amd::AdvectionOperation<dim_x, dim_v, double> gop(gmf);
   gop.reinit(a, skew, HD_BC_HOMOGENEOUS, HD_FUNC_ZERO);
   hyperdeal_amd::DeviceVector<double> s, d;
@@ -65,41 +101,46 @@ int run(int degree, bool periodic, double skew)
   s.copy_from_host(src);
   gop.apply(d, s, 0.0);
   const std::vector<double> g = d.to_host();
-  double e = 0.0, m = 0.0;
+  double e[2] = {0.0, 0.0}, m = 0.0;
   for (size_t i = 0; i < g.size(); ++i)
     {
-      e = std::max(e, std::fabs(g[i] - dst[i]));
-      m = std::max(m, std::fabs(dst[i]));
+      e[0] = std::max(e[0], std::fabs(g[i] - host_fcl[i]));
+      e[1] = std::max(e[1], std::fabs(g[i] - host_ecl[i]));
+      m = std::max(m, std::fabs(host_ecl[i]));
     }
-  std::printf("gpu %.3e\n", e / m);
+  std::printf("fcl %.3e ecl %.3e\n", e[0] / m, e[1] / m);
   return 0;
 }
 
 int main(int argc, char **argv)
 {
-  if (argc < 6)
+  if (argc < 8)
     {
-      std::printf("usage: %s dim_x dim_v degree periodic skew\n", argv[0]);
+      std::printf("usage: %s dim_x dim_v degree n_points collocation "
+                  "periodic skew\n",
+                  argv[0]);
       return 2;
     }
   const int dx = std::atoi(argv[1]), dv = std::atoi(argv[2]);
-  const int k = std::atoi(argv[3]);
-  const bool periodic = std::atoi(argv[4]) != 0;
-  const double skew = std::atof(argv[5]);
+  const int k = std::atoi(argv[3]), nq = std::atoi(argv[4]);
+  const bool coll = std::atoi(argv[5]) != 0;
+  const bool periodic = std::atoi(argv[6]) != 0;
+  const double skew = std::atof(argv[7]);
   try
     {
-      if (dx == 1 && dv == 1)
-        return run<1, 1>(k, periodic, skew);
-      if (dx == 2 && dv == 2)
-        return run<2, 2>(k, periodic, skew);
-      if (dx == 3 && dv == 3)
-        return run<3, 3>(k, periodic, skew);
+#define HD_CASE(DX, DV, K, NQ)                                               \
+  if (dx == DX && dv == DV && k == K && nq == NQ)                           \
+    return run<DX, DV, K, NQ>(coll, periodic, skew);
+      HD_CASE(1, 1, 3, 4) HD_CASE(1, 1, 3, 5)
+      HD_CASE(2, 2, 2, 3) HD_CASE(2, 2, 3, 4) HD_CASE(2, 2, 3, 5)
+      HD_CASE(3, 3, 1, 2) HD_CASE(3, 3, 2, 3) HD_CASE(3, 3, 1, 3)
+#undef HD_CASE
     }
   catch (const std::exception &ex)
     {
       std::printf("error: %s\n", ex.what());
       return 1;
     }
-  std::printf("unsupported dims\n");
+  std::printf("unsupported case\n");
   return 2;
 }

@@ -1,11 +1,13 @@
 """Both sides of the drop-in boundary: the advection operator written
 against the host loop / evaluator interface (include/hyperdeal_amd_host.hpp)
+in hyper.deal's AdvectionOperation shape, face-centric and element-centric,
 against the GPU operator of the same problem through the C++ wrapper of the
 C ABI (include/hyperdeal_amd.hpp).
 
 Bar: max |gpu - host| <= 1e-12 max |host| (FP64 on both sides), for 1+1,
 2+2 and 3+3, periodic (collapsed kernel) and homogeneous Dirichlet
-(quadrature kernel) directions, skew 0 and 0.5.
+(quadrature kernel) directions, skew 0 and 0.5, Gauss points n = k+1 and
+k+2 (over-integration) and Gauss-Lobatto collocation.
 """
 
 import os
@@ -33,18 +35,27 @@ def program(tmp_path_factory):
     return out
 
 
-CASES = [(1, 1, 3, 1, 0.0), (2, 2, 3, 1, 0.5), (2, 2, 2, 0, 0.0),
-         (3, 3, 2, 1, 0.0), (3, 3, 1, 0, 0.5)]
+# (dim_x, dim_v, degree, n_points, collocation, periodic, skew)
+CASES = [(1, 1, 3, 4, 0, 1, 0.0), (2, 2, 3, 4, 0, 1, 0.5),
+         (2, 2, 2, 3, 0, 0, 0.0), (3, 3, 2, 3, 0, 1, 0.0),
+         (3, 3, 1, 2, 0, 0, 0.5),
+         # over-integration: periodic (collapsed) and Dirichlet (quadrature)
+         (1, 1, 3, 5, 0, 1, 0.0), (2, 2, 3, 5, 0, 1, 0.5),
+         (2, 2, 3, 5, 0, 0, 0.0), (3, 3, 1, 3, 0, 0, 0.0),
+         # Gauss-Lobatto collocation
+         (2, 2, 3, 4, 1, 1, 0.0), (2, 2, 2, 3, 1, 1, 0.5),
+         (3, 3, 1, 2, 1, 1, 0.0)]
 
 
-@pytest.mark.parametrize("dx,dv,k,periodic,skew", CASES)
-def test_host_operator_equals_gpu_operator(program, dx, dv, k, periodic,
-                                           skew):
-    print("case: host vs gpu", dx, dv, k, periodic, skew, flush=True)
-    r = subprocess.run([program, str(dx), str(dv), str(k), str(periodic),
-                        str(skew)], capture_output=True, text=True,
-                       timeout=120)
+@pytest.mark.parametrize("dx,dv,k,nq,coll,periodic,skew", CASES)
+def test_host_operator_equals_gpu_operator(program, dx, dv, k, nq, coll,
+                                           periodic, skew):
+    print("case: host vs gpu", dx, dv, k, nq, coll, periodic, skew,
+          flush=True)
+    r = subprocess.run([program] + [str(x) for x in (dx, dv, k, nq, coll,
+                                                       periodic, skew)],
+                       capture_output=True, text=True, timeout=120)
     assert r.returncode == 0, r.stdout + r.stderr
     words = r.stdout.split()
-    assert words[0] == "gpu", r.stdout
-    assert float(words[1]) <= 1e-12, r.stdout
+    assert words[0] == "fcl" and words[2] == "ecl", r.stdout
+    assert float(words[1]) <= 1e-12 and float(words[3]) <= 1e-12, r.stdout
