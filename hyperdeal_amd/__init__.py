@@ -28,7 +28,7 @@ __all__ = ["MatrixFree", "AdvectionOperation", "VlasovPoissonOperation",
 
 _HERE = os.path.dirname(os.path.abspath(__file__))
 FP64, FP32 = 0, 1
-ALGO_AUTO, ALGO_QUADRATURE, ALGO_COLLAPSED = 0, 1, 2
+ALGO_AUTO, ALGO_QUADRATURE, ALGO_COLLAPSED, ALGO_FCL = 0, 1, 2, 3
 BC_HOMOGENEOUS, BC_INHOMOGENEOUS = 0, 1
 FUNC_ZERO, FUNC_HYPERRECTANGLE, FUNC_CONSTANT_ONE, FUNC_SIN_COS, \
     FUNC_PRODUCT_SINES, FUNC_LANDAU = range(6)
@@ -56,7 +56,8 @@ EXPORTED = [
     "hd_vp_learned_guess_stats", "hd_vp_learned_guess_net",
     "hd_cnn_n_params", "hd_cnn_create", "hd_cnn_destroy", "hd_cnn_params",
     "hd_cnn_forward", "hd_cnn_loss_grad", "hd_cnn_train_step",
-    "hd_cnn_last_loss", "hd_vp_exact_start",
+    "hd_cnn_last_loss", "hd_vp_exact_start", "hd_cnn_dofs_to_image",
+    "hd_cnn_image_to_dofs",
 ]
 BATCH_MAX = 64
 
@@ -156,6 +157,10 @@ def load_library(path=None):
     lib.hd_cnn_train_step.argtypes = [_vp, _vp, _vp, ctypes.c_int,
                                       ctypes.c_float, _vp]
     lib.hd_cnn_last_loss.argtypes = [_vp, _dp]
+    lib.hd_cnn_dofs_to_image.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _vp, _vp, _vp]
+    lib.hd_cnn_image_to_dofs.argtypes = [_vp, _vp, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_int, _vp, _vp]
     lib.hd_vp_x_n_dofs.restype = ctypes.c_longlong
     lib.hd_vp_x_n_dofs.argtypes = [_vp]
     lib.hd_vp_potential.argtypes = [_vp, _dp, ctypes.c_int]
@@ -559,6 +564,36 @@ class Cnn:
         if getattr(self, "_owned", False) and self._handle:
             self._lib.hd_cnn_destroy(self._handle)
             self._handle = None
+
+
+def cnn_dofs_to_image(u, n0, n1, N, sub=None, stream=None):
+    """The learned guess's x-space DoFs -> network image: ((u - sub) - mean)
+    / rms as a float32 (n1 N, n0 N) CUDA tensor, and rms (a float64 CUDA
+    tensor of one element)."""
+    import torch
+    n = n0 * n1 * N * N
+    lib = load_library()
+    img = torch.empty((n1 * N, n0 * N), dtype=torch.float32, device=u.device)
+    scale = torch.empty(1, dtype=torch.float64, device=u.device)
+    _check(lib.hd_cnn_dofs_to_image(
+        _ptr(u, torch.float64, n),
+        _ptr(sub, torch.float64, n) if sub is not None else None, n0, n1, N,
+        _ptr(img, torch.float32, n), _ptr(scale, torch.float64, 1),
+        _stream(stream)), "hd_cnn_dofs_to_image")
+    return img, scale
+
+
+def cnn_image_to_dofs(img, scale, n0, n1, N, stream=None):
+    """Network image -> x-space DoFs, scale * image with shared nodes
+    averaged over their pixels (continuous across cell faces)."""
+    import torch
+    n = n0 * n1 * N * N
+    lib = load_library()
+    u = torch.empty(n, dtype=torch.float64, device=img.device)
+    _check(lib.hd_cnn_image_to_dofs(
+        _ptr(img, torch.float32, n), _ptr(scale, torch.float64, 1), n0, n1, N,
+        _ptr(u, torch.float64, n), _stream(stream)), "hd_cnn_image_to_dofs")
+    return u
 
 
 def vp_moments(mf, f, stream=None):

@@ -35,6 +35,10 @@ int choose_algorithm(const hd_advection *op, int precision)
   const bool coll_ok = op->collapsed_ok;
   switch (op->algorithm)
     {
+      case HD_ALGO_FCL:
+        return !op->vp && hd::fcl_supported(mf->D, mf->nd, mf->nq)
+                 ? HD_ALGO_FCL
+                 : -1;
       case HD_ALGO_COLLAPSED:
         return coll_ok ? HD_ALGO_COLLAPSED : -1;
       case HD_ALGO_QUADRATURE:
@@ -127,7 +131,40 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
                   std::to_string(precision) + ", algorithm " +
                   std::to_string(op->algorithm) + ")");
   hipError_t e;
-  if (algo == HD_ALGO_COLLAPSED)
+  if (algo == HD_ALGO_FCL)
+    {
+      // the integrals are assembled in dst (apply), in ti_out (stage), or
+      // in a free vector (last stage)
+      void *asmv = mode == 0 ? dst : stage.ti_out;
+      if (!asmv)
+        asmv = stage.scratch;
+      if (!asmv)
+        {
+          if (!op->split_scratch)
+            {
+              e = hipMalloc(&op->split_scratch,
+                            (size_t)mf->n_dofs * sizeof(double));
+              if (e != hipSuccess)
+                return hip_fail(e, "FCL assembly vector");
+            }
+          asmv = op->split_scratch;
+        }
+      if (precision == HD_FP64)
+        e = hd::fcl_apply(mf->D, mf->nd, mf->nq, mode,
+                          static_cast<const double *>(src),
+                          static_cast<double *>(asmv),
+                          static_cast<double *>(dst), op->quad, op->mass,
+                          mf->mesh, stage, time, mf->dim_x, mf->collocation,
+                          stream);
+      else
+        e = hd::fcl_apply_f32(mf->D, mf->nd, mf->nq, mode,
+                              static_cast<const float *>(src),
+                              static_cast<float *>(asmv),
+                              static_cast<float *>(dst), op->quad, op->mass,
+                              mf->mesh, stage, time, mf->dim_x,
+                              mf->collocation, stream);
+    }
+  else if (algo == HD_ALGO_COLLAPSED)
     {
       // the cell and face loads are 16-byte vector loads
       if (reinterpret_cast<uintptr_t>(src) % 16 != 0)
@@ -418,6 +455,24 @@ int hd_advection_create(hd_matrix_free *mf, const double *velocity,
       q.skew = skew_factor;
       q.bc_inhomogeneous = bc_kind == HD_BC_INHOMOGENEOUS ? 1 : 0;
       q.bc_function = bc_function;
+      // the FCL inverse mass matrix: the Gauss(k+1) rule (the reference's
+      // inverse-mass evaluator uses quadrature no. 1), or the Gauss-Lobatto
+      // rule of the nodes for collocation
+      auto &m = op->mass;
+      m.collocation = mf->collocation;
+      hd::Tables1D g;
+      const hd::Tables1D *tm = &t;
+      if (!mf->collocation && nq != nd)
+        {
+          if (!g.build(mf->degree, nd, false))
+            {
+              hd_advection_destroy(op);
+              return fail(HD_ERR_INVALID, "cannot build the mass tables");
+            }
+          tm = &g;
+        }
+      std::copy(tm->Sinv.begin(), tm->Sinv.end(), m.Sinv);
+      std::copy(tm->qweights.begin(), tm->qweights.end(), m.w);
     }
   *out = op;
   return ok();
@@ -800,8 +855,11 @@ int hd_vp_field_energy(hd_advection *op, double *out)
 
 int hd_advection_set_algorithm(hd_advection *op, int algorithm)
 {
-  if (!op || algorithm < HD_ALGO_AUTO || algorithm > HD_ALGO_COLLAPSED)
+  if (!op || algorithm < HD_ALGO_AUTO || algorithm > HD_ALGO_FCL)
     return fail(HD_ERR_INVALID, "invalid algorithm");
+  if (algorithm == HD_ALGO_FCL && op->vp)
+    return fail(HD_ERR_UNSUPPORTED,
+                "the face-centric form takes a constant velocity");
   if (algorithm == HD_ALGO_COLLAPSED && !op->collapsed_ok)
     return fail(HD_ERR_UNSUPPORTED,
                 "collapsed form needs a periodic mesh in every direction");

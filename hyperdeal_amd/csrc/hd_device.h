@@ -92,6 +92,16 @@ struct QuadDev
   long long n_x_cells;
 };
 
+// the inverse mass matrix of the face-centric form: S^{-1} of the Gauss
+// (k+1) rule (ND x ND) and its weights on [0,1]; collocation: the
+// Gauss-Lobatto weights, no S
+struct MassDev
+{
+  double Sinv[64];
+  double w[8];
+  int collocation;
+};
+
 // an analytic function and the 1D nodes it is interpolated at
 struct FuncDev
 {
@@ -232,6 +242,21 @@ hipError_t quadrature_apply_f32(int D, int ND, int NQ, int mode,
                                 const StageDev &stage, double time,
                                 int dim_x, int collocation,
                                 const double *efield, hipStream_t stream);
+// the face-centric form (MatrixFree::loop + the inverse-mass cell_loop):
+// cell, face (per direction and colour) and inverse-mass kernels; asmv is
+// the vector the integrals are assembled in (dst for an apply, ti_out or a
+// free vector for a stage; never src)
+bool fcl_supported(int D, int ND, int NQ);
+hipError_t fcl_apply(int D, int ND, int NQ, int mode, const double *src,
+                     double *asmv, double *dst, const QuadDev &q,
+                     const MassDev &md, const MeshDev &mesh,
+                     const StageDev &stage, double time, int dim_x,
+                     int collocation, hipStream_t stream);
+hipError_t fcl_apply_f32(int D, int ND, int NQ, int mode, const float *src,
+                         float *asmv, float *dst, const QuadDev &q,
+                         const MassDev &md, const MeshDev &mesh,
+                         const StageDev &stage, double time, int dim_x,
+                         int collocation, hipStream_t stream);
 hipError_t launch_interpolate(double *dst, const MeshDev &mesh,
                               const FuncDev &f, double t, long long n_dofs,
                               int dofs_cell, hipStream_t stream);

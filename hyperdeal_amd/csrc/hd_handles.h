@@ -102,11 +102,14 @@ struct hd_advection
   int algorithm = HD_ALGO_AUTO;
   bool collapsed_ok = false;
   bool split = false;          // the collapsed form runs split (big cells)
-  void *split_scratch = nullptr; // the last stage's part, when none is given
+  // a free vector (n_dofs doubles) for the last stage of the split form
+  // (its first part) and of the FCL form (the assembled integrals)
+  void *split_scratch = nullptr;
   double *coef64 = nullptr;
   float *coef32 = nullptr;
   hd::InflowDev inflow{};
   hd::QuadDev quad{};
+  hd::MassDev mass{};          // inverse mass matrix of the FCL form
   // Vlasov-Poisson state (velocity (v, grad phi) from the Poisson solve)
   bool vp = false;
   bool vp_auto = true;        // stages recompute the field from their input

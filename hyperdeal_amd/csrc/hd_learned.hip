@@ -252,6 +252,26 @@ int hd_cnn_last_loss(hd_cnn *net, double *loss)
   return e == hipSuccess ? ok() : hip_fail(e, "hd_cnn_last_loss");
 }
 
+int hd_cnn_dofs_to_image(const double *u, const double *sub, int n0, int n1,
+                         int N, float *image, double *scale, void *stream)
+{
+  if (!u || !image || !scale || n0 < 1 || n1 < 1 || N < 1)
+    return fail(HD_ERR_INVALID, "hd_cnn_dofs_to_image: invalid argument");
+  const hipError_t e =
+    hd::cnn_dofs_to_image(u, sub, n0, n1, N, image, scale, as_stream(stream));
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_cnn_dofs_to_image");
+}
+
+int hd_cnn_image_to_dofs(const float *image, const double *scale, int n0,
+                         int n1, int N, double *u, void *stream)
+{
+  if (!u || !image || !scale || n0 < 1 || n1 < 1 || N < 1)
+    return fail(HD_ERR_INVALID, "hd_cnn_image_to_dofs: invalid argument");
+  const hipError_t e =
+    hd::cnn_image_to_dofs(image, scale, n0, n1, N, u, as_stream(stream));
+  return e == hipSuccess ? ok() : hip_fail(e, "hd_cnn_image_to_dofs");
+}
+
 int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
                                int capacity, int train_every, int train_batch,
                                int train_steps, float learning_rate,

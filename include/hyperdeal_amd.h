@@ -47,8 +47,14 @@ typedef enum hd_algorithm
 {
   HD_ALGO_AUTO = 0,       /* collapsed where exact, else quadrature */
   HD_ALGO_QUADRATURE = 1, /* the reference's sum-factorised quadrature steps */
-  HD_ALGO_COLLAPSED = 2   /* exact tensor factorisation (constant velocity,
+  HD_ALGO_COLLAPSED = 2,  /* exact tensor factorisation (constant velocity,
                              periodic Cartesian mesh) */
+  HD_ALGO_FCL = 3         /* the reference's face-centric loop: cell, face
+                             and boundary integrals assembled once per cell
+                             and face, then the inverse mass matrix
+                             (MatrixFree::loop + cell_loop, matrix_free.
+                             templates.h:1417-1465, 1634-1746; advection_
+                             operation.h:159-183, 573-1131) */
 } hd_algorithm;
 
 /* Boundary data of non-periodic directions (operators/advection/
@@ -121,7 +127,11 @@ int hd_advection_create(hd_matrix_free *mf, const double *velocity,
 int hd_advection_destroy(hd_advection *op);
 /* Force an algorithm (HD_ALGO_*); HD_ERR_UNSUPPORTED if not applicable.
  * HD_ALGO_COLLAPSED: periodic boxes, any n_points >= degree + 1.
- * HD_ALGO_QUADRATURE: any boundary, n_points = degree + 1 or degree + 2. */
+ * HD_ALGO_QUADRATURE: any boundary, n_points = degree + 1 or degree + 2.
+ * HD_ALGO_FCL: as HD_ALGO_QUADRATURE, constant velocity; a stage without
+ * ti_out assembles in a vector the operator allocates on first use.
+ * (AdvectionOperation chooses ECL or FCL with Matrixfree/UseECL,
+ * advection_operation.h:157-204.) */
 int hd_advection_set_algorithm(hd_advection *op, int algorithm);
 /* The algorithm apply() runs with the current settings and precision. */
 int hd_advection_algorithm(const hd_advection *op, int precision);
@@ -317,6 +327,19 @@ int hd_cnn_loss_grad(hd_cnn *net, const float *in, const float *target,
 int hd_cnn_train_step(hd_cnn *net, const float *in, const float *target,
                       int batch, float learning_rate, void *stream);
 int hd_cnn_last_loss(hd_cnn *net, double *loss);
+/* The learned guess's conversions between a 2D x-space nodal vector
+ * (n0 x n1 cells, N x N Gauss-Lobatto nodes each, cell-major) and the
+ * (n1 N) x (n0 N) image the network sees (device arrays; no counterpart in
+ * the reference):
+ *   dofs_to_image: v = u - sub (sub may be NULL); image = (v - mean v) / rms,
+ *                  *scale = rms (scale: one device double)
+ *   image_to_dofs: u = scale * image, continuous across cell faces (a node
+ *                  shared by neighbouring cells gets the mean of its two or
+ *                  four pixels) */
+int hd_cnn_dofs_to_image(const double *u, const double *sub, int n0, int n1,
+                         int N, float *image, double *scale, void *stream);
+int hd_cnn_image_to_dofs(const float *image, const double *scale, int n0,
+                         int n1, int N, double *u, void *stream);
 /* the network of an operator's learned guess (borrowed; NULL if off).  It
  * is freed by hd_vp_learned_guess_disable, by a second
  * hd_vp_learned_guess_enable and by hd_vp_exact_start(op, 1): the pointer

@@ -165,3 +165,42 @@ def test_batch_rejects_unsupported_operators():
     op2 = H.AdvectionOperation(mf2, [0.1] * 4)
     with pytest.raises(H.HyperdealError):
         H.Batch([op2])
+
+
+def test_c4_mix_graph_steps_equal_single_operator_steps():
+    """The batching workload itself: 64 instances, instance i drawn from
+    seed 20261204 + i (cells per direction from {4, 6, 8, 12, 16}, degree
+    from {2, 3, 4, 5}, velocity U(-1, 1)^4), FP32-mixed rk45.  After the
+    autotuner picks the work split, two hipGraph-replayed batch steps equal
+    two single-operator steps of every instance bit for bit."""
+    print("case: C4 mix, 64 instances", flush=True)
+    insts = _instances(64, sizes=(4, 6, 8, 12, 16), degrees=(2, 3, 4, 5))
+    assert sum(mf.n_dofs() for mf, *_ in insts) > 5e8
+    batch = H.Batch([op for _, op, *_ in insts])
+    sol, ta, tb = [], [], []
+    for mf, op, a, k, nc in insts:
+        s = mf.initialize_dof_vector()
+        H.VectorTools.interpolate(mf, s, H.FUNC_PRODUCT_SINES, [0.0] * 4)
+        H.VectorTools.add_noise(mf, s, 20261203, 1e-3)
+        sol.append(s)
+        ta.append(torch.empty(s.numel(), dtype=torch.float32, device="cuda"))
+        tb.append(torch.empty_like(ta[-1]))
+    for t, s in zip(ta, sol):
+        t.copy_(s)
+    batch.autotune(ta, tb, precision=H.FP32)
+    sol_ref = [s.clone() for s in sol]
+    dt = 1e-4
+    graph = batch.graph("rk45", sol, ta, tb, dt, precision=H.FP32)
+    for _ in range(2):
+        graph.launch()
+    torch.cuda.synchronize()
+    del graph, ta, tb
+    integ = H.LowStorageRungeKuttaIntegrator("rk45")
+    for (mf, op, *_), s, r in zip(insts, sol, sol_ref):
+        a = torch.empty(r.numel(), dtype=torch.float32, device="cuda")
+        b = torch.empty_like(a)
+        for step in range(2):
+            integ.perform_time_step(op, r, a, b, step * dt, dt,
+                                    precision=H.FP32)
+        torch.cuda.synchronize()
+        assert torch.equal(s, r)

@@ -193,8 +193,11 @@ def test_learned_guess_keeps_the_solution(precision):
     assert np.isfinite(r0) and np.isfinite(rp)
 
 
-def test_exact_start_needs_no_iteration():
-    print("case: vp exact start", flush=True)
+def test_warm_start_from_converged_potential_needs_no_iteration():
+    """The reference's warm start (the previous phi, vlasov_poisson/include/
+    application.h:575-576): a second solve with the same density starts
+    from its own answer and takes at most one CG iteration."""
+    print("case: vp warm start", flush=True)
     mf, op, f = _landau()
     op.update_field(f)
     it0, n0, _ = op.solver_stats()
@@ -257,15 +260,35 @@ def test_learned_guess_effect_is_measured_and_bounded():
     assert np.abs(phis[True] - phis[False]).max() <= 1e-8 * scale
 
 
-def test_guess_image_to_dofs_matches_oracle():
-    """x-space DoFs <-> image through the learned guess of an operator: the
-    guess of an untrained-but-stepped network equals the oracle's
-    continuous image_to_dofs of the network output (to FP32 rounding)."""
-    print("case: image to dofs", flush=True)
-    # checked indirectly on the CPU oracle: averaging is exact for a
-    # continuous field and idempotent
-    img = np.random.default_rng(0).standard_normal((16, 32)).astype(np.float32)
-    u = C.image_to_dofs(img, 8, 4, 4, 2.0, continuous=True)
-    img2, rms = C.dofs_to_image(u, 8, 4, 4)
-    u2 = C.image_to_dofs(img2, 8, 4, 4, rms, continuous=True)
-    assert np.allclose(u2 + u.mean(), u, atol=1e-5 * np.abs(u).max())
+@pytest.mark.parametrize("n0,n1,N", [(8, 4, 4), (32, 32, 4), (5, 3, 3),
+                                    (16, 8, 2)])
+def test_image_dofs_kernels_match_oracle(n0, n1, N):
+    """The learned guess's conversions on the GPU (dofs_to_image_kernel,
+    image_to_dofs_kernel through hd_cnn_dofs_to_image / _image_to_dofs)
+    against oracle/cnn.py: the image of (u - sub) to FP32 rounding (2 ulp of
+    the normalised pixel), its rms scale to 1e-13 relative, and the
+    continuous DoFs of an image (shared nodes averaged over their two or
+    four pixels) to 1e-14 of the largest value."""
+    print("case: image <-> dofs", n0, n1, N, flush=True)
+    r = np.random.default_rng(n0 * 100 + n1 * 10 + N)
+    n = n0 * n1 * N * N
+    u = r.standard_normal(n) * 3.0 + 1.5
+    sub = r.standard_normal(n)
+    ud = torch.from_numpy(u).cuda()
+    sd = torch.from_numpy(sub).cuda()
+    for s_host, s_dev in ((None, None), (sub, sd)):
+        v = u if s_host is None else u - s_host
+        img, scale = H.cnn_dofs_to_image(ud, n0, n1, N, sub=s_dev)
+        ref_img, ref_rms = C.dofs_to_image(v, n0, n1, N)
+        torch.cuda.synchronize()
+        got = img.cpu().numpy()
+        assert got.shape == ref_img.shape
+        assert np.max(np.abs(got - ref_img)) <= 2.5e-7 * np.abs(ref_img).max()
+        assert abs(scale.item() - ref_rms) <= 1e-13 * ref_rms
+    x = r.standard_normal((n1 * N, n0 * N)).astype(np.float32)
+    sc = torch.tensor([0.75], dtype=torch.float64, device="cuda")
+    back = H.cnn_image_to_dofs(torch.from_numpy(x).cuda(), sc, n0, n1, N)
+    ref = C.image_to_dofs(x, n0, n1, N, 0.75, continuous=True)
+    torch.cuda.synchronize()
+    assert np.max(np.abs(back.cpu().numpy() - ref)) <= \
+        1e-14 * np.abs(ref).max()

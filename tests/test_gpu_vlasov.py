@@ -77,6 +77,59 @@ def test_phase_space_apply_matches_oracle():
     assert np.max(np.abs(got - ref)) <= 1e-9 * np.max(np.abs(ref))
 
 
+@pytest.mark.parametrize("algo", ["phase_space", "quadrature"])
+@pytest.mark.parametrize("degree,n_cells", [(2, [4, 2, 3, 5]),
+                                            (3, [3, 4, 5, 3]),
+                                            (4, [2, 3, 3, 3])])
+def test_apply_and_stage_match_oracle_with_shared_potential(degree, n_cells,
+                                                            algo):
+    """The production 2+2 Vlasov-Poisson apply (the phase-space kernel vpk,
+    or the quadrature kernel) against the oracle at 1e-12 on one potential:
+    the oracle's converged phi (CG to 1e-14) is loaded into the operator,
+    whose own solve then starts converged and takes no CG iteration, so both
+    sides transport with the same grad phi.  Odd v cell counts on the
+    symmetric v range put a cell across v = 0 (both upwind signs in one
+    cell); two perturbed directions give fields of both signs.  One fused
+    LSRK stage (its field update also starts converged) against the stage
+    formula on the oracle's apply."""
+    print("case: vp shared potential", degree, n_cells, algo, flush=True)
+    lo, hi = [0, 0, -6, -6], [4 * PI, 2 * PI, 6, 6]
+    P = O.VlasovPoisson(2, degree, n_cells, lo, hi, [1, 1, 6, 6], eps=0.2,
+                        kx=0.5, n_perturbed=2)
+    mf = H.MatrixFree(2, 2, degree, n_cells, lo, hi)
+    op = H.VlasovPoissonOperation(mf, poisson_tol=1e-10)
+    op.set_algorithm(H.ALGO_AUTO if algo == "phase_space"
+                     else H.ALGO_QUADRATURE)
+    r = np.random.default_rng(degree)
+    f = P.interpolate() * (1.0 + 0.05 * r.standard_normal(P.n_dofs))
+    phi_ref, _ = P.poisson_solve(P.rhs(P.density(f)), rel_tol=1e-14)
+    P.set_potential(phi_ref)
+    ref = P.apply(f)
+    op.set_potential(phi_ref)
+    it0, n0, _ = op.solver_stats()
+    fd = torch.from_numpy(f).cuda()
+    out = torch.empty_like(fd)
+    op.apply(out, fd)
+    torch.cuda.synchronize()
+    it1, n1, res = op.solver_stats()
+    assert n1 == n0 + 1 and it1 == it0 and res <= 1e-10
+    scale = np.max(np.abs(ref))
+    assert np.max(np.abs(out.cpu().numpy() - ref)) <= 1e-12 * scale
+    # one stage: sol += b_dt K, ti_out = sol_old + a_dt K
+    sol0 = 0.5 * f
+    sol = torch.from_numpy(sol0.copy()).cuda()
+    ti = torch.empty_like(fd)
+    op.stage(fd, ti, sol, 0.0, 0.3, 0.2)
+    torch.cuda.synchronize()
+    it2, n2, _ = op.solver_stats()
+    assert n2 == n1 + 1 and it2 == it1
+    ssc = np.max(np.abs(sol0 + 0.2 * ref))
+    assert np.max(np.abs(sol.cpu().numpy() - (sol0 + 0.2 * ref))) <= \
+        1e-12 * ssc
+    assert np.max(np.abs(ti.cpu().numpy() - (sol0 + 0.3 * ref))) <= \
+        1e-12 * ssc
+
+
 def test_moments_match_oracle():
     P, mf, op = _landau([4, 4, 4, 4])
     f = P.interpolate()
