@@ -227,6 +227,73 @@ def landau_workload(H, torch, stream, steps):
     return out
 
 
+def landau_damping(H, torch, t_end=10.0):
+    """The Landau damping rate of the 32^4, k=3 workload to t = 10 (rk45 at
+    CFL 0.3, Poisson CG to 1e-10 from the exact start), FP64 and
+    FP32-mixed: the rate fitted to the maxima of the field energy (written
+    every step; examples/vlasov_poisson/include/application.h:630-660),
+    against the FP64 run and the linear dispersion relation, the relative
+    mass drift and the FP32-mixed run's relative L2 difference to the FP64
+    run at t_end."""
+    import numpy as np
+
+    from hyperdeal_amd.diagnostics import fit_damping_rate, landau_theory_rate
+    L, n, k = 4 * np.pi, 32, 3
+    dt = 0.3 * (L / n / 6.0) / k ** 1.5
+    n_steps = int(np.ceil(t_end / dt))
+    dt = t_end / n_steps
+    integ = H.LowStorageRungeKuttaIntegrator("rk45")
+    omega, gamma_th = landau_theory_rate(np.sqrt(0.5))
+    out = {"workload": "Landau 2D+2D, 32^4 cells, k=3, rk45, %d steps of "
+                       "dt %.6f to t = %g, exact Poisson start, CG to 1e-10"
+                       % (n_steps, dt, t_end),
+           "theory_rate": round(gamma_th, 5), "runs": {}}
+    ref = None
+    for name, prec, dtype in (("fp64", H.FP64, torch.float64),
+                              ("fp32", H.FP32, torch.float32)):
+        mf = H.MatrixFree(2, 2, k, [n] * 4, [0, 0, -6, -6], [L, L, 6, 6])
+        op = H.VlasovPoissonOperation(mf, poisson_tol=1e-10)
+        op.exact_start()
+        f = mf.initialize_dof_vector()
+        H.VectorTools.interpolate(mf, f, H.FUNC_LANDAU, [0.01, 0.5, 2.0])
+        m0 = H.vp_moments(mf, f)[0]
+        ta = torch.empty(f.numel(), dtype=dtype, device="cuda")
+        tb = torch.empty_like(ta)
+        ts, en = [], []
+        t0 = time.perf_counter()
+        last = t0
+        for s in range(n_steps):
+            integ.perform_time_step(op, f, ta, tb, s * dt, dt, precision=prec)
+            ts.append((s + 1) * dt)
+            en.append(sum(op.field_energy()))
+            if time.perf_counter() - last > 20:
+                last = time.perf_counter()
+                print("landau damping %s: step %d / %d" % (name, s + 1,
+                                                           n_steps),
+                      file=sys.stderr, flush=True)
+        torch.cuda.synchronize()
+        wall = time.perf_counter() - t0
+        gamma, peaks = fit_damping_rate(ts, en)
+        run = {"damping_rate": round(gamma, 6), "fit_maxima": peaks,
+               "mass_drift_rel": (H.vp_moments(mf, f)[0] - m0) / m0,
+               "ms_per_step_incl_diagnostics": round(1e3 * wall / n_steps, 3)}
+        if ref is None:
+            ref = f
+        else:
+            run["rel_l2_vs_fp64"] = float(torch.linalg.vector_norm(f - ref) /
+                                          torch.linalg.vector_norm(ref))
+            run["rate_minus_fp64_rate"] = gamma - out["runs"]["fp64"][
+                "damping_rate"]
+            del f
+        run["rate_minus_theory"] = gamma - gamma_th
+        out["runs"][name] = run
+        del op, ta, tb
+        torch.cuda.empty_cache()
+    del ref
+    torch.cuda.empty_cache()
+    return out
+
+
 def gl_weights(N):
     """Gauss-Lobatto quadrature weights of N points on [0, 1]."""
     import numpy as np
@@ -343,14 +410,61 @@ def c3_workload(H, torch, stream, steps):
     return out
 
 
+def quadrature_forms(H, torch, stream, steps):
+    """The kernels of the reference's algorithm itself, on a 2+2, k=3 mesh
+    of 24^4 cells (84,934,656 DoF) on [-1, 1]^4 with inhomogeneous Dirichlet
+    boundaries (the hyperrectangle case, velocity (1, 0.15, -0.05, 0)): the
+    element-centric quadrature kernel (FP64 and FP32-mixed), the
+    face-centric cell / face / inverse-mass kernels (FP64) and the
+    quadrature kernel on the deformed mesh (DeformedCubeManifold 0.1, 2;
+    FP64).  Rates in GDoF/s and the algorithmic 16 (FP64) / 8 (FP32)
+    bytes per DoF per apply; about 420 flop per DoF for the quadrature
+    steps."""
+    k, n = 3, [24] * 4
+    vel = [1.0, 0.15, -0.05, 0.0]
+    out = {"workload": "2D+2D apply, [-1,1]^4 Dirichlet, 24^4 cells, k=3 "
+                       "(84934656 DoF), Gauss k+1", "runs": {}}
+    for name, deform, algo, prec in (
+            ("ecl_fp64", False, H.ALGO_QUADRATURE, H.FP64),
+            ("ecl_fp32", False, H.ALGO_QUADRATURE, H.FP32),
+            ("fcl_fp64", False, H.ALGO_FCL, H.FP64),
+            ("ecl_deformed_fp64", True, H.ALGO_QUADRATURE, H.FP64),
+            ("fcl_deformed_fp64", True, H.ALGO_FCL, H.FP64)):
+        mf = H.MatrixFree(2, 2, k, n, [-1.0] * 4, [1.0] * 4,
+                          periodic_x=False, periodic_v=False)
+        if deform:
+            mf.set_deformation(0.1, 2)
+        op = H.AdvectionOperation(mf, vel, boundary=H.BC_INHOMOGENEOUS,
+                                  boundary_function=H.FUNC_HYPERRECTANGLE,
+                                  boundary_params=vel)
+        op.set_algorithm(algo)
+        nd = mf.n_dofs()
+        dtype = torch.float64 if prec == H.FP64 else torch.float32
+        u = mf.initialize_dof_vector()
+        H.VectorTools.interpolate(mf, u, H.FUNC_HYPERRECTANGLE, vel)
+        u = u.to(dtype)
+        d = torch.empty_like(u)
+        t = timed(lambda: op.apply(d, u, 0.1, prec, stream), steps, 2, stream)
+        bpd = 16.0 if prec == H.FP64 else 8.0
+        out["runs"][name] = {"GDoFs": round(nd / t / 1e9, 2),
+                             "ms_per_apply": round(t * 1e3, 3),
+                             "achieved_GBs": round(bpd * nd / t / 1e9, 1)}
+        del op, mf, u, d
+        torch.cuda.empty_cache()
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=50)
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-landau-rate", action="store_true",
+                    help="skip the Landau damping-rate runs to t = 10 "
+                    "(about two minutes)")
     ap.add_argument("--only", choices=["all", "fp64", "fp32", "stage",
-                                       "batch", "c3", "landau"],
+                                       "batch", "c3", "landau", "quadrature"],
                     default="all", help="restrict the timed kernels "
                     "(profiling runs)")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -430,6 +544,13 @@ def main():
     if args.only in ("all", "landau"):
         landau_result = landau_workload(H, torch, stream,
                                         max(5, args.steps // 5))
+    damping_result = None
+    if args.only == "all" and not args.no_landau_rate:
+        damping_result = landau_damping(H, torch)
+    quad_result = None
+    if args.only in ("all", "quadrature"):
+        quad_result = quadrature_forms(H, torch, stream,
+                                       max(3, args.steps // 5))
 
     def gdofs(t):
         return n / t / 1e9
@@ -499,6 +620,10 @@ def main():
         line["c3_3d3d_k5"] = c3_result
     if landau_result is not None:
         line["landau_2d2d"] = landau_result
+    if damping_result is not None:
+        line["landau_damping"] = damping_result
+    if quad_result is not None:
+        line["quadrature_forms"] = quad_result
     if not args.no_cpu_baseline:
         line["cpu_baseline"] = cpu_baseline(velocity)
     print(json.dumps(line), flush=True)

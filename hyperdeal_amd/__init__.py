@@ -57,7 +57,7 @@ EXPORTED = [
     "hd_cnn_n_params", "hd_cnn_create", "hd_cnn_destroy", "hd_cnn_params",
     "hd_cnn_forward", "hd_cnn_loss_grad", "hd_cnn_train_step",
     "hd_cnn_last_loss", "hd_vp_exact_start", "hd_cnn_dofs_to_image",
-    "hd_cnn_image_to_dofs",
+    "hd_cnn_image_to_dofs", "hd_matrix_free_set_deformation",
 ]
 BATCH_MAX = 64
 
@@ -98,6 +98,10 @@ def load_library(path=None):
                                           ctypes.c_int, ctypes.c_int,
                                           ctypes.c_int, ctypes.POINTER(_vp)]
     lib.hd_matrix_free_destroy.argtypes = [_vp]
+    lib.hd_matrix_free_set_deformation.argtypes = [_vp, ctypes.c_double,
+                                                   ctypes.c_int,
+                                                   ctypes.c_double,
+                                                   ctypes.c_int]
     for f in ("hd_matrix_free_n_dofs", "hd_matrix_free_n_cells"):
         getattr(lib, f).restype = ctypes.c_longlong
         getattr(lib, f).argtypes = [_vp]
@@ -255,6 +259,17 @@ class MatrixFree:
         if getattr(self, "handle", None):
             self._lib.hd_matrix_free_destroy(self.handle)
             self.handle = None
+
+    def set_deformation(self, deformation_x=0.1, frequency_x=2,
+                        deformation_v=None, frequency_v=None):
+        """Deform both spaces with the reference's DeformedCubeManifold
+        (grid_generator.cc:110-230; defaults 0.1 and 2) before operators
+        are created on this object."""
+        dv = deformation_x if deformation_v is None else deformation_v
+        fv = frequency_x if frequency_v is None else frequency_v
+        _check(self._lib.hd_matrix_free_set_deformation(
+            self.handle, float(deformation_x), int(frequency_x), float(dv),
+            int(fv)), "hd_matrix_free_set_deformation")
 
     def n_dofs(self):
         return self._lib.hd_matrix_free_n_dofs(self.handle)

@@ -28,89 +28,86 @@ int prec_index(int precision)
 }
 } // namespace
 
-// a work split on the device: the task list, then the blocks' task ranges
+// the work splits of one precision on the device: per degree N present, a
+// task list and the blocks' task ranges, all in one allocation (a graph
+// keeps its own copy)
 struct DevPlan
 {
   void *mem = nullptr;
   size_t bytes = 0;
-  const hd::BatchTask *tasks = nullptr;
-  hd::BatchPlan plan{};
+  const hd::BatchTask *tasks[kMaxN + 1] = {};
+  hd::BatchPlan plan[kMaxN + 1] = {};
 };
 
 struct hd_batch
 {
   std::vector<hd_advection *> ops;
   std::vector<long long> n_dofs;
+  std::vector<int> degrees; // the N (= degree + 1) present, ascending
   int D = 0;
   hd::BatchOp *dev_ops[2] = {nullptr, nullptr}; // FP64 / FP32 coefficients
-  // per precision and N: the cost of one DoF in the work split (seconds per
-  // DoF from the autotuner, 1 before it ran) and the most units per task
-  // (0: no limit)
+  // per precision and N: the measured cost of one DoF (seconds, from the
+  // autotuner; 1 before it ran) and the most units per task (0: no limit)
   double weight[2][kMaxN + 1];
   int cap[2][kMaxN + 1];
   DevPlan plan[2];
-  std::vector<hd::BatchTask> host_tasks[2];
+  std::vector<hd::BatchTask> host_tasks[2][kMaxN + 1];
   bool dirty[2] = {true, true};
+  // one stream per degree: the degrees' kernels run concurrently (forked
+  // from and joined into the caller's stream with events)
+  hipStream_t side[kMaxN + 1] = {};
+  hipEvent_t ev_fork = nullptr, ev_join[kMaxN + 1] = {};
 };
 
 struct hd_graph
 {
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
-  void *plan = nullptr; // the graph's own copy of the work split
+  void *plan = nullptr; // the graph's own copy of the work splits
 };
 
 namespace
 {
-// The work split of the operators in `which` (all if empty) over 8 x per
-// blocks.  Operators are cut into fragments of contiguous logical units, no
-// more than about a sixteenth of the total cost each (cost: weight[N] per
-// DoF), and the fragments go to the XCDs, largest first, each to the least
-// loaded one.  An XCD's blocks walk its fragments the way the single-operator
-// kernel walks its range: interleaved, unit u of the XCD's concatenated list
-// on block u mod per, so that the blocks of an XCD work on a window of
-// neighbouring cells (their face neighbours in that XCD's L2) and each block
-// restarts its load pipeline once per fragment.  Tasks hold at most cap[N]
-// units when a cap is set.
-void make_tasks(const hd_batch *b, int pi, const std::vector<int> &which,
-                int per, std::vector<hd::BatchTask> &tasks,
+// The work split of the operators of degree N (N = degree + 1) over 8 x per
+// blocks of that degree's kernel.  Operators are cut into fragments of
+// contiguous logical units, no more than about a sixteenth of the degree's
+// DoFs each, and the fragments go to the XCDs, largest first, each to the
+// least loaded one.  An XCD's blocks walk its fragments the way the
+// single-operator kernel walks its range: interleaved, unit u of the XCD's
+// concatenated list on block u mod per, so that the blocks of an XCD work on
+// a window of neighbouring cells (their face neighbours in that XCD's L2)
+// and each block restarts its load pipeline once per fragment.  Tasks hold
+// at most cap units when cap > 0.
+void make_tasks(const hd_batch *b, int pi, int N, int per, int cap,
+                std::vector<hd::BatchTask> &tasks,
                 std::vector<int> &block_start)
 {
   struct Frag
   {
-    int op, N;
+    int op;
     long long lb, le;
     double cost;
   };
   std::vector<Frag> frags;
+  const int cpb = hd::collapsed_batch_cells_per_block(b->D, N, pi);
   double total = 0.0;
   for (int i = 0; i < (int)b->ops.size(); ++i)
-    {
-      if (!which.empty() &&
-          std::find(which.begin(), which.end(), i) == which.end())
-        continue;
-      const hd_matrix_free *mf = b->ops[i]->mf;
-      const int cpb = hd::collapsed_batch_cells_per_block(b->D, mf->nd, pi);
-      const long long nl = (mf->mesh.n_cells + cpb - 1) / cpb;
-      total += b->weight[pi][mf->nd] * (double)nl * cpb * mf->dofs_cell;
-    }
+    if (b->ops[i]->mf->nd == N)
+      total += (double)b->ops[i]->mf->n_dofs;
   const double frag_cost = total / 16.0;
   for (int i = 0; i < (int)b->ops.size(); ++i)
     {
-      if (!which.empty() &&
-          std::find(which.begin(), which.end(), i) == which.end())
-        continue;
       const hd_matrix_free *mf = b->ops[i]->mf;
-      const int N = mf->nd;
-      const int cpb = hd::collapsed_batch_cells_per_block(b->D, N, pi);
+      if (mf->nd != N)
+        continue;
       const long long nl = (mf->mesh.n_cells + cpb - 1) / cpb;
-      const double uc = b->weight[pi][N] * (double)cpb * mf->dofs_cell;
+      const double uc = (double)cpb * mf->dofs_cell;
       const long long nf = std::max(
         1LL, std::min(nl, (long long)std::ceil(nl * uc / frag_cost - 1e-9)));
       for (long long f = 0; f < nf; ++f)
         {
           const long long lb = nl * f / nf, le = nl * (f + 1) / nf;
-          frags.push_back({i, N, lb, le, (double)(le - lb) * uc});
+          frags.push_back({i, lb, le, (double)(le - lb) * uc});
         }
     }
   // largest first, each to the least loaded XCD
@@ -132,14 +129,10 @@ void make_tasks(const hd_batch *b, int pi, const std::vector<int> &which,
   block_start.assign(8 * (size_t)per + 1, 0);
   for (int x = 0; x < 8; ++x)
     {
-      // this XCD's fragments by degree, then operator and position
       std::vector<int> mine;
       for (size_t f = 0; f < frags.size(); ++f)
         if (owner[f] == x)
           mine.push_back((int)f);
-      std::stable_sort(mine.begin(), mine.end(), [&](int p, int q) {
-        return frags[p].N < frags[q].N;
-      });
       for (int j = 0; j < per; ++j)
         {
           block_start[(size_t)x * per + j] = (int)tasks.size();
@@ -148,13 +141,11 @@ void make_tasks(const hd_batch *b, int pi, const std::vector<int> &which,
             {
               const Frag &fr = frags[f];
               const long long n = fr.le - fr.lb;
-              // the fragment's first unit on block j
               const long long k0 = ((j - offset) % per + per) % per;
               offset += n;
               if (k0 >= n)
                 continue;
               const long long n_mine = (n - 1 - k0) / per + 1;
-              const int cap = b->cap[pi][fr.N];
               const long long piece = cap > 0 ? cap : n_mine;
               for (long long m = 0; m < n_mine; m += piece)
                 {
@@ -177,28 +168,50 @@ void free_plan(DevPlan &p)
   p = DevPlan{};
 }
 
-int upload_plan(const std::vector<hd::BatchTask> &tasks,
-                const std::vector<int> &block_start, int per, DevPlan &p)
+// the splits of every degree present (caps from b->cap unless `caps`)
+int build_plans(hd_batch *b, int pi, const int *caps, DevPlan &p,
+                std::vector<hd::BatchTask> *host_tasks)
 {
   free_plan(p);
-  const size_t tb = (tasks.size() * sizeof(hd::BatchTask) + 15) / 16 * 16;
-  p.bytes = tb + block_start.size() * sizeof(int);
-  hipError_t e = hipMalloc(&p.mem, p.bytes);
-  if (e == hipSuccess && !tasks.empty())
-    e = hipMemcpy(p.mem, tasks.data(), tasks.size() * sizeof(hd::BatchTask),
-                  hipMemcpyHostToDevice);
-  if (e == hipSuccess)
-    e = hipMemcpy(static_cast<char *>(p.mem) + tb, block_start.data(),
-                  block_start.size() * sizeof(int), hipMemcpyHostToDevice);
+  std::vector<hd::BatchTask> tasks[kMaxN + 1];
+  std::vector<int> starts[kMaxN + 1];
+  int per[kMaxN + 1] = {};
+  size_t bytes = 0;
+  for (int N : b->degrees)
+    {
+      per[N] = hd::collapsed_batch_blocks_per_xcd(N, pi);
+      make_tasks(b, pi, N, per[N], caps ? caps[N] : b->cap[pi][N], tasks[N],
+                 starts[N]);
+      bytes += (tasks[N].size() * sizeof(hd::BatchTask) + 15) / 16 * 16;
+      bytes += (starts[N].size() * sizeof(int) + 15) / 16 * 16;
+    }
+  hipError_t e = hipMalloc(&p.mem, std::max<size_t>(bytes, 16));
+  size_t off = 0;
+  char *base = static_cast<char *>(p.mem);
+  for (int N : b->degrees)
+    {
+      if (e != hipSuccess)
+        break;
+      const size_t tb = tasks[N].size() * sizeof(hd::BatchTask);
+      if (tb)
+        e = hipMemcpy(base + off, tasks[N].data(), tb, hipMemcpyHostToDevice);
+      p.tasks[N] = reinterpret_cast<const hd::BatchTask *>(base + off);
+      off += (tb + 15) / 16 * 16;
+      const size_t sb = starts[N].size() * sizeof(int);
+      if (e == hipSuccess)
+        e = hipMemcpy(base + off, starts[N].data(), sb, hipMemcpyHostToDevice);
+      p.plan[N].block_start = reinterpret_cast<const int *>(base + off);
+      p.plan[N].per = per[N];
+      off += (sb + 15) / 16 * 16;
+      if (host_tasks)
+        host_tasks[N] = tasks[N];
+    }
   if (e != hipSuccess)
     {
       free_plan(p);
       return hip_fail(e, "batch work split");
     }
-  p.tasks = static_cast<const hd::BatchTask *>(p.mem);
-  p.plan.block_start =
-    reinterpret_cast<const int *>(static_cast<char *>(p.mem) + tb);
-  p.plan.per = per;
+  p.bytes = bytes;
   return ok();
 }
 
@@ -206,18 +219,53 @@ int ensure_tasks(hd_batch *b, int pi)
 {
   if (!b->dirty[pi])
     return ok();
-  const int per = hd::collapsed_batch_blocks_per_xcd(pi);
-  std::vector<int> block_start;
-  make_tasks(b, pi, {}, per, b->host_tasks[pi], block_start);
-  const int rc = upload_plan(b->host_tasks[pi], block_start, per, b->plan[pi]);
+  const int rc = build_plans(b, pi, nullptr, b->plan[pi], b->host_tasks[pi]);
   if (rc != HD_OK)
     return rc;
   b->dirty[pi] = false;
   return ok();
 }
 
-// the vectors of an LSRK step: distinct stage registers, 16-byte aligned
-// (the batched copy and the stage kernels use 16-byte accesses)
+// the same splits at another address (a graph's copy): pointers rebased
+DevPlan rebase(const hd_batch *b, const DevPlan &p, void *mem)
+{
+  DevPlan q = p;
+  q.mem = mem;
+  auto move = [&](const void *ptr) {
+    return static_cast<char *>(mem) +
+           (static_cast<const char *>(ptr) - static_cast<const char *>(p.mem));
+  };
+  for (int N : b->degrees)
+    {
+      q.tasks[N] = reinterpret_cast<const hd::BatchTask *>(move(p.tasks[N]));
+      q.plan[N].block_start =
+        reinterpret_cast<const int *>(move(p.plan[N].block_start));
+    }
+  return q;
+}
+
+// fn(N, s) for every degree present, the degrees on their own streams
+// forked from and joined into `stream` (one degree: on `stream` itself)
+template <class F>
+hipError_t for_degrees(hd_batch *b, hipStream_t stream, F fn)
+{
+  if (b->degrees.size() == 1)
+    return fn(b->degrees[0], stream);
+  hipError_t e = hipEventRecord(b->ev_fork, stream);
+  for (int N : b->degrees)
+    {
+      if (e == hipSuccess)
+        e = hipStreamWaitEvent(b->side[N], b->ev_fork, 0);
+      if (e == hipSuccess)
+        e = fn(N, b->side[N]);
+      if (e == hipSuccess)
+        e = hipEventRecord(b->ev_join[N], b->side[N]);
+      if (e == hipSuccess)
+        e = hipStreamWaitEvent(stream, b->ev_join[N], 0);
+    }
+  return e;
+}
+
 bool step_vectors_ok(const hd_batch *b, double *const *sol,
                      void *const *ti_a, void *const *ti_b)
 {
@@ -239,11 +287,11 @@ bool valid_ptrs(const hd_batch *b, const void *const *p)
   return true;
 }
 
-// the step: copy sol into ti_a, then the stages ping-ponging ti_a / ti_b
+// one LSRK step: the FP64 -> precision copy of every solution, then per
+// degree (concurrently) the stages of that degree's operators
 int batch_step(hd_batch *b, const char *type, int precision,
                double *const *sol, void *const *ti_a, void *const *ti_b,
-               double dt, const hd::BatchTask *tasks,
-               const hd::BatchPlan &plan, hipStream_t stream)
+               double dt, const DevPlan &plan, hipStream_t stream)
 {
   double a[16], bb[16];
   const int n_stages = hd_lsrk_coefficients(type, a, bb);
@@ -257,31 +305,31 @@ int batch_step(hd_batch *b, const char *type, int precision,
       p.sol[i] = sol[i];
       p.dst[i] = ti_a[i];
     }
-  // the first stage reads a copy of the solution (time_integrators.
-  // templates.h:142-157)
-  hipError_t e =
-    hd::batch_convert(pi, n, b->n_dofs.data(), p, stream);
+  hipError_t e = hd::batch_convert(pi, n, b->n_dofs.data(), p, stream);
   if (e != hipSuccess)
     return hip_fail(e, "batch copy");
-  bool in_a = true;
-  for (int s = 0; s < n_stages; ++s)
-    {
-      const bool last = s == n_stages - 1;
-      for (int i = 0; i < n; ++i)
-        {
-          p.src[i] = in_a ? ti_a[i] : ti_b[i];
-          p.dst[i] = nullptr;
-          p.ti[i] = last ? nullptr : (in_a ? ti_b[i] : ti_a[i]);
-        }
-      p.a_dt = last ? 0.0 : a[s] * dt;
-      p.b_dt = bb[s] * dt;
-      e = hd::collapsed_batch(b->D, pi, last ? 2 : 1, b->dev_ops[pi], tasks,
-                              plan, p, stream);
-      if (e != hipSuccess)
-        return hip_fail(e, "batch stage");
-      in_a = !in_a;
-    }
-  return ok();
+  e = for_degrees(b, stream, [&](int N, hipStream_t s) {
+    hd::BatchPtrs q = p;
+    bool in_a = true;
+    hipError_t err = hipSuccess;
+    for (int st = 0; st < n_stages && err == hipSuccess; ++st)
+      {
+        const bool last = st == n_stages - 1;
+        for (int i = 0; i < n; ++i)
+          {
+            q.src[i] = in_a ? ti_a[i] : ti_b[i];
+            q.dst[i] = nullptr;
+            q.ti[i] = last ? nullptr : (in_a ? ti_b[i] : ti_a[i]);
+          }
+        q.a_dt = last ? 0.0 : a[st] * dt;
+        q.b_dt = bb[st] * dt;
+        err = hd::collapsed_batch(b->D, N, pi, last ? 2 : 1, b->dev_ops[pi],
+                                  plan.tasks[N], plan.plan[N], q, s);
+        in_a = !in_a;
+      }
+    return err;
+  });
+  return e == hipSuccess ? ok() : hip_fail(e, "batch stage");
 }
 } // namespace
 
@@ -321,6 +369,10 @@ int hd_batch_create(hd_advection *const *ops, int n_ops, hd_batch **out)
     {
       b->ops.push_back(ops[i]);
       b->n_dofs.push_back(ops[i]->mf->n_dofs);
+      const int N = ops[i]->mf->nd;
+      if (std::find(b->degrees.begin(), b->degrees.end(), N) ==
+          b->degrees.end())
+        b->degrees.push_back(N);
       for (int p = 0; p < 2; ++p)
         {
           hd::BatchOp o{};
@@ -328,22 +380,34 @@ int hd_batch_create(hd_advection *const *ops, int n_ops, hd_batch **out)
                           : (const void *)ops[i]->coef32;
           o.mesh = ops[i]->mf->mesh;
           o.inflow = ops[i]->inflow;
-          o.N = ops[i]->mf->nd;
+          o.N = N;
           o.n_dofs = ops[i]->mf->n_dofs;
           table[p].push_back(o);
         }
     }
-  for (int p = 0; p < 2; ++p)
+  std::sort(b->degrees.begin(), b->degrees.end());
+  hipError_t e = hipSuccess;
+  for (int p = 0; p < 2 && e == hipSuccess; ++p)
     {
-      hipError_t e = hipMalloc(&b->dev_ops[p], n_ops * sizeof(hd::BatchOp));
+      e = hipMalloc(&b->dev_ops[p], n_ops * sizeof(hd::BatchOp));
       if (e == hipSuccess)
         e = hipMemcpy(b->dev_ops[p], table[p].data(),
                       n_ops * sizeof(hd::BatchOp), hipMemcpyHostToDevice);
-      if (e != hipSuccess)
-        {
-          hd_batch_destroy(b);
-          return hip_fail(e, "hd_batch_create");
-        }
+    }
+  // default-priority streams and events of the per-degree fork / join
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
+  for (int N : b->degrees)
+    {
+      if (e == hipSuccess)
+        e = hipStreamCreateWithFlags(&b->side[N], hipStreamNonBlocking);
+      if (e == hipSuccess)
+        e = hipEventCreateWithFlags(&b->ev_join[N], hipEventDisableTiming);
+    }
+  if (e != hipSuccess)
+    {
+      hd_batch_destroy(b);
+      return hip_fail(e, "hd_batch_create");
     }
   *out = b;
   return ok();
@@ -353,11 +417,24 @@ int hd_batch_destroy(hd_batch *batch)
 {
   if (!batch)
     return ok();
+  // work queued on the side streams must finish before they go
+  for (int N = 0; N <= kMaxN; ++N)
+    if (batch->side[N])
+      (void)hipStreamSynchronize(batch->side[N]);
   for (int p = 0; p < 2; ++p)
     {
       (void)hipFree(batch->dev_ops[p]);
       free_plan(batch->plan[p]);
     }
+  for (int N = 0; N <= kMaxN; ++N)
+    {
+      if (batch->side[N])
+        (void)hipStreamDestroy(batch->side[N]);
+      if (batch->ev_join[N])
+        (void)hipEventDestroy(batch->ev_join[N]);
+    }
+  if (batch->ev_fork)
+    (void)hipEventDestroy(batch->ev_fork);
   delete batch;
   return ok();
 }
@@ -387,10 +464,12 @@ int hd_batch_apply(hd_batch *batch, const void *const *src, void *const *dst,
       p.src[i] = src[i];
       p.dst[i] = dst[i];
     }
+  const DevPlan &pl = batch->plan[pi];
   const hipError_t e =
-    hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi],
-                        batch->plan[pi].tasks, batch->plan[pi].plan, p,
-                        as_stream(stream));
+    for_degrees(batch, as_stream(stream), [&](int N, hipStream_t s) {
+      return hd::collapsed_batch(batch->D, N, pi, 0, batch->dev_ops[pi],
+                                 pl.tasks[N], pl.plan[N], p, s);
+    });
   return e == hipSuccess ? ok() : hip_fail(e, "hd_batch_apply");
 }
 
@@ -412,8 +491,7 @@ int hd_batch_lsrk_step(hd_batch *batch, const char *type, int precision,
   if (rc != HD_OK)
     return rc;
   return batch_step(batch, type, precision, sol, ti_a, ti_b, dt,
-                    batch->plan[pi].tasks, batch->plan[pi].plan,
-                    as_stream(stream));
+                    batch->plan[pi], as_stream(stream));
 }
 
 int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
@@ -424,7 +502,6 @@ int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
       (precision != HD_FP64 && precision != HD_FP32))
     return fail(HD_ERR_INVALID, "hd_batch_autotune: invalid argument");
   const int pi = prec_index(precision);
-  const int per = hd::collapsed_batch_blocks_per_xcd(pi);
   hipStream_t s = as_stream(stream);
   hd::BatchPtrs p{};
   for (size_t i = 0; i < batch->ops.size(); ++i)
@@ -438,76 +515,63 @@ int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
     e = hipEventCreate(&t1);
   if (e != hipSuccess)
     return hip_fail(e, "hd_batch_autotune");
-  // milliseconds of one apply of the operators in `which` under the current
-  // weights (one warm-up, the mean of three)
-  auto time_split = [&](const std::vector<int> &which, float &ms) -> int {
-    std::vector<hd::BatchTask> tasks;
-    std::vector<int> block_start;
-    make_tasks(batch, pi, which, per, tasks, block_start);
-    DevPlan dp;
-    int rc = upload_plan(tasks, block_start, per, dp);
-    if (rc != HD_OK)
-      return rc;
-    hipError_t err = hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi],
-                                         dp.tasks, dp.plan, p, s);
-    if (err == hipSuccess)
-      err = hipEventRecord(t0, s);
-    for (int r = 0; r < 3 && err == hipSuccess; ++r)
-      err = hd::collapsed_batch(batch->D, pi, 0, batch->dev_ops[pi], dp.tasks,
-                                dp.plan, p, s);
-    if (err == hipSuccess)
-      err = hipEventRecord(t1, s);
-    if (err == hipSuccess)
-      err = hipEventSynchronize(t1);
-    if (err == hipSuccess)
-      err = hipEventElapsedTime(&ms, t0, t1);
-    free_plan(dp);
-    ms /= 3.0f;
-    return err == hipSuccess ? ok() : hip_fail(err, "hd_batch_autotune");
-  };
+  // per degree, the task size (units per task; 0: a block's whole share of
+  // a fragment) that applies that degree's operators fastest: one warm-up,
+  // then the mean of three applies
+  constexpr int kCaps[] = {0, 1, 4, 16};
+  int best_cap[kMaxN + 1] = {};
   int rc = ok();
-  // 1) each degree alone: its time per DoF is its cost weight
-  std::vector<int> degrees;
-  for (int N = 2; N <= kMaxN && rc == HD_OK; ++N)
+  for (int N : batch->degrees)
     {
-      std::vector<int> which;
+      float best = -1.0f;
       double dofs = 0.0;
       for (size_t i = 0; i < batch->ops.size(); ++i)
         if (batch->ops[i]->mf->nd == N)
-          {
-            which.push_back((int)i);
-            dofs += (double)batch->ops[i]->mf->n_dofs;
-          }
-      if (which.empty())
-        continue;
-      float ms = 0.0f;
-      rc = time_split(which, ms);
-      if (rc == HD_OK)
+          dofs += (double)batch->ops[i]->mf->n_dofs;
+      for (int cap : kCaps)
         {
-          batch->weight[pi][N] = std::max(1e-3 * ms / dofs, 1e-18);
-          degrees.push_back(N);
-        }
-    }
-  // 2) the whole batch: scale each degree's weight while that helps
-  if (rc == HD_OK && degrees.size() > 1)
-    {
-      float best = 0.0f;
-      rc = time_split({}, best);
-      for (int N : degrees)
-        for (double f : kRefine)
-          {
-            if (rc != HD_OK)
+          if (rc != HD_OK)
+            break;
+          int caps[kMaxN + 1] = {};
+          caps[N] = cap;
+          DevPlan dp;
+          rc = build_plans(batch, pi, caps, dp, nullptr);
+          if (rc != HD_OK)
+            break;
+          hipError_t err = hd::collapsed_batch(batch->D, N, pi, 0,
+                                               batch->dev_ops[pi],
+                                               dp.tasks[N], dp.plan[N], p, s);
+          if (err == hipSuccess)
+            err = hipEventRecord(t0, s);
+          for (int r = 0; r < 3 && err == hipSuccess; ++r)
+            err = hd::collapsed_batch(batch->D, N, pi, 0, batch->dev_ops[pi],
+                                      dp.tasks[N], dp.plan[N], p, s);
+          if (err == hipSuccess)
+            err = hipEventRecord(t1, s);
+          if (err == hipSuccess)
+            err = hipEventSynchronize(t1);
+          float ms = 0.0f;
+          if (err == hipSuccess)
+            err = hipEventElapsedTime(&ms, t0, t1);
+          free_plan(dp);
+          if (err != hipSuccess)
+            {
+              rc = hip_fail(err, "hd_batch_autotune");
               break;
-            const double w0 = batch->weight[pi][N];
-            batch->weight[pi][N] = w0 * f;
-            float ms = 0.0f;
-            rc = time_split({}, ms);
-            if (rc == HD_OK && ms < best)
+            }
+          ms /= 3.0f;
+          if (best < 0.0f || ms < best)
+            {
               best = ms;
-            else
-              batch->weight[pi][N] = w0;
-          }
+              best_cap[N] = cap;
+            }
+        }
+      if (rc == HD_OK)
+        batch->weight[pi][N] = std::max(1e-3 * best / dofs, 1e-18);
     }
+  if (rc == HD_OK)
+    for (int N : batch->degrees)
+      batch->cap[pi][N] = best_cap[N];
   (void)hipEventDestroy(t0);
   (void)hipEventDestroy(t1);
   batch->dirty[pi] = true;
@@ -524,23 +588,24 @@ int hd_batch_cells_per_task(const hd_batch *batch, int degree, int precision)
   const int cpb = hd::collapsed_batch_cells_per_block(batch->D, N, pi);
   if (cpb <= 0)
     return fail(HD_ERR_UNSUPPORTED, "no batch kernel for this degree");
-  // the mean task of this degree in the current work split
+  if (std::find(batch->degrees.begin(), batch->degrees.end(), N) ==
+      batch->degrees.end())
+    return (batch->cap[pi][N] > 0 ? batch->cap[pi][N] : 1) * cpb;
   std::vector<hd::BatchTask> fresh;
-  const std::vector<hd::BatchTask> *tasks = &batch->host_tasks[pi];
+  const std::vector<hd::BatchTask> *tasks = &batch->host_tasks[pi][N];
   if (batch->dirty[pi])
     {
       std::vector<int> block_start;
-      make_tasks(batch, pi, {}, hd::collapsed_batch_blocks_per_xcd(pi), fresh,
-                 block_start);
+      make_tasks(batch, pi, N, hd::collapsed_batch_blocks_per_xcd(N, pi),
+                 batch->cap[pi][N], fresh, block_start);
       tasks = &fresh;
     }
   long long units = 0, count = 0;
   for (const hd::BatchTask &t : *tasks)
-    if (batch->ops[t.op]->mf->nd == N)
-      {
-        units += (t.le - t.lb + t.stride - 1) / t.stride;
-        ++count;
-      }
+    {
+      units += (t.le - t.lb + t.stride - 1) / t.stride;
+      ++count;
+    }
   if (count == 0)
     return (batch->cap[pi][N] > 0 ? batch->cap[pi][N] : 1) * cpb;
   return (int)std::max(1LL, units * cpb / count);
@@ -584,19 +649,11 @@ int hd_batch_graph_create(hd_batch *batch, const char *type, int precision,
   auto *g = new (std::nothrow) hd_graph();
   if (!g)
     return fail(HD_ERR_NOMEM, "out of host memory");
-  // the graph keeps its own work split: later tuning does not change it
   const DevPlan &bp = batch->plan[pi];
-  hipError_t e = hipMalloc(&g->plan, bp.bytes);
-  if (e == hipSuccess)
+  hipError_t e = hipMalloc(&g->plan, std::max<size_t>(bp.bytes, 16));
+  if (e == hipSuccess && bp.bytes)
     e = hipMemcpy(g->plan, bp.mem, bp.bytes, hipMemcpyDeviceToDevice);
-  const char *base = static_cast<const char *>(g->plan);
-  const auto *gtasks = reinterpret_cast<const hd::BatchTask *>(
-    base + (reinterpret_cast<const char *>(bp.tasks) -
-            static_cast<const char *>(bp.mem)));
-  hd::BatchPlan gplan = bp.plan;
-  gplan.block_start = reinterpret_cast<const int *>(
-    base + (reinterpret_cast<const char *>(bp.plan.block_start) -
-            static_cast<const char *>(bp.mem)));
+  const DevPlan gp = rebase(batch, bp, g->plan);
   hipStream_t s = nullptr;
   if (e == hipSuccess)
     e = hipStreamCreateWithFlags(&s, hipStreamNonBlocking);
@@ -605,11 +662,12 @@ int hd_batch_graph_create(hd_batch *batch, const char *type, int precision,
       hd_graph_destroy(g);
       return hip_fail(e, "hd_batch_graph_create");
     }
+  // the per-degree streams join the capture through the fork / join events:
+  // the graph holds one branch per degree
   e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess)
     {
-      rc = batch_step(batch, type, precision, sol, ti_a, ti_b, dt, gtasks,
-                      gplan, s);
+      rc = batch_step(batch, type, precision, sol, ti_a, ti_b, dt, gp, s);
       const hipError_t e2 = hipStreamEndCapture(s, &g->graph);
       if (rc == HD_OK && e2 != hipSuccess)
         rc = hip_fail(e2, "hd_batch_graph_create: capture");

@@ -36,7 +36,9 @@ int choose_algorithm(const hd_advection *op, int precision)
   switch (op->algorithm)
     {
       case HD_ALGO_FCL:
-        return !op->vp && hd::fcl_supported(mf->D, mf->nd, mf->nq)
+        // deformed: the inverse mass matrix uses the operator's points
+        return !op->vp && hd::fcl_supported(mf->D, mf->nd, mf->nq) &&
+                   (!mf->deformed || mf->nq == mf->nd)
                  ? HD_ALGO_FCL
                  : -1;
       case HD_ALGO_COLLAPSED:
@@ -155,14 +157,14 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
                           static_cast<double *>(asmv),
                           static_cast<double *>(dst), op->quad, op->mass,
                           mf->mesh, stage, time, mf->dim_x, mf->collocation,
-                          stream);
+                          op->geo, stream);
       else
         e = hd::fcl_apply_f32(mf->D, mf->nd, mf->nq, mode,
                               static_cast<const float *>(src),
                               static_cast<float *>(asmv),
                               static_cast<float *>(dst), op->quad, op->mass,
                               mf->mesh, stage, time, mf->dim_x,
-                              mf->collocation, stream);
+                              mf->collocation, op->geo, stream);
     }
   else if (algo == HD_ALGO_COLLAPSED)
     {
@@ -216,14 +218,15 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
                              static_cast<const double *>(src),
                              static_cast<double *>(dst), op->quad, mf->mesh,
                              stage, time, mf->dim_x, mf->collocation,
-                             op->vp ? op->efield : nullptr, stream);
+                             op->vp ? op->efield : nullptr, op->geo, stream);
   else
     e = hd::quadrature_apply_f32(mf->D, mf->nd, mf->nq, mode,
                                  static_cast<const float *>(src),
                                  static_cast<float *>(dst), op->quad,
                                  mf->mesh, stage, time, mf->dim_x,
                                  mf->collocation,
-                                 op->vp ? op->efield : nullptr, stream);
+                                 op->vp ? op->efield : nullptr, op->geo,
+                                 stream);
   if (e != hipSuccess)
     return hip_fail(e, "kernel launch");
   return ok();
@@ -353,9 +356,81 @@ int hd_matrix_free_destroy(hd_matrix_free *mf)
 {
   if (!mf)
     return ok();
+  if (mf->n_ops > 0)
+    return fail(HD_ERR_INVALID,
+                "hd_matrix_free_destroy: operators of this object are alive");
   if (mf->partial)
     (void)hipFree(mf->partial);
+  if (mf->geo_jw)
+    (void)hipFree(mf->geo_jw);
   delete mf;
+  return ok();
+}
+
+int hd_matrix_free_set_deformation(hd_matrix_free *mf, double deformation_x,
+                                   int frequency_x, double deformation_v,
+                                   int frequency_v)
+{
+  if (!mf || frequency_x < 1 || frequency_v < 1)
+    return fail(HD_ERR_INVALID, "hd_matrix_free_set_deformation: invalid "
+                                "argument");
+  if (mf->n_ops > 0)
+    return fail(HD_ERR_INVALID,
+                "hd_matrix_free_set_deformation: operators already exist on "
+                "this object (deform the mesh before creating them)");
+  if (mf->geo_jw)
+    (void)hipFree(mf->geo_jw);
+  mf->geo_jw = nullptr;
+  mf->geo = hd::DeformDev{};
+  mf->deformed = deformation_x != 0.0 || deformation_v != 0.0;
+  if (!mf->deformed)
+    return ok();
+  hd::SpaceMap *maps[2] = {&mf->xmap, &mf->vmap};
+  const int b[2] = {0, mf->dim_x}, dims[2] = {mf->dim_x, mf->dim_v};
+  const double delta[2] = {deformation_x, deformation_v};
+  const int freq[2] = {frequency_x, frequency_v};
+  std::vector<double> jw[2], unused;
+  for (int s = 0; s < 2; ++s)
+    {
+      hd::SpaceMap &m = *maps[s];
+      m.dim = dims[s];
+      for (int j = 0; j < m.dim; ++j)
+        {
+          const int d = b[s] + j;
+          m.n[j] = mf->mesh.n[d];
+          m.lo[j] = mf->mesh.lo[d];
+          m.h[j] = mf->mesh.h[d];
+          m.hi[j] = mf->mesh.lo[d] + mf->mesh.n[d] * mf->mesh.h[d];
+        }
+      m.delta = delta[s];
+      m.freq = freq[s];
+      hd::space_tables(m, mf->nq, mf->tables.qpoints.data(),
+                       mf->tables.qweights.data(), nullptr, jw[s], unused);
+    }
+  const size_t nx = jw[0].size(), nv = jw[1].size();
+  hipError_t e = hipMalloc(&mf->geo_jw, (nx + nv) * sizeof(double));
+  if (e == hipSuccess)
+    e = hipMemcpy(mf->geo_jw, jw[0].data(), nx * sizeof(double),
+                  hipMemcpyHostToDevice);
+  if (e == hipSuccess)
+    e = hipMemcpy(mf->geo_jw + nx, jw[1].data(), nv * sizeof(double),
+                  hipMemcpyHostToDevice);
+  if (e != hipSuccess)
+    {
+      mf->deformed = false;
+      return hip_fail(e, "hd_matrix_free_set_deformation");
+    }
+  hd::DeformDev &g = mf->geo;
+  g.on = 1;
+  g.dim_x = mf->dim_x;
+  g.dim_v = mf->dim_v;
+  g.delta[0] = deformation_x;
+  g.delta[1] = deformation_v;
+  g.freq[0] = frequency_x;
+  g.freq[1] = frequency_v;
+  g.n_x_cells = mf->xmap.n_cells();
+  g.xjw = mf->geo_jw;
+  g.vjw = mf->geo_jw + nx;
   return ok();
 }
 
@@ -395,8 +470,8 @@ int hd_advection_create(hd_matrix_free *mf, const double *velocity,
       op->bc_params[d] = bc_params ? bc_params[d] : velocity[d];
     }
 
-  // collapsed form: periodic in every direction
-  bool all_periodic = true;
+  // collapsed form: periodic in every direction, Cartesian
+  bool all_periodic = !mf->deformed;
   for (int d = 0; d < D; ++d)
     all_periodic = all_periodic && mf->periodic[d];
   op->split = !hd::collapsed_supported(D, nd) &&
@@ -474,6 +549,43 @@ int hd_advection_create(hd_matrix_free *mf, const double *velocity,
       std::copy(tm->Sinv.begin(), tm->Sinv.end(), m.Sinv);
       std::copy(tm->qweights.begin(), tm->qweights.end(), m.w);
     }
+
+  // deformed mesh: J^{-1} a and JxW at the cell points, (a . n) JxW_face
+  // at the face points, per space
+  if (mf->deformed)
+    {
+      std::vector<double> xc, xf, vc, vf;
+      const auto &t = mf->tables;
+      hd::space_tables(mf->xmap, nq, t.qpoints.data(), t.qweights.data(),
+                       op->a, xc, xf);
+      hd::space_tables(mf->vmap, nq, t.qpoints.data(), t.qweights.data(),
+                       op->a + mf->dim_x, vc, vf);
+      const size_t n = xc.size() + xf.size() + vc.size() + vf.size();
+      hipError_t e = hipMalloc(&op->geo_tab, n * sizeof(double));
+      size_t off = 0;
+      double *parts[4] = {nullptr, nullptr, nullptr, nullptr};
+      const std::vector<double> *src[4] = {&xc, &xf, &vc, &vf};
+      for (int i = 0; i < 4 && e == hipSuccess; ++i)
+        {
+          parts[i] = op->geo_tab + off;
+          e = hipMemcpy(parts[i], src[i]->data(),
+                        src[i]->size() * sizeof(double),
+                        hipMemcpyHostToDevice);
+          off += src[i]->size();
+        }
+      if (e != hipSuccess)
+        {
+          hd_advection_destroy(op);
+          return hip_fail(e, "hd_advection_create: deformed-mesh tables");
+        }
+      op->geo = mf->geo;
+      op->geo.xca = parts[0];
+      op->geo.xfg = parts[1];
+      op->geo.vca = parts[2];
+      op->geo.vfg = parts[3];
+    }
+  mf->n_ops += 1;
+  op->counted = true;
   *out = op;
   return ok();
 }
@@ -486,9 +598,11 @@ int hd_advection_destroy(hd_advection *op)
                   (void *)op->rho, (void *)op->phi, (void *)op->work,
                   (void *)op->partial, (void *)op->stats,
                   (void *)op->ptab, (void *)op->vtab, (void *)op->rho_part,
-                  op->split_scratch})
+                  op->split_scratch, (void *)op->geo_tab})
     if (p)
       (void)hipFree(p);
+  if (op->mf && op->mf->n_ops > 0 && op->counted)
+    op->mf->n_ops -= 1;
   hd_abi::learned_release(op->learned);
   if (op->fdm)
     {
@@ -512,6 +626,9 @@ int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
   for (int d = 0; d < mf->D; ++d)
     if (!mf->periodic[d])
       return fail(HD_ERR_UNSUPPORTED, "Vlasov-Poisson needs a periodic box");
+  if (mf->deformed)
+    return fail(HD_ERR_UNSUPPORTED,
+                "Vlasov-Poisson runs on Cartesian meshes only");
   if (mf->nq != mf->nd || !hd::quadrature_supported(mf->D, mf->nd, mf->nq))
     return fail(HD_ERR_UNSUPPORTED,
                 "Vlasov-Poisson runs the quadrature kernel: n_points must "
@@ -998,8 +1115,8 @@ int hd_interpolate(hd_matrix_free *mf, int function, const double *params,
   f.nd = mf->nd;
   f.dim_x = mf->dim_x;
   const long long n = mf->n_dofs;
-  const hipError_t e = hd::launch_interpolate(dst, mf->mesh, f, time, n,
-                                              mf->dofs_cell, as_stream(stream));
+  const hipError_t e = hd::launch_interpolate(
+    dst, mf->mesh, f, time, n, mf->dofs_cell, mf->geo, as_stream(stream));
   return e == hipSuccess ? ok() : hip_fail(e, "hd_interpolate");
 }
 
@@ -1060,7 +1177,7 @@ int hd_norm_and_error(hd_matrix_free *mf, const double *src, int function,
   const size_t lds = (2 * (size_t)vol + 512) * sizeof(double);
   if (lds > 160 * 1024)
     return fail(HD_ERR_UNSUPPORTED, "cell too large for hd_norm_and_error");
-  e = hd::launch_norm(src, mf->mesh, nt, f, time, vol, mf->partial,
+  e = hd::launch_norm(src, mf->mesh, nt, f, time, vol, mf->partial, mf->geo,
                       as_stream(stream));
   if (e != hipSuccess)
     return hip_fail(e, "hd_norm_and_error");
@@ -1126,14 +1243,26 @@ double hd_critical_time_step(const hd_matrix_free *mf, const double *velocity)
   if (!mf || !velocity)
     return -1.0;
   double vx = 0.0, vv = 0.0;
-  for (int d = 0; d < mf->D; ++d)
+  if (mf->deformed)
     {
-      const double v = std::fabs(velocity[d] / mf->mesh.h[d]);
-      if (d < mf->dim_x)
-        vx = std::max(vx, v);
-      else
-        vv = std::max(vv, v);
+      // cfl.h:56-90: the largest |(J^{-T} a)_j| at the Gauss points
+      // (n_points) of each space's cells
+      hd::Tables1D g;
+      if (!g.build(mf->degree, mf->nq, false))
+        return -1.0;
+      vx = hd::space_max_speed(mf->xmap, mf->nq, g.qpoints.data(), velocity);
+      vv = hd::space_max_speed(mf->vmap, mf->nq, g.qpoints.data(),
+                               velocity + mf->dim_x);
     }
+  else
+    for (int d = 0; d < mf->D; ++d)
+      {
+        const double v = std::fabs(velocity[d] / mf->mesh.h[d]);
+        if (d < mf->dim_x)
+          vx = std::max(vx, v);
+        else
+          vv = std::max(vv, v);
+      }
   return std::min(1.0 / vx, 1.0 / vv);
 }
 

@@ -30,6 +30,7 @@ struct CnnNet
   uint32_t *pk = nullptr; // BF16 weights, packed per layer (see pack_offset)
   double *loss_dev = nullptr;
   float *act = nullptr; // activations and gradient buffers for act_batch
+  float *part = nullptr; // weight-gradient partial sums per block
   int act_batch = 0;
   int steps = 0; // Adam steps taken
 

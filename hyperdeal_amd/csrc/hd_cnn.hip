@@ -120,6 +120,246 @@ __global__ __launch_bounds__(kTile *kTile) void conv_kernel(
     }
 }
 
+// ---------------------------------------------------------------------------
+// The 32 -> 32 layers on the matrix cores.  A convolution is a GEMM over
+// K = 32 input channels x 9 taps: out[o][pixel] = sum_k W[o][k] in[k][pixel]
+// (in shifted by the tap).  v_mfma_f32_16x16x4_f32 takes FP32 operands, so
+// the BF16 weights (exact in FP32) and the FP32 activations enter unrounded
+// and every product is accumulated in FP32, as in the direct stencils.
+// Operand layout of one 16x16x4 step (64 lanes, l = lane):
+//   A[m][k]: lane m + 16 k supplies A[m][k]   (m < 16, k < 4)
+//   B[k][n]: lane n + 16 k supplies B[k][n]
+//   D[m][n]: lane n + 16 (m / 4) holds D[m][n] in element m % 4
+typedef float mfma_f4 __attribute__((ext_vector_type(4)));
+
+__device__ inline mfma_f4 mfma16x16x4(float a, float b, mfma_f4 c)
+{
+  return __builtin_amdgcn_mfma_f32_16x16x4f32(a, b, c, 0, 0, 0);
+}
+
+constexpr int kMTX = 16, kMTY = 8; // pixels per block (x, y)
+constexpr int kMIT = 8;            // input channels staged per pass
+// LDS strides chosen so that the four 16-lane groups of a read (one channel
+// each) fall on disjoint banks (stride = 16 or 48 mod 64 floats)
+constexpr int kTileStride = 208; // >= (kMTY + 2) * (kMTX + 2) = 180
+constexpr int kWStride = 304;    // >= 9 * 32 = 288
+
+// out = act(bias + conv(in)) for 32 -> 32 channels; grid (W / 16, H / 8,
+// batch), four waves per block, wave w owns rows 2w, 2w + 1 of the block's
+// 16 x 8 pixels and all 32 output channels (2 x 2 tiles of 16 x 16)
+template <bool ACT>
+__global__ __launch_bounds__(256) void conv32_mfma_kernel(
+  const float *__restrict__ in, const uint32_t *__restrict__ w,
+  const float *__restrict__ bias, float *__restrict__ out, int H, int W)
+{
+  constexpr int C = kChannels;
+  constexpr int WPR = C / 2; // packed words per (input channel, tap)
+  __shared__ float tile[kMIT * kTileStride];
+  __shared__ float wl[kMIT * kWStride];
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n16 = lane & 15, k4 = lane >> 4;
+  const int b = blockIdx.z;
+  const int x0 = blockIdx.x * kMTX, y0 = blockIdx.y * kMTY;
+  const long long plane = (long long)H * W;
+  mfma_f4 acc[2][2];
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+      acc[r][t] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (int i0 = 0; i0 < C; i0 += kMIT)
+    {
+      // input channels i0 .. i0 + 7 with a one-pixel periodic halo
+      constexpr int HW = (kMTY + 2) * (kMTX + 2);
+      for (int e = threadIdx.x; e < kMIT * HW; e += 256)
+        {
+          const int i = e / HW, r = e % HW;
+          const int yy = wrap(y0 + r / (kMTX + 2) - 1, H);
+          const int xx = wrap(x0 + r % (kMTX + 2) - 1, W);
+          tile[i * kTileStride + r] =
+            in[((long long)b * C + i0 + i) * plane + (long long)yy * W + xx];
+        }
+      // their weights as FP32: wl[i][tap][o]
+      for (int e = threadIdx.x; e < kMIT * 9 * WPR; e += 256)
+        {
+          const int i = e / (9 * WPR), r = e % (9 * WPR);
+          const int tap = r / WPR, pr = r % WPR;
+          const uint32_t u = w[((i0 + i) * 9 + tap) * WPR + pr];
+          wl[i * kWStride + tap * C + 2 * pr] = bits_to_float(u << 16);
+          wl[i * kWStride + tap * C + 2 * pr + 1] =
+            bits_to_float(u & 0xffff0000u);
+        }
+      __syncthreads();
+#pragma unroll
+      for (int tap = 0; tap < 9; ++tap)
+        {
+          const int dy = tap / 3, dx = tap % 3;
+#pragma unroll
+          for (int q = 0; q < kMIT / 4; ++q)
+            {
+              const int ci = q * 4 + k4;
+              // A = weights (m = output channel), B = pixels (n = x)
+              const float a0 = wl[ci * kWStride + tap * C + n16];
+              const float a1 = wl[ci * kWStride + tap * C + 16 + n16];
+#pragma unroll
+              for (int r = 0; r < 2; ++r)
+                {
+                  const int row = wave * 2 + r;
+                  const float bv =
+                    tile[ci * kTileStride + (row + dy) * (kMTX + 2) + n16 + dx];
+                  acc[r][0] = mfma16x16x4(a0, bv, acc[r][0]);
+                  acc[r][1] = mfma16x16x4(a1, bv, acc[r][1]);
+                }
+            }
+        }
+      __syncthreads();
+    }
+  // lane holds D[o = 16 t + 4 k4 + e][x = n16] of rows 2 wave + r
+#pragma unroll
+  for (int r = 0; r < 2; ++r)
+#pragma unroll
+    for (int t = 0; t < 2; ++t)
+#pragma unroll
+      for (int e = 0; e < 4; ++e)
+        {
+          const int oc = 16 * t + 4 * k4 + e;
+          float v = acc[r][t][e] + (bias ? ((cptr<float>)bias)[oc] : 0.0f);
+          if (ACT)
+            v = v > 0.0f ? v : kSlope * v;
+          const int y = y0 + wave * 2 + r, x = x0 + n16;
+          out[((long long)b * C + oc) * plane + (long long)y * W + x] = v;
+        }
+}
+
+// Weight gradient of a 32 -> 32 layer: gw[o][i][tap] = sum over the batch
+// and pixels of g[o][pixel] in[i][pixel + tap]; a GEMM with M = 32 (o),
+// N = 288 (i, tap), K = pixels.  Block p sums the 16 x 16 pixel tiles p,
+// p + gridDim.x, ... into part[p][o][i][tap] (wave w owns the N tiles
+// w, w + 4, ... of the 18, for both M tiles); the blocks' partials are then
+// added in a fixed order (wgrad_reduce_kernel), so the result is
+// deterministic.
+constexpr int kGStride = 272;   // 256 pixels of one gradient channel, +16
+constexpr int kInStride = 336;  // 18 x 18 = 324 input pixels, 336 = 16 mod 64
+__global__ __launch_bounds__(256) void wgrad32_mfma_kernel(
+  const float *__restrict__ in, const float *__restrict__ g,
+  float *__restrict__ part, int B, int H, int W)
+{
+  constexpr int C = kChannels;
+  constexpr int NT = C * 9 / 16; // 18 N tiles
+  extern __shared__ float sm[];
+  float *gs = sm;                  // [C][kGStride]
+  float *is = sm + C * kGStride;   // [C][kInStride]
+  const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+  const int n16 = lane & 15, k4 = lane >> 4;
+  const int tx = W / 16, ty = H / 16;
+  const long long n_tiles = (long long)B * tx * ty;
+  const long long plane = (long long)H * W;
+  // this lane's B column in every N tile it owns: (input channel, tap)
+  constexpr int NW = (NT + 3) / 4; // N tiles per wave (at most)
+  mfma_f4 acc[2][NW];
+#pragma unroll
+  for (int m = 0; m < 2; ++m)
+#pragma unroll
+    for (int j = 0; j < NW; ++j)
+      acc[m][j] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
+  for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x)
+    {
+      const int b = (int)(t / (tx * ty));
+      const int rem = (int)(t % (tx * ty));
+      const int y0 = (rem / tx) * 16, x0 = (rem % tx) * 16;
+      __syncthreads(); // the previous tile's LDS reads are done
+      for (int e = threadIdx.x; e < C * 256; e += 256)
+        {
+          const int o = e / 256, p = e % 256;
+          gs[o * kGStride + p] = g[((long long)b * C + o) * plane +
+                                   (long long)(y0 + p / 16) * W + x0 + p % 16];
+        }
+      for (int e = threadIdx.x; e < C * 324; e += 256)
+        {
+          const int i = e / 324, r = e % 324;
+          const int yy = wrap(y0 + r / 18 - 1, H), xx = wrap(x0 + r % 18 - 1, W);
+          is[i * kInStride + r] =
+            in[((long long)b * C + i) * plane + (long long)yy * W + xx];
+        }
+      __syncthreads();
+      // K steps of 4 pixels: pixel p = 4 s + k4 of the tile
+      for (int s = 0; s < 64; ++s)
+        {
+          const int p = 4 * s + k4;
+          const int py = p / 16, px = p % 16;
+          // A[m = o][k = pixel] = g[o][pixel]
+          const float a0 = gs[n16 * kGStride + p];
+          const float a1 = gs[(16 + n16) * kGStride + p];
+#pragma unroll
+          for (int j = 0; j < NW; ++j)
+            {
+              const int nt = wave + 4 * j;
+              if (nt < NT)
+                {
+                  // B[k = pixel][n = (i, tap)] = in[i][pixel + tap]
+                  const int col = nt * 16 + n16;
+                  const int i = col / 9, tap = col % 9;
+                  const float bv =
+                    is[i * kInStride + (py + tap / 3) * 18 + px + tap % 3];
+                  acc[0][j] = mfma16x16x4(a0, bv, acc[0][j]);
+                  acc[1][j] = mfma16x16x4(a1, bv, acc[1][j]);
+                }
+            }
+        }
+    }
+  // D[o = 16 m + 4 k4 + e][col = 16 nt + n16] -> part[block][o][col]
+  float *pb = part + (long long)blockIdx.x * C * C * 9;
+#pragma unroll
+  for (int j = 0; j < NW; ++j)
+    {
+      const int nt = wave + 4 * j;
+      if (nt < NT)
+#pragma unroll
+        for (int m = 0; m < 2; ++m)
+#pragma unroll
+          for (int e = 0; e < 4; ++e)
+            pb[(16 * m + 4 * k4 + e) * (C * 9) + nt * 16 + n16] = acc[m][j][e];
+    }
+}
+
+// gw[o][col] = sum over the blocks' partials in block order; gb[o] = sum of
+// g[o] over the batch and pixels (fixed order: a thread per (o, chunk) then
+// a block reduction)
+__global__ __launch_bounds__(256) void wgrad_reduce_kernel(
+  const float *__restrict__ part, int n_part, float *__restrict__ gw,
+  int n_out)
+{
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_out)
+    return;
+  float s = 0.0f;
+  for (int p = 0; p < n_part; ++p)
+    s += part[(long long)p * n_out + e];
+  gw[e] = s;
+}
+
+__global__ __launch_bounds__(256) void bias_grad_kernel(
+  const float *__restrict__ g, float *__restrict__ gb, int B, int C,
+  long long plane)
+{
+  __shared__ float red[256];
+  const int o = blockIdx.x;
+  float s = 0.0f;
+  for (int b = 0; b < B; ++b)
+    for (long long p = threadIdx.x; p < plane; p += 256)
+      s += g[((long long)b * C + o) * plane + p];
+  red[threadIdx.x] = s;
+  __syncthreads();
+  for (int st = 128; st > 0; st >>= 1)
+    {
+      if ((int)threadIdx.x < st)
+        red[threadIdx.x] += red[threadIdx.x + st];
+      __syncthreads();
+    }
+  if (threadIdx.x == 0)
+    gb[o] = red[0];
+}
+
 // g *= leaky-ReLU'(pre-activation); a is the post-activation output, which
 // has the sign of the pre-activation
 __global__ void act_grad_kernel(float *__restrict__ g,
@@ -394,6 +634,15 @@ int CnnNet::n_params() { return param_offset_w(cnn::kLayers); }
 
 namespace
 {
+// blocks of the weight-gradient kernel (each sums a strided set of 16 x 16
+// pixel tiles); fixed by the problem size, so the sums are deterministic
+constexpr int kWgradBlocks = 256;
+int wgrad_blocks(int batch, int H, int W)
+{
+  const long long tiles = (long long)batch * (H / 16) * (W / 16);
+  return (int)std::min<long long>(tiles, kWgradBlocks);
+}
+
 int fwd_words(int l)
 {
   return CnnNet::in_channels(l) * 9 * ((CnnNet::out_channels(l) + 1) / 2);
@@ -476,7 +725,8 @@ void CnnNet::release()
   (void)hipFree(pk);
   (void)hipFree(loss_dev);
   (void)hipFree(act);
-  p = m = v = grad = act = nullptr;
+  (void)hipFree(part);
+  p = m = v = grad = act = part = nullptr;
   pk = nullptr;
   loss_dev = nullptr;
   act_batch = 0;
@@ -504,12 +754,18 @@ hipError_t CnnNet::reserve(int batch)
   if (e != hipSuccess)
     return e;
   (void)hipFree(act);
+  (void)hipFree(part);
   act = nullptr;
+  part = nullptr;
   act_batch = 0;
   // a1, a2, a3 (C channels), out (1), g0, g1 (C channels)
   const size_t per =
     (size_t)H * W * (3 * cnn::kChannels + 1 + 2 * cnn::kChannels);
   e = hipMalloc(&act, per * batch * sizeof(float));
+  // the weight-gradient blocks' partial sums
+  if (e == hipSuccess)
+    e = hipMalloc(&part, (size_t)wgrad_blocks(batch, H, W) * cnn::kChannels *
+                           cnn::kChannels * 9 * sizeof(float));
   if (e == hipSuccess)
     act_batch = batch;
   return e;
@@ -548,6 +804,37 @@ void conv(const float *in, const uint32_t *w, const float *bias, float *out,
                      W);
 }
 
+template <bool ACT>
+void conv32(const float *in, const uint32_t *w, const float *bias, float *out,
+            int batch, int H, int W, hipStream_t s)
+{
+  hipLaunchKernelGGL((cnn::conv32_mfma_kernel<ACT>),
+                     dim3(W / cnn::kMTX, H / cnn::kMTY, batch), dim3(256), 0,
+                     s, in, w, bias, out, H, W);
+}
+
+// the weight and bias gradients of a 32 -> 32 layer on the matrix cores
+hipError_t wgrad32(const float *in, const float *g, float *part, float *gw,
+                   float *gb, int batch, int H, int W, hipStream_t s)
+{
+  constexpr int C = cnn::kChannels;
+  const size_t lds = sizeof(float) * C * (cnn::kGStride + cnn::kInStride);
+  hipError_t e = hipFuncSetAttribute(
+    reinterpret_cast<const void *>(&cnn::wgrad32_mfma_kernel),
+    hipFuncAttributeMaxDynamicSharedMemorySize, (int)lds);
+  if (e != hipSuccess)
+    return e;
+  const int nb = wgrad_blocks(batch, H, W);
+  hipLaunchKernelGGL(cnn::wgrad32_mfma_kernel, dim3(nb), dim3(256), lds, s,
+                     in, g, part, batch, H, W);
+  const int n_out = C * C * 9;
+  hipLaunchKernelGGL(cnn::wgrad_reduce_kernel, dim3((n_out + 255) / 256),
+                     dim3(256), 0, s, part, nb, gw, n_out);
+  hipLaunchKernelGGL(cnn::bias_grad_kernel, dim3(C), dim3(256), 0, s, g, gb,
+                     batch, C, (long long)H * W);
+  return hipGetLastError();
+}
+
 void act_grad(float *g, const float *a, long long n, hipStream_t s)
 {
   const unsigned blocks = (unsigned)std::min<long long>((n + 255) / 256, 8192);
@@ -565,10 +852,10 @@ hipError_t CnnNet::forward(const float *in, float *out, int batch,
   const Bufs b = buffers(act, batch, H, W);
   conv<1, 32, true>(in, pk + pack_offset_fwd(0), p + param_offset_b(0), b.a1,
                     batch, H, W, s);
-  conv<32, 32, true>(b.a1, pk + pack_offset_fwd(1), p + param_offset_b(1),
-                     b.a2, batch, H, W, s);
-  conv<32, 32, true>(b.a2, pk + pack_offset_fwd(2), p + param_offset_b(2),
-                     b.a3, batch, H, W, s);
+  conv32<true>(b.a1, pk + pack_offset_fwd(1), p + param_offset_b(1), b.a2,
+               batch, H, W, s);
+  conv32<true>(b.a2, pk + pack_offset_fwd(2), p + param_offset_b(2), b.a3,
+               batch, H, W, s);
   conv<32, 1, false>(b.a3, pk + pack_offset_fwd(3), p + param_offset_b(3),
                      out, batch, H, W, s);
   return hipGetLastError();
@@ -597,18 +884,18 @@ hipError_t CnnNet::loss_and_grad(const float *in, const float *target,
                      s);
   act_grad(b.g1, b.a3, nC, s);
   // layer 2
-  hipLaunchKernelGGL((cnn::wgrad_kernel<32, 32>), dim3(C * C), dim3(256), 0, s,
-                     b.a2, b.g1, grad + param_offset_w(2),
-                     grad + param_offset_b(2), batch, H, W);
-  conv<32, 32, false>(b.g1, pk + pack_offset_bwd(2), nullptr, b.g0, batch, H,
-                      W, s);
+  e = wgrad32(b.a2, b.g1, part, grad + param_offset_w(2),
+              grad + param_offset_b(2), batch, H, W, s);
+  if (e != hipSuccess)
+    return e;
+  conv32<false>(b.g1, pk + pack_offset_bwd(2), nullptr, b.g0, batch, H, W, s);
   act_grad(b.g0, b.a2, nC, s);
   // layer 1
-  hipLaunchKernelGGL((cnn::wgrad_kernel<32, 32>), dim3(C * C), dim3(256), 0, s,
-                     b.a1, b.g0, grad + param_offset_w(1),
-                     grad + param_offset_b(1), batch, H, W);
-  conv<32, 32, false>(b.g0, pk + pack_offset_bwd(1), nullptr, b.g1, batch, H,
-                      W, s);
+  e = wgrad32(b.a1, b.g0, part, grad + param_offset_w(1),
+              grad + param_offset_b(1), batch, H, W, s);
+  if (e != hipSuccess)
+    return e;
+  conv32<false>(b.g0, pk + pack_offset_bwd(1), nullptr, b.g1, batch, H, W, s);
   act_grad(b.g1, b.a1, nC, s);
   // layer 0 (1 -> C)
   hipLaunchKernelGGL((cnn::wgrad_kernel<1, 32>), dim3(C), dim3(256), 0, s, in,

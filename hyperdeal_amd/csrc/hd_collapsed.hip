@@ -895,146 +895,82 @@ hipError_t dispatch(int D, int N, int mode, const T *src, T *dst,
   return hipErrorInvalidValue;
 }
 
-// ---------------------------------------------------------------------------
-// Batches: independent operators of one dimension D (any supported N) in one
-// persistent launch.  A task is a range of logical units of one operator; a
-// block loops over tasks, switching on the operator's N into the same block
-// code as the single-operator kernel, with one LDS buffer sized for the
-// largest cell.
-// The batch kernel's blocks: one size for every degree of a launch, kBlock
-// threads and two blocks per CU (two waves per SIMD: 256 VGPRs for the 6 x 6
-// tiles of N = 6), each with half of the LDS (FP32: 7 cells at N = 6).  On
-// MI355X, 64 FP32-mixed 2+2 operators of degrees 2..5 stepped 2% faster so
-// than with four blocks of 128 threads (a quarter of the LDS each).
-template <typename T>
-constexpr int kBatchBlock = kBlock;
-template <typename T>
-constexpr int kBatchBlocksPerCu = 2;
-template <typename T>
-constexpr int kBatchWavesPerEu =
-  (kBatchBlocksPerCu<T> * kBatchBlock<T> / 64 + 3) / 4;
-template <int N, typename T>
-using BatchShape = Shape<4, N, T, 1, kBatchBlock<T>, kBatchBlocksPerCu<T>>;
-
-template <int D, typename T, int N>
-constexpr int lds_elems()
-{
-  using S = BatchShape<N, T>;
-  return S::NBUF * S::CPB * S::CELLP;
-}
-
-template <int D, typename T>
-constexpr int batch_lds_elems()
-{
-  static_assert(D == 4, "batches of 2+2 operators");
-  int m = lds_elems<D, T, 2>();
-  m = m > lds_elems<D, T, 3>() ? m : lds_elems<D, T, 3>();
-  m = m > lds_elems<D, T, 4>() ? m : lds_elems<D, T, 4>();
-  m = m > lds_elems<D, T, 5>() ? m : lds_elems<D, T, 5>();
-  m = m > lds_elems<D, T, 6>() ? m : lds_elems<D, T, 6>();
-  return m;
-}
-
-// 64-bit values the caller holds uniform over the block: function arguments
-// arrive in VGPRs, readfirstlane puts them back on the scalar unit
-__device__ inline unsigned long long uniform64(unsigned long long v)
-{
-  const unsigned lo = __builtin_amdgcn_readfirstlane((unsigned)v);
-  const unsigned hi = __builtin_amdgcn_readfirstlane((unsigned)(v >> 32));
-  return (unsigned long long)hi << 32 | lo;
-}
-template <typename P>
-__device__ inline P *uniform_ptr(P *p)
-{
-  return reinterpret_cast<P *>(uniform64(reinterpret_cast<uintptr_t>(p)));
-}
-__device__ inline double uniform_f64(double x)
-{
-  return __builtin_bit_cast(double,
-                            uniform64(__builtin_bit_cast(unsigned long long,
-                                                         x)));
-}
-
-// One task of a batch.  Kept out of line: each degree's block code then gets
-// its own register allocation (inlined into one body, the five degrees
-// spilled where each alone does not); the call costs once per task.  Its
-// arguments arrive in VGPRs and are made uniform again; apply_block types
-// the global pointers itself.
+// Batches: one kernel per degree N in the batch, with the block shape and
+// residency of the single-operator kernel of that degree (a thread owns an
+// N x N tile, the block holds as many cells as LDS and the register budget
+// allow).  A block walks its tasks -- ranges of logical units of one
+// operator of degree N -- in order, each through the same block code as the
+// single-operator kernel, so a batched apply or stage equals the
+// single-operator call bit for bit.  The operator's mesh and inflow sides
+// are read field by field from the device table (uniform scalar loads).
 template <int N, typename T, int MODE>
-__device__ __noinline__ void
-batch_task(T *lds, const T *src, T *dst, const T *coef, const BatchOp *op,
-           StageDev stage, long long lb, long long le, int stride)
-{
-  using S = BatchShape<N, T>;
-  stage.a_dt = uniform_f64(stage.a_dt);
-  stage.b_dt = uniform_f64(stage.b_dt);
-  stage.sol = MODE == 0 ? nullptr : uniform_ptr(stage.sol);
-  stage.ti_out = MODE == 1 ? uniform_ptr(stage.ti_out) : nullptr;
-  stage.scratch = nullptr;
-  stage.rho_part = nullptr;
-  // the operator's mesh and inflow sides, field by field (uniform loads)
-  const gptr<const BatchOp> o = (gptr<const BatchOp>)uniform_ptr(op);
-  MeshDev mesh;
-  InflowDev inflow;
-  mesh.dim = o->mesh.dim;
-  mesh.n_cells = o->mesh.n_cells;
-  mesh.inner = o->mesh.inner;
-#pragma unroll
-  for (int d = 0; d < kMaxDim; ++d)
-    {
-      mesh.n[d] = o->mesh.n[d];
-      mesh.stride[d] = o->mesh.stride[d];
-      mesh.lo[d] = o->mesh.lo[d];
-      mesh.h[d] = o->mesh.h[d];
-      inflow.side[d] = o->inflow.side[d];
-    }
-  lds = uniform_ptr(lds);
-  apply_block<4, N, T, MODE, 1, kBatchBlock<T>, kBatchBlocksPerCu<T>>(
-    lds, lds + S::CPB * S::CELLP, uniform_ptr(src),
-    MODE == 0 ? uniform_ptr(dst) : nullptr, uniform_ptr(coef), mesh, inflow,
-    stage,
-    (long long)uniform64((unsigned long long)lb),
-    (long long)uniform64((unsigned long long)le),
-    __builtin_amdgcn_readfirstlane(stride));
-}
-
-template <int D, typename T, int MODE>
-__global__ __launch_bounds__(kBatchBlock<T>, kBatchWavesPerEu<T>) void
+__global__ __launch_bounds__((Shape<4, N, T, 1>::BLOCK),
+                             (Residency<4, N, T, MODE, 1>::kWavesPerEu)) void
   batch_kernel(const BatchOp *__restrict__ ops,
                const BatchTask *__restrict__ tasks, BatchPlan plan,
                BatchPtrs ptrs)
 {
-  __shared__ __align__(16) T lds[batch_lds_elems<D, T>()];
+  using S = Shape<4, N, T, 1>;
+  __shared__ __align__(16) T U[S::CPB * S::CELLP];
+  __shared__ __align__(16) T R[S::P > 1 ? S::CPB * S::CELLP : 1];
   // this block's tasks: its strided share of each operator fragment its
   // XCD walks
   const int b = (blockIdx.x % 8) * plan.per + blockIdx.x / 8;
   const int t_end = plan.block_start[b + 1];
   for (int t = plan.block_start[b]; t < t_end; ++t)
     {
-      const BatchTask task = tasks[t];
-      const BatchOp *op = ops + task.op;
-      const StageDev stage{ptrs.a_dt, ptrs.b_dt, ptrs.sol[task.op],
-                           ptrs.ti[task.op]};
-      const T *src = static_cast<const T *>(ptrs.src[task.op]);
-      T *dst = static_cast<T *>(ptrs.dst[task.op]);
-      const T *coef = static_cast<const T *>(op->coef);
-      static_assert(D == 4, "batches of 2+2 operators");
-      switch (op->N)
+      const gptr<const BatchTask> tk = (gptr<const BatchTask>)(tasks + t);
+      const int op = tk->op;
+      const gptr<const BatchOp> o = (gptr<const BatchOp>)(ops + op);
+      MeshDev mesh;
+      InflowDev inflow;
+      mesh.dim = o->mesh.dim;
+      mesh.n_cells = o->mesh.n_cells;
+      mesh.inner = o->mesh.inner;
+#pragma unroll
+      for (int d = 0; d < kMaxDim; ++d)
         {
-#define HD_BCASE(NN_)                                                        \
-  case NN_:                                                                  \
-    batch_task<NN_, T, MODE>(lds, src, dst, coef, op, stage, task.lb,        \
-                             task.le, task.stride);                          \
-    break;
-          HD_BCASE(2) HD_BCASE(3) HD_BCASE(4) HD_BCASE(5) HD_BCASE(6)
-#undef HD_BCASE
-          default:
-            break;
+          mesh.n[d] = o->mesh.n[d];
+          mesh.stride[d] = o->mesh.stride[d];
+          mesh.lo[d] = o->mesh.lo[d];
+          mesh.h[d] = o->mesh.h[d];
+          inflow.side[d] = o->inflow.side[d];
         }
-      // the next task may lay out U and R differently: every read of this
-      // task's buffers precedes its first write
+      StageDev stage{};
+      stage.a_dt = ptrs.a_dt;
+      stage.b_dt = ptrs.b_dt;
+      stage.sol = MODE == 0 ? nullptr : ptrs.sol[op];
+      stage.ti_out = MODE == 1 ? ptrs.ti[op] : nullptr;
+      apply_block<4, N, T, MODE, 1>(
+        U, R, static_cast<const T *>(ptrs.src[op]),
+        MODE == 0 ? static_cast<T *>(ptrs.dst[op]) : nullptr,
+        static_cast<const T *>(o->coef), mesh, inflow, stage, tk->lb, tk->le,
+        tk->stride);
+      // the next task starts writing U: every read of this task's buffers
+      // comes first
       lds_barrier();
     }
+}
+
+template <int N, typename T>
+hipError_t batch_launch(int mode, const BatchOp *ops, const BatchTask *tasks,
+                        const BatchPlan &plan, const BatchPtrs &ptrs,
+                        hipStream_t stream)
+{
+  if (plan.per < 1 || !plan.block_start)
+    return hipErrorInvalidValue;
+  auto go = [&](auto kernel) {
+    hipLaunchKernelGGL(kernel, dim3(8 * plan.per),
+                       dim3(Shape<4, N, T, 1>::BLOCK), 0, stream, ops, tasks,
+                       plan, ptrs);
+  };
+  if (mode == 0)
+    go(batch_kernel<N, T, 0>);
+  else if (mode == 1)
+    go(batch_kernel<N, T, 1>);
+  else
+    go(batch_kernel<N, T, 2>);
+  return hipGetLastError();
 }
 
 // dst[op] = (T) sol[op] for every operator: grid row op converts operator
@@ -1063,26 +999,6 @@ __global__ void batch_convert_kernel(BatchPtrs ptrs, BatchSizes sizes)
     }
   if (blockIdx.x == 0 && threadIdx.x < n - 4 * n4)
     dst[4 * n4 + threadIdx.x] = (T)src[4 * n4 + threadIdx.x];
-}
-
-template <typename T>
-hipError_t batch_launch(int D, int mode, const BatchOp *ops,
-                        const BatchTask *tasks, const BatchPlan &plan,
-                        const BatchPtrs &ptrs, hipStream_t stream)
-{
-  if (D != 4 || plan.per < 1 || !plan.block_start)
-    return hipErrorInvalidValue;
-  auto go = [&](auto kernel) {
-    hipLaunchKernelGGL(kernel, dim3(8 * plan.per), dim3(kBatchBlock<T>), 0,
-                       stream, ops, tasks, plan, ptrs);
-  };
-  if (mode == 0)
-    go(batch_kernel<4, T, 0>);
-  else if (mode == 1)
-    go(batch_kernel<4, T, 1>);
-  else
-    go(batch_kernel<4, T, 2>);
-  return hipGetLastError();
 }
 
 } // namespace collapsed
@@ -1117,43 +1033,80 @@ bool collapsed_batch_supported(int D, int N)
   return D == 4 && N >= 2 && N <= 6;
 }
 
+namespace
+{
+template <typename T>
+int batch_cpb(int N)
+{
+  using collapsed::Shape;
+  switch (N)
+    {
+      case 2:
+        return Shape<4, 2, T, 1>::CPB;
+      case 3:
+        return Shape<4, 3, T, 1>::CPB;
+      case 4:
+        return Shape<4, 4, T, 1>::CPB;
+      case 5:
+        return Shape<4, 5, T, 1>::CPB;
+      default:
+        return Shape<4, 6, T, 1>::CPB;
+    }
+}
+
+// resident blocks per CU of the stage kernel of degree N
+template <typename T>
+int batch_bpc(int N)
+{
+  using collapsed::Residency;
+  switch (N)
+    {
+      case 2:
+        return Residency<4, 2, T, 1, 1>::kBlocks;
+      case 3:
+        return Residency<4, 3, T, 1, 1>::kBlocks;
+      case 4:
+        return Residency<4, 4, T, 1, 1>::kBlocks;
+      case 5:
+        return Residency<4, 5, T, 1, 1>::kBlocks;
+      default:
+        return Residency<4, 6, T, 1, 1>::kBlocks;
+    }
+}
+} // namespace
+
 int collapsed_batch_cells_per_block(int D, int N, int precision)
 {
   if (!collapsed_batch_supported(D, N))
     return 0;
-  using collapsed::BatchShape;
-  const bool f64 = precision == 0;
-  switch (N)
-    {
-      case 2:
-        return f64 ? BatchShape<2, double>::CPB : BatchShape<2, float>::CPB;
-      case 3:
-        return f64 ? BatchShape<3, double>::CPB : BatchShape<3, float>::CPB;
-      case 4:
-        return f64 ? BatchShape<4, double>::CPB : BatchShape<4, float>::CPB;
-      case 5:
-        return f64 ? BatchShape<5, double>::CPB : BatchShape<5, float>::CPB;
-      default:
-        return f64 ? BatchShape<6, double>::CPB : BatchShape<6, float>::CPB;
-    }
+  return precision == 0 ? batch_cpb<double>(N) : batch_cpb<float>(N);
 }
 
-int collapsed_batch_blocks_per_xcd(int precision)
+int collapsed_batch_blocks_per_xcd(int N, int precision)
 {
-  return (precision == 0 ? collapsed::kBatchBlocksPerCu<double>
-                         : collapsed::kBatchBlocksPerCu<float>) *
+  if (N < 2 || N > 6)
+    return 0;
+  return (precision == 0 ? batch_bpc<double>(N) : batch_bpc<float>(N)) *
          collapsed::cus_per_xcd();
 }
 
-hipError_t collapsed_batch(int D, int precision, int mode, const BatchOp *ops,
-                           const BatchTask *tasks, const BatchPlan &plan,
-                           const BatchPtrs &ptrs, hipStream_t stream)
+hipError_t collapsed_batch(int D, int N, int precision, int mode,
+                           const BatchOp *ops, const BatchTask *tasks,
+                           const BatchPlan &plan, const BatchPtrs &ptrs,
+                           hipStream_t stream)
 {
-  return precision == 0
-           ? collapsed::batch_launch<double>(D, mode, ops, tasks, plan, ptrs,
-                                             stream)
-           : collapsed::batch_launch<float>(D, mode, ops, tasks, plan, ptrs,
-                                            stream);
+  if (!collapsed_batch_supported(D, N))
+    return hipErrorInvalidValue;
+#define HD_BN(NN_)                                                           \
+  if (N == NN_)                                                              \
+    return precision == 0                                                    \
+             ? collapsed::batch_launch<NN_, double>(mode, ops, tasks, plan,  \
+                                                    ptrs, stream)            \
+             : collapsed::batch_launch<NN_, float>(mode, ops, tasks, plan,   \
+                                                   ptrs, stream);
+  HD_BN(2) HD_BN(3) HD_BN(4) HD_BN(5) HD_BN(6)
+#undef HD_BN
+  return hipErrorInvalidValue;
 }
 
 hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,

@@ -92,6 +92,41 @@ struct QuadDev
   long long n_x_cells;
 };
 
+// A deformed phase-space mesh (see hd_geometry.h): each space maps
+// x^ -> x^ + delta prod_j sin(freq pi (x^_j - lo_j) / (hi_j - lo_j)) (1..1).
+// The per-point tables are per space (x: [x cell][x point], v likewise):
+//   jw: |det J| w at the cell points (mesh tables)
+//   ca: operator tables [cell][point][dim + 1] = (J^{-1} a, |det J| w)
+//   fg: operator tables [cell][2 dim][face point] = +-(J^{-1} a)_d |det J| w
+// on = 0: the Cartesian mesh (no tables).
+struct DeformDev
+{
+  int on;
+  int dim_x, dim_v;
+  double delta[2];
+  int freq[2];
+  long long n_x_cells;
+  const double *xjw, *vjw;
+  const double *xca, *vca;
+  const double *xfg, *vfg;
+};
+
+// the physical point of the phase-space point x^ (in place)
+__host__ __device__ inline void deform_point(const DeformDev &g,
+                                             const MeshDev &m, double *x)
+{
+  const double pi = 3.14159265358979323846;
+  for (int s = 0; s < 2; ++s)
+    {
+      const int b = s == 0 ? 0 : g.dim_x, e = s == 0 ? g.dim_x : g.dim_x + g.dim_v;
+      double sv = g.delta[s];
+      for (int j = b; j < e; ++j)
+        sv *= sin(g.freq[s] * pi * (x[j] - m.lo[j]) / (m.n[j] * m.h[j]));
+      for (int j = b; j < e; ++j)
+        x[j] += sv;
+    }
+}
+
 // the inverse mass matrix of the face-centric form: S^{-1} of the Gauss
 // (k+1) rule (ND x ND) and its weights on [0,1]; collocation: the
 // Gauss-Lobatto weights, no S
@@ -203,16 +238,18 @@ hipError_t collapsed_apply_f32(int D, int N, int mode, const float *src,
                                float *dst, const float *coef,
                                const MeshDev &mesh, const InflowDev &inflow,
                                const StageDev &stage, hipStream_t stream);
-// batches: dimensions with a batch kernel, the cells one block handles at a
-// time (a task's units are multiples of it), the batched apply / stage
-// (mode as above; ptrs.ti[i] == nullptr selects the last-stage form for
-// every operator) and the batched FP64 -> precision copy of sol into dst
+// batches: one kernel per degree N.  The dimensions and degrees with a
+// batch kernel, the cells one block handles at a time (a task's units are
+// multiples of it), the blocks per XCD of degree N's kernel, the batched
+// apply / stage of the operators of degree N listed in `tasks` (mode as
+// above) and the batched FP64 -> precision copy of sol into dst
 bool collapsed_batch_supported(int D, int N);
 int collapsed_batch_cells_per_block(int D, int N, int precision);
-int collapsed_batch_blocks_per_xcd(int precision);
-hipError_t collapsed_batch(int D, int precision, int mode, const BatchOp *ops,
-                           const BatchTask *tasks, const BatchPlan &plan,
-                           const BatchPtrs &ptrs, hipStream_t stream);
+int collapsed_batch_blocks_per_xcd(int N, int precision);
+hipError_t collapsed_batch(int D, int N, int precision, int mode,
+                           const BatchOp *ops, const BatchTask *tasks,
+                           const BatchPlan &plan, const BatchPtrs &ptrs,
+                           hipStream_t stream);
 hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,
                          const BatchPtrs &ptrs, hipStream_t stream);
 // operators too large for one cell in LDS (D >= 5, N beyond the
@@ -230,18 +267,21 @@ hipError_t collapsed_split_f32(int D, int N, int mode, const float *src,
                                const StageDev &stage, hipStream_t stream);
 // ND = k+1 nodes, NQ quadrature points per direction (NQ = ND or ND + 1)
 bool quadrature_supported(int D, int ND, int NQ);
+// geo: the deformed mesh's tables (geo.on = 0: Cartesian)
 hipError_t quadrature_apply(int D, int ND, int NQ, int mode,
                             const double *src, double *dst, const QuadDev &q,
                             const MeshDev &mesh, const StageDev &stage,
                             double time, int dim_x, int collocation,
-                            const double *efield, hipStream_t stream);
+                            const double *efield, const DeformDev &geo,
+                            hipStream_t stream);
 // the FP32-mixed form: FP32 vectors and contractions, FP64 LSRK update
 hipError_t quadrature_apply_f32(int D, int ND, int NQ, int mode,
                                 const float *src, float *dst,
                                 const QuadDev &q, const MeshDev &mesh,
                                 const StageDev &stage, double time,
                                 int dim_x, int collocation,
-                                const double *efield, hipStream_t stream);
+                                const double *efield, const DeformDev &geo,
+                                hipStream_t stream);
 // the face-centric form (MatrixFree::loop + the inverse-mass cell_loop):
 // cell, face (per direction and colour) and inverse-mass kernels; asmv is
 // the vector the integrals are assembled in (dst for an apply, ti_out or a
@@ -251,15 +291,19 @@ hipError_t fcl_apply(int D, int ND, int NQ, int mode, const double *src,
                      double *asmv, double *dst, const QuadDev &q,
                      const MassDev &md, const MeshDev &mesh,
                      const StageDev &stage, double time, int dim_x,
-                     int collocation, hipStream_t stream);
+                     int collocation, const DeformDev &geo,
+                     hipStream_t stream);
 hipError_t fcl_apply_f32(int D, int ND, int NQ, int mode, const float *src,
                          float *asmv, float *dst, const QuadDev &q,
                          const MassDev &md, const MeshDev &mesh,
                          const StageDev &stage, double time, int dim_x,
-                         int collocation, hipStream_t stream);
+                         int collocation, const DeformDev &geo,
+                         hipStream_t stream);
+// geo (interpolation, norm): the deformed mesh's map and JxW tables
 hipError_t launch_interpolate(double *dst, const MeshDev &mesh,
                               const FuncDev &f, double t, long long n_dofs,
-                              int dofs_cell, hipStream_t stream);
+                              int dofs_cell, const DeformDev &geo,
+                              hipStream_t stream);
 hipError_t launch_noise(double *dst, uint64_t seed, double amplitude,
                         long long n, hipStream_t stream);
 hipError_t launch_convert_f32(const double *src, float *dst, long long n,
@@ -326,9 +370,15 @@ hipError_t vp_fdm_solve(const FdmDev &f, const double *tab, const double *rho,
                         hipStream_t stream);
 hipError_t vp_moments(const double *f, const MeshDev &mesh, const NormDev &nt,
                       int dim_x, int vol, double *partial, hipStream_t stream);
+// blocks <= 0: one block per cell (partial: 2 per cell); otherwise `blocks`
+// blocks striding over the cells (partial: 2 per block)
 hipError_t launch_norm(const double *src, const MeshDev &mesh,
                        const NormDev &nt, const FuncDev &f, double t, int vol,
-                       double *partial, hipStream_t stream);
+                       double *partial, const DeformDev &geo,
+                       hipStream_t stream, int blocks = 0);
+// out[0..1] = sqrt of the sums of n partial pairs (device pointers)
+hipError_t launch_norm_finish(const double *partial, int n, double *out,
+                              hipStream_t stream);
 
 // SplitMix64 output for counter i (state seed + (i+1) golden increments),
 // mapped to [-1, 1)

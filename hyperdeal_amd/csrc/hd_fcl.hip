@@ -67,7 +67,7 @@ __device__ inline QTab<T> tables(const QuadDev &q, T *tabs)
 template <int D, int ND, int NQ, class T>
 __global__ __launch_bounds__(kBlock) void cell_kernel(
   const T *__restrict__ src, T *__restrict__ asmv, QuadDev q, MeshDev mesh,
-  int collocation)
+  int collocation, DeformDev geo)
 {
   using Sh = quad::Shape<D, ND, NQ>;
   constexpr int CELL = Sh::CELL, CELLD = Sh::CELLD, FACE = Sh::FACE;
@@ -129,7 +129,12 @@ __global__ __launch_bounds__(kBlock) void cell_kernel(
           for (int j = 0; j < NQ; ++j)
             {
               uq[j] = u[j * sd];
-              jw[j] = (T)quad::jxw_point<D, NQ>(q, mesh, base + j * sd) * ad;
+              // deformed: the contravariant velocity (J^{-1} a)_d times JxW
+              jw[j] = geo.on ? (T)quad::deformed_velocity_jxw<NQ>(
+                                 geo, cell0 + lc, base + j * sd, d)
+                             : (T)quad::jxw_point<D, NQ>(q, mesh,
+                                                         base + j * sd) *
+                                 ad;
               acc[j] = 0;
             }
           if (q.skew != 0.0)
@@ -292,7 +297,7 @@ template <int D, int ND, int NQ, class T>
 __global__ __launch_bounds__(kBlock) void face_kernel(
   const T *__restrict__ src, T *__restrict__ asmv, QuadDev q, MeshDev mesh,
   int d, int start, int step, int count, long long n_faces, double time,
-  int dim_x, int collocation)
+  int dim_x, int collocation, DeformDev geo)
 {
   using FS = FaceShape<D, ND, NQ, T>;
   constexpr int FACE = FS::FACE, FACED = FS::FACED, FPB = FS::FPB;
@@ -365,10 +370,18 @@ __global__ __launch_bounds__(kBlock) void face_kernel(
           }
       }
       const bool has_m = cm[lf] >= 0, has_p = cp[lf] >= 0;
+      // deformed: (a . n) JxW_face of the cell whose face it is, from the
+      // tables (the face JxW is in it: jf = 1)
+      const int base = quad::line_base<D, NQ>(fi, d);
       if (has_m && has_p)
         {
           const double um = Fm[i], up = Fp[i];
-          const double an = a; // normal of the minus side: +e_d
+          double an = a; // normal of the minus side: +e_d
+          if (geo.on)
+            {
+              an = quad::deformed_face<NQ>(geo, cm[lf], base, d, 1);
+              jf = 1.0;
+            }
           const double flux = 0.5 * ((um + up) * an + fabs(an) * (um - up));
           Fm[i] = (T)(-(flux - q.skew * um * an) * jf);
           Fp[i] = (T)(-(-flux + q.skew * up * an) * jf);
@@ -400,10 +413,18 @@ __global__ __launch_bounds__(kBlock) void face_kernel(
                     }
                   x[j] = mesh.lo[j] + (cj + xi) * mesh.h[j];
                 }
+              if (geo.on)
+                deform_point(geo, mesh, x);
               g = eval_function(q.bc_function, q.bc_params, dim_x, D, x, time);
             }
           const double up = -um + 2.0 * g;
-          const double an = side ? a : -a;
+          double an = side ? a : -a;
+          if (geo.on)
+            {
+              an = quad::deformed_face<NQ>(geo, has_m ? cm[lf] : cp[lf], base,
+                                           d, side);
+              jf = 1.0;
+            }
           const double flux = 0.5 * ((um + up) * an + fabs(an) * (um - up));
           const T v = (T)(-(flux - q.skew * um * an) * jf);
           if (has_m)
@@ -446,7 +467,8 @@ __global__ __launch_bounds__(kBlock) void face_kernel(
 template <int D, int ND, int MODE, class T>
 __global__ __launch_bounds__(kBlock) void mass_kernel(const T *asmv, T *dst,
                                                       MassDev md, MeshDev mesh,
-                                                      StageDev stage)
+                                                      StageDev stage,
+                                                      DeformDev geo)
 {
   constexpr int CELLD = ipow(ND, D);
   constexpr int CPB = CELLD >= 1024 ? 1 : 1024 / CELLD;
@@ -477,12 +499,15 @@ __global__ __launch_bounds__(kBlock) void mass_kernel(const T *asmv, T *dst,
       const int lc = i / CELLD;
       int li = i - lc * CELLD;
       double jxw = 1.0;
+      if (geo.on) // the mass rule is the operator's (ND points)
+        jxw = quad::deformed_jxw<ND>(geo, cell0 + lc, li);
+      else
 #pragma unroll
-      for (int j = 0; j < D; ++j)
-        {
-          jxw *= mesh.h[j] * md.w[li % ND];
-          li /= ND;
-        }
+        for (int j = 0; j < D; ++j)
+          {
+            jxw *= mesh.h[j] * md.w[li % ND];
+            li /= ND;
+          }
       U[lc * CELLP + (i - lc * CELLD)] /= (T)jxw;
     }
   __syncthreads();
@@ -540,7 +565,7 @@ template <int D, int ND, int NQ, class T>
 hipError_t launch(int mode, const T *src, T *asmv, T *dst, const QuadDev &q,
                   const MassDev &md, const MeshDev &mesh,
                   const StageDev &stage, double time, int dim_x,
-                  int collocation, hipStream_t stream)
+                  int collocation, const DeformDev &geo, hipStream_t stream)
 {
   using Sh = quad::Shape<D, ND, NQ>;
   // 1) cells
@@ -554,7 +579,7 @@ hipError_t launch(int mode, const T *src, T *asmv, T *dst, const QuadDev &q,
     const long long grid = (mesh.n_cells + Sh::CPB - 1) / Sh::CPB;
     hipLaunchKernelGGL((cell_kernel<D, ND, NQ, T>), dim3((unsigned)grid),
                        dim3(kBlock), lds, stream, src, asmv, q, mesh,
-                       collocation);
+                       collocation, geo);
     e = hipGetLastError();
     if (e != hipSuccess)
       return e;
@@ -575,7 +600,7 @@ hipError_t launch(int mode, const T *src, T *asmv, T *dst, const QuadDev &q,
           hipLaunchKernelGGL((face_kernel<D, ND, NQ, T>), dim3((unsigned)grid),
                              dim3(kBlock), 0, stream, src, asmv, q, mesh, d,
                              c[k].start, c[k].step, c[k].count, n_faces, time,
-                             dim_x, collocation);
+                             dim_x, collocation, geo);
           const hipError_t e = hipGetLastError();
           if (e != hipSuccess)
             return e;
@@ -587,10 +612,12 @@ hipError_t launch(int mode, const T *src, T *asmv, T *dst, const QuadDev &q,
   const long long grid = (mesh.n_cells + CPB - 1) / CPB;
   if (mode == 0)
     hipLaunchKernelGGL((mass_kernel<D, ND, 0, T>), dim3((unsigned)grid),
-                       dim3(kBlock), 0, stream, asmv, dst, md, mesh, stage);
+                       dim3(kBlock), 0, stream, asmv, dst, md, mesh, stage,
+                       geo);
   else
     hipLaunchKernelGGL((mass_kernel<D, ND, 1, T>), dim3((unsigned)grid),
-                       dim3(kBlock), 0, stream, asmv, dst, md, mesh, stage);
+                       dim3(kBlock), 0, stream, asmv, dst, md, mesh, stage,
+                       geo);
   return hipGetLastError();
 }
 
@@ -598,12 +625,13 @@ template <class T>
 hipError_t dispatch(int D, int ND, int NQ, int mode, const T *src, T *asmv,
                     T *dst, const QuadDev &q, const MassDev &md,
                     const MeshDev &mesh, const StageDev &stage, double time,
-                    int dim_x, int collocation, hipStream_t stream)
+                    int dim_x, int collocation, const DeformDev &geo,
+                    hipStream_t stream)
 {
 #define HD_CASE(DD, ND_, NQ_)                                                \
   if (D == DD && ND == ND_ && NQ == NQ_)                                     \
     return launch<DD, ND_, NQ_, T>(mode, src, asmv, dst, q, md, mesh, stage, \
-                                   time, dim_x, collocation, stream);
+                                   time, dim_x, collocation, geo, stream);
 #define HD_CASE2(DD, ND_) HD_CASE(DD, ND_, ND_) HD_CASE(DD, ND_, ND_ + 1)
   HD_CASE2(2, 2) HD_CASE2(2, 3) HD_CASE2(2, 4) HD_CASE2(2, 5) HD_CASE(2, 6, 6)
   HD_CASE2(3, 2) HD_CASE2(3, 3) HD_CASE2(3, 4) HD_CASE2(3, 5) HD_CASE(3, 6, 6)
@@ -625,19 +653,21 @@ hipError_t fcl_apply(int D, int ND, int NQ, int mode, const double *src,
                      double *asmv, double *dst, const QuadDev &q,
                      const MassDev &md, const MeshDev &mesh,
                      const StageDev &stage, double time, int dim_x,
-                     int collocation, hipStream_t stream)
+                     int collocation, const DeformDev &geo,
+                     hipStream_t stream)
 {
   return fcl::dispatch<double>(D, ND, NQ, mode, src, asmv, dst, q, md, mesh,
-                               stage, time, dim_x, collocation, stream);
+                               stage, time, dim_x, collocation, geo, stream);
 }
 
 hipError_t fcl_apply_f32(int D, int ND, int NQ, int mode, const float *src,
                          float *asmv, float *dst, const QuadDev &q,
                          const MassDev &md, const MeshDev &mesh,
                          const StageDev &stage, double time, int dim_x,
-                         int collocation, hipStream_t stream)
+                         int collocation, const DeformDev &geo,
+                         hipStream_t stream)
 {
   return fcl::dispatch<float>(D, ND, NQ, mode, src, asmv, dst, q, md, mesh,
-                              stage, time, dim_x, collocation, stream);
+                              stage, time, dim_x, collocation, geo, stream);
 }
 } // namespace hd

@@ -5,6 +5,7 @@
 #include "../../include/hyperdeal_amd.h"
 #include "hd_cnn.h"
 #include "hd_device.h"
+#include "hd_geometry.h"
 #include "hd_tables.h"
 
 #include <string>
@@ -63,6 +64,13 @@ struct hd_matrix_free
   long long n_dofs = 0;
   int dofs_cell = 0;
   double *partial = nullptr; // norm partial sums (2 per cell)
+  // deformed mesh: the maps of the two spaces, |det J| w per space, cell and
+  // quadrature point (device, x then v) and their view for the kernels
+  bool deformed = false;
+  hd::SpaceMap xmap, vmap;
+  double *geo_jw = nullptr;
+  hd::DeformDev geo{};
+  int n_ops = 0; // operators created on this object and not destroyed
 };
 
 // a standalone network (hd_cnn_*), also the one inside a learned guess
@@ -110,6 +118,11 @@ struct hd_advection
   hd::InflowDev inflow{};
   hd::QuadDev quad{};
   hd::MassDev mass{};          // inverse mass matrix of the FCL form
+  // deformed mesh: the operator's per-space tables (J^{-1} a and JxW at the
+  // cell points, (a . n) JxW_face at the face points) and their view
+  double *geo_tab = nullptr;
+  hd::DeformDev geo{};
+  bool counted = false; // counted in mf->n_ops
   // Vlasov-Poisson state (velocity (v, grad phi) from the Poisson solve)
   bool vp = false;
   bool vp_auto = true;        // stages recompute the field from their input

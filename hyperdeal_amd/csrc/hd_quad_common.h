@@ -135,5 +135,85 @@ __device__ inline double jxw_point(const QuadDev &q, const MeshDev &m, int i)
   return v;
 }
 
+// ---------------------------------------------------------------------------
+// deformed meshes: the per-point factors from the per-space tables of
+// DeformDev (the Jacobian is block diagonal over x and v, so every factor is
+// an x-space value times a v-space value)
+
+// x and v parts of a point index of the NQ^D cell array
+template <int NQ>
+__device__ inline void split_point(const DeformDev &g, int pt, int &px,
+                                   int &pv, int &nqx, int &nqv)
+{
+  nqx = 1;
+  nqv = 1;
+  for (int j = 0; j < g.dim_x; ++j)
+    nqx *= NQ;
+  for (int j = 0; j < g.dim_v; ++j)
+    nqv *= NQ;
+  px = pt % nqx;
+  pv = pt / nqx;
+}
+
+// |det J| w at cell point pt: the JxW of the deformed cell
+template <int NQ>
+__device__ inline double deformed_jxw(const DeformDev &g, long long cell,
+                                      int pt)
+{
+  int px, pv, nqx, nqv;
+  split_point<NQ>(g, pt, px, pv, nqx, nqv);
+  const long long cx = cell % g.n_x_cells, cv = cell / g.n_x_cells;
+  return g.xca[(cx * nqx + px) * (g.dim_x + 1) + g.dim_x] *
+         g.vca[(cv * nqv + pv) * (g.dim_v + 1) + g.dim_v];
+}
+
+// (J^{-1} a)_d JxW at cell point pt (direction d of phase space)
+template <int NQ>
+__device__ inline double deformed_velocity_jxw(const DeformDev &g,
+                                               long long cell, int pt, int d)
+{
+  int px, pv, nqx, nqv;
+  split_point<NQ>(g, pt, px, pv, nqx, nqv);
+  const long long cx = cell % g.n_x_cells, cv = cell / g.n_x_cells;
+  const double *xc = g.xca + (cx * nqx + px) * (g.dim_x + 1);
+  const double *vc = g.vca + (cv * nqv + pv) * (g.dim_v + 1);
+  const double jw = xc[g.dim_x] * vc[g.dim_v];
+  return (d < g.dim_x ? xc[d] : vc[d - g.dim_x]) * jw;
+}
+
+// (a . n) JxW_face at the face point of the cell array position `base`
+// (digit d unused) of the face of direction d, side `side`
+template <int NQ>
+__device__ inline double deformed_face(const DeformDev &g, long long cell,
+                                       int base, int d, int side)
+{
+  int px, pv, nqx, nqv;
+  split_point<NQ>(g, base, px, pv, nqx, nqv);
+  const long long cx = cell % g.n_x_cells, cv = cell / g.n_x_cells;
+  const bool in_x = d < g.dim_x;
+  const int dl = in_x ? d : d - g.dim_x;      // direction within its space
+  const int dim = in_x ? g.dim_x : g.dim_v;
+  int p = in_x ? px : pv;
+  // the face point: the space's point digits without digit dl
+  int fp = 0, pw = 1;
+  for (int j = 0; j < dim; ++j)
+    {
+      const int dg = p % NQ;
+      p /= NQ;
+      if (j != dl)
+        {
+          fp += dg * pw;
+          pw *= NQ;
+        }
+    }
+  const int nfp = (in_x ? nqx : nqv) / NQ;
+  const long long c = in_x ? cx : cv;
+  const double gf =
+    (in_x ? g.xfg : g.vfg)[(c * 2 * dim + 2 * dl + side) * nfp + fp];
+  // the other space's JxW at its point
+  const double jw = in_x ? g.vca[(cv * nqv + pv) * (g.dim_v + 1) + g.dim_v]
+                         : g.xca[(cx * nqx + px) * (g.dim_x + 1) + g.dim_x];
+  return gf * jw;
+}
 } // namespace quad
 } // namespace hd

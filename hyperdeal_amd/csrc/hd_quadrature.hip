@@ -35,7 +35,7 @@ template <int D, int ND, int NQ, int MODE, class T>
 __global__ __launch_bounds__(kBlock) void apply_kernel(
   const T *__restrict__ src, T *__restrict__ dst, QuadDev q, MeshDev mesh,
   StageDev stage, double time, int dim_x, int collocation,
-  const double *__restrict__ efield)
+  const double *__restrict__ efield, DeformDev geo)
 {
   using Sh = Shape<D, ND, NQ>;
   constexpr int CELL = Sh::CELL, CELLD = Sh::CELLD, FACE = Sh::FACE;
@@ -111,11 +111,15 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
           for (int j = 0; j < NQ; ++j)
             {
               uq[j] = u[j * sd];
-              // velocity / h folded into the quadrature weight
-              jw[j] = (T)(jxw_point<D, NQ>(q, mesh, base + j * sd) *
-                          point_velocity<D, NQ>(q, mesh, efield, dim_x, d,
-                                                cell0 + lc, base + j * sd) /
-                          mesh.h[d]);
+              // velocity / h folded into the quadrature weight (deformed:
+              // the contravariant velocity (J^{-1} a)_d times JxW)
+              jw[j] = (T)(geo.on ? deformed_velocity_jxw<NQ>(
+                                     geo, cell0 + lc, base + j * sd, d)
+                                 : jxw_point<D, NQ>(q, mesh, base + j * sd) *
+                                     point_velocity<D, NQ>(
+                                       q, mesh, efield, dim_x, d, cell0 + lc,
+                                       base + j * sd) /
+                                     mesh.h[d]);
               acc[j] = 0;
             }
           if (q.skew != 0.0)
@@ -286,14 +290,24 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
                              (cj + (j == d ? (double)side : q.qp[ij])) *
                                mesh.h[j];
                     }
+                  if (geo.on)
+                    deform_point(geo, mesh, x);
                   g = eval_function(q.bc_function, q.bc_params, dim_x, D, x,
                                     time);
                 }
               up = -um + 2.0 * g;
             }
-          const double an =
-            point_velocity<D, NQ>(q, mesh, efield, dim_x, d, cell, base) *
-            nrm;
+          // deformed: (a . n) JxW_face from the tables (the face JxW is in
+          // it, jf = 1)
+          double an;
+          if (geo.on)
+            {
+              an = deformed_face<NQ>(geo, cell, base, d, side);
+              jf = 1.0;
+            }
+          else
+            an = point_velocity<D, NQ>(q, mesh, efield, dim_x, d, cell, base) *
+                 nrm;
           const double flux =
             0.5 * ((um + up) * an + fabs(an) * (um - up));
           const T fd = (T)(-(flux - q.skew * um * an) * jf);
@@ -309,7 +323,8 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   for (int i = tid; i < ncell * CELL; i += kBlock)
     {
       const int lc = i / CELL, li = i - lc * CELL;
-      R[lc * CELLP + li] /= (T)jxw_point<D, NQ>(q, mesh, li);
+      R[lc * CELLP + li] /= (T)(geo.on ? deformed_jxw<NQ>(geo, cell0 + lc, li)
+                                       : jxw_point<D, NQ>(q, mesh, li));
     }
   __syncthreads();
   if (!collocation)
@@ -341,7 +356,7 @@ template <int D, int ND, int NQ, class T>
 hipError_t launch(int mode, const T *src, T *dst, const QuadDev &q,
                   const MeshDev &mesh, const StageDev &stage, double time,
                   int dim_x, int collocation, const double *efield,
-                  hipStream_t stream)
+                  const DeformDev &geo, hipStream_t stream)
 {
   using Sh = Shape<D, ND, NQ>;
   const size_t lds = sizeof(T) *
@@ -358,11 +373,11 @@ hipError_t launch(int mode, const T *src, T *dst, const QuadDev &q,
   if (mode == 0)
     hipLaunchKernelGGL((apply_kernel<D, ND, NQ, 0, T>), dim3((unsigned)grid),
                        dim3(kBlock), lds, stream, src, dst, q, mesh, stage,
-                       time, dim_x, collocation, efield);
+                       time, dim_x, collocation, efield, geo);
   else
     hipLaunchKernelGGL((apply_kernel<D, ND, NQ, 1, T>), dim3((unsigned)grid),
                        dim3(kBlock), lds, stream, src, dst, q, mesh, stage,
-                       time, dim_x, collocation, efield);
+                       time, dim_x, collocation, efield, geo);
   return hipGetLastError();
 }
 } // namespace quad
@@ -374,13 +389,14 @@ template <class T>
 hipError_t dispatch(int D, int ND, int NQ, int mode, const T *src, T *dst,
                     const QuadDev &q, const MeshDev &mesh,
                     const StageDev &stage, double time, int dim_x,
-                    int collocation, const double *efield, hipStream_t stream)
+                    int collocation, const double *efield,
+                    const DeformDev &geo, hipStream_t stream)
 {
 #define HD_CASE(DD, ND_, NQ_)                                                \
   if (D == DD && ND == ND_ && NQ == NQ_)                                     \
     return quad::launch<DD, ND_, NQ_, T>(mode, src, dst, q, mesh, stage,     \
                                          time, dim_x, collocation, efield,   \
-                                         stream);
+                                         geo, stream);
 #define HD_CASE2(DD, ND_) HD_CASE(DD, ND_, ND_) HD_CASE(DD, ND_, ND_ + 1)
   HD_CASE2(2, 2) HD_CASE2(2, 3) HD_CASE2(2, 4) HD_CASE2(2, 5) HD_CASE(2, 6, 6)
   HD_CASE2(3, 2) HD_CASE2(3, 3) HD_CASE2(3, 4) HD_CASE2(3, 5) HD_CASE(3, 6, 6)
@@ -412,10 +428,11 @@ hipError_t quadrature_apply(int D, int ND, int NQ, int mode,
                             const double *src, double *dst, const QuadDev &q,
                             const MeshDev &mesh, const StageDev &stage,
                             double time, int dim_x, int collocation,
-                            const double *efield, hipStream_t stream)
+                            const double *efield, const DeformDev &geo,
+                            hipStream_t stream)
 {
   return dispatch<double>(D, ND, NQ, mode, src, dst, q, mesh, stage, time,
-                          dim_x, collocation, efield, stream);
+                          dim_x, collocation, efield, geo, stream);
 }
 
 hipError_t quadrature_apply_f32(int D, int ND, int NQ, int mode,
@@ -423,9 +440,10 @@ hipError_t quadrature_apply_f32(int D, int ND, int NQ, int mode,
                                 const QuadDev &q, const MeshDev &mesh,
                                 const StageDev &stage, double time,
                                 int dim_x, int collocation,
-                                const double *efield, hipStream_t stream)
+                                const double *efield, const DeformDev &geo,
+                                hipStream_t stream)
 {
   return dispatch<float>(D, ND, NQ, mode, src, dst, q, mesh, stage, time,
-                         dim_x, collocation, efield, stream);
+                         dim_x, collocation, efield, geo, stream);
 }
 } // namespace hd

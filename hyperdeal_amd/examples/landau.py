@@ -46,30 +46,8 @@ sys.path.insert(0, os.path.dirname(os.path.dirname(os.path.dirname(
     os.path.abspath(__file__)))))
 
 
-def theory_rate(k):
-    from scipy.optimize import fsolve
-    from scipy.special import wofz
-
-    def eps(w):
-        z = w / (np.sqrt(2.0) * k)
-        zz = 1j * np.sqrt(np.pi) * wofz(z)
-        return 1.0 + (1.0 + z * zz) / k ** 2
-
-    sol = fsolve(lambda x: [eps(x[0] + 1j * x[1]).real,
-                            eps(x[0] + 1j * x[1]).imag], [1.7, -0.4])
-    return float(sol[0]), float(sol[1])
-
-
-def fit_rate(t, energy, t0=0.5):
-    """gamma from log(E) at the local maxima of E(t) after t0 (E ~ |E|^2
-    decays like exp(2 gamma t))."""
-    t, e = np.asarray(t), np.asarray(energy)
-    idx = [i for i in range(1, len(e) - 1)
-           if e[i] >= e[i - 1] and e[i] > e[i + 1] and t[i] >= t0]
-    if len(idx) < 3:
-        return float("nan"), len(idx)
-    slope = np.polyfit(t[idx], np.log(e[idx]), 1)[0]
-    return float(slope / 2.0), len(idx)
+from hyperdeal_amd.diagnostics import fit_damping_rate as fit_rate  # noqa
+from hyperdeal_amd.diagnostics import landau_theory_rate as theory_rate  # noqa
 
 
 def run(H, torch, name, args, ref_f=None):

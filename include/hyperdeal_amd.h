@@ -106,7 +106,26 @@ int hd_version(int *major, int *minor);
  */
 int hd_matrix_free_create(const hd_mesh_desc *mesh, int degree, int n_points,
                           int collocation, hd_matrix_free **out);
+/* Fails (HD_ERR_INVALID, nothing freed) while operators created on mf are
+ * alive: destroy them first. */
 int hd_matrix_free_destroy(hd_matrix_free *mf);
+/*
+ * Deform the mesh: replaces subdivided_hyper_rectangle(...,
+ * with_internal_deformation = true) with DeformedCubeManifold
+ * (source/grid/grid_generator.cc:110-230, 240-290).  Each space is mapped by
+ *   x^ -> x^ + deformation prod_j sin(frequency pi (x^_j - lo_j) /
+ *                                     (hi_j - lo_j)) (1, ..., 1)
+ * (the reference's defaults: deformation 0.1, frequency 2), used as the
+ * exact map of every cell (the reference interpolates it with its
+ * MappingQ).  Operators then run the quadrature (ECL) or FCL kernels with
+ * per-point Jacobians, JxW and normals; interpolation, norms and the
+ * critical time step use the mapped points.  Call before creating
+ * operators on mf (HD_ERR_INVALID otherwise); 0, 0 restores the Cartesian
+ * mesh.  Not for Vlasov-Poisson; the FCL form needs n_points = degree + 1.
+ */
+int hd_matrix_free_set_deformation(hd_matrix_free *mf, double deformation_x,
+                                   int frequency_x, double deformation_v,
+                                   int frequency_v);
 /* n_dofs, n_cells, dofs per cell: MatrixFree::initialize_dof_vector sizes
  * (matrix_free.templates.h:1367-1413). */
 long long hd_matrix_free_n_dofs(const hd_matrix_free *mf);

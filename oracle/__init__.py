@@ -76,6 +76,12 @@ def load(path=None):
                                        _dp]
     lib.hdo_critical_time_step.restype = ctypes.c_double
     lib.hdo_critical_time_step.argtypes = [ctypes.c_void_p]
+    lib.hdo_set_deformation.argtypes = [ctypes.c_void_p, ctypes.c_double,
+                                        ctypes.c_int, ctypes.c_double,
+                                        ctypes.c_int]
+    lib.hdo_geometry.argtypes = [ctypes.c_void_p,
+                                 ctypes.POINTER(ctypes.c_int), _dp, _dp, _dp,
+                                 _dp]
     lib.hdo_lsrk_step.argtypes = [ctypes.c_void_p, ctypes.c_char_p, _dp, _dp,
                                   _dp, ctypes.c_double, ctypes.c_double,
                                   ctypes.c_int]
@@ -186,6 +192,28 @@ class Problem:
 
     def critical_time_step(self):
         return self.lib.hdo_critical_time_step(self.h)
+
+    def set_deformation(self, deformation_x, frequency_x=2,
+                        deformation_v=None, frequency_v=None):
+        """The deformed mesh of grid_generator.cc:110-230 (exact map)."""
+        dv = deformation_x if deformation_v is None else deformation_v
+        fv = frequency_x if frequency_v is None else frequency_v
+        assert self.lib.hdo_set_deformation(self.h, float(deformation_x),
+                                            int(frequency_x), float(dv),
+                                            int(fv)) == 0
+
+    def geometry(self, cell_index, xi):
+        """(physical point, inverse Jacobian d xi / d x, det d x / d xi) at
+        reference point xi of the cell with the given multi-index."""
+        D = self.D
+        ci = (ctypes.c_int * D)(*cell_index)
+        xi = np.ascontiguousarray(xi, dtype=np.float64)
+        x = np.zeros(D)
+        ji = np.zeros((D, D))
+        det = np.zeros(1)
+        assert self.lib.hdo_geometry(self.h, ci, _ptr(xi), _ptr(x), _ptr(ji),
+                                     _ptr(det)) == 0
+        return x, ji, float(det[0])
 
     def lsrk_step(self, rk, sol, ki, ti, t, dt, ecl=True):
         rc = self.lib.hdo_lsrk_step(self.h, rk.encode(), _ptr(sol), _ptr(ki),

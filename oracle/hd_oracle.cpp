@@ -495,6 +495,13 @@ struct hdo_problem
   std::vector<double> efield; // n_x_cells x nq^dim_x x dim_x
   std::vector<int> qidx;      // nq^D x D multi-indices of quadrature points
 
+  // deformed mesh (grid_generator.cc:110-230, DeformedCubeManifold): each
+  // space s (0 = x, 1 = v) maps x^ -> x^ + deform[s] prod_j sin(freq[s] pi
+  // (x^_j - lo_j) / (hi_j - lo_j)) (1, ..., 1); deform = 0: Cartesian
+  bool deformed = false;
+  double deform[2] = {0.0, 0.0};
+  int freq[2] = {2, 2};
+
   Tables1D t;       // tables of the operator quadrature
   Tables1D tg;      // Gauss(k+1) tables (FCL inverse mass, quad no 1)
   std::vector<double> jxw;     // nq^D
@@ -648,6 +655,150 @@ void scatter_add_face_nodes(const hdo_problem &P, double *dst,
     }
 }
 
+// ---------------------------------------------------------------------------
+// Deformed phase-space geometry (evaluated per point, textbook formulas):
+// the Jacobian J of the cell map xi -> x (block diagonal over the x and v
+// spaces), its inverse by Gauss-Jordan elimination, det J, the physical
+// normal J^{-T} n^ / |J^{-T} n^| and the face measure det J |J^{-T} n^|
+// (the quantities FEEvaluation / FEFaceEvaluation take from deal.II's
+// mapping: fe_evaluation_cell.h:196-271, fe_evaluation_face.h:434-463).
+struct PointGeom
+{
+  double Ji[6][6]; // inverse Jacobian, Ji[i][j] = d xi_i / d x_j
+  double det;
+};
+
+// physical point of the reference point x^ (per direction) of the mesh map
+void map_point(const hdo_problem &P, const double *xh, double *x)
+{
+  for (int s = 0; s < 2; ++s)
+    {
+      const int b = s == 0 ? 0 : P.dim_x, e = s == 0 ? P.dim_x : P.D;
+      double sv = P.deform[s];
+      for (int j = b; j < e; ++j)
+        sv *= std::sin(P.freq[s] * PI * (xh[j] - P.lo[j]) / (P.hi[j] - P.lo[j]));
+      for (int j = b; j < e; ++j)
+        x[j] = xh[j] + sv;
+    }
+}
+
+void point_geometry(const hdo_problem &P, const int *idx, const double *xi,
+                    PointGeom &g)
+{
+  const int D = P.D;
+  double J[6][6] = {};
+  for (int s = 0; s < 2; ++s)
+    {
+      const int b = s == 0 ? 0 : P.dim_x, e = s == 0 ? P.dim_x : P.D;
+      double sn[6], cs[6];
+      for (int j = b; j < e; ++j)
+        {
+          const double xh = P.lo[j] + (idx[j] + xi[j]) * P.h[j];
+          const double arg = P.freq[s] * PI * (xh - P.lo[j]) / (P.hi[j] - P.lo[j]);
+          sn[j] = std::sin(arg);
+          cs[j] = std::cos(arg);
+        }
+      for (int j = b; j < e; ++j)
+        {
+          // d/dx^_j of deform prod_k sin(...)
+          double ds = P.deform[s] * P.freq[s] * PI / (P.hi[j] - P.lo[j]) * cs[j];
+          for (int k = b; k < e; ++k)
+            if (k != j)
+              ds *= sn[k];
+          for (int i = b; i < e; ++i)
+            J[i][j] = ((i == j ? 1.0 : 0.0) + ds) * P.h[j];
+        }
+    }
+  // Gauss-Jordan with partial pivoting on [J | I]
+  double A[6][12];
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < 2 * D; ++j)
+      A[i][j] = j < D ? J[i][j] : (j - D == i ? 1.0 : 0.0);
+  double det = 1.0;
+  for (int c = 0; c < D; ++c)
+    {
+      int piv = c;
+      for (int r = c + 1; r < D; ++r)
+        if (std::fabs(A[r][c]) > std::fabs(A[piv][c]))
+          piv = r;
+      if (piv != c)
+        {
+          for (int j = 0; j < 2 * D; ++j)
+            std::swap(A[c][j], A[piv][j]);
+          det = -det;
+        }
+      const double pv = A[c][c];
+      det *= pv;
+      for (int j = 0; j < 2 * D; ++j)
+        A[c][j] /= pv;
+      for (int r = 0; r < D; ++r)
+        if (r != c && A[r][c] != 0.0)
+          {
+            const double f = A[r][c];
+            for (int j = 0; j < 2 * D; ++j)
+              A[r][j] -= f * A[c][j];
+          }
+    }
+  for (int i = 0; i < D; ++i)
+    for (int j = 0; j < D; ++j)
+      g.Ji[i][j] = A[i][D + j];
+  g.det = det;
+}
+
+// JxW at the cell quadrature point with per-direction indices qi (points
+// xq, weights wq of the given table)
+double cell_jxw(const hdo_problem &P, const int *idx, const int *qi,
+                const std::vector<double> &xq, const std::vector<double> &wq,
+                PointGeom *out = nullptr)
+{
+  double xi[6] = {0, 0, 0, 0, 0, 0};
+  double w = 1.0;
+  for (int j = 0; j < P.D; ++j)
+    {
+      xi[j] = xq[qi[j]];
+      w *= wq[qi[j]];
+    }
+  PointGeom g;
+  point_geometry(P, idx, xi, g);
+  if (out)
+    *out = g;
+  return std::fabs(g.det) * w;
+}
+
+// a . n (outward normal of side `side` of direction d) and the face JxW at
+// face point fq of the cell idx
+void face_geometry(const hdo_problem &P, const int *idx, int d, int side,
+                   int fq, double &an, double &jxwf)
+{
+  double xi[6] = {0, 0, 0, 0, 0, 0};
+  double w = 1.0;
+  int rem = fq;
+  for (int j = 0; j < P.D; ++j)
+    {
+      if (j == d)
+        {
+          xi[j] = side;
+          continue;
+        }
+      const int qj = rem % P.nq;
+      rem /= P.nq;
+      xi[j] = P.t.xq[qj];
+      w *= P.t.wq[qj];
+    }
+  PointGeom g;
+  point_geometry(P, idx, xi, g);
+  // J^{-T} n^ with n^ = +-e_d: component i is Ji[d][i]
+  const double sg = side ? 1.0 : -1.0;
+  double nn = 0.0;
+  for (int i = 0; i < P.D; ++i)
+    nn += g.Ji[d][i] * g.Ji[d][i];
+  nn = std::sqrt(nn);
+  an = 0.0;
+  for (int i = 0; i < P.D; ++i)
+    an += P.a[i] * sg * g.Ji[d][i] / nn;
+  jxwf = std::fabs(g.det) * nn * w;
+}
+
 // Cell integral (a u, grad phi) in quadrature space, including the
 // skew-symmetric part (advection_operation.h:318-403 / 713-787):
 //   data = -skew (a . grad u) JxW + sum_d D_d^T [ (1-skew) u a_d / h_d JxW ]
@@ -685,6 +836,44 @@ void cell_term(const hdo_problem &P, Work &w, const int (*idx)[6] = nullptr)
   for (int j = 0; j < D; ++j)
     ext[j] = nq;
   std::fill(w.data.begin(), w.data.begin() + nvol * L, 0.0);
+  if (P.deformed)
+    {
+      // a . grad v = (J^{-1} a) . grad^ v at every point:
+      //   -skew (a . grad u) JxW + sum_d D_d^T [(1-skew) u (J^{-1} a)_d JxW]
+      std::vector<double> ah(nvol * L * D), jw(nvol * L);
+      for (size_t q = 0; q < nvol; ++q)
+        for (int l = 0; l < L; ++l)
+          {
+            PointGeom g;
+            jw[q * L + l] = cell_jxw(P, idx[l], &P.qidx[q * D], P.t.xq,
+                                     P.t.wq, &g);
+            for (int d = 0; d < D; ++d)
+              {
+                double v = 0.0;
+                for (int i = 0; i < D; ++i)
+                  v += g.Ji[d][i] * P.a[i];
+                ah[(q * L + l) * D + d] = v;
+              }
+          }
+      for (int d = 0; d < D; ++d)
+        {
+          if (P.skew != 0.0)
+            contract(D, ext, d, nq, P.t.D.data(), nq, 1, w.buf.data(),
+                     w.tmp2.data(), false);
+          for (size_t q = 0; q < nvol; ++q)
+            for (int l = 0; l < L; ++l)
+              {
+                const double vel = ah[(q * L + l) * D + d] * jw[q * L + l];
+                if (P.skew != 0.0)
+                  w.data[q * L + l] += -P.skew * w.tmp2[q * L + l] * vel;
+                w.tmp[q * L + l] = (1.0 - P.skew) * w.buf[q * L + l] * vel;
+              }
+          if (P.skew != 1.0)
+            contract(D, ext, d, nq, P.t.D.data(), 1, nq, w.tmp.data(),
+                     w.data.data(), true);
+        }
+      return;
+    }
   if (P.vp)
     {
       // per-point velocity: -skew (a . grad u) JxW + D_d^T[(1-skew) u a_d
@@ -841,15 +1030,27 @@ void ecl_cells(const hdo_problem &P, Work &w, const double *src, double *dst,
               double g = 0.0;
               if (P.bc_kind == HDO_BC_INHOMOGENEOUS)
                 {
-                  double x[6];
-                  face_point(P, idx[l], d, side, q, x);
+                  double xh[6], x[6];
+                  face_point(P, idx[l], d, side, q, xh);
+                  map_point(P, xh, x);
                   g = eval_function(P.func_id, P.func_params, P.dim_x, D, x,
                                     t);
                 }
               up[q * L + l] = -w.fm[q * L + l] + 2.0 * g;
             }
 
-      if (!P.vp)
+      if (P.deformed)
+        {
+          for (int q = 0; q < nfq; ++q)
+            for (int l = 0; l < L; ++l)
+              {
+                double an, jf;
+                face_geometry(P, idx[l], d, side, q, an, jf);
+                w.fd[q * L + l] =
+                  face_value(w.fm[q * L + l], up[q * L + l], an, P.skew, jf);
+              }
+        }
+      else if (!P.vp)
         {
           const double an = P.a[d] * nrm;
           for (int q = 0; q < nfq; ++q)
@@ -884,7 +1085,9 @@ void ecl_cells(const hdo_problem &P, Work &w, const double *src, double *dst,
   // 4) inverse mass matrix: divide by JxW, then Sinv along each direction
   for (size_t q = 0; q < nvol; ++q)
     for (int l = 0; l < L; ++l)
-      w.data[q * L + l] /= P.jxw[q];
+      w.data[q * L + l] /=
+        P.deformed ? cell_jxw(P, idx[l], &P.qidx[q * D], P.t.xq, P.t.wq)
+                   : P.jxw[q];
   for (int j = 0; j < D; ++j)
     ext[j] = nq;
   double *res = w.data.data();
@@ -920,7 +1123,7 @@ void fcl_cell_macro(const hdo_problem &P, Work &w, const double *src,
                        w.uq.data(), false);
   const size_t nvol = (size_t)ipow(nq, D);
   std::copy(uq, uq + nvol * L, w.buf.begin());
-  cell_term(P, w);
+  cell_term(P, w, idx);
   for (int j = 0; j < D; ++j)
     ext[j] = nq;
   double *res = w.data.data();
@@ -997,14 +1200,16 @@ void fcl_faces_macro(const hdo_problem &P, Work &w, const double *src,
       double *up = to_quad(w.fw.data(), w.fp.data());
       if (up != w.fp.data())
         std::copy(up, up + (size_t)nfq * L, w.fp.begin());
-      const double an = P.a[d]; // normal of the minus side is +e_d
       for (int q = 0; q < nfq; ++q)
         for (int l = 0; l < L; ++l)
           {
+            // the normal of the minus side (+e_d on the reference cell)
+            double an = P.a[d], jf = P.jxwf[d][q];
+            if (P.deformed)
+              face_geometry(P, idx[l], d, 1, q, an, jf);
             const double u_m = w.fm[q * L + l], u_p = w.fp[q * L + l];
             const double flux =
               0.5 * ((u_m + u_p) * an + std::fabs(an) * (u_m - u_p));
-            const double jf = P.jxwf[d][q];
             w.fm[q * L + l] = -(flux - P.skew * u_m * an) * jf;
             w.fp[q * L + l] = -(-flux + P.skew * u_p * an) * jf;
           }
@@ -1032,21 +1237,24 @@ void fcl_faces_macro(const hdo_problem &P, Work &w, const double *src,
         double *um = to_quad(w.fw.data(), w.fm.data());
         if (um != w.fm.data())
           std::copy(um, um + (size_t)nfq * L, w.fm.begin());
-        const double an = P.a[d] * (side ? 1.0 : -1.0);
         for (int q = 0; q < nfq; ++q)
           for (int l = 0; l < L; ++l)
             {
+              double an = P.a[d] * (side ? 1.0 : -1.0), jf = P.jxwf[d][q];
+              if (P.deformed)
+                face_geometry(P, idx[l], d, side, q, an, jf);
               double g = 0.0;
               if (P.bc_kind == HDO_BC_INHOMOGENEOUS && vb[l])
                 {
-                  double x[6];
-                  face_point(P, idx[l], d, side, q, x);
+                  double xh[6], x[6];
+                  face_point(P, idx[l], d, side, q, xh);
+                  map_point(P, xh, x);
                   g = eval_function(P.func_id, P.func_params, P.dim_x, D, x,
                                     t);
                 }
               const double u_m = w.fm[q * L + l];
               const double u_p = -u_m + 2.0 * g;
-              w.fd[q * L + l] = face_value(u_m, u_p, an, P.skew, P.jxwf[d][q]);
+              w.fd[q * L + l] = face_value(u_m, u_p, an, P.skew, jf);
             }
         double *r = to_nodal(w.fd.data(), w.fw.data());
         scatter_add_face_nodes(P, dst, cells, vb, d, side ? nd - 1 : 0, r);
@@ -1075,8 +1283,18 @@ void fcl_mass_macro(const hdo_problem &P, Work &w, double *dst, long long m)
     cur = contract_dirs(P, ext, -1, nd, tg.Sinv.data(), 1, nd, w.u.data(),
                         w.uq.data(), false);
   for (size_t q = 0; q < ndofs; ++q)
-    for (int l = 0; l < L; ++l)
-      cur[q * L + l] /= P.jxw_g[q];
+    {
+      int qi[6];
+      int rem = (int)q;
+      for (int j = 0; j < D; ++j)
+        {
+          qi[j] = rem % nd;
+          rem /= nd;
+        }
+      for (int l = 0; l < L; ++l)
+        cur[q * L + l] /=
+          P.deformed ? cell_jxw(P, idx[l], qi, tg.xq, tg.wq) : P.jxw_g[q];
+    }
   double *other = (cur == w.u.data()) ? w.uq.data() : w.u.data();
   if (!P.collocation)
     cur = contract_dirs(P, ext, -1, nd, tg.Sinv.data(), nd, 1, cur, other,
@@ -1696,13 +1914,14 @@ int hdo_interpolate(hdo_problem *p, double t, double *dst)
       P.cell_index(c, idx);
       for (long long i = 0; i < P.dofs_cell; ++i)
         {
-          double x[6];
+          double xh[6], x[6];
           long long rem = i;
           for (int j = 0; j < P.D; ++j)
             {
-              x[j] = P.lo[j] + (idx[j] + P.t.xd[rem % P.nd]) * P.h[j];
+              xh[j] = P.lo[j] + (idx[j] + P.t.xd[rem % P.nd]) * P.h[j];
               rem /= P.nd;
             }
+          map_point(P, xh, x);
           dst[c * P.dofs_cell + i] =
             eval_function(P.func_id, P.func_params, P.dim_x, P.D, x, t);
         }
@@ -1747,18 +1966,23 @@ int hdo_norm_and_error(hdo_problem *p, const double *src, double t,
             {
               if (!valid[l])
                 continue;
-              double x[6];
+              double xh[6], x[6];
               int rem = q;
               for (int j = 0; j < D; ++j)
                 {
-                  x[j] = P.lo[j] + (idx[l][j] + P.t.xq[rem % nq]) * P.h[j];
+                  xh[j] = P.lo[j] + (idx[l][j] + P.t.xq[rem % nq]) * P.h[j];
                   rem /= nq;
                 }
+              map_point(P, xh, x);
+              const double jxw =
+                P.deformed
+                  ? cell_jxw(P, idx[l], &P.qidx[(size_t)q * D], P.t.xq, P.t.wq)
+                  : P.jxw[q];
               const double f = uq[q * L + l];
               const double e =
                 f - eval_function(P.func_id, P.func_params, P.dim_x, D, x, t);
-              s0 += f * f * P.jxw[q];
-              s1 += e * e * P.jxw[q];
+              s0 += f * f * jxw;
+              s1 += e * e * jxw;
             }
       }
   }
@@ -1769,17 +1993,100 @@ int hdo_norm_and_error(hdo_problem *p, const double *src, double t,
 
 double hdo_critical_time_step(const hdo_problem *p)
 {
-  // cfl.h:56-124 on a Cartesian mesh: J^{-T} a = a_d / h_d
+  // cfl.h:56-124: per space, the largest |(J^{-T} a)_j| over its cells and
+  // Gauss points (n_points); Cartesian: a_d / h_d
   double vx = 0.0, vv = 0.0;
-  for (int d = 0; d < p->D; ++d)
+  if (!p->deformed)
+    for (int d = 0; d < p->D; ++d)
+      {
+        const double v = std::fabs(p->a[d] / p->h[d]);
+        if (d < p->dim_x)
+          vx = std::max(vx, v);
+        else
+          vv = std::max(vv, v);
+      }
+  else
     {
-      const double v = std::fabs(p->a[d] / p->h[d]);
-      if (d < p->dim_x)
-        vx = std::max(vx, v);
-      else
-        vv = std::max(vv, v);
+      const hdo_problem &P = *p;
+      std::vector<double> xg, wg;
+      gauss_rule(P.nq, xg, wg);
+      for (int s = 0; s < 2; ++s)
+        {
+          const int b = s == 0 ? 0 : P.dim_x, e = s == 0 ? P.dim_x : P.D;
+          long long nc = 1;
+          int nqs = 1;
+          for (int j = b; j < e; ++j)
+            {
+              nc *= P.n[j];
+              nqs *= P.nq;
+            }
+          double vmax = 0.0;
+          for (long long c = 0; c < nc; ++c)
+            for (int q = 0; q < nqs; ++q)
+              {
+                int idx[6] = {0, 0, 0, 0, 0, 0};
+                double xi[6] = {0.5, 0.5, 0.5, 0.5, 0.5, 0.5};
+                long long cr = c;
+                int qr = q;
+                for (int j = b; j < e; ++j)
+                  {
+                    idx[j] = (int)(cr % P.n[j]);
+                    cr /= P.n[j];
+                    xi[j] = xg[qr % P.nq];
+                    qr /= P.nq;
+                  }
+                PointGeom g;
+                point_geometry(P, idx, xi, g);
+                for (int j = b; j < e; ++j)
+                  {
+                    double v = 0.0;
+                    for (int i = b; i < e; ++i)
+                      v += g.Ji[i][j] * P.a[i];
+                    vmax = std::max(vmax, std::fabs(v));
+                  }
+              }
+          (s == 0 ? vx : vv) = vmax;
+        }
     }
   return std::min(1.0 / vx, 1.0 / vv);
+}
+
+int hdo_geometry(const hdo_problem *p, const int *cell_index,
+                 const double *xi, double *x, double *inverse_jacobian,
+                 double *det)
+{
+  if (!p || !cell_index || !xi)
+    return -1;
+  const hdo_problem &P = *p;
+  if (x)
+    {
+      double xh[6];
+      for (int j = 0; j < P.D; ++j)
+        xh[j] = P.lo[j] + (cell_index[j] + xi[j]) * P.h[j];
+      map_point(P, xh, x);
+    }
+  PointGeom g;
+  point_geometry(P, cell_index, xi, g);
+  if (inverse_jacobian)
+    for (int i = 0; i < P.D; ++i)
+      for (int j = 0; j < P.D; ++j)
+        inverse_jacobian[i * P.D + j] = g.Ji[i][j];
+  if (det)
+    *det = g.det;
+  return 0;
+}
+
+int hdo_set_deformation(hdo_problem *p, double deformation_x, int frequency_x,
+                        double deformation_v, int frequency_v)
+{
+  if (!p || p->vp || frequency_x < 1 || frequency_v < 1)
+    return -1;
+  p->deform[0] = deformation_x;
+  p->deform[1] = deformation_v;
+  p->freq[0] = frequency_x;
+  p->freq[1] = frequency_v;
+  p->deformed = deformation_x != 0.0 || deformation_v != 0.0;
+  return 0;
 }
 
 int hdo_lsrk_n_stages(const char *rk_type)
@@ -1923,6 +2230,8 @@ int hdo_add_noise(const hdo_problem *p, double *dst, unsigned long long seed,
 
 int hdo_set_vlasov_poisson(hdo_problem *p, int enable)
 {
+  if (p && p->deformed && enable)
+    return -1;
   if (!p || (enable && p->dim_x != p->dim_v))
     return -1; // the reference's v-integration assumes dim_x == dim_v
   p->vp = enable != 0;

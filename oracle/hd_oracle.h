@@ -79,6 +79,18 @@ int hdo_norm_and_error(hdo_problem *p, const double *src, double t,
 
 /* dt_crit = min over x and v of 1/max |J^{-T} a|. */
 double hdo_critical_time_step(const hdo_problem *p);
+/* Deformed mesh (grid_generator.cc:110-230, DeformedCubeManifold, applied
+ * as the exact map): space x maps x^ -> x^ + dx prod_j sin(fx pi (x^_j -
+ * lo_j) / (hi_j - lo_j)) (1, ..., 1), space v likewise with (dv, fv);
+ * 0, 0 restores the Cartesian mesh.  Not with Vlasov-Poisson. */
+int hdo_set_deformation(hdo_problem *p, double deformation_x, int frequency_x,
+                        double deformation_v, int frequency_v);
+/* The mesh map at reference point xi of the cell with multi-index
+ * cell_index: physical point x (D), inverse Jacobian d xi_i / d x_j
+ * (D x D, row-major) and det of d x / d xi; any output may be NULL. */
+int hdo_geometry(const hdo_problem *p, const int *cell_index,
+                 const double *xi, double *x, double *inverse_jacobian,
+                 double *det);
 
 /* One low-storage Runge-Kutta step (rk33, rk45, rk47, rk59) of du/dt = op(u),
  * op = apply (ECL when use_ecl else FCL).  ki, ti: caller-owned registers. */
