@@ -57,7 +57,8 @@ EXPORTED = [
     "hd_cnn_n_params", "hd_cnn_create", "hd_cnn_destroy", "hd_cnn_params",
     "hd_cnn_forward", "hd_cnn_loss_grad", "hd_cnn_train_step",
     "hd_cnn_last_loss", "hd_vp_exact_start", "hd_cnn_dofs_to_image",
-    "hd_cnn_image_to_dofs", "hd_matrix_free_set_deformation",
+    "hd_cnn_image_to_dofs", "hd_norm_and_error_noalloc",
+    "hd_matrix_free_set_deformation",
 ]
 BATCH_MAX = 64
 
@@ -127,6 +128,8 @@ def load_library(path=None):
     lib.hd_norm_and_error.argtypes = [_vp, _vp, ctypes.c_int, _dp,
                                       ctypes.c_double, _dp, _vp]
     lib.hd_vector_convert.argtypes = [_vp, _vp, _vp, ctypes.c_int, _vp]
+    lib.hd_norm_and_error_noalloc.argtypes = [_vp, _vp, ctypes.c_int, _dp,
+                                              ctypes.c_double, _vp, _vp]
     lib.hd_critical_time_step.restype = ctypes.c_double
     lib.hd_critical_time_step.argtypes = [_vp, _dp]
     lib.hd_device_synchronize.argtypes = []
@@ -780,6 +783,19 @@ class VectorTools:
                                          function, p, float(time), out,
                                          _stream(stream)), "hd_norm_and_error")
         return out[0], out[1]
+
+    @staticmethod
+    def norm_and_error_noalloc(mf, src, function, out, params=None, time=0.0,
+                               stream=None):
+        """norm_and_error into `out` (a float64 CUDA tensor of two values),
+        stream ordered, with no allocation (hd_norm_and_error_noalloc)."""
+        import torch
+        p = _darr(params) if params is not None else None
+        _check(mf._lib.hd_norm_and_error_noalloc(
+            mf.handle, _ptr(src, torch.float64, mf.n_dofs()), function, p,
+            float(time), _ptr(out, torch.float64, 2), _stream(stream)),
+            "hd_norm_and_error_noalloc")
+        return out
 
     @staticmethod
     def convert(mf, src, dst, precision, stream=None):

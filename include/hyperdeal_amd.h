@@ -381,6 +381,25 @@ int hd_norm_and_error(hd_matrix_free *mf, const double *src, int function,
                       const double *params, double time, double *out,
                       void *stream);
 
+/* hd_norm_and_error for callers that cannot allocate (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc); no
+ * counterpart in the reference.  It calls no allocating routine and does
+ * not synchronise: out_dev is a caller-owned device array of two doubles
+ * that receives (||u||_L2, ||u - f(t)||_L2), stream ordered.  Its scratch
+ * (the norm kernel's block partial sums) is a fixed device array of
+ * 2 x HD_NOALLOC_NORM_BLOCKS doubles (32 KiB) per GPU, held until the
+ * process ends; the kernel's blocks stride over the cells, so meshes of any
+ * size fit.  Cells too large for the kernel's LDS are rejected
+ * (HD_ERR_UNSUPPORTED), as by hd_norm_and_error.  Calls share their GPU's
+ * scratch: calls that overlap on one GPU (different streams, or threads)
+ * race and must be serialised by the caller.  The first call on a device
+ * looks up the scratch's device address (make it outside a graph
+ * capture). */
+#define HD_NOALLOC_NORM_BLOCKS 2048
+int hd_norm_and_error_noalloc(hd_matrix_free *mf, const double *src,
+                              int function, const double *params, double time,
+                              double *out_dev, void *stream);
+
 /* dst (precision) = src (FP64), element-wise conversion / copy. */
 int hd_vector_convert(hd_matrix_free *mf, const double *src, void *dst,
                       int precision, void *stream);
