@@ -50,7 +50,8 @@ enum hdo_boundary
  *   n_cells[D], lo[D], hi[D]: cells per direction and the box.
  *   n_points: 1D quadrature points (Gauss, or Gauss-Lobatto when collocation).
  *   a[D]: constant transport velocity.
- *   func_id/func_params: analytic function for Dirichlet data and errors.
+ *   func_id/func_params: analytic function for Dirichlet data and errors;
+ *   func_params holds D values (3 for HDO_FUNC_LANDAU), or NULL for a.
  */
 hdo_problem *hdo_create(int dim_x, int dim_v, int degree, int n_points,
                         int collocation, const int *n_cells, const double *lo,

@@ -116,7 +116,8 @@ int run(bool collocation, bool periodic, double skew)
         return 1;
       }
 
-  const double fp[1] = {0.0};
+  // the function parameters: one per direction (HDO_FUNC_ZERO ignores them)
+  const double fp[6] = {0.0, 0.0, 0.0, 0.0, 0.0, 0.0};
   hdo_problem *p =
     hdo_create(dim_x, dim_v, degree, n_points, collocation ? 1 : 0, n_cells,
                lo6, hi6, periodic ? 1 : 0, periodic ? 1 : 0, skew, a6,
