@@ -1,6 +1,6 @@
 // hd_batch.hip -- batches of independent advection operators (C ABI):
-// one launch per apply / LSRK stage for all of them, an autotuner for the
-// work split, and hipGraph capture of a whole LSRK step.
+// one launch per degree and apply / LSRK stage for all of them, an
+// autotuner for the task size, and hipGraph capture of a whole LSRK step.
 #include "../../include/hyperdeal_amd.h"
 #include "hd_device.h"
 #include "hd_handles.h"
@@ -19,8 +19,6 @@ using hd_abi::ok;
 namespace
 {
 constexpr int kMaxN = 8;
-// the autotuner's refinement: factors tried on each degree's cost weight
-constexpr double kRefine[] = {0.8, 1.25};
 
 int prec_index(int precision)
 {
@@ -53,10 +51,6 @@ struct hd_batch
   DevPlan plan[2];
   std::vector<hd::BatchTask> host_tasks[2][kMaxN + 1];
   bool dirty[2] = {true, true};
-  // one stream per degree: the degrees' kernels run concurrently (forked
-  // from and joined into the caller's stream with events)
-  hipStream_t side[kMaxN + 1] = {};
-  hipEvent_t ev_fork = nullptr, ev_join[kMaxN + 1] = {};
 };
 
 struct hd_graph
@@ -69,96 +63,53 @@ struct hd_graph
 namespace
 {
 // The work split of the operators of degree N (N = degree + 1) over 8 x per
-// blocks of that degree's kernel.  Operators are cut into fragments of
-// contiguous logical units, no more than about a sixteenth of the degree's
-// DoFs each, and the fragments go to the XCDs, largest first, each to the
-// least loaded one.  An XCD's blocks walk its fragments the way the
-// single-operator kernel walks its range: interleaved, unit u of the XCD's
-// concatenated list on block u mod per, so that the blocks of an XCD work on
-// a window of neighbouring cells (their face neighbours in that XCD's L2)
-// and each block restarts its load pipeline once per fragment.  Tasks hold
-// at most cap units when cap > 0.
+// blocks of that degree's kernel, the single-operator kernel's split for
+// every operator: each operator's logical units are cut into eight
+// contiguous parts, part x on XCD x (so every XCD gets an eighth of every
+// operator: balanced, and its blocks work on neighbouring cells whose faces
+// its L2 holds).  An XCD's blocks walk the concatenation of its parts
+// interleaved, unit u of the list on block u mod per, and each block
+// restarts its load pipeline once per part.  Tasks hold at most cap units
+// when cap > 0.
 void make_tasks(const hd_batch *b, int pi, int N, int per, int cap,
                 std::vector<hd::BatchTask> &tasks,
                 std::vector<int> &block_start)
 {
-  struct Frag
-  {
-    int op;
-    long long lb, le;
-    double cost;
-  };
-  std::vector<Frag> frags;
   const int cpb = hd::collapsed_batch_cells_per_block(b->D, N, pi);
-  double total = 0.0;
-  for (int i = 0; i < (int)b->ops.size(); ++i)
-    if (b->ops[i]->mf->nd == N)
-      total += (double)b->ops[i]->mf->n_dofs;
-  const double frag_cost = total / 16.0;
-  for (int i = 0; i < (int)b->ops.size(); ++i)
-    {
-      const hd_matrix_free *mf = b->ops[i]->mf;
-      if (mf->nd != N)
-        continue;
-      const long long nl = (mf->mesh.n_cells + cpb - 1) / cpb;
-      const double uc = (double)cpb * mf->dofs_cell;
-      const long long nf = std::max(
-        1LL, std::min(nl, (long long)std::ceil(nl * uc / frag_cost - 1e-9)));
-      for (long long f = 0; f < nf; ++f)
-        {
-          const long long lb = nl * f / nf, le = nl * (f + 1) / nf;
-          frags.push_back({i, lb, le, (double)(le - lb) * uc});
-        }
-    }
-  // largest first, each to the least loaded XCD
-  std::vector<int> order(frags.size());
-  for (size_t f = 0; f < frags.size(); ++f)
-    order[f] = (int)f;
-  std::stable_sort(order.begin(), order.end(), [&](int p, int q) {
-    return frags[p].cost > frags[q].cost;
-  });
-  std::vector<int> owner(frags.size(), 0);
-  double load[8] = {0, 0, 0, 0, 0, 0, 0, 0};
-  for (int f : order)
-    {
-      const int x = (int)(std::min_element(load, load + 8) - load);
-      owner[f] = x;
-      load[x] += frags[f].cost;
-    }
   tasks.clear();
   block_start.assign(8 * (size_t)per + 1, 0);
   for (int x = 0; x < 8; ++x)
-    {
-      std::vector<int> mine;
-      for (size_t f = 0; f < frags.size(); ++f)
-        if (owner[f] == x)
-          mine.push_back((int)f);
-      for (int j = 0; j < per; ++j)
-        {
-          block_start[(size_t)x * per + j] = (int)tasks.size();
-          long long offset = 0; // units of the XCD's list before fragment
-          for (int f : mine)
-            {
-              const Frag &fr = frags[f];
-              const long long n = fr.le - fr.lb;
-              const long long k0 = ((j - offset) % per + per) % per;
-              offset += n;
-              if (k0 >= n)
-                continue;
-              const long long n_mine = (n - 1 - k0) / per + 1;
-              const long long piece = cap > 0 ? cap : n_mine;
-              for (long long m = 0; m < n_mine; m += piece)
-                {
-                  hd::BatchTask task{};
-                  task.op = fr.op;
-                  task.stride = per;
-                  task.lb = fr.lb + k0 + m * per;
-                  task.le = std::min(fr.le, task.lb + piece * per);
-                  tasks.push_back(task);
-                }
-            }
-        }
-    }
+    for (int j = 0; j < per; ++j)
+      {
+        block_start[(size_t)x * per + j] = (int)tasks.size();
+        long long offset = 0; // units of the XCD's list before this part
+        for (int i = 0; i < (int)b->ops.size(); ++i)
+          {
+            const hd_matrix_free *mf = b->ops[i]->mf;
+            if (mf->nd != N)
+              continue;
+            const long long nl = (mf->mesh.n_cells + cpb - 1) / cpb;
+            const long long lb = nl * x / 8, le = nl * (x + 1) / 8;
+            const long long n = le - lb;
+            if (n == 0)
+              continue;
+            const long long k0 = ((j - offset) % per + per) % per;
+            offset += n;
+            if (k0 >= n)
+              continue;
+            const long long n_mine = (n - 1 - k0) / per + 1;
+            const long long piece = cap > 0 ? cap : n_mine;
+            for (long long m = 0; m < n_mine; m += piece)
+              {
+                hd::BatchTask task{};
+                task.op = i;
+                task.stride = per;
+                task.lb = lb + k0 + m * per;
+                task.le = std::min(le, task.lb + piece * per);
+                tasks.push_back(task);
+              }
+          }
+      }
   block_start[8 * (size_t)per] = (int)tasks.size();
 }
 
@@ -244,25 +195,17 @@ DevPlan rebase(const hd_batch *b, const DevPlan &p, void *mem)
   return q;
 }
 
-// fn(N, s) for every degree present, the degrees on their own streams
-// forked from and joined into `stream` (one degree: on `stream` itself)
+// fn(N, s) for every degree present, in order on `stream`.  (Forking the
+// degrees onto concurrent streams measured slower on MI355X: 136-140 against
+// 143 GDoF/s for the 64-operator FP32-mixed step -- concurrent kernels no
+// longer get the blocks-to-XCD mapping the work split assumes.)
 template <class F>
 hipError_t for_degrees(hd_batch *b, hipStream_t stream, F fn)
 {
-  if (b->degrees.size() == 1)
-    return fn(b->degrees[0], stream);
-  hipError_t e = hipEventRecord(b->ev_fork, stream);
+  hipError_t e = hipSuccess;
   for (int N : b->degrees)
-    {
-      if (e == hipSuccess)
-        e = hipStreamWaitEvent(b->side[N], b->ev_fork, 0);
-      if (e == hipSuccess)
-        e = fn(N, b->side[N]);
-      if (e == hipSuccess)
-        e = hipEventRecord(b->ev_join[N], b->side[N]);
-      if (e == hipSuccess)
-        e = hipStreamWaitEvent(stream, b->ev_join[N], 0);
-    }
+    if (e == hipSuccess)
+      e = fn(N, stream);
   return e;
 }
 
@@ -288,7 +231,7 @@ bool valid_ptrs(const hd_batch *b, const void *const *p)
 }
 
 // one LSRK step: the FP64 -> precision copy of every solution, then per
-// degree (concurrently) the stages of that degree's operators
+// degree the stages of that degree's operators
 int batch_step(hd_batch *b, const char *type, int precision,
                double *const *sol, void *const *ti_a, void *const *ti_b,
                double dt, const DevPlan &plan, hipStream_t stream)
@@ -394,16 +337,6 @@ int hd_batch_create(hd_advection *const *ops, int n_ops, hd_batch **out)
         e = hipMemcpy(b->dev_ops[p], table[p].data(),
                       n_ops * sizeof(hd::BatchOp), hipMemcpyHostToDevice);
     }
-  // default-priority streams and events of the per-degree fork / join
-  if (e == hipSuccess)
-    e = hipEventCreateWithFlags(&b->ev_fork, hipEventDisableTiming);
-  for (int N : b->degrees)
-    {
-      if (e == hipSuccess)
-        e = hipStreamCreateWithFlags(&b->side[N], hipStreamNonBlocking);
-      if (e == hipSuccess)
-        e = hipEventCreateWithFlags(&b->ev_join[N], hipEventDisableTiming);
-    }
   if (e != hipSuccess)
     {
       hd_batch_destroy(b);
@@ -417,24 +350,11 @@ int hd_batch_destroy(hd_batch *batch)
 {
   if (!batch)
     return ok();
-  // work queued on the side streams must finish before they go
-  for (int N = 0; N <= kMaxN; ++N)
-    if (batch->side[N])
-      (void)hipStreamSynchronize(batch->side[N]);
   for (int p = 0; p < 2; ++p)
     {
       (void)hipFree(batch->dev_ops[p]);
       free_plan(batch->plan[p]);
     }
-  for (int N = 0; N <= kMaxN; ++N)
-    {
-      if (batch->side[N])
-        (void)hipStreamDestroy(batch->side[N]);
-      if (batch->ev_join[N])
-        (void)hipEventDestroy(batch->ev_join[N]);
-    }
-  if (batch->ev_fork)
-    (void)hipEventDestroy(batch->ev_fork);
   delete batch;
   return ok();
 }
@@ -662,8 +582,6 @@ int hd_batch_graph_create(hd_batch *batch, const char *type, int precision,
       hd_graph_destroy(g);
       return hip_fail(e, "hd_batch_graph_create");
     }
-  // the per-degree streams join the capture through the fork / join events:
-  // the graph holds one branch per degree
   e = hipStreamBeginCapture(s, hipStreamCaptureModeThreadLocal);
   if (e == hipSuccess)
     {

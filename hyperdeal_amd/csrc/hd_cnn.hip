@@ -138,7 +138,6 @@ __device__ inline mfma_f4 mfma16x16x4(float a, float b, mfma_f4 c)
 }
 
 constexpr int kMTX = 16, kMTY = 8; // pixels per block (x, y)
-constexpr int kMIT = 8;            // input channels staged per pass
 // LDS strides chosen so that the four 16-lane groups of a read (one channel
 // each) fall on disjoint banks (stride = 16 or 48 mod 64 floats)
 constexpr int kTileStride = 208; // >= (kMTY + 2) * (kMTX + 2) = 180
@@ -146,7 +145,12 @@ constexpr int kWStride = 304;    // >= 9 * 32 = 288
 
 // out = act(bias + conv(in)) for 32 -> 32 channels; grid (W / 16, H / 8,
 // batch), four waves per block, wave w owns rows 2w, 2w + 1 of the block's
-// 16 x 8 pixels and all 32 output channels (2 x 2 tiles of 16 x 16)
+// 16 x 8 pixels and all 32 output channels (2 x 2 tiles of 16 x 16).  The
+// block stages all 32 input channels (with a one-pixel periodic halo) and
+// all weights (as FP32) in LDS once, then runs its 9 x 8 x 4 MFMA steps
+// without further barriers (dynamic LDS: kConvLds bytes).
+constexpr int kConvLds =
+  kChannels * (kTileStride + kWStride) * (int)sizeof(float);
 template <bool ACT>
 __global__ __launch_bounds__(256) void conv32_mfma_kernel(
   const float *__restrict__ in, const uint32_t *__restrict__ w,
@@ -154,65 +158,59 @@ __global__ __launch_bounds__(256) void conv32_mfma_kernel(
 {
   constexpr int C = kChannels;
   constexpr int WPR = C / 2; // packed words per (input channel, tap)
-  __shared__ float tile[kMIT * kTileStride];
-  __shared__ float wl[kMIT * kWStride];
+  constexpr int HW = (kMTY + 2) * (kMTX + 2);
+  extern __shared__ float sm[];
+  float *tile = sm;                 // [C][kTileStride]
+  float *wl = sm + C * kTileStride; // [C][kWStride]: wl[i][tap][o]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n16 = lane & 15, k4 = lane >> 4;
   const int b = blockIdx.z;
   const int x0 = blockIdx.x * kMTX, y0 = blockIdx.y * kMTY;
   const long long plane = (long long)H * W;
+  for (int e = threadIdx.x; e < C * HW; e += 256)
+    {
+      const int i = e / HW, r = e % HW;
+      const int yy = wrap(y0 + r / (kMTX + 2) - 1, H);
+      const int xx = wrap(x0 + r % (kMTX + 2) - 1, W);
+      tile[i * kTileStride + r] =
+        in[((long long)b * C + i) * plane + (long long)yy * W + xx];
+    }
+  for (int e = threadIdx.x; e < C * 9 * WPR; e += 256)
+    {
+      const int i = e / (9 * WPR), r = e % (9 * WPR);
+      const int tap = r / WPR, pr = r % WPR;
+      const uint32_t u = w[e];
+      wl[i * kWStride + tap * C + 2 * pr] = bits_to_float(u << 16);
+      wl[i * kWStride + tap * C + 2 * pr + 1] = bits_to_float(u & 0xffff0000u);
+    }
+  __syncthreads();
   mfma_f4 acc[2][2];
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
     for (int t = 0; t < 2; ++t)
       acc[r][t] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
-  for (int i0 = 0; i0 < C; i0 += kMIT)
+#pragma unroll
+  for (int tap = 0; tap < 9; ++tap)
     {
-      // input channels i0 .. i0 + 7 with a one-pixel periodic halo
-      constexpr int HW = (kMTY + 2) * (kMTX + 2);
-      for (int e = threadIdx.x; e < kMIT * HW; e += 256)
+      const int dy = tap / 3, dx = tap % 3;
+#pragma unroll 2
+      for (int q = 0; q < C / 4; ++q)
         {
-          const int i = e / HW, r = e % HW;
-          const int yy = wrap(y0 + r / (kMTX + 2) - 1, H);
-          const int xx = wrap(x0 + r % (kMTX + 2) - 1, W);
-          tile[i * kTileStride + r] =
-            in[((long long)b * C + i0 + i) * plane + (long long)yy * W + xx];
-        }
-      // their weights as FP32: wl[i][tap][o]
-      for (int e = threadIdx.x; e < kMIT * 9 * WPR; e += 256)
-        {
-          const int i = e / (9 * WPR), r = e % (9 * WPR);
-          const int tap = r / WPR, pr = r % WPR;
-          const uint32_t u = w[((i0 + i) * 9 + tap) * WPR + pr];
-          wl[i * kWStride + tap * C + 2 * pr] = bits_to_float(u << 16);
-          wl[i * kWStride + tap * C + 2 * pr + 1] =
-            bits_to_float(u & 0xffff0000u);
-        }
-      __syncthreads();
+          const int ci = q * 4 + k4;
+          // A = weights (m = output channel), B = pixels (n = x)
+          const float a0 = wl[ci * kWStride + tap * C + n16];
+          const float a1 = wl[ci * kWStride + tap * C + 16 + n16];
 #pragma unroll
-      for (int tap = 0; tap < 9; ++tap)
-        {
-          const int dy = tap / 3, dx = tap % 3;
-#pragma unroll
-          for (int q = 0; q < kMIT / 4; ++q)
+          for (int r = 0; r < 2; ++r)
             {
-              const int ci = q * 4 + k4;
-              // A = weights (m = output channel), B = pixels (n = x)
-              const float a0 = wl[ci * kWStride + tap * C + n16];
-              const float a1 = wl[ci * kWStride + tap * C + 16 + n16];
-#pragma unroll
-              for (int r = 0; r < 2; ++r)
-                {
-                  const int row = wave * 2 + r;
-                  const float bv =
-                    tile[ci * kTileStride + (row + dy) * (kMTX + 2) + n16 + dx];
-                  acc[r][0] = mfma16x16x4(a0, bv, acc[r][0]);
-                  acc[r][1] = mfma16x16x4(a1, bv, acc[r][1]);
-                }
+              const int row = wave * 2 + r;
+              const float bv =
+                tile[ci * kTileStride + (row + dy) * (kMTX + 2) + n16 + dx];
+              acc[r][0] = mfma16x16x4(a0, bv, acc[r][0]);
+              acc[r][1] = mfma16x16x4(a1, bv, acc[r][1]);
             }
         }
-      __syncthreads();
     }
   // lane holds D[o = 16 t + 4 k4 + e][x = n16] of rows 2 wave + r
 #pragma unroll
@@ -236,9 +234,10 @@ __global__ __launch_bounds__(256) void conv32_mfma_kernel(
 // N = 288 (i, tap), K = pixels.  Block p sums the 16 x 16 pixel tiles p,
 // p + gridDim.x, ... into part[p][o][i][tap] (wave w owns the N tiles
 // w, w + 4, ... of the 18, for both M tiles); the blocks' partials are then
-// added in a fixed order (wgrad_reduce_kernel), so the result is
+// added in a fixed order (part_reduce_kernel), so the result is
 // deterministic.
 constexpr int kGStride = 272;   // 256 pixels of one gradient channel, +16
+constexpr int kWgradOut = kChannels * kChannels * 9 + kChannels; // per block
 constexpr int kInStride = 336;  // 18 x 18 = 324 input pixels, 336 = 16 mod 64
 __global__ __launch_bounds__(256) void wgrad32_mfma_kernel(
   const float *__restrict__ in, const float *__restrict__ g,
@@ -262,6 +261,7 @@ __global__ __launch_bounds__(256) void wgrad32_mfma_kernel(
 #pragma unroll
     for (int j = 0; j < NW; ++j)
       acc[m][j] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
+  float bsum = 0.0f;
   for (long long t = blockIdx.x; t < n_tiles; t += gridDim.x)
     {
       const int b = (int)(t / (tx * ty));
@@ -282,6 +282,14 @@ __global__ __launch_bounds__(256) void wgrad32_mfma_kernel(
             in[((long long)b * C + i) * plane + (long long)yy * W + xx];
         }
       __syncthreads();
+      // this thread's share of the bias gradient: channel t / 8, pixels
+      // 32 (t % 8) .. + 31 of the tile
+      {
+        const float *gr = gs + (threadIdx.x / 8) * kGStride + 32 * (threadIdx.x % 8);
+#pragma unroll 8
+        for (int j = 0; j < 32; ++j)
+          bsum += gr[j];
+      }
       // K steps of 4 pixels: pixel p = 4 s + k4 of the tile
       for (int s = 0; s < 64; ++s)
         {
@@ -307,8 +315,22 @@ __global__ __launch_bounds__(256) void wgrad32_mfma_kernel(
             }
         }
     }
-  // D[o = 16 m + 4 k4 + e][col = 16 nt + n16] -> part[block][o][col]
-  float *pb = part + (long long)blockIdx.x * C * C * 9;
+  // D[o = 16 m + 4 k4 + e][col = 16 nt + n16] -> part[block][o][col]; the
+  // bias gradient's block partials after them: part[block][C * C * 9 + o]
+  float *pb = part + (long long)blockIdx.x * kWgradOut;
+  {
+    __shared__ float bred[256];
+    bred[threadIdx.x] = bsum;
+    __syncthreads();
+    if (threadIdx.x < C)
+      {
+        float t = 0.0f;
+#pragma unroll
+        for (int j = 0; j < 8; ++j)
+          t += bred[threadIdx.x * 8 + j];
+        pb[C * C * 9 + threadIdx.x] = t;
+      }
+  }
 #pragma unroll
   for (int j = 0; j < NW; ++j)
     {
@@ -322,72 +344,55 @@ __global__ __launch_bounds__(256) void wgrad32_mfma_kernel(
     }
 }
 
-// gw[o][col] = sum over the blocks' partials in block order; gb[o] = sum of
-// g[o] over the batch and pixels (fixed order: a thread per (o, chunk) then
-// a block reduction)
-__global__ __launch_bounds__(256) void wgrad_reduce_kernel(
-  const float *__restrict__ part, int n_part, float *__restrict__ gw,
-  int n_out)
+// out[e] = sum over n_part partials (stride n_out) of element e, in a fixed
+// order: four threads per element each sum every fourth partial, then the
+// four sums are added in order; e < n_w goes to gw[e], the rest to
+// gb[e - n_w]
+__global__ __launch_bounds__(256) void part_reduce_kernel(
+  const float *__restrict__ part, int n_part, int n_out, int n_w,
+  float *__restrict__ gw, float *__restrict__ gb)
 {
-  const int e = blockIdx.x * blockDim.x + threadIdx.x;
-  if (e >= n_out)
-    return;
+  __shared__ float red[4][64];
+  const int o = blockIdx.x * 64 + threadIdx.x % 64, q = threadIdx.x / 64;
   float s = 0.0f;
-  for (int p = 0; p < n_part; ++p)
-    s += part[(long long)p * n_out + e];
-  gw[e] = s;
-}
-
-__global__ __launch_bounds__(256) void bias_grad_kernel(
-  const float *__restrict__ g, float *__restrict__ gb, int B, int C,
-  long long plane)
-{
-  __shared__ float red[256];
-  const int o = blockIdx.x;
-  float s = 0.0f;
-  for (int b = 0; b < B; ++b)
-    for (long long p = threadIdx.x; p < plane; p += 256)
-      s += g[((long long)b * C + o) * plane + p];
-  red[threadIdx.x] = s;
+  if (o < n_out)
+    for (int p = q; p < n_part; p += 4)
+      s += part[(long long)p * n_out + o];
+  red[q][threadIdx.x % 64] = s;
   __syncthreads();
-  for (int st = 128; st > 0; st >>= 1)
+  if (q == 0 && o < n_out)
     {
-      if ((int)threadIdx.x < st)
-        red[threadIdx.x] += red[threadIdx.x + st];
-      __syncthreads();
+      const int l = threadIdx.x;
+      const float t = ((red[0][l] + red[1][l]) + red[2][l]) + red[3][l];
+      if (o < n_w)
+        gw[o] = t;
+      else
+        gb[o - n_w] = t;
     }
-  if (threadIdx.x == 0)
-    gb[o] = red[0];
 }
 
-// g *= leaky-ReLU'(pre-activation); a is the post-activation output, which
-// has the sign of the pre-activation
-__global__ void act_grad_kernel(float *__restrict__ g,
-                                const float *__restrict__ a, long long n)
-{
-  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
-       i += (long long)gridDim.x * blockDim.x)
-    g[i] *= a[i] > 0.0f ? 1.0f : kSlope;
-}
-
-// Weight and bias gradients of one layer, block (o, i): four pixels of a
-// row per thread and step, then a fixed-order block reduction; blocks with
-// i == 0 also write the bias gradient of o.
+// Weight and bias gradients of the 1 -> 32 and 32 -> 1 layers, split over
+// the pixels: block (pair (o, i), chunk) sums the nine tap products and
+// (for i == 0) the bias term over its share of the pixel groups into
+// part[chunk][pair][10]; part_reduce_kernel adds the chunks in order
 template <int CIN, int COUT>
-__global__ __launch_bounds__(256) void wgrad_kernel(
+__global__ __launch_bounds__(256) void wgrad_split_kernel(
   const float *__restrict__ in, const float *__restrict__ g,
-  float *__restrict__ gw, float *__restrict__ gb, int B, int H, int W)
+  float *__restrict__ part, int B, int H, int W)
 {
   __shared__ float red[10][256];
-  const int o = blockIdx.x / CIN, i = blockIdx.x % CIN;
+  const int pair = blockIdx.x, chunk = blockIdx.y, n_chunks = gridDim.y;
+  const int o = pair / CIN, i = pair % CIN;
   const long long plane = (long long)H * W;
   const int W4 = W / 4;
   const long long groups = (long long)B * H * W4;
+  const long long e0 = groups * chunk / n_chunks;
+  const long long e1 = groups * (chunk + 1) / n_chunks;
   float s[10];
 #pragma unroll
   for (int k = 0; k < 10; ++k)
     s[k] = 0.0f;
-  for (long long e = threadIdx.x; e < groups; e += blockDim.x)
+  for (long long e = e0 + threadIdx.x; e < e1; e += blockDim.x)
     {
       const int b = (int)(e / ((long long)H * W4));
       const int r = (int)(e % ((long long)H * W4));
@@ -421,21 +426,44 @@ __global__ __launch_bounds__(256) void wgrad_kernel(
           red[k][threadIdx.x] += red[k][threadIdx.x + st];
       __syncthreads();
     }
-  if (threadIdx.x < 9)
-    gw[((long long)o * CIN + i) * 9 + threadIdx.x] = red[threadIdx.x][0];
-  if (threadIdx.x == 9 && i == 0)
-    gb[o] = red[9][0];
+  if (threadIdx.x < 10)
+    part[((long long)chunk * gridDim.x + pair) * 10 + threadIdx.x] =
+      red[threadIdx.x][0];
 }
 
-// loss = mean (out - target)^2 (FP64 sum), g = d loss / d out; one block
-__global__ __launch_bounds__(1024) void mse_kernel(
-  const float *__restrict__ out, const float *__restrict__ target,
-  float *__restrict__ g, long long n, double *__restrict__ loss)
+// the split partials, summed over the chunks in order, to the canonical
+// layout: gw[o][i][tap], gb[o] (from the pairs with i == 0)
+__global__ __launch_bounds__(256) void wgrad_split_reduce_kernel(
+  const float *__restrict__ part, int n_chunks, int cin, int n_pairs,
+  float *__restrict__ gw, float *__restrict__ gb)
 {
-  __shared__ double red[1024];
+  const int e = blockIdx.x * blockDim.x + threadIdx.x;
+  if (e >= n_pairs * 10)
+    return;
+  const int pair = e / 10, k = e % 10;
+  float s = 0.0f;
+  for (int c = 0; c < n_chunks; ++c)
+    s += part[((long long)c * n_pairs + pair) * 10 + k];
+  const int o = pair / cin, i = pair % cin;
+  if (k < 9)
+    gw[((long long)o * cin + i) * 9 + k] = s;
+  else if (i == 0)
+    gb[o] = s;
+}
+
+// loss partials: g = 2 (out - target) / n and, per block, the FP64 sum of
+// (out - target)^2 over its strided share; mse_finish_kernel adds the
+// blocks' sums in order
+constexpr int kMseBlocks = 64;
+__global__ __launch_bounds__(256) void mse_partial_kernel(
+  const float *__restrict__ out, const float *__restrict__ target,
+  float *__restrict__ g, long long n, double *__restrict__ partial)
+{
+  __shared__ double red[256];
   double s = 0.0;
   const float sc = (float)(2.0 / (double)n);
-  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+  for (long long i = blockIdx.x * 256LL + threadIdx.x; i < n;
+       i += (long long)gridDim.x * 256)
     {
       const float d = out[i] - target[i];
       g[i] = sc * d;
@@ -443,14 +471,36 @@ __global__ __launch_bounds__(1024) void mse_kernel(
     }
   red[threadIdx.x] = s;
   __syncthreads();
-  for (int st = blockDim.x / 2; st > 0; st >>= 1)
+  for (int st = 128; st > 0; st >>= 1)
     {
       if ((int)threadIdx.x < st)
         red[threadIdx.x] += red[threadIdx.x + st];
       __syncthreads();
     }
   if (threadIdx.x == 0)
-    *loss = red[0] / (double)n;
+    partial[blockIdx.x] = red[0];
+}
+
+__global__ void mse_finish_kernel(const double *__restrict__ partial, int nb,
+                                  long long n, double *__restrict__ loss)
+{
+  if (threadIdx.x == 0)
+    {
+      double s = 0.0;
+      for (int b = 0; b < nb; ++b)
+        s += partial[b];
+      *loss = s / (double)n;
+    }
+}
+
+// g *= leaky-ReLU'(pre-activation); a is the post-activation output, which
+// has the sign of the pre-activation
+__global__ void act_grad_kernel(float *__restrict__ g,
+                                const float *__restrict__ a, long long n)
+{
+  for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
+       i += (long long)gridDim.x * blockDim.x)
+    g[i] *= a[i] > 0.0f ? 1.0f : kSlope;
 }
 
 // Adam on the FP32 master parameters (bias-corrected step sizes c1, c2)
@@ -537,16 +587,44 @@ __global__ __launch_bounds__(1024) void dofs_to_image_kernel(
 {
   __shared__ double red[1024];
   auto val = [&](long long i) { return sub ? u[i] - sub[i] : u[i]; };
+  // the first kKeep values of each thread stay in registers between the
+  // passes (all of them up to 16,384 DoFs: 32^2 cells of 4^2 nodes)
+  constexpr int kKeep = 16;
+  double keep[kKeep];
   double s = 0.0;
-  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+#pragma unroll
+  for (int k = 0; k < kKeep; ++k)
+    {
+      const long long i = threadIdx.x + (long long)k * blockDim.x;
+      keep[k] = i < n ? val(i) : 0.0;
+      s += keep[k];
+    }
+  for (long long i = threadIdx.x + (long long)kKeep * blockDim.x; i < n;
+       i += blockDim.x)
     s += val(i);
   const double mean = block_sum(s, red) / (double)n;
   s = 0.0;
-  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+#pragma unroll
+  for (int k = 0; k < kKeep; ++k)
+    {
+      const long long i = threadIdx.x + (long long)k * blockDim.x;
+      if (i < n)
+        s += (keep[k] - mean) * (keep[k] - mean);
+    }
+  for (long long i = threadIdx.x + (long long)kKeep * blockDim.x; i < n;
+       i += blockDim.x)
     s += (val(i) - mean) * (val(i) - mean);
   const double rms = sqrt(block_sum(s, red) / (double)n);
   const double inv = rms > 0.0 ? 1.0 / rms : 0.0;
-  for (long long i = threadIdx.x; i < n; i += blockDim.x)
+#pragma unroll
+  for (int k = 0; k < kKeep; ++k)
+    {
+      const long long i = threadIdx.x + (long long)k * blockDim.x;
+      if (i < n)
+        image[pixel_of(i, n0, N)] = (float)((keep[k] - mean) * inv);
+    }
+  for (long long i = threadIdx.x + (long long)kKeep * blockDim.x; i < n;
+       i += blockDim.x)
     image[pixel_of(i, n0, N)] = (float)((val(i) - mean) * inv);
   if (threadIdx.x == 0)
     *scale = rms;
@@ -637,6 +715,8 @@ namespace
 // blocks of the weight-gradient kernel (each sums a strided set of 16 x 16
 // pixel tiles); fixed by the problem size, so the sums are deterministic
 constexpr int kWgradBlocks = 256;
+// pixel chunks of the small layers' weight gradients
+constexpr int kSplitChunks = 32;
 int wgrad_blocks(int batch, int H, int W)
 {
   const long long tiles = (long long)batch * (H / 16) * (W / 16);
@@ -762,10 +842,12 @@ hipError_t CnnNet::reserve(int batch)
   const size_t per =
     (size_t)H * W * (3 * cnn::kChannels + 1 + 2 * cnn::kChannels);
   e = hipMalloc(&act, per * batch * sizeof(float));
-  // the weight-gradient blocks' partial sums
+  // the partial sums of the weight gradients (and of the loss)
   if (e == hipSuccess)
-    e = hipMalloc(&part, (size_t)wgrad_blocks(batch, H, W) * cnn::kChannels *
-                           cnn::kChannels * 9 * sizeof(float));
+    e = hipMalloc(&part, std::max<size_t>(
+                           (size_t)wgrad_blocks(batch, H, W) * cnn::kWgradOut,
+                           (size_t)kSplitChunks * cnn::kChannels * 10) *
+                           sizeof(float));
   if (e == hipSuccess)
     act_batch = batch;
   return e;
@@ -805,12 +887,18 @@ void conv(const float *in, const uint32_t *w, const float *bias, float *out,
 }
 
 template <bool ACT>
-void conv32(const float *in, const uint32_t *w, const float *bias, float *out,
-            int batch, int H, int W, hipStream_t s)
+hipError_t conv32(const float *in, const uint32_t *w, const float *bias,
+                  float *out, int batch, int H, int W, hipStream_t s)
 {
+  hipError_t e = hipFuncSetAttribute(
+    reinterpret_cast<const void *>(&cnn::conv32_mfma_kernel<ACT>),
+    hipFuncAttributeMaxDynamicSharedMemorySize, cnn::kConvLds);
+  if (e != hipSuccess)
+    return e;
   hipLaunchKernelGGL((cnn::conv32_mfma_kernel<ACT>),
-                     dim3(W / cnn::kMTX, H / cnn::kMTY, batch), dim3(256), 0,
-                     s, in, w, bias, out, H, W);
+                     dim3(W / cnn::kMTX, H / cnn::kMTY, batch), dim3(256),
+                     cnn::kConvLds, s, in, w, bias, out, H, W);
+  return hipGetLastError();
 }
 
 // the weight and bias gradients of a 32 -> 32 layer on the matrix cores
@@ -827,11 +915,26 @@ hipError_t wgrad32(const float *in, const float *g, float *part, float *gw,
   const int nb = wgrad_blocks(batch, H, W);
   hipLaunchKernelGGL(cnn::wgrad32_mfma_kernel, dim3(nb), dim3(256), lds, s,
                      in, g, part, batch, H, W);
-  const int n_out = C * C * 9;
-  hipLaunchKernelGGL(cnn::wgrad_reduce_kernel, dim3((n_out + 255) / 256),
-                     dim3(256), 0, s, part, nb, gw, n_out);
-  hipLaunchKernelGGL(cnn::bias_grad_kernel, dim3(C), dim3(256), 0, s, g, gb,
-                     batch, C, (long long)H * W);
+  const int n_out = cnn::kWgradOut;
+  // gw and gb are adjacent in the parameter layout (w[Co][Ci][9], b[Co])
+  hipLaunchKernelGGL(cnn::part_reduce_kernel, dim3((n_out + 63) / 64),
+                     dim3(256), 0, s, part, nb, n_out, C * C * 9, gw, gb);
+  return hipGetLastError();
+}
+
+// the weight and bias gradients of the 1 -> 32 and 32 -> 1 layers
+template <int CIN, int COUT>
+hipError_t wgrad_split(const float *in, const float *g, float *part,
+                       float *gw, float *gb, int batch, int H, int W,
+                       hipStream_t s)
+{
+  constexpr int pairs = CIN * COUT;
+  hipLaunchKernelGGL((cnn::wgrad_split_kernel<CIN, COUT>),
+                     dim3(pairs, kSplitChunks), dim3(256), 0, s, in, g, part,
+                     batch, H, W);
+  hipLaunchKernelGGL(cnn::wgrad_split_reduce_kernel,
+                     dim3((pairs * 10 + 255) / 256), dim3(256), 0, s, part,
+                     kSplitChunks, CIN, pairs, gw, gb);
   return hipGetLastError();
 }
 
@@ -852,10 +955,13 @@ hipError_t CnnNet::forward(const float *in, float *out, int batch,
   const Bufs b = buffers(act, batch, H, W);
   conv<1, 32, true>(in, pk + pack_offset_fwd(0), p + param_offset_b(0), b.a1,
                     batch, H, W, s);
-  conv32<true>(b.a1, pk + pack_offset_fwd(1), p + param_offset_b(1), b.a2,
-               batch, H, W, s);
-  conv32<true>(b.a2, pk + pack_offset_fwd(2), p + param_offset_b(2), b.a3,
-               batch, H, W, s);
+  e = conv32<true>(b.a1, pk + pack_offset_fwd(1), p + param_offset_b(1), b.a2,
+                   batch, H, W, s);
+  if (e == hipSuccess)
+    e = conv32<true>(b.a2, pk + pack_offset_fwd(2), p + param_offset_b(2),
+                     b.a3, batch, H, W, s);
+  if (e != hipSuccess)
+    return e;
   conv<32, 1, false>(b.a3, pk + pack_offset_fwd(3), p + param_offset_b(3),
                      out, batch, H, W, s);
   return hipGetLastError();
@@ -874,12 +980,18 @@ hipError_t CnnNet::loss_and_grad(const float *in, const float *target,
   const long long plane = (long long)H * W;
   const long long nC = (long long)batch * cnn::kChannels * plane;
   const int C = cnn::kChannels;
-  hipLaunchKernelGGL(cnn::mse_kernel, dim3(1), dim3(1024), 0, s, b.out, target,
-                     b.g0, batch * plane, loss_dev);
+  // the loss partials sit in the weight-gradient buffer, consumed at once
+  double *mse_part = reinterpret_cast<double *>(part);
+  hipLaunchKernelGGL(cnn::mse_partial_kernel, dim3(cnn::kMseBlocks),
+                     dim3(256), 0, s, b.out, target, b.g0, batch * plane,
+                     mse_part);
+  hipLaunchKernelGGL(cnn::mse_finish_kernel, dim3(1), dim3(64), 0, s,
+                     mse_part, cnn::kMseBlocks, batch * plane, loss_dev);
   // layer 3 (C -> 1, linear)
-  hipLaunchKernelGGL((cnn::wgrad_kernel<32, 1>), dim3(C), dim3(256), 0, s,
-                     b.a3, b.g0, grad + param_offset_w(3),
-                     grad + param_offset_b(3), batch, H, W);
+  e = wgrad_split<32, 1>(b.a3, b.g0, part, grad + param_offset_w(3),
+                         grad + param_offset_b(3), batch, H, W, s);
+  if (e != hipSuccess)
+    return e;
   conv<1, 32, false>(b.g0, pk + pack_offset_bwd(3), nullptr, b.g1, batch, H, W,
                      s);
   act_grad(b.g1, b.a3, nC, s);
@@ -888,20 +1000,24 @@ hipError_t CnnNet::loss_and_grad(const float *in, const float *target,
               grad + param_offset_b(2), batch, H, W, s);
   if (e != hipSuccess)
     return e;
-  conv32<false>(b.g1, pk + pack_offset_bwd(2), nullptr, b.g0, batch, H, W, s);
+  e = conv32<false>(b.g1, pk + pack_offset_bwd(2), nullptr, b.g0, batch, H, W,
+                    s);
+  if (e != hipSuccess)
+    return e;
   act_grad(b.g0, b.a2, nC, s);
   // layer 1
   e = wgrad32(b.a1, b.g0, part, grad + param_offset_w(1),
               grad + param_offset_b(1), batch, H, W, s);
   if (e != hipSuccess)
     return e;
-  conv32<false>(b.g0, pk + pack_offset_bwd(1), nullptr, b.g1, batch, H, W, s);
+  e = conv32<false>(b.g0, pk + pack_offset_bwd(1), nullptr, b.g1, batch, H, W,
+                    s);
+  if (e != hipSuccess)
+    return e;
   act_grad(b.g1, b.a1, nC, s);
   // layer 0 (1 -> C)
-  hipLaunchKernelGGL((cnn::wgrad_kernel<1, 32>), dim3(C), dim3(256), 0, s, in,
-                     b.g1, grad + param_offset_w(0), grad + param_offset_b(0),
-                     batch, H, W);
-  return hipGetLastError();
+  return wgrad_split<1, 32>(in, b.g1, part, grad + param_offset_w(0),
+                            grad + param_offset_b(0), batch, H, W, s);
 }
 
 hipError_t CnnNet::adam(float lr, hipStream_t s)

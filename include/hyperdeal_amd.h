@@ -195,7 +195,9 @@ int hd_lsrk_step(hd_advection *op, const char *type, int precision,
  * counterpart (it runs one operator per MPI rank); a batch replaces many
  * calls of AdvectionOperation::apply (advection_operation.h:134-209) and
  * LowStorageRungeKuttaIntegrator::perform_time_step (time_integrators.
- * templates.h:91-184) on small meshes by one launch per apply / stage.  The
+ * templates.h:91-184) on small meshes by one launch per degree and apply /
+ * stage (the operators of one degree share a kernel with the block shape of
+ * that degree's single-operator kernel).  The
  * operators must use the collapsed kernel on 2+2 periodic meshes (degree
  * 1..5, meshes and degrees may differ); at most HD_BATCH_MAX of them.  Array
  * arguments hold one pointer per operator, in the order of hd_batch_create.
@@ -212,17 +214,17 @@ int hd_batch_size(const hd_batch *batch);
 int hd_batch_apply(hd_batch *batch, const void *const *src, void *const *dst,
                    int precision, void *stream);
 /* one LSRK step of every operator: sol[i] FP64, ti_a[i] != ti_b[i] stage
- * registers of `precision`; one copy launch and one launch per stage */
+ * registers of `precision`; one copy launch and one launch per degree and
+ * stage */
 int hd_batch_lsrk_step(hd_batch *batch, const char *type, int precision,
                        double *const *sol, void *const *ti_a,
                        void *const *ti_b, double dt, void *stream);
-/* Work split: the operators are cut into fragments of contiguous cells
- * (about 1/16 of the batch's cost each), the fragments are spread over the
- * XCDs by cost, and each XCD's blocks walk its fragments interleaved.  The
- * cost of a DoF depends on its degree; the autotuner measures it: it times
- * the batched apply of each degree's operators alone on src -> dst, then
- * scales each degree's cost while that makes the whole batch faster.
- * Synchronous on `stream`. */
+/* Work split: every operator's cells are cut into eight contiguous parts,
+ * one per XCD, and each XCD's blocks walk its parts interleaved (a task: one
+ * block's share of one part).  The autotuner times each degree's batched
+ * apply on src -> dst with tasks capped at 1, 4 or 16 cell groups or
+ * uncapped, and keeps the fastest cap per degree.  Synchronous on
+ * `stream`. */
 int hd_batch_autotune(hd_batch *batch, int precision, const void *const *src,
                       void *const *dst, void *stream);
 /* the mean cells per task of a degree in the current work split (a task: one
