@@ -118,9 +118,9 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
   if (vp_kernel_active(op))
     {
       const hipError_t e =
-        hd::vp_apply(mf->nd, mode, precision == HD_FP64 ? 0 : 1, src, dst,
-                     op->vtab, mf->mesh, op->n_x_cells, op->efield, stage,
-                     stream);
+        hd::vp_wave_apply(mf->nd, mode, precision == HD_FP64 ? 0 : 1, src,
+                          dst, op->vtab, mf->mesh, op->n_x_cells, op->efield,
+                          stage, stream);
       return e == hipSuccess ? ok() : hip_fail(e, "kernel launch");
     }
   const int algo = choose_algorithm(op, precision);
@@ -780,6 +780,31 @@ int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
                   }
                 tab.push_back(acc);
               }
+      // per d and v_d cell: bit 0 when the + factor is nonzero (a Gauss
+      // point with v_d > 0), bit 1 when the - factor is
+      const size_t vfac0 = (size_t)n_tab + (n_tab + 1) / 2;
+      const size_t n_vfac = tab.size() - vfac0;
+      for (int d = 0; d < 2; ++d)
+        for (int c = 0; c < mf->mesh.n[2 + d]; ++c)
+          {
+            const double lo = mf->mesh.lo[2 + d] + c * mf->mesh.h[2 + d];
+            int flag = 0;
+            for (int q = 0; q < nd; ++q)
+              {
+                const double v = lo + t.qpoints[q] * mf->mesh.h[2 + d];
+                flag |= (v > 0.0 ? 1 : 0) | (v < 0.0 ? 2 : 0);
+              }
+            tab.push_back((double)flag);
+          }
+      // and the FP32 copy of the v factors
+      {
+        std::vector<float> vf(n_vfac);
+        for (size_t i = 0; i < n_vfac; ++i)
+          vf[i] = (float)tab[vfac0 + i];
+        const size_t at = tab.size();
+        tab.resize(at + (n_vfac + 1) / 2, 0.0);
+        std::memcpy(tab.data() + at, vf.data(), n_vfac * sizeof(float));
+      }
       e = hipMalloc(&op->vtab, tab.size() * sizeof(double));
       if (e == hipSuccess)
         e = hipMemcpy(op->vtab, tab.data(), tab.size() * sizeof(double),

@@ -342,10 +342,11 @@ hipError_t vp_density_from_partials(const double *part, long long n_v_cells,
                                     long long n_x_dofs, double *rho,
                                     hipStream_t stream);
 int vp_apply_table_size(int N);
-hipError_t vp_apply(int N, int mode, int precision, const void *src, void *dst,
-                    const double *tab, const MeshDev &mesh,
-                    long long n_x_cells, const double *efield,
-                    const StageDev &stage, hipStream_t stream);
+// the 2+2 phase-space apply / LSRK stage (hd_vpwave.hip)
+hipError_t vp_wave_apply(int N, int mode, int precision, const void *src,
+                         void *dst, const double *tab, const MeshDev &mesh,
+                         long long n_x_cells, const double *efield,
+                         const StageDev &stage, hipStream_t stream);
 void vp_poisson2d_table(const XDev &xd, int N, double *tab);
 // The exact start of the 2D solver by fast diagonalisation (hd_fdm.hip):
 // the generalised eigenvectors of the two 1D periodic SIPG operators on the

@@ -15,6 +15,7 @@
 // there), dot products are block reductions, and the iteration count stays
 // on the device -- no host round trip inside a Runge-Kutta stage.
 #include "hd_device.h"
+#include "hd_vp_table.h"
 
 #include <algorithm>
 
@@ -927,561 +928,7 @@ __global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
     }
 }
 
-// ---- the 2+2 phase-space apply with the velocity (v, grad phi) ----
-// The reference's quadrature algorithm (advection_operation.h:221-566 with
-// the velocity of vlasov_poisson/include/velocity_field_view.h:111-175)
-// rewritten exactly per direction, for affine Cartesian cells and n_points
-// = k + 1:
-//   x_d (d = 0, 1): the speed v_d is constant along every x line, so the
-//     x_d term is the unit-speed 1D DG operator along x_d (the collapsed
-//     A, b of the constant-velocity kernels) tensored with the v_d factor
-//     S^-1 diag(v_d(q)) S, split by sign for the upwind choice:
-//       out += V_d+ (x) (A_d+ u + b_d+ u_left) + V_d- (x) (A_d- u + b_d- u_right)
-//   v_e (e = 0, 1): E_e is given at the x Gauss points; in the x Gauss basis
-//     (S (x) S along x0, x1) every v_e line has the constant speed
-//     E_e(q_x): the unit-speed operator scaled by E_e(q_x), upwind by its
-//     sign, then back with S^-1 (x) S^-1.
-// A_d+ is the collapsed operator for speed +1, A_d- minus the one for speed
-// -1, so that the operator for speed c is c A_d+ (c >= 0) or c A_d- (c < 0).
-// Every quadrature the reference applies is reproduced (Gauss k+1 points in
-// every factor), so the results agree to roundoff.
-//
-// One block of 256 threads per cell, one DoF per thread and step (N^4 <=
-// 625: threads loop), five LDS barriers per cell.
-// Table (doubles): per direction d: A+ (N^2), b+ (N), A- (N^2), b- (N);
-// then S (N^2, row q: Gauss point), S^-1 (N^2), Gauss points (N).
-namespace vpk
-{
-template <int N>
-struct Tab
-{
-  static constexpr int NN = N * N, PER = 2 * (NN + N);
-  static constexpr int Ap(int d) { return d * PER; }
-  static constexpr int bp(int d) { return d * PER + NN; }
-  static constexpr int An(int d) { return d * PER + NN + N; }
-  static constexpr int bn(int d) { return d * PER + 2 * NN + N; }
-  static constexpr int S = 4 * PER, Si = S + NN, qp = Si + NN;
-  static constexpr int wv = qp + N;          // JxW_v of the v nodes (N^2)
-  static constexpr int size = wv + NN;
-};
-
-// An LDS read of the line mappings below.  FP64: a single ds_read_b64 the
-// compiler neither pairs (ds_read2) nor widens (b128) -- on gfx950 those
-// forms conflict for these lane strides, the single form does not (bank
-// conflicts measured at about 6 cycles per LDS instruction before).  FP32:
-// the compiler's wide reads measured faster than single ones.
-template <typename T>
-__device__ inline T ld(const T *p, int i)
-{
-  if constexpr (sizeof(T) == 8)
-    return ((const volatile __attribute__((address_space(3))) T *)p)[i];
-  else
-    return p[i];
-}
-
-// a workgroup barrier that orders LDS only: the next cell's global loads
-// stay in flight across it (__syncthreads would wait for them)
-__device__ inline void lds_sync()
-{
-  __builtin_amdgcn_fence(__ATOMIC_RELEASE, "workgroup", "local");
-  __builtin_amdgcn_s_barrier();
-  __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "workgroup", "local");
-}
-
-// Work mapping of every contraction step: element e = i * FP + l with i
-// the output value of the contracted digit (uniform over a wave: FP is a
-// multiple of 64) and l the line (the other three digits, ascending), so
-// the coefficient rows are scalar loads.
-template <int N>
-struct Map
-{
-  static constexpr int F = N * N * N;
-  static constexpr int FP = (F + 63) / 64 * 64;
-  static constexpr int E = N * FP;
-};
-
-// base index (digit d = 0) of line l along direction d
-template <int N>
-__device__ inline int line_base(int l, int d)
-{
-  int r = 0, p = 1;
-#pragma unroll
-  for (int k = 0; k < 4; ++k)
-    {
-      if (k != d)
-        {
-          r += (l % N) * p;
-          l /= N;
-        }
-      p *= N;
-    }
-  return r;
-}
-
-__device__ inline int uniform(int v)
-{
-  return __builtin_amdgcn_readfirstlane(v);
-}
-
-template <int N, typename T, int MODE>
-__global__ __launch_bounds__(256) void apply_kernel(
-  const T *__restrict__ src, T *__restrict__ dst,
-  const double *__restrict__ tab_in, MeshDev mesh, long long n_x_cells,
-  const double *__restrict__ efield, StageDev stage)
-{
-  using TB = Tab<N>;
-  using MP = Map<N>;
-  constexpr int NN = N * N, C = NN * NN, F = MP::F, FP = MP::FP;
-  constexpr int s1 = N, s2 = NN, s3 = N * NN; // digit strides
-  const cptr<double> tab = (cptr<double>)tab_in;
-  // the 1D coefficients in the arithmetic's precision: FP32 reads the float
-  // copy that follows the table (no conversion per use)
-  const cptr<float> tabf = (cptr<float>)(tab_in + TB::size);
-  auto tc = [&](int i) -> T {
-    if constexpr (sizeof(T) == 4)
-      return tabf[i];
-    else
-      return tab[i];
-  };
-  __shared__ T U[C], T0p[C], T0n[C], T1p[C], T1n[C], Q1[C], Q2[C];
-  // the inflow faces, left (FL) and right (FR) of each direction; after S1
-  // the 8 N^3 >= N^4 values hold R1
-  // FP32: the cell and the v faces go to the x Gauss points in one pass
-  // each (S x S along x0, x1), so the x-part products and the v operators
-  // share a phase (five barriers per cell).  FP64: one direction per pass
-  // (Q1, G1), six barriers -- the one-pass form needs more registers and
-  // 64-bit LDS reads, which cost FP64 a resident block (measured slower).
-  constexpr bool kDirect = sizeof(T) == 4;
-  // G1Y: the x0 transform of the v faces (FP64, [2][2][F]) or Y1 (FP32)
-  __shared__ T FLR[8][F], G2[2][2][F], G1Y[kDirect ? C : 4 * F];
-  T(*FL)[F] = FLR;
-  T(*FR)[F] = FLR + 4;
-  __shared__ T V[2][2][NN]; // [d][+/-][i][l]
-  // the v factors of the x terms per v cell follow the two coefficient
-  // tables: [d][v_d cell][sign][i][l]
-  const double *vfac = tab_in + TB::size + (TB::size + 1) / 2;
-  __shared__ T E[2][NN];    // E_e at the x Gauss points
-  __shared__ double RD[MODE == 1 ? C : 1]; // the stage value, for rho
-  // the next cell's values, field and eight inflow faces (both sides of
-  // every direction) travel into registers while this cell computes
-  constexpr int PU = (C + 255) / 256, PE = (2 * NN + 255) / 256;
-  constexpr int PF = (8 * F + 255) / 256, PV = (4 * NN + 255) / 256;
-  T pu[PU], pe[PE], pf[PF], pv[PV];
-  auto prefetch = [&](long long cell) {
-#pragma unroll
-    for (int k = 0; k < PU; ++k)
-      {
-        const int t = threadIdx.x + 256 * k;
-        if (t < C)
-          pu[k] = src[cell * C + t];
-      }
-    const long long xc = cell % n_x_cells;
-#pragma unroll
-    for (int k = 0; k < PE; ++k)
-      {
-        const int t = threadIdx.x + 256 * k;
-        if (t < 2 * NN)
-          pe[k] = (T)efield[(xc * NN + t % NN) * 2 + t / NN];
-      }
-    long long nbl[4], nbr[4];
-    int cv[2];
-    {
-      unsigned rem = (unsigned)cell;
-#pragma unroll
-      for (int d = 0; d < 4; ++d)
-        {
-          const unsigned n = (unsigned)mesh.n[d];
-          const unsigned q = rem / n;
-          const unsigned c = rem - q * n;
-          rem = q;
-          if (d >= 2)
-            cv[d - 2] = (int)c;
-          const long long st = mesh.stride[d];
-          nbl[d] = c == 0 ? cell + (long long)(n - 1) * st : cell - st;
-          nbr[d] = c == n - 1 ? cell - (long long)(n - 1) * st : cell + st;
-        }
-    }
-#pragma unroll
-    for (int k = 0; k < PV; ++k)
-      {
-        const int t = threadIdx.x + 256 * k;
-        if (t < 4 * NN)
-          {
-            const int d = t / (2 * NN);
-            pv[k] = (T)vfac[(d == 0 ? 0 : mesh.n[2] * 2 * NN) +
-                            (d == 0 ? cv[0] : cv[1]) * 2 * NN + t % (2 * NN)];
-          }
-      }
-#pragma unroll
-    for (int k = 0; k < PF; ++k)
-      {
-        const int t = threadIdx.x + 256 * k;
-        if (t < 8 * F)
-          {
-            const int ds = t / F, f = t % F, d = ds / 2, side = ds % 2;
-            long long nb = 0;
-            int sd = 1;
-#pragma unroll
-            for (int dd = 0; dd < 4; ++dd)
-              if (dd == d)
-                {
-                  nb = side == 0 ? nbl[dd] : nbr[dd];
-                  sd = dd == 0 ? 1 : dd == 1 ? s1 : dd == 2 ? s2 : s3;
-                }
-            const int node = side == 0 ? N - 1 : 0;
-            pf[k] = src[nb * C + line_base<N>(f, d) + node * sd];
-          }
-      }
-  };
-  // reuse: R0 = U and R1 = FLR (both consumed by S1), Z = T1p; Y0, Y1 =
-  // Q1, G1Y (FP32) or T0p, T0n (FP64: consumed by the x-part products)
-  T *R0 = U, *R1 = &FLR[0][0], *Z = T1p;
-  T *Y0 = kDirect ? Q1 : T0p, *Y1 = kDirect ? G1Y : T0n;
-
-  if ((long long)blockIdx.x < mesh.n_cells)
-    prefetch(blockIdx.x);
-  for (long long cell = blockIdx.x; cell < mesh.n_cells; cell += gridDim.x)
-    {
-      // S0: the prefetched cell, field, faces and v_d factors into LDS
-#pragma unroll
-      for (int k = 0; k < PU; ++k)
-        {
-          const int t = threadIdx.x + 256 * k;
-          if (t < C)
-            U[t] = pu[k];
-        }
-#pragma unroll
-      for (int k = 0; k < PE; ++k)
-        {
-          const int t = threadIdx.x + 256 * k;
-          if (t < 2 * NN)
-            E[t / NN][t % NN] = pe[k];
-        }
-#pragma unroll
-      for (int k = 0; k < PF; ++k)
-        {
-          const int t = threadIdx.x + 256 * k;
-          if (t < 8 * F)
-            {
-              const int ds = t / F, f = t % F;
-              if (ds % 2 == 0)
-                FL[ds / 2][f] = pf[k];
-              else
-                FR[ds / 2][f] = pf[k];
-            }
-        }
-#pragma unroll
-      for (int k = 0; k < PV; ++k)
-        {
-          const int t = threadIdx.x + 256 * k;
-          if (t < 4 * NN)
-            (&V[0][0][0])[t] = pv[k];
-        }
-      lds_sync();
-      // the next cell's loads go out now and land during the arithmetic
-      {
-        const long long next = cell + gridDim.x;
-        if (next < mesh.n_cells)
-          prefetch(next);
-      }
-      // S1: x0 operators (i0 uniform), x1 operators (i1 uniform); the cell
-      // and the v faces at the x Gauss points (FP32) or along x0 (FP64)
-      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
-        {
-          const int i = uniform(e / FP), l = e % FP;
-          if (l >= F)
-            continue;
-          const int b = line_base<N>(l, 0);
-          T u[N];
-#pragma unroll
-          for (int j = 0; j < N; ++j)
-            u[j] = ld(U, b + j);
-          {
-              T a = tc(TB::bp(0) + i) * FL[0][l];
-#pragma unroll
-              for (int j = 0; j < N; ++j)
-                a += tc(TB::Ap(0) + i * N + j) * u[j];
-              T0p[b + i] = a;
-            }
-          {
-              T a = tc(TB::bn(0) + i) * FR[0][l];
-#pragma unroll
-              for (int j = 0; j < N; ++j)
-                a += tc(TB::An(0) + i * N + j) * u[j];
-              T0n[b + i] = a;
-            }
-          if constexpr (!kDirect)
-            {
-              T a = 0;
-#pragma unroll
-              for (int j = 0; j < N; ++j)
-                a += tc(TB::S + i * N + j) * u[j];
-              Q1[b + i] = a;
-            }
-        }
-      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
-        {
-          const int i = uniform(e / FP), l = e % FP;
-          if (l >= F)
-            continue;
-          const int b = line_base<N>(l, 1);
-          T u[N];
-#pragma unroll
-          for (int j = 0; j < N; ++j)
-            u[j] = ld(U, b + j * s1);
-          {
-              T a = tc(TB::bp(1) + i) * FL[1][l];
-#pragma unroll
-              for (int j = 0; j < N; ++j)
-                a += tc(TB::Ap(1) + i * N + j) * u[j];
-              T1p[b + i * s1] = a;
-            }
-          {
-              T a = tc(TB::bn(1) + i) * FR[1][l];
-#pragma unroll
-              for (int j = 0; j < N; ++j)
-                a += tc(TB::An(1) + i * N + j) * u[j];
-              T1n[b + i * s1] = a;
-            }
-        }
-      if constexpr (kDirect)
-        {
-          // the cell and the v faces at the x Gauss points, (S x S) along
-          // x0, x1 in one pass (i1 uniform): Q2 = (S x S) u, G2 likewise
-          for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
-            {
-              const int i1 = uniform(e / FP), l = e % FP;
-              if (l >= F)
-                continue;
-              const int i0 = l % N, v = l / N; // v = v0 + N v1
-              const T *ub = U + NN * v;
-              T a = 0;
-#pragma unroll
-              for (int j1 = 0; j1 < N; ++j1)
-                {
-                  T t = 0;
-#pragma unroll
-                  for (int j0 = 0; j0 < N; ++j0)
-                    t += tc(TB::S + i0 * N + j0) * ld(ub, j0 + N * j1);
-                  a += tc(TB::S + i1 * N + j1) * t;
-                }
-              Q2[i0 + N * i1 + NN * v] = a;
-            }
-          for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
-            {
-              const int i1 = e / (4 * NN), r = e % (4 * NN);
-              const int fs = r / NN, rest = r % NN; // rest = x0 + N * v_other
-              const int ee = fs / 2, side = fs % 2;
-              const int i0 = rest % N, vo = rest / N;
-              const T *fsrc = (side == 0 ? FL[2 + ee] : FR[2 + ee]) + NN * vo;
-              T a = 0;
-#pragma unroll
-              for (int j1 = 0; j1 < N; ++j1)
-                {
-                  T t = 0;
-#pragma unroll
-                  for (int j0 = 0; j0 < N; ++j0)
-                    t += tc(TB::S + i0 * N + j0) * ld(fsrc, j0 + N * j1);
-                  a += tc(TB::S + i1 * N + j1) * t;
-                }
-              G2[ee][side][i0 + N * (i1 + N * vo)] = a;
-            }
-        }
-      else
-        // v faces (digits x0, x1, other v): along x0, i0 uniform; lines are
-        // (face, side, x1, other v)
-        for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
-          {
-            const int i = e / (4 * NN), r = e % (4 * NN);
-            const int fs = r / NN, rest = r % NN; // rest = x1 + N * v_other
-            const int ee = fs / 2, side = fs % 2;
-            const T *fsrc = side == 0 ? FL[2 + ee] : FR[2 + ee];
-            T a = 0;
-#pragma unroll
-            for (int j = 0; j < N; ++j)
-              a += tc(TB::S + i * N + j) * ld(fsrc, j + N * rest);
-            G1Y[fs * F + i + N * rest] = a;
-          }
-      lds_sync();
-      // S2 (FP64): the x1 transform of Q1 and of the v faces (i1 uniform)
-      if constexpr (!kDirect)
-        {
-          for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
-            {
-              const int i = uniform(e / FP), l = e % FP;
-              if (l >= F)
-                continue;
-              const int b = line_base<N>(l, 1);
-              T a = 0;
-#pragma unroll
-              for (int j = 0; j < N; ++j)
-                a += tc(TB::S + i * N + j) * ld(Q1, b + j * s1);
-              Q2[b + i * s1] = a;
-            }
-          for (int e = threadIdx.x; e < N * 4 * NN; e += blockDim.x)
-            {
-              const int i = e / (4 * NN), r = e % (4 * NN);
-              const int fs = r / NN, rest = r % NN; // x0 + N * v_other
-              const int ee = fs / 2, side = fs % 2;
-              const int x0 = rest % N, vo = rest / N;
-              T a = 0;
-#pragma unroll
-              for (int j = 0; j < N; ++j)
-                a += tc(TB::S + i * N + j) *
-                     ld(G1Y + fs * F, x0 + N * (j + N * vo));
-              G2[ee][side][x0 + N * (i + N * vo)] = a;
-            }
-        }
-      // the v_d factors of the x part (i2, i3 uniform) into R0, R1 (U and
-      // FLR: S1 has consumed them); FP32: the v_e operators follow in the
-      // same phase
-      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
-        {
-          const int i = uniform(e / FP), l = e % FP;
-          if (l >= F)
-            continue;
-          const int b = line_base<N>(l, 2);
-          T a = 0;
-#pragma unroll
-            for (int j = 0; j < N; ++j)
-              a += V[0][0][i * N + j] * ld(T0p, b + j * s2);
-#pragma unroll
-            for (int j = 0; j < N; ++j)
-              a += V[0][1][i * N + j] * ld(T0n, b + j * s2);
-          R0[b + i * s2] = a;
-        }
-      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
-        {
-          const int i = uniform(e / FP), l = e % FP;
-          if (l >= F)
-            continue;
-          const int b = line_base<N>(l, 3);
-          T a = 0;
-#pragma unroll
-            for (int j = 0; j < N; ++j)
-              a += V[1][0][i * N + j] * ld(T1p, b + j * s3);
-#pragma unroll
-            for (int j = 0; j < N; ++j)
-              a += V[1][1][i * N + j] * ld(T1n, b + j * s3);
-          R1[b + i * s3] = a;
-        }
-      // FP64: R0, R1 have read T0p, T0n, which become Y0, Y1
-      if constexpr (!kDirect)
-        lds_sync();
-      // the v_e operators at the x Gauss points (i2 / i3 uniform), into Y0,
-      // Y1
-#pragma unroll
-      for (int ee = 0; ee < 2; ++ee)
-        {
-          const int d = 2 + ee, sd = ee == 0 ? s2 : s3;
-          T *Yd = ee == 0 ? Y0 : Y1;
-          for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
-            {
-              const int i = uniform(e / FP), l = e % FP;
-              if (l >= F)
-                continue;
-              const int b = line_base<N>(l, d);
-              const int qx = b % NN; // digits x0, x1 of the line
-              const T c = E[ee][qx];
-              T q[N];
-#pragma unroll
-              for (int j = 0; j < N; ++j)
-                q[j] = ld(Q2, b + j * sd);
-              // face index of the line: (x0, x1, other v) = (qx, v_other)
-              const int f = qx + NN * (b / (ee == 0 ? s3 : s2) % N);
-              T yp = tc(TB::bp(d) + i) * G2[ee][0][f];
-              T yn = tc(TB::bn(d) + i) * G2[ee][1][f];
-#pragma unroll
-              for (int j = 0; j < N; ++j)
-                {
-                  yp += tc(TB::Ap(d) + i * N + j) * q[j];
-                  yn += tc(TB::An(d) + i * N + j) * q[j];
-                }
-              Yd[b + i * sd] = c * (c >= (T)0 ? yp : yn);
-            }
-        }
-      lds_sync();
-      // S4: back along x0 (i0 uniform) into Z
-      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
-        {
-          const int i = uniform(e / FP), l = e % FP;
-          if (l >= F)
-            continue;
-          const int b = line_base<N>(l, 0);
-          T a = 0;
-#pragma unroll
-          for (int j = 0; j < N; ++j)
-            a += tc(TB::Si + i * N + j) * (ld(Y0, b + j) + ld(Y1, b + j));
-          Z[b + i] = a;
-        }
-      lds_sync();
-      // S5: back along x1 (i1 uniform), plus the x part, straight to HBM
-      // (a wave writes runs of N values along x0); with rho_part set, the
-      // next stage input also goes to LDS unrounded, for its density
-      const bool dens = MODE == 1 && stage.rho_part && stage.ti_out;
-      for (int e = threadIdx.x; e < MP::E; e += blockDim.x)
-        {
-          const int i = uniform(e / FP), l = e % FP;
-          if (l >= F)
-            continue;
-          const int b = line_base<N>(l, 1);
-          T a = ld(R0, b + i * s1) + ld(R1, b + i * s1);
-#pragma unroll
-          for (int j = 0; j < N; ++j)
-            a += tc(TB::Si + i * N + j) * ld(Z, b + j * s1);
-          const int t = b + i * s1;
-          const long long g = cell * C + t;
-          if constexpr (MODE == 0)
-            dst[g] = a;
-          else
-            {
-              const double so = stage.sol[g];
-              stage.sol[g] = so + stage.b_dt * (double)a;
-              if (stage.ti_out)
-                {
-                  const double tn = so + stage.a_dt * (double)a;
-                  static_cast<T *>(stage.ti_out)[g] = (T)tn;
-                  if (dens)
-                    RD[t] = tn;
-                }
-            }
-        }
-      lds_sync(); // the LDS buffers are reused by the next cell
-      if (dens)
-        {
-          // int f dv over this cell per x node (the Gauss-Lobatto v rule on
-          // the nodes, numerics/vector_tools.h:230-315), in FP64 from the
-          // unrounded stage value: one partial per (v cell, x node)
-          const long long vc = cell / n_x_cells, xc = cell % n_x_cells;
-          for (int t = threadIdx.x; t < NN; t += blockDim.x)
-            {
-              double acc = 0.0;
-#pragma unroll
-              for (int q = 0; q < NN; ++q)
-                acc += tab[TB::wv + q] * RD[t + NN * q];
-              stage.rho_part[vc * n_x_cells * NN + xc * NN + t] = acc;
-            }
-        }
-    }
-}
-
-template <int N, typename T>
-hipError_t launch(int mode, const T *src, T *dst, const double *tab,
-                  const MeshDev &mesh, long long n_x_cells,
-                  const double *efield, const StageDev &stage,
-                  hipStream_t stream)
-{
-  const unsigned grid =
-    (unsigned)std::min<long long>(mesh.n_cells, 256LL * 64);
-  if (mode == 0)
-    hipLaunchKernelGGL((apply_kernel<N, T, 0>), dim3(grid), dim3(256), 0,
-                       stream, src, dst, tab, mesh, n_x_cells, efield, stage);
-  else
-    hipLaunchKernelGGL((apply_kernel<N, T, 1>), dim3(grid), dim3(256), 0,
-                       stream, src, dst, tab, mesh, n_x_cells, efield, stage);
-  return hipGetLastError();
-}
-} // namespace vpk
+// ---- the 2+2 phase-space apply: hd_vpwave.hip ----
 
 // ---- diagnostics: mass, ||f||^2, kinetic energy, momentum (3), per cell ----
 __global__ __launch_bounds__(256) void moments_kernel(
@@ -1765,34 +1212,16 @@ int vp_apply_table_size(int N)
   switch (N)
     {
       case 2:
-        return vlasov::vpk::Tab<2>::size;
+        return vlasov::vptab::Tab<2>::size;
       case 3:
-        return vlasov::vpk::Tab<3>::size;
+        return vlasov::vptab::Tab<3>::size;
       case 4:
-        return vlasov::vpk::Tab<4>::size;
+        return vlasov::vptab::Tab<4>::size;
       case 5:
-        return vlasov::vpk::Tab<5>::size;
+        return vlasov::vptab::Tab<5>::size;
       default:
         return 0;
     }
 }
 
-hipError_t vp_apply(int N, int mode, int precision, const void *src, void *dst,
-                    const double *tab, const MeshDev &mesh,
-                    long long n_x_cells, const double *efield,
-                    const StageDev &stage, hipStream_t stream)
-{
-#define HD_VPK(NN_)                                                            \
-  if (N == NN_)                                                                \
-    return precision == 0                                                      \
-             ? vlasov::vpk::launch<NN_, double>(mode, (const double *)src,             \
-                                        (double *)dst, tab, mesh, n_x_cells,   \
-                                        efield, stage, stream)                 \
-             : vlasov::vpk::launch<NN_, float>(mode, (const float *)src,               \
-                                       (float *)dst, tab, mesh, n_x_cells,     \
-                                       efield, stage, stream);
-  HD_VPK(2) HD_VPK(3) HD_VPK(4) HD_VPK(5)
-#undef HD_VPK
-  return hipErrorInvalidValue;
-}
 } // namespace hd

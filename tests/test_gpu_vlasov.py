@@ -83,7 +83,7 @@ def test_phase_space_apply_matches_oracle():
                                             (4, [2, 3, 3, 3])])
 def test_apply_and_stage_match_oracle_with_shared_potential(degree, n_cells,
                                                             algo):
-    """The production 2+2 Vlasov-Poisson apply (the phase-space kernel vpk,
+    """The production 2+2 Vlasov-Poisson apply (the phase-space kernel,
     or the quadrature kernel) against the oracle at 1e-12 on one potential:
     the oracle's converged phi (CG to 1e-14) is loaded into the operator,
     whose own solve then starts converged and takes no CG iteration, so both
@@ -185,14 +185,17 @@ def test_vlasov_poisson_golden_on_gpu():
     assert np.allclose(rows[1:, 1], gold[1:, 1], rtol=1e-7)
 
 
-@pytest.mark.parametrize("n_cells", [[4, 2, 4, 4], [2, 3, 3, 5], [3, 2, 2, 4]])
-@pytest.mark.parametrize("degree", [2, 3, 4])
+@pytest.mark.parametrize("n_cells", [[4, 2, 4, 4], [2, 3, 3, 5], [3, 2, 2, 4],
+                                     [3, 3, 2, 3]])
+@pytest.mark.parametrize("degree", [1, 2, 3, 4])
 def test_phase_space_kernel_matches_quadrature_kernel(n_cells, degree):
     """The 2+2 phase-space kernel (per-direction rewrite) against the
     quadrature kernel (the reference's algorithm step by step) on the same
     field: FP64 apply and stage to 1e-12, FP32-mixed apply to 2e-6.  Odd v
     cell counts put a cell across v = 0 (both upwind signs in one cell); the
-    perturbed density gives fields of both signs."""
+    perturbed density gives fields of both signs; x cell counts that are not
+    a multiple of a wave's cells (16, 7, 4, 2 for k = 1..4) leave a partial
+    group of cells at the end of every v cell."""
     print("case: vp kernel", n_cells, degree, flush=True)
     lo, hi = [0, 0, -6, -6], [4 * PI, 2 * PI, 5, 6]
     mf = H.MatrixFree(2, 2, degree, n_cells, lo, hi)
