@@ -746,6 +746,16 @@ int hd_vlasov_poisson_create(hd_matrix_free *mf, double skew_factor,
               o[nd * nd + i] = cp.b[i];
               o[2 * nd * nd + nd + i] = -cn.b[i];
             }
+          // difference form: A u + b f = A^ (u - u_o 1) + b (f - u_o) with
+          // A^ = A + b e_o^T and o the outflow node (N-1 for +, 0 for -);
+          // A^ 1 = A 1 + b = 0, so both parts act on differences of nearby
+          // values and the mass of a cell changes by exactly b-weighted
+          // (f - u_o), which telescopes over a periodic row of cells
+          for (int i = 0; i < nd; ++i)
+            {
+              o[i * nd + nd - 1] += cp.b[i];
+              o[nd * nd + nd + i * nd] += -cn.b[i];
+            }
         }
       double *o = tab.data() + 4 * per;
       std::copy(t.S.begin(), t.S.begin() + nd * nd, o);

@@ -4,7 +4,9 @@
 // Doubles, in order:
 //   per direction d (x0, x1, v0, v1): A+ (N^2), b+ (N), A- (N^2), b- (N) --
 //     the collapsed 1D operator for speed +1 and minus the one for speed -1,
-//     so that the operator for speed c is c A+ (c >= 0) or c A- (c < 0);
+//     so that the operator for speed c is c A+ (c >= 0) or c A- (c < 0),
+//     with A stored in difference form A^ = A + b e_o^T (o: the outflow
+//     node, N-1 for +, 0 for -): A u + b f = A^ (u - u_o) + b (f - u_o);
 //   S (N^2, row q: Gauss point), S^-1 (N^2), the Gauss points (N), JxW_v at
 //     the Gauss-Lobatto v nodes (N^2);
 // then the FP32 copy of all of the above (size / 2 rounded up doubles);

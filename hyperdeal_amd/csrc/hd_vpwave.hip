@@ -176,6 +176,7 @@ __global__ __launch_bounds__(64) void stage_kernel(
   const long long n_vfac = (long long)(nv0 + nv1) * 2 * N2;
   const cptr<double> vflag = vfac + n_vfac;
   const cptr<float> vfacf = (cptr<float>)(vflag + nv0 + nv1);
+
   auto vcoef = [&](long long i) -> T {
     if constexpr (sizeof(T) == 4)
       return vfacf[i];
@@ -405,25 +406,60 @@ __global__ __launch_bounds__(64) void stage_kernel(
             {
               const int oA = sg0 == 0 ? TB::Ap(0) : TB::An(0);
               const int ob = sg0 == 0 ? TB::bp(0) : TB::bn(0);
-              T y[N2];
-              along0<N>(u, y, [&](int r) { return tc(oA + r); });
+              // difference form (A^ 1 = 0): A^ u + b (f - u_o); FP32 applies
+              // A^ to u - u_o as well, so that no product carries the
+              // magnitude of u
+              T dd[N2], uo[N];
 #pragma unroll
               for (int b = 0; b < N; ++b)
+                {
+                  uo[b] = sg0 == 0 ? u[N - 1 + N * b] : u[N * b];
 #pragma unroll
-                for (int i = 0; i < N; ++i)
-                  B0[t * P + i + N * b] = y[i + N * b] + tc(ob + i) * fx[0][b];
+                  for (int k = 0; k < N; ++k)
+                    dd[k + N * b] =
+                      sizeof(T) == 4 ? u[k + N * b] - uo[b] : u[k + N * b];
+                }
+              T y[N2];
+              along0<N>(dd, y, [&](int r) { return tc(oA + r); });
+#pragma unroll
+              for (int b = 0; b < N; ++b)
+                {
+                  const T df = fx[0][b] - uo[b];
+#pragma unroll
+                  for (int i = 0; i < N; ++i)
+                    {
+                      T v = y[i + N * b] + tc(ob + i) * df;
+                      B0[t * P + i + N * b] = v;
+                    }
+                }
             }
           if (busy && sg1 >= 0)
             {
               const int oA = sg1 == 0 ? TB::Ap(1) : TB::An(1);
               const int ob = sg1 == 0 ? TB::bp(1) : TB::bn(1);
+              T dd[N2], uo[N];
+#pragma unroll
+              for (int a = 0; a < N; ++a)
+                {
+                  uo[a] = sg1 == 0 ? u[a + N * (N - 1)] : u[a];
+#pragma unroll
+                  for (int k = 0; k < N; ++k)
+                    dd[a + N * k] =
+                      sizeof(T) == 4 ? u[a + N * k] - uo[a] : u[a + N * k];
+                }
               T y[N2];
-              along1<N>(u, y, [&](int r) { return tc(oA + r); });
+              along1<N>(dd, y, [&](int r) { return tc(oA + r); });
 #pragma unroll
-              for (int i = 0; i < N; ++i)
+              for (int a = 0; a < N; ++a)
+                {
+                  const T df = fx[1][a] - uo[a];
 #pragma unroll
-                for (int a = 0; a < N; ++a)
-                  B1[t * P + a + N * i] = y[a + N * i] + tc(ob + i) * fx[1][a];
+                  for (int i = 0; i < N; ++i)
+                    {
+                      T v = y[a + N * i] + tc(ob + i) * df;
+                      B1[t * P + a + N * i] = v;
+                    }
+                }
             }
           wave_sync();
           // V1: nodal += V_0 P_0 (along v0) + V_1 P_1 (along v1), lane =
@@ -493,20 +529,38 @@ __global__ __launch_bounds__(64) void stage_kernel(
             constexpr int ob = side == 0 ? TB::bp(2 + e) : TB::bn(2 + e);
             const bool use = side == 0 ? ee[e] >= (T)0 : ee[e] < (T)0;
             const T *fh = FH + (2 * e + side) * N * P + t;
-            T y[N2];
-            if constexpr (e == 0)
-              along0<N>(uh, y, [&](int r) { return tc(oA + r); });
-            else
-              along1<N>(uh, y, [&](int r) { return tc(oA + r); });
+            // difference form along v_e (FP32: lines minus their outflow
+            // value)
+            constexpr int o = side == 0 ? N - 1 : 0;
+            T dd[N2], uo[N];
 #pragma unroll
             for (int m = 0; m < N; ++m)
               {
-                T f = fh[m * P];
+                uo[m] = e == 0 ? uh[o + N * m] : uh[m + N * o];
+#pragma unroll
+                for (int k = 0; k < N; ++k)
+                  {
+                    const int j = e == 0 ? k + N * m : m + N * k;
+                    dd[j] = sizeof(T) == 4 ? uh[j] - uo[m] : uh[j];
+                  }
+              }
+            T y[N2];
+            if constexpr (e == 0)
+              along0<N>(dd, y, [&](int r) { return tc(oA + r); });
+            else
+              along1<N>(dd, y, [&](int r) { return tc(oA + r); });
+#pragma unroll
+            for (int m = 0; m < N; ++m)
+              {
+                T df = fh[m * P] - uo[m];
                 if (masked && !use)
-                  f = 0;
+                  df = 0;
 #pragma unroll
                 for (int i = 0; i < N; ++i)
-                  y[e == 0 ? i + N * m : m + N * i] += tc(ob + i) * f;
+                  {
+                    T v = tc(ob + i) * df;
+                    y[e == 0 ? i + N * m : m + N * i] += v;
+                  }
               }
 #pragma unroll
             for (int j = 0; j < N2; ++j)
