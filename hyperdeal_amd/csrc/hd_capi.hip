@@ -202,6 +202,17 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
               static_cast<float *>(dst), static_cast<float *>(part),
               op->coef32, mf->mesh, op->inflow, stage, stream);
         }
+      else if (hd::collapsed_wave_supported(mf->D, mf->nd))
+        e = precision == HD_FP64
+              ? hd::collapsed_wave_f64(mf->nd, mode,
+                                       static_cast<const double *>(src),
+                                       static_cast<double *>(dst),
+                                       op->coef64, mf->mesh, op->inflow,
+                                       stage, stream)
+              : hd::collapsed_wave_f32(mf->nd, mode,
+                                       static_cast<const float *>(src),
+                                       static_cast<float *>(dst), op->coef32,
+                                       mf->mesh, op->inflow, stage, stream);
       else if (precision == HD_FP64)
         e = hd::collapsed_apply_f64(mf->D, mf->nd, mode,
                                     static_cast<const double *>(src),

@@ -1075,10 +1075,14 @@ int batch_bpc(int N)
 }
 } // namespace
 
+// degrees with the wave-per-group kernel (hd_cvwave.hip) batch through it,
+// so that a batch equals its single-operator calls bit for bit
 int collapsed_batch_cells_per_block(int D, int N, int precision)
 {
   if (!collapsed_batch_supported(D, N))
     return 0;
+  if (collapsed_wave_supported(D, N))
+    return collapsed_wave_cells_per_unit(N);
   return precision == 0 ? batch_cpb<double>(N) : batch_cpb<float>(N);
 }
 
@@ -1086,6 +1090,9 @@ int collapsed_batch_blocks_per_xcd(int N, int precision)
 {
   if (N < 2 || N > 6)
     return 0;
+  if (collapsed_wave_supported(4, N))
+    return collapsed_wave_blocks_per_cu(N, precision) *
+           collapsed::cus_per_xcd();
   return (precision == 0 ? batch_bpc<double>(N) : batch_bpc<float>(N)) *
          collapsed::cus_per_xcd();
 }
@@ -1097,6 +1104,9 @@ hipError_t collapsed_batch(int D, int N, int precision, int mode,
 {
   if (!collapsed_batch_supported(D, N))
     return hipErrorInvalidValue;
+  if (collapsed_wave_supported(D, N))
+    return collapsed_wave_batch(N, precision, mode, ops, tasks, plan, ptrs,
+                                stream);
 #define HD_BN(NN_)                                                           \
   if (N == NN_)                                                              \
     return precision == 0                                                    \

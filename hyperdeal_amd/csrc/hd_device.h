@@ -238,6 +238,20 @@ hipError_t collapsed_apply_f32(int D, int N, int mode, const float *src,
                                float *dst, const float *coef,
                                const MeshDev &mesh, const InflowDev &inflow,
                                const StageDev &stage, hipStream_t stream);
+// 2+2 (D = 4) of low degree through the wave-per-group kernel
+// (hd_cvwave.hip); batches of such operators likewise (units: groups of
+// collapsed_wave_cells_per_unit cells, one wave per block)
+bool collapsed_wave_supported(int D, int N);
+int collapsed_wave_cells_per_unit(int N);
+int collapsed_wave_blocks_per_cu(int N, int precision);
+hipError_t collapsed_wave_f64(int N, int mode, const double *src, double *dst,
+                              const double *coef, const MeshDev &mesh,
+                              const InflowDev &inflow, const StageDev &stage,
+                              hipStream_t stream);
+hipError_t collapsed_wave_f32(int N, int mode, const float *src, float *dst,
+                              const float *coef, const MeshDev &mesh,
+                              const InflowDev &inflow, const StageDev &stage,
+                              hipStream_t stream);
 // batches: one kernel per degree N.  The dimensions and degrees with a
 // batch kernel, the cells one block handles at a time (a task's units are
 // multiples of it), the blocks per XCD of degree N's kernel, the batched
@@ -250,6 +264,10 @@ hipError_t collapsed_batch(int D, int N, int precision, int mode,
                            const BatchOp *ops, const BatchTask *tasks,
                            const BatchPlan &plan, const BatchPtrs &ptrs,
                            hipStream_t stream);
+hipError_t collapsed_wave_batch(int N, int precision, int mode,
+                                const BatchOp *ops, const BatchTask *tasks,
+                                const BatchPlan &plan, const BatchPtrs &ptrs,
+                                hipStream_t stream);
 hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,
                          const BatchPtrs &ptrs, hipStream_t stream);
 // operators too large for one cell in LDS (D >= 5, N beyond the
