@@ -224,6 +224,17 @@ def landau_workload(H, torch, stream, steps):
                                              2)}
         del op, f, ta, tb
         torch.cuda.empty_cache()
+    # the learned start against the reference's previous-phi start
+    # (vlasov_poisson/include/application.h:575-576), both FP32-mixed
+    pl, ln = out["runs"]["fp32"], out["runs"]["fp32-learned"]
+    out["learned_vs_previous_phi"] = {
+        "cg_iterations_ratio": round(pl["cg_iterations_per_solve"] and
+                                     ln["cg_iterations_per_solve"] /
+                                     pl["cg_iterations_per_solve"], 4),
+        "step_time_ratio": round(ln["ms_per_step"] / pl["ms_per_step"], 4),
+        "note": "below 1: the learned start wins; the network's training "
+                "and inference cost more per solve than the CG iterations "
+                "they save when the step time ratio is above 1"}
     return out
 
 
