@@ -202,7 +202,8 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
               static_cast<float *>(dst), static_cast<float *>(part),
               op->coef32, mf->mesh, op->inflow, stage, stream);
         }
-      else if (hd::collapsed_wave_supported(mf->D, mf->nd))
+      else if (hd::collapsed_wave_supported(mf->D, mf->nd,
+                                            precision == HD_FP64 ? 0 : 1))
         e = precision == HD_FP64
               ? hd::collapsed_wave_f64(mf->nd, mode,
                                        static_cast<const double *>(src),

@@ -1081,7 +1081,7 @@ int collapsed_batch_cells_per_block(int D, int N, int precision)
 {
   if (!collapsed_batch_supported(D, N))
     return 0;
-  if (collapsed_wave_supported(D, N))
+  if (collapsed_wave_supported(D, N, precision))
     return collapsed_wave_cells_per_unit(N);
   return precision == 0 ? batch_cpb<double>(N) : batch_cpb<float>(N);
 }
@@ -1090,7 +1090,7 @@ int collapsed_batch_blocks_per_xcd(int N, int precision)
 {
   if (N < 2 || N > 6)
     return 0;
-  if (collapsed_wave_supported(4, N))
+  if (collapsed_wave_supported(4, N, precision))
     return collapsed_wave_blocks_per_cu(N, precision) *
            collapsed::cus_per_xcd();
   return (precision == 0 ? batch_bpc<double>(N) : batch_bpc<float>(N)) *
@@ -1104,7 +1104,7 @@ hipError_t collapsed_batch(int D, int N, int precision, int mode,
 {
   if (!collapsed_batch_supported(D, N))
     return hipErrorInvalidValue;
-  if (collapsed_wave_supported(D, N))
+  if (collapsed_wave_supported(D, N, precision))
     return collapsed_wave_batch(N, precision, mode, ops, tasks, plan, ptrs,
                                 stream);
 #define HD_BN(NN_)                                                           \

@@ -389,6 +389,12 @@ hipError_t dispatch(int N, int mode, const T *src, T *dst, const T *coef,
       case 3:
         return launch<3, T>(mode, src, dst, coef, mesh, inflow, stage,
                             stream);
+      case 5:
+        if constexpr (sizeof(T) == 4)
+          return launch<5, T>(mode, src, dst, coef, mesh, inflow, stage,
+                              stream);
+        else
+          return hipErrorInvalidValue;
       default:
         return hipErrorInvalidValue;
     }
@@ -462,19 +468,24 @@ hipError_t batch_launch(int mode, const BatchOp *ops, const BatchTask *tasks,
 }
 } // namespace cvw
 
-// 2+2 operators of degree k <= 2 (N <= 3) run here: measured on MI355X
-// (GDoF/s, 2+2, 10^8 DoF, against the tile kernel of hd_collapsed.hip):
-// k = 2 FP64 stage 114.6 vs 84.7, FP32 stage 154.5 vs 101.6, FP32 apply
-// 300.8 vs 280.4, FP64 apply 171.2 vs 171.5.  At k >= 3 the tile kernel is
-// as fast or faster (k = 5: a wave holds one cell on 36 of its 64 lanes).
-bool collapsed_wave_supported(int D, int N)
+// Which 2+2 operators run here, measured on MI355X (GDoF/s, 2+2, about
+// 10^8 DoF, this kernel against the tile kernel of hd_collapsed.hip):
+//   k = 2 (N = 3): FP64 stage 114.9 vs 84.7, FP32 stage 154.7 vs 101.6,
+//                  FP32 apply 292.7 vs 280.4, FP64 apply 171.6 vs 171.5
+//   k = 4 (N = 5), FP32: stage 178.3 vs 140.6, apply 312.4 vs 373.4 (the
+//                  stage is what an LSRK run repeats); FP64 stage 109.6 vs
+//                  113.9 and apply 184.5 vs 195.9 stay on the tile kernel
+//   k = 3: within 5 %; k = 5: slower throughout (a wave holds one cell on
+//   36 of its 64 lanes)
+bool collapsed_wave_supported(int D, int N, int precision)
 {
-  return D == 4 && N >= 2 && N <= 3;
+  return D == 4 && (N == 2 || N == 3 || (N == 5 && precision != 0));
 }
 
 int collapsed_wave_cells_per_unit(int N)
 {
-  return N == 2 ? cvw::Geo<2>::CPW : cvw::Geo<3>::CPW;
+  return N == 2 ? cvw::Geo<2>::CPW
+                : (N == 3 ? cvw::Geo<3>::CPW : cvw::Geo<5>::CPW);
 }
 
 int collapsed_wave_blocks_per_cu(int N, int precision)
@@ -482,8 +493,10 @@ int collapsed_wave_blocks_per_cu(int N, int precision)
   if (N == 2)
     return precision == 0 ? cvw::waves_per_cu<2, double, 1>()
                           : cvw::waves_per_cu<2, float, 1>();
-  return precision == 0 ? cvw::waves_per_cu<3, double, 1>()
-                        : cvw::waves_per_cu<3, float, 1>();
+  if (N == 3)
+    return precision == 0 ? cvw::waves_per_cu<3, double, 1>()
+                          : cvw::waves_per_cu<3, float, 1>();
+  return cvw::waves_per_cu<5, float, 1>();
 }
 
 hipError_t collapsed_wave_batch(int N, int precision, int mode,
@@ -503,6 +516,8 @@ hipError_t collapsed_wave_batch(int N, int precision, int mode,
                                             stream)
              : cvw::batch_launch<3, float>(mode, ops, tasks, plan, ptrs,
                                            stream);
+  if (N == 5 && precision != 0)
+    return cvw::batch_launch<5, float>(mode, ops, tasks, plan, ptrs, stream);
   return hipErrorInvalidValue;
 }
 

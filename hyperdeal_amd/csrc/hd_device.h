@@ -241,7 +241,7 @@ hipError_t collapsed_apply_f32(int D, int N, int mode, const float *src,
 // 2+2 (D = 4) of low degree through the wave-per-group kernel
 // (hd_cvwave.hip); batches of such operators likewise (units: groups of
 // collapsed_wave_cells_per_unit cells, one wave per block)
-bool collapsed_wave_supported(int D, int N);
+bool collapsed_wave_supported(int D, int N, int precision);
 int collapsed_wave_cells_per_unit(int N);
 int collapsed_wave_blocks_per_cu(int N, int precision);
 hipError_t collapsed_wave_f64(int N, int mode, const double *src, double *dst,
