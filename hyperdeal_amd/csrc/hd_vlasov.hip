@@ -814,6 +814,14 @@ __global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
     }
   rr = block_sum<W, 1>(rr, red); // also: every thread is done with Pt
   const double rr_start = rr;
+  // the iterate of this thread's cells stays in registers during the
+  // iterations (no global read-modify-write on the critical path)
+  double xk[CPT][NN];
+#pragma unroll
+  for (int k = 0; k < CPT; ++k)
+#pragma unroll
+    for (int i = 0; i < NN; ++i)
+      xk[k][i] = own[k] ? phit[(long long)i * nc + cl[k]] : 0.0;
 #pragma unroll
   for (int k = 0; k < CPT; ++k)
     if (own[k])
@@ -846,8 +854,7 @@ __global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
 #pragma unroll
           for (int i = 0; i < NN; ++i)
             {
-              phit[(long long)i * nc + cl[k]] +=
-                alpha * Pt[i * kPStride + cl[k]];
+              xk[k][i] += alpha * Pt[i * kPStride + cl[k]];
               r[k][i] -= alpha * y[k][i];
               rr2 += r[k][i] * r[k][i];
             }
@@ -875,7 +882,7 @@ __global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
         double x[NN], g[NN];
 #pragma unroll
         for (int i = 0; i < NN; ++i)
-          x[i] = phit[(long long)i * nc + cl[k]];
+          x[i] = xk[k][i];
 #pragma unroll
         for (int i = 0; i < NN; ++i)
           phi[(long long)cl[k] * NN + i] = x[i];
