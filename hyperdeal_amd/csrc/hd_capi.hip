@@ -173,7 +173,13 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
         return fail(HD_ERR_INVALID,
                     "the collapsed kernel needs a 16-byte aligned source "
                     "vector");
-      if (op->split)
+      if (op->split && precision != HD_FP64 &&
+          hd::collapsed_cres_supported(mf->D, mf->nd))
+        e = hd::collapsed_cres_f32(mf->nd, mode,
+                                   static_cast<const float *>(src),
+                                   static_cast<float *>(dst), op->coef32,
+                                   mf->mesh, op->inflow, stage, stream);
+      else if (op->split)
         {
           // the first part lands in dst (apply), in ti_out (stage), or in a
           // free vector (last stage)

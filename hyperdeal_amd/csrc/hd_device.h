@@ -283,6 +283,13 @@ hipError_t collapsed_split_f32(int D, int N, int mode, const float *src,
                                float *dst, float *part, const float *coef,
                                const MeshDev &mesh, const InflowDev &inflow,
                                const StageDev &stage, hipStream_t stream);
+// FP32 3+3 cells larger than LDS in one pass, the cell's result held in a
+// block's registers (hd_cres.hip)
+bool collapsed_cres_supported(int D, int N);
+hipError_t collapsed_cres_f32(int N, int mode, const float *src, float *dst,
+                              const float *coef, const MeshDev &mesh,
+                              const InflowDev &inflow, const StageDev &stage,
+                              hipStream_t stream);
 // ND = k+1 nodes, NQ quadrature points per direction (NQ = ND or ND + 1)
 bool quadrature_supported(int D, int ND, int NQ);
 // geo: the deformed mesh's tables (geo.on = 0: Cartesian)

@@ -114,3 +114,36 @@ def test_split_lsrk_step_matches_recurrence():
                  0.0 if last else ca[s] * dt, cb[s] * dt)
     torch.cuda.synchronize()
     assert _rel(sol2.cpu().numpy(), ref.cpu().numpy()) <= 1e-12
+
+
+@pytest.mark.parametrize("case", [0, 1])
+def test_split_fp32_stages_match_fp64(case):
+    """FP32-mixed LSRK stages of 3+3 cells larger than LDS (the one-pass
+    kernel whose cell result lives in a block's registers) against the FP64
+    stages on the same FP32-rounded input: sol += b dt K, ti = sol + a dt K
+    (middle stage) and sol += b dt K alone (last stage), within 2e-6 of
+    b dt max |K| (FP32 arithmetic of K)."""
+    dims, k, n_cells, a = CASES[case]
+    print("case: split fp32 stages", dims, k, n_cells, flush=True)
+    P, mf, op = _pair(dims, k, n_cells, a, seed=7 + case)
+    r = np.random.default_rng(11)
+    u32 = torch.from_numpy(r.standard_normal(P.n_dofs)).cuda().float()
+    u64 = u32.double()
+    sol0 = torch.from_numpy(r.standard_normal(P.n_dofs)).cuda()
+    kref = torch.empty_like(u64)
+    op.apply(kref, u64)
+    torch.cuda.synchronize()
+    kmax = kref.abs().max().item()
+    ad, bd = 0.37, 0.21
+    for last in (False, True):
+        s64, s32 = sol0.clone(), sol0.clone()
+        t64 = None if last else torch.empty_like(u64)
+        t32 = None if last else torch.empty_like(u32)
+        op.stage(u64, t64, s64, 0.0, 0.0 if last else ad, bd)
+        op.stage(u32, t32, s32, 0.0, 0.0 if last else ad, bd,
+                 precision=H.FP32)
+        torch.cuda.synchronize()
+        assert (s32 - s64).abs().max().item() <= 2e-6 * bd * kmax
+        if not last:
+            err = (t32.double() - t64).abs().max().item()
+            assert err <= 2e-6 * ad * kmax + 6e-8 * t64.abs().max().item()
