@@ -195,7 +195,7 @@ def landau_workload(H, torch, stream, steps):
             op.exact_start()
         if name.endswith("-learned"):
             op.enable_learned_guess(seed=20261205, capacity=64,
-                                    train_every=1, train_batch=8,
+                                    train_every=1, train_batch=4,
                                     train_steps=1, learning_rate=1e-3,
                                     min_pairs=8)
         prec = H.FP64 if name == "fp64" else H.FP32

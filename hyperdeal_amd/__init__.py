@@ -54,6 +54,7 @@ EXPORTED = [
     "hd_graph_launch", "hd_graph_destroy", "hd_vp_solver_start",
     "hd_vp_learned_guess_enable", "hd_vp_learned_guess_disable",
     "hd_vp_learned_guess_stats", "hd_vp_learned_guess_net",
+    "hd_vp_learned_guess_schedule",
     "hd_cnn_n_params", "hd_cnn_create", "hd_cnn_destroy", "hd_cnn_params",
     "hd_cnn_forward", "hd_cnn_loss_grad", "hd_cnn_train_step",
     "hd_cnn_last_loss", "hd_vp_exact_start", "hd_cnn_dofs_to_image",
@@ -149,6 +150,8 @@ def load_library(path=None):
         _vp, ctypes.c_ulonglong, ctypes.c_int, ctypes.c_int, ctypes.c_int,
         ctypes.c_int, ctypes.c_float, ctypes.c_int]
     lib.hd_vp_learned_guess_disable.argtypes = [_vp]
+    lib.hd_vp_learned_guess_schedule.argtypes = [_vp, ctypes.c_longlong,
+                                                 ctypes.c_int]
     lib.hd_vp_learned_guess_stats.argtypes = [_vp, _llp, _llp, _llp, _dp]
     lib.hd_vp_learned_guess_net.restype = _vp
     lib.hd_vp_learned_guess_net.argtypes = [_vp]
@@ -424,7 +427,7 @@ class VlasovPoissonOperation(AdvectionOperation):
         self._learned_gen = getattr(self, "_learned_gen", 0) + 1
 
     def enable_learned_guess(self, seed=20261205, capacity=64, train_every=1,
-                             train_batch=8, train_steps=1,
+                             train_batch=4, train_steps=1,
                              learning_rate=1e-3, min_pairs=4):
         """Start the potential solves from the previous phi combined with
         the CNN's correction for its residual; the network trains on this
@@ -434,6 +437,14 @@ class VlasovPoissonOperation(AdvectionOperation):
             self.handle, int(seed), int(capacity), int(train_every),
             int(train_batch), int(train_steps), float(learning_rate),
             int(min_pairs)), "hd_vp_learned_guess_enable")
+
+    def learned_guess_schedule(self, dense_solves, train_every):
+        """Train on every one of the first dense_solves solves (counted
+        from enabling), then every train_every solves (0: keep the network
+        as trained)."""
+        _check(self._lib.hd_vp_learned_guess_schedule(
+            self.handle, int(dense_solves), int(train_every)),
+            "hd_vp_learned_guess_schedule")
 
     def disable_learned_guess(self):
         self._release_learned_net()

@@ -75,11 +75,13 @@ hipError_t cnn_dofs_to_image(const double *u, const double *sub, int n0,
 hipError_t cnn_image_to_dofs(const float *image, const double *scale, int n0,
                              int n1, int N, double *u, hipStream_t s);
 //   a training batch: dst[j] = src[i_j], dst2[j] = src2[i_j] for n images of
-//   `plane` floats, i_j uniform in [0, count) from SplitMix64 (seed,
-//   counter0 + j), drawn on the device
+//   `plane` floats, i_j uniform over the `count` images [0, count] without
+//   `skip` (skip < 0: [0, count)) from SplitMix64 (seed, counter0 + j),
+//   drawn on the device
 hipError_t cnn_sample(const float *src, const float *src2, uint64_t seed,
-                      uint64_t counter0, int count, int n, long long plane,
-                      float *dst, float *dst2, hipStream_t s);
+                      uint64_t counter0, int count, int skip, int n,
+                      long long plane, float *dst, float *dst2,
+                      hipStream_t s);
 
 uint16_t cnn_to_bf16(float x);
 } // namespace hd

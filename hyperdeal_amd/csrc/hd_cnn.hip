@@ -151,20 +151,22 @@ constexpr int kWStride = 304;    // >= 9 * 32 = 288
 // without further barriers (dynamic LDS: kConvLds bytes).
 constexpr int kConvLds =
   kChannels * (kTileStride + kWStride) * (int)sizeof(float);
-template <bool ACT>
+template <bool ACT, int OH>
 __global__ __launch_bounds__(256) void conv32_mfma_kernel(
   const float *__restrict__ in, const uint32_t *__restrict__ w,
   const float *__restrict__ bias, float *__restrict__ out, int H, int W)
 {
   constexpr int C = kChannels;
-  constexpr int WPR = C / 2; // packed words per (input channel, tap)
+  constexpr int WPR = C / 2;        // packed words per (input channel, tap)
+  constexpr int WPH = WPR / OH;     // of them for this block's channels
+  constexpr int TO = 2 / OH;        // 16-channel output tiles per block
   constexpr int HW = (kMTY + 2) * (kMTX + 2);
   extern __shared__ float sm[];
   float *tile = sm;                 // [C][kTileStride]
   float *wl = sm + C * kTileStride; // [C][kWStride]: wl[i][tap][o]
   const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
   const int n16 = lane & 15, k4 = lane >> 4;
-  const int b = blockIdx.z;
+  const int b = blockIdx.z / OH, half = blockIdx.z % OH;
   const int x0 = blockIdx.x * kMTX, y0 = blockIdx.y * kMTY;
   const long long plane = (long long)H * W;
   for (int e = threadIdx.x; e < C * HW; e += 256)
@@ -175,20 +177,20 @@ __global__ __launch_bounds__(256) void conv32_mfma_kernel(
       tile[i * kTileStride + r] =
         in[((long long)b * C + i) * plane + (long long)yy * W + xx];
     }
-  for (int e = threadIdx.x; e < C * 9 * WPR; e += 256)
+  for (int e = threadIdx.x; e < C * 9 * WPH; e += 256)
     {
-      const int i = e / (9 * WPR), r = e % (9 * WPR);
-      const int tap = r / WPR, pr = r % WPR;
-      const uint32_t u = w[e];
+      const int i = e / (9 * WPH), r = e % (9 * WPH);
+      const int tap = r / WPH, pr = half * WPH + r % WPH;
+      const uint32_t u = w[(i * 9 + tap) * WPR + pr];
       wl[i * kWStride + tap * C + 2 * pr] = bits_to_float(u << 16);
       wl[i * kWStride + tap * C + 2 * pr + 1] = bits_to_float(u & 0xffff0000u);
     }
   __syncthreads();
-  mfma_f4 acc[2][2];
+  mfma_f4 acc[2][TO];
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < TO; ++t)
       acc[r][t] = mfma_f4{0.0f, 0.0f, 0.0f, 0.0f};
 #pragma unroll
   for (int tap = 0; tap < 9; ++tap)
@@ -199,34 +201,87 @@ __global__ __launch_bounds__(256) void conv32_mfma_kernel(
         {
           const int ci = q * 4 + k4;
           // A = weights (m = output channel), B = pixels (n = x)
-          const float a0 = wl[ci * kWStride + tap * C + n16];
-          const float a1 = wl[ci * kWStride + tap * C + 16 + n16];
+          float a[TO];
+#pragma unroll
+          for (int t = 0; t < TO; ++t)
+            a[t] = wl[ci * kWStride + tap * C + 16 * (half + t) + n16];
 #pragma unroll
           for (int r = 0; r < 2; ++r)
             {
               const int row = wave * 2 + r;
               const float bv =
                 tile[ci * kTileStride + (row + dy) * (kMTX + 2) + n16 + dx];
-              acc[r][0] = mfma16x16x4(a0, bv, acc[r][0]);
-              acc[r][1] = mfma16x16x4(a1, bv, acc[r][1]);
+#pragma unroll
+              for (int t = 0; t < TO; ++t)
+                acc[r][t] = mfma16x16x4(a[t], bv, acc[r][t]);
             }
         }
     }
-  // lane holds D[o = 16 t + 4 k4 + e][x = n16] of rows 2 wave + r
+  // lane holds D[o = 16 (half + t) + 4 k4 + e][x = n16] of rows 2 wave + r
 #pragma unroll
   for (int r = 0; r < 2; ++r)
 #pragma unroll
-    for (int t = 0; t < 2; ++t)
+    for (int t = 0; t < TO; ++t)
 #pragma unroll
       for (int e = 0; e < 4; ++e)
         {
-          const int oc = 16 * t + 4 * k4 + e;
+          const int oc = 16 * (half + t) + 4 * k4 + e;
           float v = acc[r][t][e] + (bias ? ((cptr<float>)bias)[oc] : 0.0f);
           if (ACT)
             v = v > 0.0f ? v : kSlope * v;
           const int y = y0 + wave * 2 + r, x = x0 + n16;
           out[((long long)b * C + oc) * plane + (long long)y * W + x] = v;
         }
+}
+
+// out[b] = bias + sum_{i,k} w[i][k] in[b][i] shifted by k for CIN -> 1
+// channels (the last layer, no activation).  A block of 256 threads owns a
+// 16 x 4 pixel tile; wave g sums the channels 8 g .. 8 g + 7 (taps inner)
+// for the lane's pixel with the inputs read straight from global memory
+// (L2: every value is read by 9 lanes of neighbouring tiles), the weights
+// through scalar loads; the four partial sums are added in wave order
+// through LDS, so the result is deterministic
+template <int CIN>
+__global__ __launch_bounds__(256) void conv_to1_kernel(
+  const float *__restrict__ in, const uint32_t *__restrict__ w,
+  const float *__restrict__ bias, float *__restrict__ out, int H, int W)
+{
+  constexpr int G = 4, CG = CIN / G;
+  static_assert(CIN % G == 0, "channel groups");
+  __shared__ float part[G][64];
+  const cptr<uint32_t> wc = (cptr<uint32_t>)w;
+  const int lane = threadIdx.x & 63, g = threadIdx.x >> 6;
+  const int tiles_x = W / 16, tiles = tiles_x * (H / 4);
+  const int b = blockIdx.x / tiles, tl = blockIdx.x % tiles;
+  const int y = (tl / tiles_x) * 4 + lane / 16;
+  const int x = (tl % tiles_x) * 16 + lane % 16;
+  const int plane = H * W;
+  const int rows[3] = {wrap(y - 1, H) * W, y * W, wrap(y + 1, H) * W};
+  const int cols[3] = {wrap(x - 1, W), x, wrap(x + 1, W)};
+  const float *ib = in + ((long long)b * CIN + g * CG) * plane;
+  float v[CG][9];
+#pragma unroll
+  for (int i = 0; i < CG; ++i)
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      v[i][k] = ib[(long long)i * plane + rows[k / 3] + cols[k % 3]];
+  float acc = 0.0f;
+#pragma unroll
+  for (int i = 0; i < CG; ++i)
+#pragma unroll
+    for (int k = 0; k < 9; ++k)
+      acc += bits_to_float(wc[(g * CG + i) * 9 + k] << 16) * v[i][k];
+  part[g][lane] = acc;
+  __syncthreads();
+  if (g == 0)
+    {
+      float t = part[0][lane];
+#pragma unroll
+      for (int j = 1; j < G; ++j)
+        t += part[j][lane];
+      out[(long long)b * plane + y * W + x] =
+        t + (bias ? ((cptr<float>)bias)[0] : 0.0f);
+    }
 }
 
 // Weight gradient of a 32 -> 32 layer: gw[o][i][tap] = sum over the batch
@@ -555,14 +610,15 @@ __global__ void pack_kernel(const float *__restrict__ w, int Ci, int Co,
 }
 
 // ---- x-space nodal values <-> images ----
-__device__ inline long long pixel_of(long long dof, int n0, int N)
+// (an image holds fewer than 2^31 pixels: shape_ok and the callers' sizes)
+__device__ inline int pixel_of(int dof, int n0, int N)
 {
   const int NN = N * N;
-  const long long c = dof / NN;
-  const int i = (int)(dof - c * NN);
-  const long long cx = c % n0, cy = c / n0;
-  const int ix = i % N, iy = i / N;
-  return (cy * N + iy) * ((long long)n0 * N) + cx * N + ix;
+  const int c = dof / NN;
+  const int i = dof - c * NN;
+  const int cy = c / n0, cx = c - cy * n0;
+  const int iy = i / N, ix = i - iy * N;
+  return (cy * N + iy) * (n0 * N) + cx * N + ix;
 }
 
 __device__ inline double block_sum(double v, double *red)
@@ -621,11 +677,11 @@ __global__ __launch_bounds__(1024) void dofs_to_image_kernel(
     {
       const long long i = threadIdx.x + (long long)k * blockDim.x;
       if (i < n)
-        image[pixel_of(i, n0, N)] = (float)((keep[k] - mean) * inv);
+        image[pixel_of((int)i, n0, N)] = (float)((keep[k] - mean) * inv);
     }
   for (long long i = threadIdx.x + (long long)kKeep * blockDim.x; i < n;
        i += blockDim.x)
-    image[pixel_of(i, n0, N)] = (float)((val(i) - mean) * inv);
+    image[pixel_of((int)i, n0, N)] = (float)((val(i) - mean) * inv);
   if (threadIdx.x == 0)
     *scale = rms;
 }
@@ -644,7 +700,7 @@ __global__ void image_to_dofs_kernel(const float *__restrict__ image,
   for (long long i = blockIdx.x * (long long)blockDim.x + threadIdx.x; i < n;
        i += (long long)gridDim.x * blockDim.x)
     {
-      const long long p = pixel_of(i, n0, N);
+      const long long p = pixel_of((int)i, n0, N);
       const int Y = (int)(p / W), X = (int)(p % W);
       const int ix = X % N, iy = Y % N;
       int xs[2] = {X, X}, ys[2] = {Y, Y};
@@ -666,10 +722,11 @@ __global__ void image_to_dofs_kernel(const float *__restrict__ image,
 }
 
 // dst[j] = src[idx_j] for n images of `plane` floats; idx_j drawn on the
-// device from SplitMix64 (seed, counter0 + j) uniformly in [0, count)
+// device from SplitMix64 (seed, counter0 + j) uniformly over `count` images:
+// [0, count), or [0, count] without image `skip` when skip >= 0
 __global__ void gather_kernel(const float *__restrict__ src,
                               const float *__restrict__ src2, uint64_t seed,
-                              uint64_t counter0, int count, int n,
+                              uint64_t counter0, int count, int skip, int n,
                               long long plane, float *__restrict__ dst,
                               float *__restrict__ dst2)
 {
@@ -679,7 +736,9 @@ __global__ void gather_kernel(const float *__restrict__ src,
     {
       const long long j = e / plane, p = e - j * plane;
       const double u = 0.5 * (splitmix_uniform(seed, counter0 + j) + 1.0);
-      const int idx = min((int)(u * count), count - 1);
+      int idx = min((int)(u * count), count - 1);
+      if (skip >= 0 && idx >= skip)
+        ++idx;
       dst[e] = src[(long long)idx * plane + p];
       dst2[e] = src2[(long long)idx * plane + p];
     }
@@ -886,19 +945,40 @@ void conv(const float *in, const uint32_t *w, const float *bias, float *out,
                      W);
 }
 
+template <bool ACT, int OH>
+hipError_t conv32_launch(const float *in, const uint32_t *w,
+                         const float *bias, float *out, int batch, int H,
+                         int W, hipStream_t s)
+{
+  hipError_t e = hipFuncSetAttribute(
+    reinterpret_cast<const void *>(&cnn::conv32_mfma_kernel<ACT, OH>),
+    hipFuncAttributeMaxDynamicSharedMemorySize, cnn::kConvLds);
+  if (e != hipSuccess)
+    return e;
+  hipLaunchKernelGGL((cnn::conv32_mfma_kernel<ACT, OH>),
+                     dim3(W / cnn::kMTX, H / cnn::kMTY, batch * OH),
+                     dim3(256), cnn::kConvLds, s, in, w, bias, out, H, W);
+  return hipGetLastError();
+}
+
+// small batches split the output channels over two blocks per pixel tile,
+// so that a 128 x 128 image still fills the GPU's CUs
 template <bool ACT>
 hipError_t conv32(const float *in, const uint32_t *w, const float *bias,
                   float *out, int batch, int H, int W, hipStream_t s)
 {
-  hipError_t e = hipFuncSetAttribute(
-    reinterpret_cast<const void *>(&cnn::conv32_mfma_kernel<ACT>),
-    hipFuncAttributeMaxDynamicSharedMemorySize, cnn::kConvLds);
-  if (e != hipSuccess)
-    return e;
-  hipLaunchKernelGGL((cnn::conv32_mfma_kernel<ACT>),
-                     dim3(W / cnn::kMTX, H / cnn::kMTY, batch), dim3(256),
-                     cnn::kConvLds, s, in, w, bias, out, H, W);
-  return hipGetLastError();
+  const long long tiles = (long long)batch * (H / cnn::kMTY) * (W / cnn::kMTX);
+  return tiles < 512
+           ? conv32_launch<ACT, 2>(in, w, bias, out, batch, H, W, s)
+           : conv32_launch<ACT, 1>(in, w, bias, out, batch, H, W, s);
+}
+
+void conv_to1(const float *in, const uint32_t *w, const float *bias,
+              float *out, int batch, int H, int W, hipStream_t s)
+{
+  hipLaunchKernelGGL(cnn::conv_to1_kernel<cnn::kChannels>,
+                     dim3(batch * (W / 16) * (H / 4)), dim3(256), 0, s, in, w,
+                     bias, out, H, W);
 }
 
 // the weight and bias gradients of a 32 -> 32 layer on the matrix cores
@@ -962,8 +1042,8 @@ hipError_t CnnNet::forward(const float *in, float *out, int batch,
                      b.a3, batch, H, W, s);
   if (e != hipSuccess)
     return e;
-  conv<32, 1, false>(b.a3, pk + pack_offset_fwd(3), p + param_offset_b(3),
-                     out, batch, H, W, s);
+  conv_to1(b.a3, pk + pack_offset_fwd(3), p + param_offset_b(3), out, batch,
+           H, W, s);
   return hipGetLastError();
 }
 
@@ -1057,14 +1137,15 @@ hipError_t cnn_image_to_dofs(const float *image, const double *scale, int n0,
 }
 
 hipError_t cnn_sample(const float *src, const float *src2, uint64_t seed,
-                      uint64_t counter0, int count, int n, long long plane,
-                      float *dst, float *dst2, hipStream_t s)
+                      uint64_t counter0, int count, int skip, int n,
+                      long long plane, float *dst, float *dst2,
+                      hipStream_t s)
 {
   const long long total = (long long)n * plane;
   const unsigned blocks =
     (unsigned)std::min<long long>((total + 255) / 256, 4096);
   hipLaunchKernelGGL(cnn::gather_kernel, dim3(blocks), dim3(256), 0, s, src,
-                     src2, seed, counter0, count, n, plane, dst, dst2);
+                     src2, seed, counter0, count, skip, n, plane, dst, dst2);
   return hipGetLastError();
 }
 

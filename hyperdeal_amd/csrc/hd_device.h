@@ -344,15 +344,22 @@ bool vp_poisson_supported(int dim_x, int N, long long n_x_cells);
 // potential and of the guess in the start (doubles in slots 2-4, 6, 7)
 // tab2d: the 2D tensor-product SIPG table (vp_poisson2d_table, 8 N^2
 // doubles on the device) selecting the fast 2D solver where it applies, or
-// null.  residual_out (2D solver only): write b - A phi there and stop
-// (phi, efield and stats untouched).  guess (2D solver: its nodal mean
-// removed): CG starts from the energy-minimising combination of phi and the
-// guess, or from the guess alone when guess_alone (an exact start).
+// null.  residual (2D solver only): write r = b - A phi as the image
+// (r - mean r) / scale of the x nodes, scale = rms(r - mean r) (the layout
+// of cnn_dofs_to_image), and stop (phi, efield and stats untouched).  guess
+// (2D solver: its nodal mean removed): CG starts from the
+// energy-minimising combination of phi and the guess, or from the guess
+// alone when guess_alone (an exact start).
+struct ResidualImage
+{
+  float *image = nullptr;  // (n1 N) x (n0 N) floats
+  double *scale = nullptr; // one double
+};
 hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
                       const double *guess, double *efield, double *work,
                       double rel_tol, int max_it, long long *stats,
                       hipStream_t stream, const double *tab2d = nullptr,
-                      double *residual_out = nullptr,
+                      const ResidualImage *residual = nullptr,
                       bool guess_alone = false);
 bool vp_poisson2d_supported(int dim_x, int N, long long n_x_cells);
 // the 2+2 Vlasov-Poisson apply with the velocity (v, grad phi): a table of

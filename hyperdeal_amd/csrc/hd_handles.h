@@ -92,11 +92,17 @@ struct hd_learned_guess
   float *ring_in = nullptr, *ring_tgt = nullptr;
   float *img_out = nullptr, *batch_in = nullptr, *batch_tgt = nullptr;
   double *scale = nullptr, *scale_t = nullptr, *guess = nullptr;
-  double *phi_prev = nullptr, *resid = nullptr; // x-space DoFs
+  double *phi_prev = nullptr; // x-space DoFs
   uint64_t seed = 0, draws = 0;
   long long solves = 0, guessed = 0;
+  long long dense_solves = 0; // the first solves all train (then train_every)
   int train_every = 1, train_batch = 8, train_steps = 1, min_pairs = 4;
   float lr = 1e-3f;
+  // training runs on its own stream while the solve's CG (one workgroup)
+  // runs on the caller's: forked after the guess, joined before the next
+  hipStream_t train_stream = nullptr;
+  hipEvent_t ev_fork = nullptr, ev_done = nullptr;
+  bool pending = false; // training enqueued and not yet joined
 };
 
 struct hd_advection

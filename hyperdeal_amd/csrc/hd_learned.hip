@@ -17,8 +17,12 @@
 //                                               correction CG found, the
 //                                               solution for right-hand
 //                                               side r0)
-//   every train_every solves: train_steps Adam steps on batches drawn from
-//   the ring on the device
+// and, on every solve of the first dense_solves, then every train_every
+// solves, train_steps Adam steps on batches drawn on the device from the
+// ring's complete pairs.  The training and the target image run on a stream
+// of their own, forked after the guess (and after the CG) and joined before
+// the next solve: the CG of a solve runs in one workgroup, so the rest of
+// the GPU trains meanwhile.
 // The network learns the shape of A^-1 on the residuals this run produces;
 // the combination sets the scale.  Nothing here synchronises with the host.
 #include "hd_handles.h"
@@ -40,71 +44,114 @@ hipError_t learned_before_solve(hd_advection &op, const double **guess,
 {
   hd_learned_guess &lg = *op.learned;
   *guess = nullptr;
+  hipError_t e = hipSuccess;
+  // the training stream's work (the previous target image, the previous
+  // training step) is done: the weights are updated, phi_prev is free and
+  // no training batch reads the ring slot this solve writes
+  if (lg.pending)
+    {
+      e = hipStreamWaitEvent(s, lg.ev_done, 0);
+      lg.pending = false;
+    }
   const size_t bytes = (size_t)op.n_x_dofs * sizeof(double);
-  hipError_t e =
-    hipMemcpyAsync(lg.phi_prev, op.phi, bytes, hipMemcpyDeviceToDevice, s);
+  if (e == hipSuccess)
+    e = hipMemcpyAsync(lg.phi_prev, op.phi, bytes, hipMemcpyDeviceToDevice,
+                       s);
+  // the residual image straight from the solver's kernel
+  float *slot = lg.ring_in + (long long)lg.head * lg.plane;
+  hd::ResidualImage resid;
+  resid.image = slot;
+  resid.scale = lg.scale;
   if (e == hipSuccess)
     e = hd::vp_poisson(op.xdev, op.mf->nd, op.rho, op.phi, nullptr,
                        op.efield, op.work, op.poisson_tol, op.poisson_max_it,
-                       op.stats, s, op.ptab, lg.resid);
-  float *slot = lg.ring_in + (long long)lg.head * lg.plane;
-  if (e == hipSuccess)
-    e = hd::cnn_dofs_to_image(lg.resid, nullptr, lg.n0, lg.n1, lg.N, slot,
-                              lg.scale, s);
-  if (e != hipSuccess || lg.cnn.net.steps == 0)
-    return e;
-  e = lg.cnn.net.forward(slot, lg.img_out, 1, s);
-  if (e == hipSuccess)
-    e = hd::cnn_image_to_dofs(lg.img_out, lg.scale, lg.n0, lg.n1, lg.N,
-                              lg.guess, s);
-  if (e == hipSuccess)
+                       op.stats, s, op.ptab, &resid);
+  if (e == hipSuccess && lg.cnn.net.steps > 0)
     {
-      *guess = lg.guess;
-      ++lg.guessed;
+      e = lg.cnn.net.forward(slot, lg.img_out, 1, s);
+      if (e == hipSuccess)
+        e = hd::cnn_image_to_dofs(lg.img_out, lg.scale, lg.n0, lg.n1, lg.N,
+                                  lg.guess, s);
+      if (e == hipSuccess)
+        {
+          *guess = lg.guess;
+          ++lg.guessed;
+        }
     }
+  if (e != hipSuccess)
+    return e;
+  // training on the complete pairs (every slot but the head, which this
+  // solve fills), overlapped with the CG
+  const long long n = lg.solves + 1;
+  const int every = n <= lg.dense_solves ? 1 : lg.train_every;
+  const bool full = lg.count == lg.capacity;
+  const int complete = full ? lg.capacity - 1 : lg.count;
+  if (every <= 0 || complete < lg.min_pairs || complete < 1 || n % every != 0)
+    return hipSuccess;
+  e = hipEventRecord(lg.ev_fork, s);
+  if (e == hipSuccess)
+    e = hipStreamWaitEvent(lg.train_stream, lg.ev_fork, 0);
+  for (int t = 0; t < lg.train_steps && e == hipSuccess; ++t)
+    {
+      e = hd::cnn_sample(lg.ring_in, lg.ring_tgt, lg.seed, lg.draws, complete,
+                         full ? lg.head : -1, lg.train_batch, lg.plane,
+                         lg.batch_in, lg.batch_tgt, lg.train_stream);
+      lg.draws += (uint64_t)lg.train_batch;
+      if (e == hipSuccess)
+        e = lg.cnn.net.loss_and_grad(lg.batch_in, lg.batch_tgt,
+                                     lg.train_batch, lg.train_stream);
+      if (e == hipSuccess)
+        e = lg.cnn.net.adam(lg.lr, lg.train_stream);
+    }
+  if (e == hipSuccess)
+    e = hipEventRecord(lg.ev_done, lg.train_stream);
+  if (e == hipSuccess)
+    lg.pending = true;
   return e;
 }
 
 hipError_t learned_after_solve(hd_advection &op, hipStream_t s)
 {
   hd_learned_guess &lg = *op.learned;
-  hipError_t e = hd::cnn_dofs_to_image(
-    op.phi, lg.phi_prev, lg.n0, lg.n1, lg.N,
-    lg.ring_tgt + (long long)lg.head * lg.plane, lg.scale_t, s);
+  // the target image on the training stream too: phi and phi_prev stay as
+  // they are until the next solve, which joins that stream first
+  hipError_t e = hipEventRecord(lg.ev_fork, s);
+  if (e == hipSuccess)
+    e = hipStreamWaitEvent(lg.train_stream, lg.ev_fork, 0);
+  if (e == hipSuccess)
+    e = hd::cnn_dofs_to_image(op.phi, lg.phi_prev, lg.n0, lg.n1, lg.N,
+                              lg.ring_tgt + (long long)lg.head * lg.plane,
+                              lg.scale_t, lg.train_stream);
+  if (e == hipSuccess)
+    e = hipEventRecord(lg.ev_done, lg.train_stream);
   if (e != hipSuccess)
     return e;
+  lg.pending = true;
   lg.head = (lg.head + 1) % lg.capacity;
   lg.count = std::min(lg.count + 1, lg.capacity);
   ++lg.solves;
-  if (lg.train_every <= 0 || lg.count < lg.min_pairs ||
-      lg.solves % lg.train_every != 0)
-    return hipSuccess;
-  for (int t = 0; t < lg.train_steps && e == hipSuccess; ++t)
-    {
-      e = hd::cnn_sample(lg.ring_in, lg.ring_tgt, lg.seed, lg.draws, lg.count,
-                         lg.train_batch, lg.plane, lg.batch_in, lg.batch_tgt,
-                         s);
-      lg.draws += (uint64_t)lg.train_batch;
-      if (e == hipSuccess)
-        e = lg.cnn.net.loss_and_grad(lg.batch_in, lg.batch_tgt,
-                                     lg.train_batch, s);
-      if (e == hipSuccess)
-        e = lg.cnn.net.adam(lg.lr, s);
-    }
-  return e;
+  return hipSuccess;
 }
 
 void learned_release(hd_learned_guess *lg)
 {
   if (!lg)
     return;
+  if (lg->train_stream)
+    (void)hipStreamSynchronize(lg->train_stream);
   lg->cnn.net.release();
   for (void *p : {(void *)lg->ring_in, (void *)lg->ring_tgt,
                   (void *)lg->img_out, (void *)lg->batch_in,
                   (void *)lg->batch_tgt, (void *)lg->scale,
                   (void *)lg->scale_t, (void *)lg->guess,
-                  (void *)lg->phi_prev, (void *)lg->resid})
+                  (void *)lg->phi_prev})
     (void)hipFree(p);
+  if (lg->ev_fork)
+    (void)hipEventDestroy(lg->ev_fork);
+  if (lg->ev_done)
+    (void)hipEventDestroy(lg->ev_done);
+  if (lg->train_stream)
+    (void)hipStreamDestroy(lg->train_stream);
   delete lg;
 }
 } // namespace hd_abi
@@ -255,7 +302,8 @@ int hd_cnn_last_loss(hd_cnn *net, double *loss)
 int hd_cnn_dofs_to_image(const double *u, const double *sub, int n0, int n1,
                          int N, float *image, double *scale, void *stream)
 {
-  if (!u || !image || !scale || n0 < 1 || n1 < 1 || N < 1)
+  if (!u || !image || !scale || n0 < 1 || n1 < 1 || N < 1 ||
+      (long long)n0 * n1 * N * N > 0x7fffffffLL)
     return fail(HD_ERR_INVALID, "hd_cnn_dofs_to_image: invalid argument");
   const hipError_t e =
     hd::cnn_dofs_to_image(u, sub, n0, n1, N, image, scale, as_stream(stream));
@@ -265,7 +313,8 @@ int hd_cnn_dofs_to_image(const double *u, const double *sub, int n0, int n1,
 int hd_cnn_image_to_dofs(const float *image, const double *scale, int n0,
                          int n1, int N, double *u, void *stream)
 {
-  if (!u || !image || !scale || n0 < 1 || n1 < 1 || N < 1)
+  if (!u || !image || !scale || n0 < 1 || n1 < 1 || N < 1 ||
+      (long long)n0 * n1 * N * N > 0x7fffffffLL)
     return fail(HD_ERR_INVALID, "hd_cnn_image_to_dofs: invalid argument");
   const hipError_t e =
     hd::cnn_image_to_dofs(image, scale, n0, n1, N, u, as_stream(stream));
@@ -316,6 +365,12 @@ int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
   const size_t img = (size_t)lg->plane * sizeof(float);
   hipError_t e = lg->cnn.net.init(n1 * N, n0 * N, seed);
   if (e == hipSuccess)
+    e = hipStreamCreateWithFlags(&lg->train_stream, hipStreamNonBlocking);
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&lg->ev_fork, hipEventDisableTiming);
+  if (e == hipSuccess)
+    e = hipEventCreateWithFlags(&lg->ev_done, hipEventDisableTiming);
+  if (e == hipSuccess)
     e = hipMalloc(&lg->ring_in, img * capacity);
   if (e == hipSuccess)
     e = hipMalloc(&lg->ring_tgt, img * capacity);
@@ -333,8 +388,6 @@ int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
     e = hipMalloc(&lg->guess, (size_t)op->n_x_dofs * sizeof(double));
   if (e == hipSuccess)
     e = hipMalloc(&lg->phi_prev, (size_t)op->n_x_dofs * sizeof(double));
-  if (e == hipSuccess)
-    e = hipMalloc(&lg->resid, (size_t)op->n_x_dofs * sizeof(double));
   // the activation buffers for training and inference, up front
   if (e == hipSuccess)
     e = lg->cnn.net.reserve(train_batch);
@@ -345,6 +398,20 @@ int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
     }
   op->learned = lg;
   op->fdm_on = false; // the two starts exclude each other
+  return ok();
+}
+
+int hd_vp_learned_guess_schedule(hd_advection *op, long long dense_solves,
+                                 int train_every)
+{
+  if (!op || !op->learned)
+    return fail(HD_ERR_INVALID, "hd_vp_learned_guess_schedule: no learned "
+                                "guess on this operator");
+  if (dense_solves < 0 || train_every < 0)
+    return fail(HD_ERR_INVALID, "hd_vp_learned_guess_schedule: negative "
+                                "count");
+  op->learned->dense_solves = dense_solves;
+  op->learned->train_every = train_every;
   return ok();
 }
 

@@ -590,8 +590,8 @@ __global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
   const double *__restrict__ rho_in, double *__restrict__ phi,
   const double *__restrict__ guess, double *__restrict__ efield,
   double *__restrict__ work, double rel_tol, int max_it,
-  long long *__restrict__ stats, double *__restrict__ residual_out,
-  int guess_alone)
+  long long *__restrict__ stats, float *__restrict__ res_image,
+  double *__restrict__ res_scale, int guess_alone)
 {
   constexpr int NN = N * N, T = kSolverThreads / CPT, W = T / 64;
   extern __shared__ double sm[];
@@ -801,15 +801,40 @@ __global__ __launch_bounds__(kSolverThreads / CPT) void poisson2d_kernel(
         r[k][i] -= y[k][i];
         rr += r[k][i] * r[k][i];
       }
-  if (residual_out)
+  if (res_image)
     {
-      // residual of the start only (no solve): b - A phi
+      // the residual of the start only (no solve), r = b - A phi, as the
+      // image (r - mean r) / rms(r - mean r) of the x nodes (row = x1 node,
+      // x0 node fastest) and its scale rms
+      double sr = 0.0;
 #pragma unroll
       for (int k = 0; k < CPT; ++k)
         if (own[k])
 #pragma unroll
           for (int i = 0; i < NN; ++i)
-            residual_out[(long long)cl[k] * NN + i] = r[k][i];
+            sr += r[k][i];
+      const double mean = block_sum<W, 1>(sr, red) / nx;
+      double sq = 0.0;
+#pragma unroll
+      for (int k = 0; k < CPT; ++k)
+        if (own[k])
+#pragma unroll
+          for (int i = 0; i < NN; ++i)
+            sq += (r[k][i] - mean) * (r[k][i] - mean);
+      const double rms = sqrt(block_sum<W, 0>(sq, red) / nx);
+      const double inv = rms > 0.0 ? 1.0 / rms : 0.0;
+      const int row = n0 * N;
+#pragma unroll
+      for (int k = 0; k < CPT; ++k)
+        if (own[k])
+#pragma unroll
+          for (int iy = 0; iy < N; ++iy)
+#pragma unroll
+            for (int ix = 0; ix < N; ++ix)
+              res_image[(cy[k] * N + iy) * row + cx[k] * N + ix] =
+                (float)((r[k][ix + N * iy] - mean) * inv);
+      if (t == 0)
+        *res_scale = rms;
       return;
     }
   rr = block_sum<W, 1>(rr, red); // also: every thread is done with Pt
@@ -1097,9 +1122,9 @@ hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
                       const double *guess, double *efield, double *work,
                       double rel_tol, int max_it, long long *stats,
                       hipStream_t stream, const double *tab2d,
-                      double *residual_out, bool guess_alone)
+                      const ResidualImage *residual, bool guess_alone)
 {
-  if (residual_out &&
+  if (residual &&
       !(tab2d && vp_poisson2d_supported(xd.dim, N, xd.n_cells)))
     return hipErrorInvalidValue;
   if (tab2d && vp_poisson2d_supported(xd.dim, N, xd.n_cells))
@@ -1119,7 +1144,10 @@ hipError_t vp_poisson(const XDev &xd, int N, const double *rho, double *phi,
       hipLaunchKernelGGL((vlasov::poisson2d_kernel<NN_, CPT>), dim3(1),        \
                          dim3(vlasov::kSolverThreads / CPT), lds, stream, xd,  \
                          tab2d, rho, phi, guess, efield, work, rel_tol,        \
-                         max_it, stats, residual_out, guess_alone ? 1 : 0);    \
+                         max_it, stats,                                        \
+                         residual ? residual->image : nullptr,                 \
+                         residual ? residual->scale : nullptr,                 \
+                         guess_alone ? 1 : 0);                                 \
       return hipGetLastError();                                                \
     }
       HD_VP2(2) HD_VP2(3) HD_VP2(4)

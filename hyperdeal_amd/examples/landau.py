@@ -60,7 +60,7 @@ def run(H, torch, name, args, ref_f=None):
         op.exact_start()
     if name.endswith("-learned"):
         op.enable_learned_guess(seed=20261205, capacity=64, train_every=1,
-                                train_batch=8, train_steps=1,
+                                train_batch=4, train_steps=1,
                                 learning_rate=1e-3, min_pairs=8)
     f = mf.initialize_dof_vector()
     H.VectorTools.interpolate(mf, f, H.FUNC_LANDAU, [0.01, 0.5, 2.0])

@@ -320,6 +320,11 @@ int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
                                int capacity, int train_every, int train_batch,
                                int train_steps, float learning_rate,
                                int min_pairs);
+/* The training schedule (no counterpart in the reference): every one of
+ * the first dense_solves solves counted from enabling trains, the later ones
+ * every train_every solves (0: the network stays as trained). */
+int hd_vp_learned_guess_schedule(hd_advection *op, long long dense_solves,
+                                 int train_every);
 int hd_vp_learned_guess_disable(hd_advection *op);
 /* Adam steps taken, solves that used the guess, pairs stored in total,
  * last training loss (sync) */
