@@ -314,7 +314,9 @@ int hd_vp_moments(hd_matrix_free *mf, const double *f, double *out,
  * once min_pairs are stored (train_every 0: no training).  CG starts from
  * the energy-minimising combination of phi_prev and the guess and still runs
  * to poisson_tol, so the guess changes the iteration count, not the
- * result's tolerance.  Enabling it switches the exact start off.
+ * result's tolerance.  The training runs on a stream the operator owns
+ * (default priority, non-blocking), beside the solve's CG, ordered against
+ * the caller's stream by events.  Enabling it switches the exact start off.
  */
 int hd_vp_learned_guess_enable(hd_advection *op, unsigned long long seed,
                                int capacity, int train_every, int train_batch,
