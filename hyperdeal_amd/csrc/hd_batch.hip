@@ -291,7 +291,7 @@ int hd_batch_create(hd_advection *const *ops, int n_ops, hd_batch **out)
       const hd_advection *op = ops[i];
       if (!op)
         return fail(HD_ERR_INVALID, "hd_batch_create: null operator");
-      if (op->vp || !op->collapsed_ok || op->mf->D != D ||
+      if (op->vp || !op->collapsed_ok || op->dirichlet || op->mf->D != D ||
           !hd::collapsed_batch_supported(D, op->mf->nd))
         return fail(HD_ERR_UNSUPPORTED,
                     "batched operators: constant-velocity periodic 2+2 "

@@ -230,6 +230,17 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
                                     static_cast<const float *>(src),
                                     static_cast<float *>(dst), op->coef32,
                                     mf->mesh, op->inflow, stage, stream);
+      // Dirichlet faces: the kernel read the periodic images there
+      if (e == hipSuccess && op->dirichlet)
+        e = precision == HD_FP64
+              ? hd::dirichlet_correct_f64(
+                  mf->D, mf->nd, mf->nq, mode, static_cast<const double *>(src),
+                  static_cast<double *>(dst), op->coef64, mf->mesh, op->inflow,
+                  op->quad, stage, time, mf->dim_x, stream)
+              : hd::dirichlet_correct_f32(
+                  mf->D, mf->nd, mf->nq, mode, static_cast<const float *>(src),
+                  static_cast<float *>(dst), op->coef32, mf->mesh, op->inflow,
+                  op->quad, stage, time, mf->dim_x, stream);
     }
   else if (precision == HD_FP64)
     e = hd::quadrature_apply(mf->D, mf->nd, mf->nq, mode,
@@ -488,14 +499,20 @@ int hd_advection_create(hd_matrix_free *mf, const double *velocity,
       op->bc_params[d] = bc_params ? bc_params[d] : velocity[d];
     }
 
-  // collapsed form: periodic in every direction, Cartesian
-  bool all_periodic = !mf->deformed;
+  // collapsed form: Cartesian; Dirichlet directions through the boundary
+  // correction after the collapsed kernel
+  bool all_periodic = true;
   for (int d = 0; d < D; ++d)
     all_periodic = all_periodic && mf->periodic[d];
+  op->dirichlet = !all_periodic;
   op->split = !hd::collapsed_supported(D, nd) &&
               hd::collapsed_split_supported(D, nd);
   op->collapsed_ok =
-    all_periodic && (hd::collapsed_supported(D, nd) || op->split);
+    !mf->deformed &&
+    (all_periodic ||
+     hd::dirichlet_collapsed_supported(D, nq,
+                                       bc_kind == HD_BC_INHOMOGENEOUS)) &&
+    (hd::collapsed_supported(D, nd) || op->split);
   if (op->collapsed_ok)
     {
       const int per = nd * nd + nd;

@@ -293,6 +293,24 @@ hipError_t collapsed_cres_f32(int N, int mode, const float *src, float *dst,
 // ND = k+1 nodes, NQ quadrature points per direction (NQ = ND or ND + 1)
 bool quadrature_supported(int D, int ND, int NQ);
 // geo: the deformed mesh's tables (geo.on = 0: Cartesian)
+// Dirichlet boundaries of the collapsed form (hd_dirichlet.hip): after a
+// collapsed apply (mode 0) or stage (mode 1; ti_out null: the last stage)
+// that read periodic images at the box edges, replace the inflow faces of
+// the non-periodic directions by the reference's exterior value
+// u+ = -u- + 2 g (q.bc_inhomogeneous) or -u-, at time `time`
+bool dirichlet_collapsed_supported(int D, int NQ, bool inhomogeneous);
+hipError_t dirichlet_correct_f64(int D, int N, int NQ, int mode,
+                                 const double *src, double *dst,
+                                 const double *coef, const MeshDev &mesh,
+                                 const InflowDev &inflow, const QuadDev &q,
+                                 const StageDev &stage, double time,
+                                 int dim_x, hipStream_t stream);
+hipError_t dirichlet_correct_f32(int D, int N, int NQ, int mode,
+                                 const float *src, float *dst,
+                                 const float *coef, const MeshDev &mesh,
+                                 const InflowDev &inflow, const QuadDev &q,
+                                 const StageDev &stage, double time,
+                                 int dim_x, hipStream_t stream);
 hipError_t quadrature_apply(int D, int ND, int NQ, int mode,
                             const double *src, double *dst, const QuadDev &q,
                             const MeshDev &mesh, const StageDev &stage,

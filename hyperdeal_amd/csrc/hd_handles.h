@@ -115,6 +115,7 @@ struct hd_advection
   double bc_params[8] = {0};
   int algorithm = HD_ALGO_AUTO;
   bool collapsed_ok = false;
+  bool dirichlet = false;      // some direction has Dirichlet boundaries
   bool split = false;          // the collapsed form runs split (big cells)
   // a free vector (n_dofs doubles) for the last stage of the split form
   // (its first part) and of the FCL form (the assembled integrals)

@@ -145,7 +145,9 @@ int hd_advection_create(hd_matrix_free *mf, const double *velocity,
                         const double *bc_params, hd_advection **out);
 int hd_advection_destroy(hd_advection *op);
 /* Force an algorithm (HD_ALGO_*); HD_ERR_UNSUPPORTED if not applicable.
- * HD_ALGO_COLLAPSED: periodic boxes, any n_points >= degree + 1.
+ * HD_ALGO_COLLAPSED: Cartesian boxes, any n_points >= degree + 1;
+ * Dirichlet directions through a boundary correction after the kernel
+ * (inhomogeneous data: at most 7776 face points, n_points^(dim-1)).
  * HD_ALGO_QUADRATURE: any boundary, n_points = degree + 1 or degree + 2.
  * HD_ALGO_FCL: as HD_ALGO_QUADRATURE, constant velocity; a stage without
  * ti_out assembles in a vector the operator allocates on first use.

@@ -5,8 +5,9 @@ against the GPU operator of the same problem through the C++ wrapper of the
 C ABI (include/hyperdeal_amd.hpp).
 
 Bar: max |gpu - host| <= 1e-12 max |host| (FP64 on both sides), for 1+1,
-2+2 and 3+3, periodic (collapsed kernel) and homogeneous Dirichlet
-(quadrature kernel) directions, skew 0 and 0.5, Gauss points n = k+1 and
+2+2 and 3+3, periodic and homogeneous Dirichlet directions (both through
+the collapsed kernel, Dirichlet with its boundary correction), skew 0 and
+0.5, Gauss points n = k+1 and
 k+2 (over-integration) and Gauss-Lobatto collocation.
 """
 
@@ -39,7 +40,7 @@ def program(tmp_path_factory):
 CASES = [(1, 1, 3, 4, 0, 1, 0.0), (2, 2, 3, 4, 0, 1, 0.5),
          (2, 2, 2, 3, 0, 0, 0.0), (3, 3, 2, 3, 0, 1, 0.0),
          (3, 3, 1, 2, 0, 0, 0.5),
-         # over-integration: periodic (collapsed) and Dirichlet (quadrature)
+         # over-integration: periodic and Dirichlet
          (1, 1, 3, 5, 0, 1, 0.0), (2, 2, 3, 5, 0, 1, 0.5),
          (2, 2, 3, 5, 0, 0, 0.0), (3, 3, 1, 3, 0, 0, 0.0),
          # Gauss-Lobatto collocation

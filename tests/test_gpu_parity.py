@@ -144,10 +144,16 @@ def test_apply_fp32_matches_oracle(dims, degree, n_cells):
 
 @pytest.mark.parametrize("skew", [0.0, 0.5, 1.0])
 @pytest.mark.parametrize("collocation", [False, True])
-def test_quadrature_dirichlet_skew_collocation(skew, collocation):
+@pytest.mark.parametrize("algo", ["quadrature", "collapsed"])
+def test_dirichlet_skew_collocation(skew, collocation, algo):
+    """Dirichlet data with the skew form and collocation, through the
+    quadrature kernel and through the collapsed kernel with its boundary
+    correction."""
     P, mf, op, a = _pair((2, 2), 3, [3, 2, 2, 3], seed=11, periodic=False,
                          skew=skew, collocation=collocation,
                          lo=[-1.0] * 4, hi=[1.0] * 4)
+    op.set_algorithm(H.ALGO_QUADRATURE if algo == "quadrature"
+                     else H.ALGO_COLLAPSED)
     u = _rng(3).standard_normal(P.n_dofs)
     for t in (0.0, 0.37):
         ref = P.apply(u, t=t)
@@ -158,9 +164,12 @@ def test_quadrature_dirichlet_skew_collocation(skew, collocation):
         assert _rel_err(dst.cpu().numpy(), ref) <= 1e-12
 
 
-def test_homogeneous_dirichlet_3d3d():
+@pytest.mark.parametrize("algo", ["quadrature", "collapsed"])
+def test_homogeneous_dirichlet_3d3d(algo):
     P, mf, op, a = _pair((3, 3), 3, [2, 2, 3, 2, 2, 2], seed=12,
                          periodic=False, bc=O.BC_HOMOGENEOUS)
+    op.set_algorithm(H.ALGO_QUADRATURE if algo == "quadrature"
+                     else H.ALGO_COLLAPSED)
     u = _rng(4).standard_normal(P.n_dofs)
     ref = P.apply(u)
     src = torch.from_numpy(u).cuda()
@@ -225,12 +234,13 @@ def test_quadrature_overintegration_matches_oracle(dims, degree, n_cells,
                                rtol=0, atol=1e-12 * np.max(np.abs(k)) + 1e-14)
 
 
-@pytest.mark.parametrize("dims,degree,n_points", [((2, 2), 2, 5),
-                                                 ((3, 3), 3, 5)])
+@pytest.mark.parametrize("dims,degree,n_points", [((3, 3), 5, 7),
+                                                 ((3, 3), 4, 7)])
 def test_quadrature_unsupported_points_rejected(dims, degree, n_points):
-    """Dirichlet boundaries need the quadrature kernel, which runs k+1 and
-    k+2 points (k+2 in 3+3 only up to k=2): other sizes are an error before
-    any launch, not a silent fallback."""
+    """Inhomogeneous Dirichlet data: the collapsed kernel's boundary
+    correction takes at most 7776 face quadrature points (n_points^(D-1)),
+    the quadrature kernel k+1 and k+2 points (k+2 in 3+3 only up to k=2):
+    other sizes are an error before any launch, not a silent fallback."""
     D = sum(dims)
     mf = H.MatrixFree(dims[0], dims[1], degree, [2] * D, [0.0] * D,
                       [1.0] * D, periodic_x=False, periodic_v=False,
@@ -329,10 +339,11 @@ def test_lsrk_fp32_mixed_tracks_fp64():
         np.sqrt(np.mean(sols[H.FP64].cpu().numpy() ** 2)) <= 1e-6
 
 
-def _gpu_run_advection(c):
+def _gpu_run_advection(c, algo=None):
     """examples/advection end to end on the GPU path: interpolate f0 at the
     nodes, dt rule (application.h:363-396), time loop (time_loop.cc:33-67)
-    with fused LSRK steps, diagnostics rows (application.h:464-506)."""
+    with fused LSRK steps, diagnostics rows (application.h:464-506); algo:
+    the algorithm forced on the operator (None: the automatic choice)."""
     D = c["dim_x"] + c["dim_v"]
     lo, hi = AC.HYPERRECTANGLE_BOX
     a = AC.HYPERRECTANGLE_VELOCITY[:D]
@@ -344,6 +355,8 @@ def _gpu_run_advection(c):
                               boundary=H.BC_INHOMOGENEOUS,
                               boundary_function=H.FUNC_HYPERRECTANGLE,
                               boundary_params=a)
+    if algo is not None:
+        op.set_algorithm(algo)
     sol = mf.initialize_dof_vector()
     ta, tb = torch.empty_like(sol), torch.empty_like(sol)
     H.VectorTools.interpolate(mf, sol, H.FUNC_HYPERRECTANGLE, a,
@@ -378,11 +391,18 @@ GOLD = AC.cases()
 
 @pytest.mark.parametrize("c", GOLD, ids=[c["name"] for c in GOLD])
 def test_reference_golden_end_to_end_on_gpu(c):
+    """The automatic choice (the collapsed kernel, with the boundary
+    correction for Dirichlet cases) and, for Dirichlet cases, the
+    quadrature kernel too."""
     print("case: golden %s" % c["name"], flush=True)
-    rows = _gpu_run_advection(c)
-    ours = AC.parse_rows(AC.format_rows(rows))
-    assert ours.shape == c["golden"].shape
-    assert AC.numdiff_ok(ours, c["golden"])
+    algos = [None]
+    if not (c["periodic_x"] and c["periodic_v"]):
+        algos.append(H.ALGO_QUADRATURE)
+    for algo in algos:
+        rows = _gpu_run_advection(c, algo)
+        ours = AC.parse_rows(AC.format_rows(rows))
+        assert ours.shape == c["golden"].shape
+        assert AC.numdiff_ok(ours, c["golden"]), algo
 
 
 # --------------------------------------------------------------------------
