@@ -40,7 +40,6 @@ struct Geo
   static constexpr int BUF = CH + 4 * FC;        // LDS floats per chunk
   static constexpr long long CELL = (long long)CH * N;
   static constexpr int LPT = (CH + BLOCK - 1) / BLOCK;     // chunk loads
-  static constexpr int FPT = (4 * FC + BLOCK - 1) / BLOCK; // face loads
 };
 
 // 4 bytes per lane from the lane's own global address into LDS at
@@ -61,7 +60,7 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
 {
   using G = Geo<N>;
   constexpr int NN = G::NN, P = G::P, TH = G::TH, CH = G::CH;
-  constexpr int LPT = G::LPT, FPT = G::FPT, BLOCK = G::BLOCK;
+  constexpr int LPT = G::LPT, BLOCK = G::BLOCK;
   constexpr int FC = G::FC, BUF = G::BUF, PN = P / N;
   constexpr long long CELL = G::CELL;
   // per chunk: the N^5 values, then the inflow faces of directions 0..3
@@ -105,13 +104,71 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
   auto B5 = [&](int i) -> float { return cc[5 * (NN + N) + NN + i]; };
   __syncthreads();
 
+  // the inflow neighbours of a cell (uniform over the block)
+  struct Nb
+  {
+    long long d0, d1, d2, d3, d4, d5;
+  };
+  auto neighbours = [&](long long cl) -> Nb {
+    Nb r;
+    r.d0 = inflow_neighbour(mesh, cl, 0, inflow.side[0]);
+    r.d1 = inflow_neighbour(mesh, cl, 1, inflow.side[1]);
+    r.d2 = inflow_neighbour(mesh, cl, 2, inflow.side[2]);
+    r.d3 = inflow_neighbour(mesh, cl, 3, inflow.side[3]);
+    r.d4 = inflow_neighbour(mesh, cl, 4, inflow.side[4]);
+    r.d5 = inflow_neighbour(mesh, cl, 5, inflow.side[5]);
+    return r;
+  };
+  // chunk j of a cell and its faces into LDS buffer b, straight from HBM /
+  // L2 (element e of a wave lands at the wave's base + lane).  Face d of
+  // the chunk: the neighbour's values at its face node, indexed by the point
+  // without digit d and by i4; one direction per pass, so that no
+  // lane-dependent choice of neighbour is needed
+  constexpr int FPD = (FC + BLOCK - 1) / BLOCK;
+  auto fetch = [&](long long cl, const Nb &nbx, int j, int b) {
+    const int wbase = t & ~63;
+    const float *uc = src + cl * CELL;
+#pragma unroll
+    for (int l = 0; l < LPT; ++l)
+      {
+        const int e = t + BLOCK * l;
+        if (e < CH)
+          load_lds(uc + (long long)CH * j + e, &buf[b][wbase + BLOCK * l]);
+      }
+#pragma unroll
+    for (int d = 0; d < 4; ++d)
+      {
+        const long long nbd = d == 0   ? nbx.d0
+                              : d == 1 ? nbx.d1
+                              : d == 2 ? nbx.d2
+                                       : nbx.d3;
+        const float *fsrc =
+          src + nbd * CELL + (long long)fnode[d] * sd[d] + (long long)CH * j;
+#pragma unroll
+        for (int l = 0; l < FPD; ++l)
+          {
+            const int e = t + BLOCK * l;
+            if (e < FC)
+              {
+                const int c = e % PN, i4 = e / PN;
+                const int pf = c % sd[d] + sd[d] * N * (c / sd[d]);
+                load_lds(fsrc + pf + (long long)P * i4,
+                         &buf[b][CH + d * FC + wbase + BLOCK * l]);
+              }
+          }
+      }
+  };
+
+  // the first cell's first chunk; every later cell's first chunk goes out
+  // before the previous cell's outputs
+  Nb nb{};
+  if ((long long)blockIdx.x < mesh.n_cells)
+    {
+      nb = neighbours(blockIdx.x);
+      fetch(blockIdx.x, nb, 0, 0);
+    }
   for (long long cell = blockIdx.x; cell < mesh.n_cells; cell += gridDim.x)
     {
-      const float *u = src + cell * CELL;
-      long long nb[6];
-#pragma unroll
-      for (int d = 0; d < 6; ++d)
-        nb[d] = inflow_neighbour(mesh, cell, d, inflow.side[d]);
       float acc[2][N][N]; // [point][i4][i5]
 #pragma unroll
       for (int q = 0; q < 2; ++q)
@@ -120,40 +177,6 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
 #pragma unroll
           for (int b = 0; b < N; ++b)
             acc[q][a][b] = 0.0f;
-      // face value e of chunk j: direction d = e / FC, face index c and i4
-      auto face_src = [&](int e, int j) -> long long {
-        const int d = e / FC, r = e % FC;
-        const int c = r % PN, i4 = r / PN;
-        const int s = d == 0 ? 1 : d == 1 ? N : d == 2 ? NN : NN * N;
-        const int fn = d == 0   ? fnode[0]
-                       : d == 1 ? fnode[1]
-                       : d == 2 ? fnode[2]
-                                : fnode[3];
-        const long long nbd = d == 0 ? nb[0] : d == 1 ? nb[1] : d == 2 ? nb[2]
-                                                                        : nb[3];
-        const int pf = c % s + s * (fn + N * (c / s));
-        return nbd * CELL + pf + (long long)P * i4 + (long long)CH * j;
-      };
-      // chunk j and its faces into LDS buffer b, straight from HBM / L2
-      // (element e of a wave lands at the wave's base + lane)
-      const int wbase = t & ~63;
-      auto fetch = [&](int j, int b) {
-#pragma unroll
-        for (int l = 0; l < LPT; ++l)
-          {
-            const int e = t + BLOCK * l;
-            if (e < CH)
-              load_lds(u + (long long)CH * j + e, &buf[b][wbase + BLOCK * l]);
-          }
-#pragma unroll
-        for (int l = 0; l < FPT; ++l)
-          {
-            const int e = t + BLOCK * l;
-            if (e < 4 * FC)
-              load_lds(src + face_src(e, j), &buf[b][CH + wbase + BLOCK * l]);
-          }
-      };
-      fetch(0, 0);
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < N; ++j)
@@ -163,7 +186,7 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
           // previous chunk)
           lane_setup();
           if (j + 1 < N)
-            fetch(j + 1, (j + 1) & 1);
+            fetch(cell, nb, j + 1, (j + 1) & 1);
           const float *ch = buf[j & 1];
 #pragma unroll
           for (int q = 0; q < 2; ++q)
@@ -186,7 +209,7 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
                 }
               // along i4, within plane j, with the i4 inflow face
               const float f4 =
-                src[nb[4] * CELL + p + (long long)P * (fnode[4] + N * j)];
+                src[nb.d4 * CELL + p + (long long)P * (fnode[4] + N * j)];
 #pragma unroll
               for (int i4 = 0; i4 < N; ++i4)
                 {
@@ -222,34 +245,52 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
           // done (the barrier waits for both)
           __syncthreads();
         }
-      // the i5 inflow face, then the outputs (for fixed (i4, i5) the points
-      // of consecutive threads are consecutive values)
+      // the next cell's first chunk travels while this cell's outputs go
+      // out (both buffers are free after the last chunk's barrier); the i5
+      // inflow face of this cell, then its outputs (for fixed (i4, i5) the
+      // points of consecutive threads are consecutive values)
+      const long long nb5 = nb.d5;
+      const long long next = cell + gridDim.x;
+      if (next < mesh.n_cells)
+        {
+          lane_setup();
+          nb = neighbours(next);
+          fetch(next, nb, 0, 0);
+        }
 #pragma unroll
       for (int q = 0; q < 2; ++q)
         {
           if (!own[q])
             continue;
           const int p = pp[q];
+          float f5[N];
+#pragma unroll
+          for (int i4 = 0; i4 < N; ++i4)
+            f5[i4] = src[nb5 * CELL + p + (long long)P * (i4 + N * fnode[5])];
+          const long long g0 = cell * CELL + p;
 #pragma unroll
           for (int i4 = 0; i4 < N; ++i4)
             {
-              const float f5 = src[nb[5] * CELL + p +
-                                   (long long)P * (i4 + N * fnode[5])];
+              // a stage's solution values of this i4 row: N loads in flight
+              // before the first store
+              double so[MODE == 0 ? 1 : N];
+              if constexpr (MODE != 0)
+#pragma unroll
+                for (int i5 = 0; i5 < N; ++i5)
+                  so[i5] = stage.sol[g0 + (long long)P * (i4 + N * i5)];
 #pragma unroll
               for (int i5 = 0; i5 < N; ++i5)
                 {
-                  const float k = acc[q][i4][i5] + B5(i5) * f5;
-                  const long long g =
-                    cell * CELL + p + (long long)P * (i4 + N * i5);
+                  const float k = acc[q][i4][i5] + B5(i5) * f5[i4];
+                  const long long g = g0 + (long long)P * (i4 + N * i5);
                   if constexpr (MODE == 0)
                     dst[g] = k;
                   else
                     {
-                      const double so = stage.sol[g];
-                      stage.sol[g] = so + stage.b_dt * (double)k;
+                      stage.sol[g] = so[i5] + stage.b_dt * (double)k;
                       if constexpr (MODE == 1)
                         static_cast<float *>(stage.ti_out)[g] =
-                          (float)(so + stage.a_dt * (double)k);
+                          (float)(so[i5] + stage.a_dt * (double)k);
                     }
                 }
             }
