@@ -177,6 +177,17 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
 #pragma unroll
           for (int b = 0; b < N; ++b)
             acc[q][a][b] = 0.0f;
+      // the i4 and i5 inflow faces of this thread's points go out now and
+      // land with the first chunk (no global load inside the chunk loop)
+      float f4[2][N], f5[2][N];
+#pragma unroll
+      for (int q = 0; q < 2; ++q)
+#pragma unroll
+        for (int k = 0; k < N; ++k)
+          {
+            f4[q][k] = src[nb.d4 * CELL + pp[q] + (long long)P * (fnode[4] + N * k)];
+            f5[q][k] = src[nb.d5 * CELL + pp[q] + (long long)P * (k + N * fnode[5])];
+          }
       __syncthreads();
 #pragma unroll
       for (int j = 0; j < N; ++j)
@@ -208,12 +219,10 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
                     acc[q][i4][i5] += a5 * v[i4];
                 }
               // along i4, within plane j, with the i4 inflow face
-              const float f4 =
-                src[nb.d4 * CELL + p + (long long)P * (fnode[4] + N * j)];
 #pragma unroll
               for (int i4 = 0; i4 < N; ++i4)
                 {
-                  float a = B4(i4) * f4;
+                  float a = B4(i4) * f4[q][j];
 #pragma unroll
                   for (int k = 0; k < N; ++k)
                     a += A4(i4, k) * v[k];
@@ -246,10 +255,9 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
           __syncthreads();
         }
       // the next cell's first chunk travels while this cell's outputs go
-      // out (both buffers are free after the last chunk's barrier); the i5
-      // inflow face of this cell, then its outputs (for fixed (i4, i5) the
-      // points of consecutive threads are consecutive values)
-      const long long nb5 = nb.d5;
+      // out (both buffers are free after the last chunk's barrier); then
+      // the outputs with the i5 inflow face (for fixed (i4, i5) the points
+      // of consecutive threads are consecutive values)
       const long long next = cell + gridDim.x;
       if (next < mesh.n_cells)
         {
@@ -263,10 +271,6 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
           if (!own[q])
             continue;
           const int p = pp[q];
-          float f5[N];
-#pragma unroll
-          for (int i4 = 0; i4 < N; ++i4)
-            f5[i4] = src[nb5 * CELL + p + (long long)P * (i4 + N * fnode[5])];
           const long long g0 = cell * CELL + p;
 #pragma unroll
           for (int i4 = 0; i4 < N; ++i4)
@@ -281,7 +285,7 @@ __global__ __launch_bounds__(Geo<N>::BLOCK) void cres_kernel(
 #pragma unroll
               for (int i5 = 0; i5 < N; ++i5)
                 {
-                  const float k = acc[q][i4][i5] + B5(i5) * f5[i4];
+                  const float k = acc[q][i4][i5] + B5(i5) * f5[q][i4];
                   const long long g = g0 + (long long)P * (i4 + N * i5);
                   if constexpr (MODE == 0)
                     dst[g] = k;
