@@ -95,12 +95,40 @@ struct QTab
   const T *S, *Dq, *F, *Sinv;
 };
 
-// transport velocity of direction d at the in-cell point pt of `cell`
+// a cell's coordinates and the indices of its x- and v-space cells (cell =
+// xc + n_x vc), computed once per cell so that the per-point helpers below
+// do no 64-bit division
+struct CellAt
+{
+  int c[kMaxDim];
+  long long xc, vc;
+};
+
+__device__ inline CellAt cell_at(const MeshDev &m, long long cell,
+                                 long long n_x_cells)
+{
+  CellAt a;
+  long long r = cell;
+#pragma unroll
+  for (int j = 0; j < kMaxDim; ++j)
+    if (j < m.dim)
+      {
+        a.c[j] = (int)(r % m.n[j]);
+        r /= m.n[j];
+      }
+    else
+      a.c[j] = 0;
+  a.xc = n_x_cells > 0 ? cell % n_x_cells : 0;
+  a.vc = n_x_cells > 0 ? cell / n_x_cells : 0;
+  return a;
+}
+
+// transport velocity of direction d at the in-cell point pt of a cell
 // (constant, or the Vlasov-Poisson field; a face point's digit d is unused)
 template <int D, int N>
-__device__ inline double point_velocity(const QuadDev &q, const MeshDev &m,
-                                        const double *efield, int dim_x,
-                                        int d, long long cell, int pt)
+__device__ inline double point_velocity_at(const QuadDev &q, const MeshDev &m,
+                                           const double *efield, int dim_x,
+                                           int d, const CellAt &ca, int pt)
 {
   if (!q.vp)
     return q.a[d];
@@ -111,15 +139,22 @@ __device__ inline double point_velocity(const QuadDev &q, const MeshDev &m,
       for (int j = 0; j < e; ++j)
         pe *= N;
       const int qe = (pt / pe) % N;
-      const int ce = (int)((cell / m.stride[e]) % m.n[e]);
-      return m.lo[e] + (ce + q.qp[qe]) * m.h[e];
+      return m.lo[e] + (ca.c[e] + q.qp[qe]) * m.h[e];
     }
   const int e = d - dim_x;
   int nqx = 1;
   for (int j = 0; j < dim_x; ++j)
     nqx *= N;
-  const long long xc = cell % q.n_x_cells;
-  return efield[(xc * nqx + pt % nqx) * dim_x + e];
+  return efield[(ca.xc * nqx + pt % nqx) * dim_x + e];
+}
+
+template <int D, int N>
+__device__ inline double point_velocity(const QuadDev &q, const MeshDev &m,
+                                        const double *efield, int dim_x,
+                                        int d, long long cell, int pt)
+{
+  return point_velocity_at<D, N>(q, m, efield, dim_x, d,
+                                 cell_at(m, cell, q.vp ? q.n_x_cells : 0), pt);
 }
 
 template <int D, int N>
@@ -155,41 +190,56 @@ __device__ inline void split_point(const DeformDev &g, int pt, int &px,
   pv = pt / nqx;
 }
 
-// |det J| w at cell point pt: the JxW of the deformed cell
+// |det J| w at cell point pt: the JxW of the deformed cell (x / v cells cx,
+// cv)
 template <int NQ>
-__device__ inline double deformed_jxw(const DeformDev &g, long long cell,
-                                      int pt)
+__device__ inline double deformed_jxw_at(const DeformDev &g, long long cx,
+                                         long long cv, int pt)
 {
   int px, pv, nqx, nqv;
   split_point<NQ>(g, pt, px, pv, nqx, nqv);
-  const long long cx = cell % g.n_x_cells, cv = cell / g.n_x_cells;
   return g.xca[(cx * nqx + px) * (g.dim_x + 1) + g.dim_x] *
          g.vca[(cv * nqv + pv) * (g.dim_v + 1) + g.dim_v];
 }
 
+template <int NQ>
+__device__ inline double deformed_jxw(const DeformDev &g, long long cell,
+                                      int pt)
+{
+  return deformed_jxw_at<NQ>(g, cell % g.n_x_cells, cell / g.n_x_cells, pt);
+}
+
 // (J^{-1} a)_d JxW at cell point pt (direction d of phase space)
 template <int NQ>
-__device__ inline double deformed_velocity_jxw(const DeformDev &g,
-                                               long long cell, int pt, int d)
+__device__ inline double deformed_velocity_jxw_at(const DeformDev &g,
+                                                  long long cx, long long cv,
+                                                  int pt, int d)
 {
   int px, pv, nqx, nqv;
   split_point<NQ>(g, pt, px, pv, nqx, nqv);
-  const long long cx = cell % g.n_x_cells, cv = cell / g.n_x_cells;
   const double *xc = g.xca + (cx * nqx + px) * (g.dim_x + 1);
   const double *vc = g.vca + (cv * nqv + pv) * (g.dim_v + 1);
   const double jw = xc[g.dim_x] * vc[g.dim_v];
   return (d < g.dim_x ? xc[d] : vc[d - g.dim_x]) * jw;
 }
 
+template <int NQ>
+__device__ inline double deformed_velocity_jxw(const DeformDev &g,
+                                               long long cell, int pt, int d)
+{
+  return deformed_velocity_jxw_at<NQ>(g, cell % g.n_x_cells,
+                                      cell / g.n_x_cells, pt, d);
+}
+
 // (a . n) JxW_face at the face point of the cell array position `base`
 // (digit d unused) of the face of direction d, side `side`
 template <int NQ>
-__device__ inline double deformed_face(const DeformDev &g, long long cell,
-                                       int base, int d, int side)
+__device__ inline double deformed_face_at(const DeformDev &g, long long cx,
+                                          long long cv, int base, int d,
+                                          int side)
 {
   int px, pv, nqx, nqv;
   split_point<NQ>(g, base, px, pv, nqx, nqv);
-  const long long cx = cell % g.n_x_cells, cv = cell / g.n_x_cells;
   const bool in_x = d < g.dim_x;
   const int dl = in_x ? d : d - g.dim_x;      // direction within its space
   const int dim = in_x ? g.dim_x : g.dim_v;
@@ -214,6 +264,14 @@ __device__ inline double deformed_face(const DeformDev &g, long long cell,
   const double jw = in_x ? g.vca[(cv * nqv + pv) * (g.dim_v + 1) + g.dim_v]
                          : g.xca[(cx * nqx + px) * (g.dim_x + 1) + g.dim_x];
   return gf * jw;
+}
+
+template <int NQ>
+__device__ inline double deformed_face(const DeformDev &g, long long cell,
+                                       int base, int d, int side)
+{
+  return deformed_face_at<NQ>(g, cell % g.n_x_cells, cell / g.n_x_cells, base,
+                              d, side);
 }
 } // namespace quad
 } // namespace hd

@@ -66,6 +66,12 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   const long long cell0 = (long long)blockIdx.x * CPB;
   const int ncell = (int)min((long long)CPB, mesh.n_cells - cell0);
   const int tid = threadIdx.x;
+  // the block's cells: coordinates and x / v cells, once (read after the
+  // barrier that follows the load of U)
+  __shared__ CellAt at[CPB];
+  if (tid < ncell)
+    at[tid] = cell_at(mesh, cell0 + tid,
+                      q.vp ? q.n_x_cells : (geo.on ? geo.n_x_cells : 0));
 
   // the positions that hold no nodal value start at zero
   if constexpr (ND < NQ)
@@ -113,11 +119,12 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
               uq[j] = u[j * sd];
               // velocity / h folded into the quadrature weight (deformed:
               // the contravariant velocity (J^{-1} a)_d times JxW)
-              jw[j] = (T)(geo.on ? deformed_velocity_jxw<NQ>(
-                                     geo, cell0 + lc, base + j * sd, d)
+              jw[j] = (T)(geo.on ? deformed_velocity_jxw_at<NQ>(
+                                     geo, at[lc].xc, at[lc].vc, base + j * sd,
+                                     d)
                                  : jxw_point<D, NQ>(q, mesh, base + j * sd) *
-                                     point_velocity<D, NQ>(
-                                       q, mesh, efield, dim_x, d, cell0 + lc,
+                                     point_velocity_at<D, NQ>(
+                                       q, mesh, efield, dim_x, d, at[lc],
                                        base + j * sd) /
                                      mesh.h[d]);
               acc[j] = 0;
@@ -188,7 +195,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
           }
           const long long cell = cell0 + lc;
           const long long s = mesh.stride[d];
-          const int c = (int)((cell / s) % mesh.n[d]);
+          const int c = at[lc].c[d];
           const int cn = c + (side ? 1 : -1);
           const int off = nbase + (side ? 0 : (ND - 1)) * sdn;
           T up = 0;
@@ -253,9 +260,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
         {
           const int lc = l / FACE, fi = l - lc * FACE;
           const int base = line_base<D, NQ>(fi, d);
-          const long long cell = cell0 + lc;
-          const long long s = mesh.stride[d];
-          const int c = (int)((cell / s) % mesh.n[d]);
+          const int c = at[lc].c[d];
           const bool boundary =
             !q.periodic[d] && (side ? c == mesh.n[d] - 1 : c == 0);
           const double um = Fm[lc * FACE + fi];
@@ -278,12 +283,10 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
               if (q.bc_inhomogeneous)
                 {
                   double x[kMaxDim];
-                  long long cc = cell;
                   int rem = base;
                   for (int j = 0; j < D; ++j)
                     {
-                      const int cj = (int)(cc % mesh.n[j]);
-                      cc /= mesh.n[j];
+                      const int cj = at[lc].c[j];
                       const int ij = rem % NQ;
                       rem /= NQ;
                       x[j] = mesh.lo[j] +
@@ -302,11 +305,13 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
           double an;
           if (geo.on)
             {
-              an = deformed_face<NQ>(geo, cell, base, d, side);
+              an = deformed_face_at<NQ>(geo, at[lc].xc, at[lc].vc, base, d,
+                                        side);
               jf = 1.0;
             }
           else
-            an = point_velocity<D, NQ>(q, mesh, efield, dim_x, d, cell, base) *
+            an = point_velocity_at<D, NQ>(q, mesh, efield, dim_x, d, at[lc],
+                                          base) *
                  nrm;
           const double flux =
             0.5 * ((um + up) * an + fabs(an) * (um - up));
@@ -323,8 +328,9 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   for (int i = tid; i < ncell * CELL; i += kBlock)
     {
       const int lc = i / CELL, li = i - lc * CELL;
-      R[lc * CELLP + li] /= (T)(geo.on ? deformed_jxw<NQ>(geo, cell0 + lc, li)
-                                       : jxw_point<D, NQ>(q, mesh, li));
+      R[lc * CELLP + li] /=
+        (T)(geo.on ? deformed_jxw_at<NQ>(geo, at[lc].xc, at[lc].vc, li)
+                   : jxw_point<D, NQ>(q, mesh, li));
     }
   __syncthreads();
   if (!collocation)

@@ -197,6 +197,27 @@ def test_learned_guess_keeps_the_solution(precision):
     assert np.isfinite(r0) and np.isfinite(rp)
 
 
+@pytest.mark.parametrize("dense,every,expect", [(8, 3, 10), (8, 0, 6),
+                                                (0, 4, 5)])
+def test_learned_guess_schedule(dense, every, expect):
+    """hd_vp_learned_guess_schedule: solve n (counted from enabling) trains
+    when n <= dense or n is a multiple of every (every 0: never after the
+    dense part), once min_pairs complete pairs (n - 1 here) are stored.  20
+    solves with min_pairs 2 take 10 Adam steps for (8, 3) (solves 3-8, 9,
+    12, 15, 18), 6 for (8, 0) (solves 3-8) and 5 for (0, 4) (solves 4, 8,
+    12, 16, 20)."""
+    print("case: vp learned guess schedule", dense, every, flush=True)
+    mf, op, f = _landau()
+    op.enable_learned_guess(seed=1, train_every=1, train_batch=2,
+                            train_steps=1, min_pairs=2)
+    op.learned_guess_schedule(dense, every)
+    _steps(mf, op, f, 4, 0.05, H.FP32)
+    steps, guessed, pairs, loss = op.learned_guess_stats()
+    assert pairs == 20 and steps == expect, (steps, pairs)
+    with pytest.raises(H.HyperdealError):
+        op.learned_guess_schedule(-1, 1)
+
+
 def test_warm_start_from_converged_potential_needs_no_iteration():
     """The reference's warm start (the previous phi, vlasov_poisson/include/
     application.h:575-576): a second solve with the same density starts
