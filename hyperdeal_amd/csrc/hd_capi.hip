@@ -55,7 +55,8 @@ enum VpDensity
 {
   kDensityInput = 0,    // the stage input (FP64, or FP32 in the mixed form)
   kDensitySolution = 1, // the FP64 solution (first stage of an LSRK step)
-  kDensityPartials = 2  // the partials the previous stage's kernel wrote
+  kDensityPartials = 2, // the partials the previous stage's kernel wrote
+  kDensityReady = 3     // rho is computed already (with the step's copy)
 };
 
 // the phase-space kernel runs (not the quadrature kernel)
@@ -67,10 +68,12 @@ bool vp_kernel_active(const hd_advection *op)
 // rho (from f, or from the partials of the previous stage), the potential
 // solve (with the learned guess when it is on), grad phi
 int vp_update(hd_advection *op, const double *f, const float *f32,
-              hipStream_t stream, bool from_partials = false)
+              hipStream_t stream, bool from_partials = false,
+              bool rho_ready = false)
 {
   hipError_t e =
-    from_partials
+    rho_ready ? hipSuccess
+    : from_partials
       ? hd::vp_density_from_partials(op->rho_part, op->n_v_cells,
                                      op->n_x_dofs, op->rho, stream)
       : hd::vp_density(f, f32, op->vpdev, op->n_x_cells, op->n_v_cells,
@@ -104,7 +107,9 @@ int run_operator(hd_advection *op, int mode, int precision, const void *src,
   if (op->vp && op->vp_auto)
     {
       int rc;
-      if (density == kDensityPartials)
+      if (density == kDensityReady)
+        rc = vp_update(op, nullptr, nullptr, stream, false, true);
+      else if (density == kDensityPartials)
         rc = vp_update(op, nullptr, nullptr, stream, true);
       else if (density == kDensitySolution)
         rc = vp_update(op, sol, nullptr, stream);
@@ -1141,8 +1146,23 @@ int hd_lsrk_step(hd_advection *op, const char *type, int precision,
   if (!rk_table(type, a, b))
     return fail(HD_ERR_INVALID, "unknown RK scheme");
   // the first stage reads a copy of the solution (time_integrators.
-  // templates.h:142-157)
-  int rc = hd_vector_convert(op->mf, sol, ti_a, precision, stream);
+  // templates.h:142-157); a Vlasov-Poisson operator on the phase-space
+  // kernel takes rho from the same pass over the solution
+  const bool fused = vp_kernel_active(op) && op->vp_auto;
+  int rc;
+  if (fused)
+    {
+      if (!valid_precision(precision))
+        return fail(HD_ERR_INVALID, "invalid precision");
+      const hipError_t e = hd::vp_density(
+        sol, nullptr, op->vpdev, op->n_x_cells, op->n_v_cells, op->partial,
+        op->density_chunks, op->rho, as_stream(stream),
+        precision == HD_FP32 ? static_cast<float *>(ti_a) : nullptr,
+        precision == HD_FP64 ? static_cast<double *>(ti_a) : nullptr);
+      rc = e == hipSuccess ? HD_OK : hip_fail(e, "hd_lsrk_step: density");
+    }
+  else
+    rc = hd_vector_convert(op->mf, sol, ti_a, precision, stream);
   if (rc != HD_OK)
     return rc;
   double sum_b = 0.0;
@@ -1158,9 +1178,8 @@ int hd_lsrk_step(hd_advection *op, const char *type, int precision,
       // rho from the FP64 solution, the others from the partials the
       // previous stage wrote of its unrounded output (no pass over f, and
       // in the mixed form no FP32 rounding in the Poisson right-hand side)
-      const bool fused = vp_kernel_active(op) && op->vp_auto;
       const int density =
-        !fused ? kDensityInput : (s == 0 ? kDensitySolution : kDensityPartials);
+        !fused ? kDensityInput : (s == 0 ? kDensityReady : kDensityPartials);
       rc = lsrk_stage(op, precision, in, last ? nullptr : out, out, sol,
                       time + c * dt, last ? 0.0 : a[s] * dt, b[s] * dt, stream,
                       density, fused && !last);

@@ -351,10 +351,14 @@ hipError_t launch_noise(double *dst, uint64_t seed, double amplitude,
                         long long n, hipStream_t stream);
 hipError_t launch_convert_f32(const double *src, float *dst, long long n,
                               hipStream_t stream);
+// rho = int f dv from f (FP64) or f32; copy32 / copy64 (may be null) get a
+// copy of the values read (one pass over f for the density and the first
+// LSRK stage's input)
 hipError_t vp_density(const double *f, const float *f32, const VpDev &vp,
                       long long n_x_cells, long long n_v_cells,
                       double *partial, int n_chunks, double *rho,
-                      hipStream_t stream);
+                      hipStream_t stream, float *copy32 = nullptr,
+                      double *copy64 = nullptr);
 bool vp_poisson_supported(int dim_x, int N, long long n_x_cells);
 // stats: [0] iterations (summed), [1] solves, [2] final relative residual,
 // [3] relative residual of the start, [4] of the previous potential,

@@ -26,10 +26,13 @@ namespace vlasov
 constexpr int kSolverThreads = 1024;
 
 // ---- rho: partial sums over chunks of v cells, then a fixed-order sum ----
+// copy32 / copy64 (may be null): the values read are also written there
+// (the first LSRK stage's input register, from the FP64 solution)
 __global__ __launch_bounds__(256) void density_partial(
   const double *__restrict__ f, const float *__restrict__ f32,
   VpDev vp, long long n_x_cells, long long n_v_cells, int chunk,
-  double *__restrict__ partial)
+  double *__restrict__ partial, float *__restrict__ copy32,
+  double *__restrict__ copy64)
 {
   // block: one x cell, v cells [blockIdx.y * chunk, ...)
   const long long xc = blockIdx.x;
@@ -45,6 +48,10 @@ __global__ __launch_bounds__(256) void density_partial(
       for (int t = threadIdx.x; t < dc; t += blockDim.x)
         {
           const double v = f ? f[cell * dc + t] : (double)f32[cell * dc + t];
+          if (copy32)
+            copy32[cell * dc + t] = (float)v;
+          if (copy64)
+            copy64[cell * dc + t] = v;
           acc += v * vp.wv[t / dcx];
         }
     }
@@ -1048,7 +1055,7 @@ __global__ __launch_bounds__(256) void moments_kernel(
 hipError_t vp_density(const double *f, const float *f32, const VpDev &vp,
                       long long n_x_cells, long long n_v_cells,
                       double *partial, int n_chunks, double *rho,
-                      hipStream_t stream)
+                      hipStream_t stream, float *copy32, double *copy64)
 {
   const int chunk = (int)((n_v_cells + n_chunks - 1) / n_chunks);
   // a multiple of the x DoFs per cell, so each thread keeps one x node
@@ -1057,7 +1064,7 @@ hipError_t vp_density(const double *f, const float *f32, const VpDev &vp,
                      dim3((unsigned)n_x_cells, (unsigned)n_chunks),
                      dim3(threads),
                      0, stream, f, f32, vp, n_x_cells, n_v_cells, chunk,
-                     partial);
+                     partial, copy32, copy64);
   hipError_t e = hipGetLastError();
   if (e != hipSuccess)
     return e;
