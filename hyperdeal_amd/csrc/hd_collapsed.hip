@@ -1165,8 +1165,10 @@ hipError_t batch_convert(int precision, int n_ops, const long long *n_dofs,
 // 48 bytes per DoF instead of 16).
 namespace collapsed
 {
+// three waves per SIMD: with the part loaded up front the compiler would
+// otherwise keep every plane value live and run one wave per SIMD
 template <int D, int N, typename T, int MODE>
-__global__ __launch_bounds__(256) void plane_kernel(
+__global__ __launch_bounds__(256, 3) void plane_kernel(
   const T *__restrict__ src, const T *part, T *dst,
   const T *__restrict__ coef, MeshDev mesh, InflowDev inflow, StageDev stage)
 {
@@ -1181,12 +1183,14 @@ __global__ __launch_bounds__(256) void plane_kernel(
       const long long cell = idx / P;
       const int pt = (int)(idx - cell * P);
       const long long g0 = cell * CELL + pt;
+      // the part of the first four directions starts the sums: its loads
+      // go out with the plane's values (one round trip per plane)
       T u[PL], k[PL];
 #pragma unroll
       for (int q = 0; q < PL; ++q)
         {
           u[q] = src[g0 + (long long)q * P];
-          k[q] = 0;
+          k[q] = part[g0 + (long long)q * P];
         }
 #pragma unroll
       for (int r = 0; r < R; ++r)
@@ -1220,7 +1224,7 @@ __global__ __launch_bounds__(256) void plane_kernel(
       for (int q = 0; q < PL; ++q)
         {
           const long long g = g0 + (long long)q * P;
-          const T kq = k[q] + part[g];
+          const T kq = k[q];
           if constexpr (MODE == 0)
             dst[g] = kq;
           else
