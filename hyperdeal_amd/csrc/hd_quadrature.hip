@@ -9,8 +9,9 @@
 // positions whose digits are all < ND):
 //   U  <- src cell (nodal), then U <- S^D U             (quadrature values)
 //   R  <- cell integral: -skew (a.grad u) JxW + sum_d D_d^T[(1-skew) u a_d/h_d JxW]
-//   per face f (2D of them): u- = F_s . U along d; u+ = S^{D-1} (neighbour's
-//       face nodes) or the Dirichlet ghost -u- + 2g; R += F_s^T (-flux JxW_f)
+//   per direction d, its two faces s together: u- = F_s . U along d;
+//       u+ = S^{D-1} (neighbour's face nodes) or the Dirichlet ghost
+//       -u- + 2g; R += F_s^T (-flux JxW_f)
 //   R <- Sinv^D (R / JxW)                                 (inverse mass)
 // S is NQ x ND, Sinv = (S^T W S)^-1 S^T W is ND x NQ.  Every 1D contraction
 // is done line by line: a thread owns whole lines along the current
@@ -44,8 +45,8 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *U = reinterpret_cast<T *>(smem_raw); // CPB x CELLP
   T *R = U + CPB * CELLP;                 // CPB x CELLP
-  T *Fm = R + CPB * CELLP;                // CPB x FACE  (u- then flux)
-  T *Fp = Fm + CPB * FACE;                // CPB x FACE  (u+)
+  T *Fm = R + CPB * CELLP;                // 2 sides x CPB x FACE (u-)
+  T *Fp = Fm + 2 * CPB * FACE;            // 2 sides x CPB x FACE (u+)
   QTab<T> tb;
   if constexpr (sizeof(T) == sizeof(double))
     tb = {q.S, q.Dq, q.F, q.Sinv};
@@ -156,25 +157,32 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
       __syncthreads();
     }
 
-  // 3) faces
-  for (int f = 0; f < 2 * D; ++f)
+  // 3) faces, both sides of a direction at once (side 0: the face towards
+  // the left neighbour, normal -e_d; side 1: towards the right one).  The
+  // lifts of the two sides go into R in the order side 0, side 1, as if the
+  // faces were taken one by one.
+  for (int d = 0; d < D; ++d)
     {
-      const int d = f / 2, side = f % 2;
       const int sd = ipow(NQ, d);
       const int sdn = ipow(ND, d); // stride of direction d in the src cell
-      const double nrm = side ? 1.0 : -1.0;
-      // u- (trace) and the neighbour's face nodes
+      const long long s = mesh.stride[d];
+      // u- (traces of both sides) and the neighbours' face nodes
       for (int l = tid; l < ncell * FACE; l += kBlock)
         {
           const int lc = l / FACE, fi = l - lc * FACE;
           const int base = line_base<D, NQ>(fi, d);
           const T *u = U + lc * CELLP + base;
-          T um = 0;
+          T um0 = 0, um1 = 0;
 #pragma unroll
           for (int p = 0; p < NQ; ++p)
-            um += tb.F[side * NQ + p] * u[p * sd];
-          Fm[lc * FACE + fi] = um;
-          // neighbour: its face nodes sit at the face positions whose
+            {
+              const T v = u[p * sd];
+              um0 += tb.F[p] * v;
+              um1 += tb.F[NQ + p] * v;
+            }
+          Fm[l] = um0;
+          Fm[CPB * FACE + l] = um1;
+          // neighbours: their face nodes sit at the face positions whose
           // digits are all < ND; the others start at zero
           int nbase = 0;
           bool node = true;
@@ -194,40 +202,45 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
               }
           }
           const long long cell = cell0 + lc;
-          const long long s = mesh.stride[d];
           const int c = at[lc].c[d];
-          const int cn = c + (side ? 1 : -1);
-          const int off = nbase + (side ? 0 : (ND - 1)) * sdn;
-          T up = 0;
+          const long long wrap = (long long)(mesh.n[d] - 1) * s;
+          T up0 = 0, up1 = 0;
           if (node)
             {
-              if (cn >= 0 && cn < mesh.n[d])
-                up = src[(cell + (side ? s : -s)) * CELLD + off];
+              // side 0: the left neighbour's last layer
+              const int off0 = nbase + (ND - 1) * sdn;
+              if (c - 1 >= 0)
+                up0 = src[(cell - s) * CELLD + off0];
               else if (q.periodic[d])
-                up = src[(cell + (side ? -(long long)(mesh.n[d] - 1) * s
-                                       : (long long)(mesh.n[d] - 1) * s)) *
-                           CELLD +
-                         off];
+                up0 = src[(cell + wrap) * CELLD + off0];
+              // side 1: the right neighbour's first layer
+              if (c + 1 < mesh.n[d])
+                up1 = src[(cell + s) * CELLD + nbase];
+              else if (q.periodic[d])
+                up1 = src[(cell - wrap) * CELLD + nbase];
             }
-          Fp[lc * FACE + fi] = up;
+          Fp[l] = up0;
+          Fp[CPB * FACE + l] = up1;
         }
       __syncthreads();
-      // interpolate u+ along the face directions (S: NQ x ND)
+      // interpolate u+ of both sides along the face directions (S: NQ x ND)
       if (!collocation)
         for (int j = 0; j < D; ++j)
           {
             if (j == d)
               continue;
-            // lines of the face array along j (the face array is the cell
+            // lines of the face arrays along j (a face array is the cell
             // array with direction d of extent 1)
-            const int sj = ipow(NQ, j < d ? j : j - 1);
+            const int jj = j < d ? j : j - 1;
+            const int sj = ipow(NQ, jj);
             const int nl = FACE / NQ;
-            for (int l = tid; l < ncell * nl; l += kBlock)
+            for (int l = tid; l < 2 * ncell * nl; l += kBlock)
               {
-                const int lc = l / nl;
-                int li = l - lc * nl;
+                const int side = l >= ncell * nl;
+                const int ls = l - side * ncell * nl;
+                const int lc = ls / nl;
+                int li = ls - lc * nl;
                 // base of line li along face-direction jj in NQ^{D-1}
-                const int jj = j < d ? j : j - 1;
                 int base = 0, pw = 1;
                 for (int k = 0; k < D - 1; ++k)
                   {
@@ -238,7 +251,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
                       }
                     pw *= NQ;
                   }
-                T *p = Fp + lc * FACE + base;
+                T *p = Fp + side * CPB * FACE + lc * FACE + base;
                 T in[ND];
 #pragma unroll
                 for (int t = 0; t < ND; ++t)
@@ -255,16 +268,12 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
               }
             __syncthreads();
           }
-      // flux at the face points, then lift into R
+      // fluxes at the face points of both sides, then their lifts into R
       for (int l = tid; l < ncell * FACE; l += kBlock)
         {
           const int lc = l / FACE, fi = l - lc * FACE;
           const int base = line_base<D, NQ>(fi, d);
           const int c = at[lc].c[d];
-          const bool boundary =
-            !q.periodic[d] && (side ? c == mesh.n[d] - 1 : c == 0);
-          const double um = Fm[lc * FACE + fi];
-          double up = Fp[lc * FACE + fi];
           // face JxW: product over j != d
           double jf = 1.0;
           {
@@ -277,49 +286,68 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
                 rem /= NQ;
               }
           }
-          if (boundary)
+          // Cartesian: the velocity component at the face point (the same
+          // for both sides; the normal flips its sign)
+          const double a_pt =
+            geo.on ? 0.0
+                   : point_velocity_at<D, NQ>(q, mesh, efield, dim_x, d,
+                                              at[lc], base);
+          T fd[2];
+#pragma unroll
+          for (int side = 0; side < 2; ++side)
             {
-              double g = 0.0;
-              if (q.bc_inhomogeneous)
+              const bool boundary =
+                !q.periodic[d] && (side ? c == mesh.n[d] - 1 : c == 0);
+              const double um = Fm[side * CPB * FACE + l];
+              double up = Fp[side * CPB * FACE + l];
+              double jfs = jf;
+              if (boundary)
                 {
-                  double x[kMaxDim];
-                  int rem = base;
-                  for (int j = 0; j < D; ++j)
+                  double g = 0.0;
+                  if (q.bc_inhomogeneous)
                     {
-                      const int cj = at[lc].c[j];
-                      const int ij = rem % NQ;
-                      rem /= NQ;
-                      x[j] = mesh.lo[j] +
-                             (cj + (j == d ? (double)side : q.qp[ij])) *
-                               mesh.h[j];
+                      double x[kMaxDim];
+                      int rem = base;
+                      for (int j = 0; j < D; ++j)
+                        {
+                          const int cj = at[lc].c[j];
+                          const int ij = rem % NQ;
+                          rem /= NQ;
+                          x[j] = mesh.lo[j] +
+                                 (cj + (j == d ? (double)side : q.qp[ij])) *
+                                   mesh.h[j];
+                        }
+                      if (geo.on)
+                        deform_point(geo, mesh, x);
+                      g = eval_function(q.bc_function, q.bc_params, dim_x, D,
+                                        x, time);
                     }
-                  if (geo.on)
-                    deform_point(geo, mesh, x);
-                  g = eval_function(q.bc_function, q.bc_params, dim_x, D, x,
-                                    time);
+                  up = -um + 2.0 * g;
                 }
-              up = -um + 2.0 * g;
+              // deformed: (a . n) JxW_face from the tables (the face JxW is
+              // in it)
+              double an;
+              if (geo.on)
+                {
+                  an = deformed_face_at<NQ>(geo, at[lc].xc, at[lc].vc, base,
+                                            d, side);
+                  jfs = 1.0;
+                }
+              else
+                an = side ? a_pt : -a_pt;
+              const double flux =
+                0.5 * ((um + up) * an + fabs(an) * (um - up));
+              fd[side] = (T)(-(flux - q.skew * um * an) * jfs);
             }
-          // deformed: (a . n) JxW_face from the tables (the face JxW is in
-          // it, jf = 1)
-          double an;
-          if (geo.on)
-            {
-              an = deformed_face_at<NQ>(geo, at[lc].xc, at[lc].vc, base, d,
-                                        side);
-              jf = 1.0;
-            }
-          else
-            an = point_velocity_at<D, NQ>(q, mesh, efield, dim_x, d, at[lc],
-                                          base) *
-                 nrm;
-          const double flux =
-            0.5 * ((um + up) * an + fabs(an) * (um - up));
-          const T fd = (T)(-(flux - q.skew * um * an) * jf);
           T *r = R + lc * CELLP + base;
 #pragma unroll
           for (int p = 0; p < NQ; ++p)
-            r[p * sd] += tb.F[side * NQ + p] * fd;
+            {
+              T v = r[p * sd];
+              v += tb.F[p] * fd[0];
+              v += tb.F[NQ + p] * fd[1];
+              r[p * sd] = v;
+            }
         }
       __syncthreads();
     }
@@ -366,7 +394,7 @@ hipError_t launch(int mode, const T *src, T *dst, const QuadDev &q,
 {
   using Sh = Shape<D, ND, NQ>;
   const size_t lds = sizeof(T) *
-                     (2 * Sh::CPB * (Sh::CELL + 1) + 2 * Sh::CPB * Sh::FACE);
+                     (2 * Sh::CPB * (Sh::CELL + 1) + 4 * Sh::CPB * Sh::FACE);
   const long long grid = (mesh.n_cells + Sh::CPB - 1) / Sh::CPB;
   // gfx950 has 160 KB of LDS per CU; opt in above the 64 KB default
   hipError_t e = hipFuncSetAttribute(
@@ -425,9 +453,10 @@ bool quadrature_supported(int D, int ND, int NQ)
     NQ == ND ? (D <= 4 ? 6 : 4) : (D <= 4 ? 5 : D == 5 ? 4 : 3);
   if (ND > max_nd)
     return false;
-  // both LDS cell buffers must fit in 160 KB
+  // both LDS cell buffers and the face buffers of two sides must fit in
+  // 160 KB
   const long long cell = (long long)std::pow(NQ, D);
-  return 2 * (cell + 1) * 8 + 2 * (cell / NQ) * 8 <= 160 * 1024;
+  return 2 * (cell + 1) * 8 + 4 * (cell / NQ) * 8 <= 160 * 1024;
 }
 
 hipError_t quadrature_apply(int D, int ND, int NQ, int mode,
