@@ -45,8 +45,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
   extern __shared__ __align__(16) unsigned char smem_raw[];
   T *U = reinterpret_cast<T *>(smem_raw); // CPB x CELLP
   T *R = U + CPB * CELLP;                 // CPB x CELLP
-  T *Fm = R + CPB * CELLP;                // 2 sides x CPB x FACE (u-)
-  T *Fp = Fm + 2 * CPB * FACE;            // 2 sides x CPB x FACE (u+)
+  T *Fp = R + CPB * CELLP;                // 2 sides x CPB x FACE (u+)
   QTab<T> tb;
   if constexpr (sizeof(T) == sizeof(double))
     tb = {q.S, q.Dq, q.F, q.Sinv};
@@ -166,114 +165,182 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
       const int sd = ipow(NQ, d);
       const int sdn = ipow(ND, d); // stride of direction d in the src cell
       const long long s = mesh.stride[d];
-      // u- (traces of both sides) and the neighbours' face nodes
-      for (int l = tid; l < ncell * FACE; l += kBlock)
+      const long long wrap = (long long)(mesh.n[d] - 1) * s;
+      // the neighbours' face nodes (they sit at the face positions whose
+      // digits are all < ND; the others are zero) at the face points of
+      // both sides: u+ = S^{D-1} (face nodes), S: NQ x ND.  The gather does
+      // the first face direction j0 itself (a thread owns a line along j0
+      // and reads the line's ND nodes from the neighbour), the passes after
+      // it the others.
+      const int j0 = d == 0 ? 1 : 0;
+      if (!collocation)
         {
-          const int lc = l / FACE, fi = l - lc * FACE;
-          const int base = line_base<D, NQ>(fi, d);
-          const T *u = U + lc * CELLP + base;
-          T um0 = 0, um1 = 0;
-#pragma unroll
-          for (int p = 0; p < NQ; ++p)
+          const int sn0 = ipow(ND, j0); // node stride of j0 in the src cell
+          const int nl = FACE / NQ;
+          for (int l = tid; l < 2 * ncell * nl; l += kBlock)
             {
-              const T v = u[p * sd];
-              um0 += tb.F[p] * v;
-              um1 += tb.F[NQ + p] * v;
+              const int side = l >= ncell * nl;
+              const int ls = l - side * ncell * nl;
+              const int lc = ls / nl;
+              // the line's digits in the face directions after j0: its
+              // base in the face array and its node offset in the
+              // neighbour
+              int rem = ls - lc * nl, fb = 0, nb = 0, pwq = 1, pwn = 1;
+              bool node = true;
+#pragma unroll
+              for (int j = 0; j < D; ++j)
+                {
+                  if (j != d)
+                    {
+                      if (j != j0)
+                        {
+                          const int dg = rem % NQ;
+                          rem /= NQ;
+                          node = node && dg < ND;
+                          fb += dg * pwq;
+                          nb += dg * pwn;
+                        }
+                      pwq *= NQ;
+                    }
+                  pwn *= ND;
+                }
+              const long long cell = cell0 + lc;
+              const int c = at[lc].c[d];
+              long long nc = -1;
+              int off;
+              if (side == 0)
+                {
+                  // the left neighbour's last layer
+                  off = nb + (ND - 1) * sdn;
+                  if (c - 1 >= 0)
+                    nc = cell - s;
+                  else if (q.periodic[d])
+                    nc = cell + wrap;
+                }
+              else
+                {
+                  // the right neighbour's first layer
+                  off = nb;
+                  if (c + 1 < mesh.n[d])
+                    nc = cell + s;
+                  else if (q.periodic[d])
+                    nc = cell - wrap;
+                }
+              T in[ND];
+#pragma unroll
+              for (int t = 0; t < ND; ++t)
+                in[t] = node && nc >= 0 ? src[nc * CELLD + off + t * sn0]
+                                        : T(0);
+              T *pl = Fp + side * CPB * FACE + lc * FACE + fb;
+#pragma unroll
+              for (int i = 0; i < NQ; ++i)
+                {
+                  T acc = 0;
+#pragma unroll
+                  for (int t = 0; t < ND; ++t)
+                    acc += tb.S[i * ND + t] * in[t];
+                  pl[i] = acc;
+                }
             }
-          Fm[l] = um0;
-          Fm[CPB * FACE + l] = um1;
-          // neighbours: their face nodes sit at the face positions whose
-          // digits are all < ND; the others start at zero
-          int nbase = 0;
-          bool node = true;
-          {
-            int rem = fi, pw = 1;
-#pragma unroll
-            for (int j = 0; j < D; ++j)
-              {
-                if (j != d)
-                  {
-                    const int dg = rem % NQ;
-                    rem /= NQ;
-                    node = node && dg < ND;
-                    nbase += dg * pw;
-                  }
-                pw *= ND;
-              }
-          }
-          const long long cell = cell0 + lc;
-          const int c = at[lc].c[d];
-          const long long wrap = (long long)(mesh.n[d] - 1) * s;
-          T up0 = 0, up1 = 0;
-          if (node)
+          __syncthreads();
+          for (int j = 0; j < D; ++j)
             {
-              // side 0: the left neighbour's last layer
+              if (j == d || j == j0)
+                continue;
+              // lines of the face arrays along j (a face array is the cell
+              // array with direction d of extent 1)
+              const int jj = j < d ? j : j - 1;
+              const int sj = ipow(NQ, jj);
+              for (int l = tid; l < 2 * ncell * nl; l += kBlock)
+                {
+                  const int side = l >= ncell * nl;
+                  const int ls = l - side * ncell * nl;
+                  const int lc = ls / nl;
+                  int li = ls - lc * nl;
+                  // base of line li along face-direction jj in NQ^{D-1}
+                  int base = 0, pw = 1;
+                  for (int k = 0; k < D - 1; ++k)
+                    {
+                      if (k != jj)
+                        {
+                          base += (li % NQ) * pw;
+                          li /= NQ;
+                        }
+                      pw *= NQ;
+                    }
+                  T *p = Fp + side * CPB * FACE + lc * FACE + base;
+                  T in[ND];
+#pragma unroll
+                  for (int t = 0; t < ND; ++t)
+                    in[t] = p[t * sj];
+#pragma unroll
+                  for (int i = 0; i < NQ; ++i)
+                    {
+                      T acc = 0;
+#pragma unroll
+                      for (int t = 0; t < ND; ++t)
+                        acc += tb.S[i * ND + t] * in[t];
+                      p[i * sj] = acc;
+                    }
+                }
+              __syncthreads();
+            }
+        }
+      else
+        {
+          // collocation: the face nodes are the face points
+          for (int l = tid; l < ncell * FACE; l += kBlock)
+            {
+              const int lc = l / FACE, fi = l - lc * FACE;
+              int nbase = 0;
+              {
+                int rem = fi, pw = 1;
+#pragma unroll
+                for (int j = 0; j < D; ++j)
+                  {
+                    if (j != d)
+                      {
+                        nbase += (rem % NQ) * pw;
+                        rem /= NQ;
+                      }
+                    pw *= ND;
+                  }
+              }
+              const long long cell = cell0 + lc;
+              const int c = at[lc].c[d];
+              T up0 = 0, up1 = 0;
               const int off0 = nbase + (ND - 1) * sdn;
               if (c - 1 >= 0)
                 up0 = src[(cell - s) * CELLD + off0];
               else if (q.periodic[d])
                 up0 = src[(cell + wrap) * CELLD + off0];
-              // side 1: the right neighbour's first layer
               if (c + 1 < mesh.n[d])
                 up1 = src[(cell + s) * CELLD + nbase];
               else if (q.periodic[d])
                 up1 = src[(cell - wrap) * CELLD + nbase];
+              Fp[l] = up0;
+              Fp[CPB * FACE + l] = up1;
             }
-          Fp[l] = up0;
-          Fp[CPB * FACE + l] = up1;
+          __syncthreads();
         }
-      __syncthreads();
-      // interpolate u+ of both sides along the face directions (S: NQ x ND)
-      if (!collocation)
-        for (int j = 0; j < D; ++j)
-          {
-            if (j == d)
-              continue;
-            // lines of the face arrays along j (a face array is the cell
-            // array with direction d of extent 1)
-            const int jj = j < d ? j : j - 1;
-            const int sj = ipow(NQ, jj);
-            const int nl = FACE / NQ;
-            for (int l = tid; l < 2 * ncell * nl; l += kBlock)
-              {
-                const int side = l >= ncell * nl;
-                const int ls = l - side * ncell * nl;
-                const int lc = ls / nl;
-                int li = ls - lc * nl;
-                // base of line li along face-direction jj in NQ^{D-1}
-                int base = 0, pw = 1;
-                for (int k = 0; k < D - 1; ++k)
-                  {
-                    if (k != jj)
-                      {
-                        base += (li % NQ) * pw;
-                        li /= NQ;
-                      }
-                    pw *= NQ;
-                  }
-                T *p = Fp + side * CPB * FACE + lc * FACE + base;
-                T in[ND];
-#pragma unroll
-                for (int t = 0; t < ND; ++t)
-                  in[t] = p[t * sj];
-#pragma unroll
-                for (int i = 0; i < NQ; ++i)
-                  {
-                    T acc = 0;
-#pragma unroll
-                    for (int t = 0; t < ND; ++t)
-                      acc += tb.S[i * ND + t] * in[t];
-                    p[i * sj] = acc;
-                  }
-              }
-            __syncthreads();
-          }
       // fluxes at the face points of both sides, then their lifts into R
       for (int l = tid; l < ncell * FACE; l += kBlock)
         {
           const int lc = l / FACE, fi = l - lc * FACE;
           const int base = line_base<D, NQ>(fi, d);
           const int c = at[lc].c[d];
+          // u- of both sides: the traces of the line along d
+          T umt[2] = {0, 0};
+          {
+            const T *u = U + lc * CELLP + base;
+#pragma unroll
+            for (int p = 0; p < NQ; ++p)
+              {
+                const T v = u[p * sd];
+                umt[0] += tb.F[p] * v;
+                umt[1] += tb.F[NQ + p] * v;
+              }
+          }
           // face JxW: product over j != d
           double jf = 1.0;
           {
@@ -298,7 +365,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
             {
               const bool boundary =
                 !q.periodic[d] && (side ? c == mesh.n[d] - 1 : c == 0);
-              const double um = Fm[side * CPB * FACE + l];
+              const double um = umt[side];
               double up = Fp[side * CPB * FACE + l];
               double jfs = jf;
               if (boundary)
@@ -394,7 +461,7 @@ hipError_t launch(int mode, const T *src, T *dst, const QuadDev &q,
 {
   using Sh = Shape<D, ND, NQ>;
   const size_t lds = sizeof(T) *
-                     (2 * Sh::CPB * (Sh::CELL + 1) + 4 * Sh::CPB * Sh::FACE);
+                     (2 * Sh::CPB * (Sh::CELL + 1) + 2 * Sh::CPB * Sh::FACE);
   const long long grid = (mesh.n_cells + Sh::CPB - 1) / Sh::CPB;
   // gfx950 has 160 KB of LDS per CU; opt in above the 64 KB default
   hipError_t e = hipFuncSetAttribute(
@@ -453,10 +520,10 @@ bool quadrature_supported(int D, int ND, int NQ)
     NQ == ND ? (D <= 4 ? 6 : 4) : (D <= 4 ? 5 : D == 5 ? 4 : 3);
   if (ND > max_nd)
     return false;
-  // both LDS cell buffers and the face buffers of two sides must fit in
+  // both LDS cell buffers and the u+ face buffers of two sides must fit in
   // 160 KB
   const long long cell = (long long)std::pow(NQ, D);
-  return 2 * (cell + 1) * 8 + 4 * (cell / NQ) * 8 <= 160 * 1024;
+  return 2 * (cell + 1) * 8 + 2 * (cell / NQ) * 8 <= 160 * 1024;
 }
 
 hipError_t quadrature_apply(int D, int ND, int NQ, int mode,
