@@ -32,8 +32,11 @@ namespace quad
 // MODE 0: dst = apply(src); MODE 1: LSRK stage epilogue
 // T: vector, LDS and contraction type (double, or float for the FP32-mixed
 // form: the LSRK update of the FP64 solution stays in FP64)
+// Three waves per SIMD (at most 168 registers): the FP32 2+2 k=3 kernel
+// needs 170 without the bound and runs at two waves 25% slower; the
+// spills the bound causes elsewhere (12-80 B per lane) cost less.
 template <int D, int ND, int NQ, int MODE, class T>
-__global__ __launch_bounds__(kBlock) void apply_kernel(
+__global__ __launch_bounds__(kBlock, 3) void apply_kernel(
   const T *__restrict__ src, T *__restrict__ dst, QuadDev q, MeshDev mesh,
   StageDev stage, double time, int dim_x, int collocation,
   const double *__restrict__ efield, DeformDev geo)
@@ -73,35 +76,61 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
     at[tid] = cell_at(mesh, cell0 + tid,
                       q.vp ? q.n_x_cells : (geo.on ? geo.n_x_cells : 0));
 
-  // the positions that hold no nodal value start at zero
-  if constexpr (ND < NQ)
+  // 1) nodal -> quadrature values (S: NQ x ND).  The load does direction 0
+  // itself: a thread owns a line along direction 0 of U, reads the line's
+  // ND nodal values (contiguous in the cell) and writes its NQ points;
+  // lines through positions that hold no nodal value (a digit >= ND in
+  // another direction) are zero.
+  if (!collocation)
     {
-      for (int i = tid; i < ncell * CELLP; i += kBlock)
-        U[i] = T(0);
+      for (int l = tid; l < ncell * FACE; l += kBlock)
+        {
+          const int lc = l / FACE, fi = l - lc * FACE;
+          int rem = fi, nb = 0, pwn = ND;
+          bool node = true;
+#pragma unroll
+          for (int j = 1; j < D; ++j)
+            {
+              const int dg = rem % NQ;
+              rem /= NQ;
+              node = node && dg < ND;
+              nb += dg * pwn;
+              pwn *= ND;
+            }
+          T in[ND];
+#pragma unroll
+          for (int t = 0; t < ND; ++t)
+            in[t] = node ? src[(cell0 + lc) * CELLD + nb + t] : T(0);
+          T *pl = U + lc * CELLP + fi * NQ;
+#pragma unroll
+          for (int i = 0; i < NQ; ++i)
+            {
+              T acc = 0;
+#pragma unroll
+              for (int t = 0; t < ND; ++t)
+                acc += tb.S[i * ND + t] * in[t];
+              pl[i] = acc;
+            }
+        }
+      __syncthreads();
+      for (int d = 1; d < D; ++d)
+        {
+          contract_lines<D, NQ, ND, NQ, T>(U, CELLP, d, tb.S, ND, 1, ncell);
+          __syncthreads();
+        }
+    }
+  else
+    {
+      // collocation: the nodes are the points (ND = NQ)
+      for (int i = tid; i < ncell * CELLD; i += kBlock)
+        {
+          const int lc = i / CELLD;
+          U[lc * CELLP + (i - lc * CELLD)] = src[cell0 * CELLD + i];
+        }
       __syncthreads();
     }
-  for (int i = tid; i < ncell * CELLD; i += kBlock)
-    {
-      const int lc = i / CELLD;
-      U[lc * CELLP + qpos<D, ND, NQ>(i - lc * CELLD)] = src[cell0 * CELLD + i];
-    }
-  __syncthreads();
 
-  // 1) nodal -> quadrature values (S: NQ x ND)
-  if (!collocation)
-    for (int d = 0; d < D; ++d)
-      {
-        contract_lines<D, NQ, ND, NQ, T>(U, CELLP, d, tb.S, ND, 1, ncell);
-        __syncthreads();
-      }
-
-  // 2) cell integral into R
-  for (int i = tid; i < ncell * CELL; i += kBlock)
-    {
-      const int lc = i / CELL;
-      R[lc * CELLP + (i - lc * CELL)] = 0;
-    }
-  __syncthreads();
+  // 2) cell integral into R (direction 0 writes R, the others add)
   for (int d = 0; d < D; ++d)
     {
       const int sd = ipow(NQ, d);
@@ -151,7 +180,7 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
               }
 #pragma unroll
           for (int i = 0; i < NQ; ++i)
-            r[i * sd] += acc[i];
+            r[i * sd] = d == 0 ? acc[i] : r[i * sd] + acc[i];
         }
       __syncthreads();
     }
@@ -428,29 +457,69 @@ __global__ __launch_bounds__(kBlock) void apply_kernel(
                    : jxw_point<D, NQ>(q, mesh, li));
     }
   __syncthreads();
-  if (!collocation)
-    for (int d = D - 1; d >= 0; --d)
+  // the result of a point: the apply's value or the LSRK stage update
+  auto store = [&](long long g, T k) {
+    if (MODE == 0)
+      dst[g] = k;
+    else
       {
-        contract_lines<D, NQ, NQ, ND, T>(R, CELLP, d, tb.Sinv, NQ, 1, ncell);
-        __syncthreads();
+        const double s = stage.sol[g];
+        stage.sol[g] = s + stage.b_dt * (double)k;
+        if (stage.ti_out)
+          static_cast<T *>(stage.ti_out)[g] = (T)(s + stage.a_dt * (double)k);
       }
-
-  // 5) write
-  for (int i = tid; i < ncell * CELLD; i += kBlock)
+  };
+  if (!collocation)
     {
-      const int lc = i / CELLD;
-      const T k = R[lc * CELLP + qpos<D, ND, NQ>(i - lc * CELLD)];
-      const long long g = cell0 * CELLD + i;
-      if (MODE == 0)
-        dst[g] = k;
-      else
+      for (int d = D - 1; d >= 1; --d)
         {
-          const double s = stage.sol[g];
-          stage.sol[g] = s + stage.b_dt * (double)k;
-          if (stage.ti_out)
-            static_cast<T *>(stage.ti_out)[g] = (T)(s + stage.a_dt * (double)k);
+          contract_lines<D, NQ, NQ, ND, T>(R, CELLP, d, tb.Sinv, NQ, 1,
+                                           ncell);
+          __syncthreads();
+        }
+      // 5) direction 0 and the write: a thread owns a line along direction
+      // 0 whose other digits are all < ND and stores its ND nodal values
+      // (contiguous in the cell)
+      for (int l = tid; l < ncell * FACE; l += kBlock)
+        {
+          const int lc = l / FACE, fi = l - lc * FACE;
+          int rem = fi, nb = 0, pwn = ND;
+          bool node = true;
+#pragma unroll
+          for (int j = 1; j < D; ++j)
+            {
+              const int dg = rem % NQ;
+              rem /= NQ;
+              node = node && dg < ND;
+              nb += dg * pwn;
+              pwn *= ND;
+            }
+          if (!node)
+            continue;
+          const T *pl = R + lc * CELLP + fi * NQ;
+          T v[NQ];
+#pragma unroll
+          for (int j = 0; j < NQ; ++j)
+            v[j] = pl[j];
+          const long long g0 = (cell0 + lc) * CELLD + nb;
+#pragma unroll
+          for (int i = 0; i < ND; ++i)
+            {
+              T acc = 0;
+#pragma unroll
+              for (int j = 0; j < NQ; ++j)
+                acc += tb.Sinv[i * NQ + j] * v[j];
+              store(g0 + i, acc);
+            }
         }
     }
+  else
+    // 5) write (collocation: the points are the nodes)
+    for (int i = tid; i < ncell * CELLD; i += kBlock)
+      {
+        const int lc = i / CELLD;
+        store(cell0 * CELLD + i, R[lc * CELLP + (i - lc * CELLD)]);
+      }
 }
 
 template <int D, int ND, int NQ, class T>
