@@ -264,7 +264,11 @@ struct FaceShape
 {
   static constexpr int FACE = ipow(NQ, D - 1);   // face quadrature points
   static constexpr int FACED = ipow(ND, D - 1);  // face nodes
-  static constexpr int FPB = FACE >= 256 ? 1 : 256 / FACE; // faces per block
+  // faces per block: about 1024 face points (four per thread), at most one
+  // face per thread (the face set-up uses thread f for face f)
+  static constexpr int FPB = FACE >= 1024 ? 1
+                             : 1024 / FACE > kBlock ? kBlock
+                                                    : 1024 / FACE;
 };
 
 // position in the face array (NQ^{D-1}) of face node i (ND^{D-1}, base ND)
