@@ -18,7 +18,11 @@ struct Shape
   static constexpr int CELL = ipow(NQ, D);  // quadrature-point array
   static constexpr int CELLD = ipow(ND, D); // nodal values of a cell
   static constexpr int FACE = ipow(NQ, D - 1);
-  static constexpr int CPB = CELL >= 1024 ? 1 : 1024 / CELL;
+  // cells per block: about 2048 points, at most one cell per thread (the
+  // per-cell set-up uses thread c for cell c)
+  static constexpr int CPB = CELL >= 2048        ? 1
+                             : 2048 / CELL > kBlock ? kBlock
+                                                    : 2048 / CELL;
 };
 
 // position in the NQ^DD array of the point with base-ND digits i
