@@ -266,9 +266,9 @@ struct FaceShape
   static constexpr int FACED = ipow(ND, D - 1);  // face nodes
   // faces per block: about 1024 face points (four per thread), at most one
   // face per thread (the face set-up uses thread f for face f)
-  static constexpr int FPB = FACE >= 1024 ? 1
-                             : 1024 / FACE > kBlock ? kBlock
-                                                    : 1024 / FACE;
+  static constexpr int FPB = FACE >= 2048 ? 1
+                             : 2048 / FACE > kBlock ? kBlock
+                                                    : 2048 / FACE;
 };
 
 // position in the face array (NQ^{D-1}) of face node i (ND^{D-1}, base ND)
@@ -468,6 +468,12 @@ __global__ __launch_bounds__(kBlock) void face_kernel(
 // epilogue (sol += b_dt K, ti_out = sol_old + a_dt K; asm may be ti_out).
 // asmv may be dst (apply) or ti_out (stage): a block reads its cells into
 // LDS before it writes them.
+// cells per block of the inverse mass kernel: about 2048 nodal values
+__host__ __device__ constexpr int mass_cpb(int celld)
+{
+  return celld >= 2048 ? 1 : 2048 / celld;
+}
+
 template <int D, int ND, int MODE, class T>
 __global__ __launch_bounds__(kBlock) void mass_kernel(const T *asmv, T *dst,
                                                       MassDev md, MeshDev mesh,
@@ -475,7 +481,7 @@ __global__ __launch_bounds__(kBlock) void mass_kernel(const T *asmv, T *dst,
                                                       DeformDev geo)
 {
   constexpr int CELLD = ipow(ND, D);
-  constexpr int CPB = CELLD >= 1024 ? 1 : 1024 / CELLD;
+  constexpr int CPB = mass_cpb(CELLD);
   constexpr int CELLP = CELLD + 1;
   __shared__ T U[CPB * CELLP];
   __shared__ T sinv[64];
@@ -611,8 +617,7 @@ hipError_t launch(int mode, const T *src, T *asmv, T *dst, const QuadDev &q,
         }
     }
   // 3) inverse mass matrix
-  constexpr int CELLD = ipow(ND, D);
-  constexpr int CPB = CELLD >= 1024 ? 1 : 1024 / CELLD;
+  constexpr int CPB = mass_cpb(ipow(ND, D));
   const long long grid = (mesh.n_cells + CPB - 1) / CPB;
   if (mode == 0)
     hipLaunchKernelGGL((mass_kernel<D, ND, 0, T>), dim3((unsigned)grid),
