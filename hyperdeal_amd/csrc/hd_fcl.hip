@@ -263,39 +263,12 @@ template <int D, int ND, int NQ, class T>
 struct FaceShape
 {
   static constexpr int FACE = ipow(NQ, D - 1);   // face quadrature points
-  static constexpr int FACED = ipow(ND, D - 1);  // face nodes
   // faces per block: about 1024 face points (four per thread), at most one
   // face per thread (the face set-up uses thread f for face f)
   static constexpr int FPB = FACE >= 2048 ? 1
                              : 2048 / FACE > kBlock ? kBlock
                                                     : 2048 / FACE;
 };
-
-// position in the face array (NQ^{D-1}) of face node i (ND^{D-1}, base ND)
-// and the offset of that node in a cell (direction d at layer `layer`)
-template <int D, int ND, int NQ>
-__device__ inline void face_node(int i, int d, int layer, int &fpos,
-                                 int &coff)
-{
-  fpos = 0;
-  coff = 0;
-  int pw_q = 1, pw_c = 1;
-#pragma unroll
-  for (int j = 0; j < D; ++j)
-    {
-      if (j == d)
-        coff += layer * pw_c;
-      else
-        {
-          const int dg = i % ND;
-          i /= ND;
-          fpos += dg * pw_q;
-          coff += dg * pw_c;
-          pw_q *= NQ;
-        }
-      pw_c *= ND;
-    }
-}
 
 template <int D, int ND, int NQ, class T>
 __global__ __launch_bounds__(kBlock) void face_kernel(
@@ -304,7 +277,7 @@ __global__ __launch_bounds__(kBlock) void face_kernel(
   int dim_x, int collocation, DeformDev geo)
 {
   using FS = FaceShape<D, ND, NQ, T>;
-  constexpr int FACE = FS::FACE, FACED = FS::FACED, FPB = FS::FPB;
+  constexpr int FACE = FS::FACE, FPB = FS::FPB;
   constexpr int CELLD = ipow(ND, D);
   __shared__ T Fm[FPB * FACE], Fp[FPB * FACE];
   __shared__ long long cm[FPB], cp[FPB];
@@ -325,29 +298,78 @@ __global__ __launch_bounds__(kBlock) void face_kernel(
       cp[tid] = pl;
       pos[tid] = p;
     }
-  for (int i = tid; i < nf * FACE; i += kBlock)
-    {
-      Fm[i] = T(0);
-      Fp[i] = T(0);
-    }
-  __syncthreads();
+  __syncthreads(); // the face set-up is visible
 
-  // the face nodes of both sides (a boundary face: the missing side stays 0)
-  for (int i = tid; i < nf * FACED; i += kBlock)
+  // the face nodes of both sides at the face points (S along the D-1 face
+  // directions; a boundary face: the missing side is 0).  The gather does
+  // the first face direction j0 itself: a thread owns a line along j0 of
+  // one side's face array, reads the line's ND nodes from the cell (zero
+  // for lines through positions with a digit >= ND in another face
+  // direction) and writes its NQ points; the passes after it the others.
+  constexpr int NL = FACE / NQ;
+  const int j0 = d == 0 ? 1 : 0;
+  const int sn0 = ipow(ND, j0); // node stride of j0 in a cell
+  const int sdn = ipow(ND, d);
+  // a line of one side: its base in the face array, the node offset of its
+  // first node in the side's cell (layer ND-1 of the minus cell, 0 of the
+  // plus cell) and whether its other digits are all < ND
+  auto face_line = [&](int ls, int side, int &fb, int &off, bool &node) {
+    int rem = ls, nb = 0, pwq = 1, pwn = 1;
+    fb = 0;
+    node = true;
+#pragma unroll
+    for (int j = 0; j < D; ++j)
+      {
+        if (j != d)
+          {
+            if (j != j0)
+              {
+                const int dg = rem % NQ;
+                rem /= NQ;
+                node = node && dg < ND;
+                fb += dg * pwq;
+                nb += dg * pwn;
+              }
+            pwq *= NQ;
+          }
+        pwn *= ND;
+      }
+    off = nb + (side ? 0 : (ND - 1) * sdn);
+  };
+  for (int l = tid; l < 2 * nf * NL; l += kBlock)
     {
-      const int lf = i / FACED, fi = i - lf * FACED;
-      int fpos, coff_m, coff_p;
-      face_node<D, ND, NQ>(fi, d, ND - 1, fpos, coff_m);
-      face_node<D, ND, NQ>(fi, d, 0, fpos, coff_p);
-      if (cm[lf] >= 0)
-        Fm[lf * FACE + fpos] = src[cm[lf] * CELLD + coff_m];
-      if (cp[lf] >= 0)
-        Fp[lf * FACE + fpos] = src[cp[lf] * CELLD + coff_p];
+      const int side = l >= nf * NL; // 0: minus cell (Fm), 1: plus (Fp)
+      const int ls = l - side * nf * NL;
+      const int lf = ls / NL;
+      int fb, off;
+      bool node;
+      face_line(ls - lf * NL, side, fb, off, node);
+      const long long cl = side ? cp[lf] : cm[lf];
+      T in[ND];
+#pragma unroll
+      for (int t = 0; t < ND; ++t)
+        in[t] = node && cl >= 0 ? src[cl * CELLD + off + t * sn0] : T(0);
+      T *pl = (side ? Fp : Fm) + lf * FACE + fb;
+      if (collocation)
+        {
+#pragma unroll
+          for (int t = 0; t < ND; ++t)
+            pl[t] = in[t];
+        }
+      else
+#pragma unroll
+        for (int i = 0; i < NQ; ++i)
+          {
+            T acc = 0;
+#pragma unroll
+            for (int t = 0; t < ND; ++t)
+              acc += tb.S[i * ND + t] * in[t];
+            pl[i] = acc;
+          }
     }
   __syncthreads();
-  // nodal -> face quadrature points (S along the D-1 face directions)
   if (!collocation)
-    for (int jj = 0; jj < D - 1; ++jj)
+    for (int jj = 1; jj < D - 1; ++jj)
       {
         contract_face_lines<D, NQ, ND, NQ, T>(Fm, jj, tb.S, ND, 1, nf);
         contract_face_lines<D, NQ, ND, NQ, T>(Fp, jj, tb.S, ND, 1, nf);
@@ -438,26 +460,48 @@ __global__ __launch_bounds__(kBlock) void face_kernel(
         }
     }
   __syncthreads();
-  // test with the face's nodal basis (S^T along the face directions)
+  // test with the face's nodal basis (S^T along the face directions), the
+  // first face direction j0 together with the add into the cells: the
+  // minus cell's last node layer along d, the plus cell's first
   if (!collocation)
-    for (int jj = D - 2; jj >= 0; --jj)
+    for (int jj = D - 2; jj >= 1; --jj)
       {
         contract_face_lines<D, NQ, NQ, ND, T>(Fm, jj, tb.S, 1, ND, nf);
         contract_face_lines<D, NQ, NQ, ND, T>(Fp, jj, tb.S, 1, ND, nf);
         __syncthreads();
       }
-  // add into the cells: the minus cell's last node layer along d, the plus
-  // cell's first
-  for (int i = tid; i < nf * FACED; i += kBlock)
+  for (int l = tid; l < 2 * nf * NL; l += kBlock)
     {
-      const int lf = i / FACED, fi = i - lf * FACED;
-      int fpos, coff_m, coff_p;
-      face_node<D, ND, NQ>(fi, d, ND - 1, fpos, coff_m);
-      face_node<D, ND, NQ>(fi, d, 0, fpos, coff_p);
-      if (cm[lf] >= 0)
-        asmv[cm[lf] * CELLD + coff_m] += Fm[lf * FACE + fpos];
-      if (cp[lf] >= 0)
-        asmv[cp[lf] * CELLD + coff_p] += Fp[lf * FACE + fpos];
+      const int side = l >= nf * NL;
+      const int ls = l - side * nf * NL;
+      const int lf = ls / NL;
+      int fb, off;
+      bool node;
+      face_line(ls - lf * NL, side, fb, off, node);
+      const long long cl = side ? cp[lf] : cm[lf];
+      if (!node || cl < 0)
+        continue;
+      const T *pl = (side ? Fp : Fm) + lf * FACE + fb;
+      T v[NQ];
+#pragma unroll
+      for (int j = 0; j < NQ; ++j)
+        v[j] = pl[j];
+      T *out = asmv + cl * CELLD + off;
+#pragma unroll
+      for (int i = 0; i < ND; ++i)
+        {
+          T acc;
+          if (collocation)
+            acc = v[i];
+          else
+            {
+              acc = 0;
+#pragma unroll
+              for (int j = 0; j < NQ; ++j)
+                acc += tb.S[j * ND + i] * v[j];
+            }
+          out[i * sn0] += acc;
+        }
     }
 }
 
