@@ -83,33 +83,68 @@ __global__ __launch_bounds__(kBlock) void cell_kernel(
   const int ncell = (int)min((long long)CPB, mesh.n_cells - cell0);
   const int tid = threadIdx.x;
 
-  if constexpr (ND < NQ)
+  // a line along direction 0 of a cell array (fi: its index among the
+  // lines): the offset of its first node in the nodal cell and whether its
+  // other digits are all < ND (it holds nodal values)
+  auto line0 = [&](int fi, int &nb, bool &node) {
+    int rem = fi, pwn = ND;
+    nb = 0;
+    node = true;
+#pragma unroll
+    for (int j = 1; j < D; ++j)
+      {
+        const int dg = rem % NQ;
+        rem /= NQ;
+        node = node && dg < ND;
+        nb += dg * pwn;
+        pwn *= ND;
+      }
+  };
+
+  // nodal -> quadrature values (S: NQ x ND); the load does direction 0
+  // itself (a thread reads a line of ND nodal values, contiguous in the
+  // cell, and writes its NQ points; lines without nodal values are zero)
+  if (!collocation)
     {
-      for (int i = tid; i < ncell * CELLP; i += kBlock)
-        U[i] = T(0);
+      for (int l = tid; l < ncell * FACE; l += kBlock)
+        {
+          const int lc = l / FACE, fi = l - lc * FACE;
+          int nb;
+          bool node;
+          line0(fi, nb, node);
+          T in[ND];
+#pragma unroll
+          for (int t = 0; t < ND; ++t)
+            in[t] = node ? src[(cell0 + lc) * CELLD + nb + t] : T(0);
+          T *pl = U + lc * CELLP + fi * NQ;
+#pragma unroll
+          for (int i = 0; i < NQ; ++i)
+            {
+              T acc = 0;
+#pragma unroll
+              for (int t = 0; t < ND; ++t)
+                acc += tb.S[i * ND + t] * in[t];
+              pl[i] = acc;
+            }
+        }
+      __syncthreads();
+      for (int d = 1; d < D; ++d)
+        {
+          quad::contract_lines<D, NQ, ND, NQ, T>(U, CELLP, d, tb.S, ND, 1,
+                                                 ncell);
+          __syncthreads();
+        }
+    }
+  else
+    {
+      // collocation: the nodes are the points
+      for (int i = tid; i < ncell * CELLD; i += kBlock)
+        {
+          const int lc = i / CELLD;
+          U[lc * CELLP + (i - lc * CELLD)] = src[cell0 * CELLD + i];
+        }
       __syncthreads();
     }
-  for (int i = tid; i < ncell * CELLD; i += kBlock)
-    {
-      const int lc = i / CELLD;
-      U[lc * CELLP + quad::qpos<D, ND, NQ>(i - lc * CELLD)] =
-        src[cell0 * CELLD + i];
-    }
-  for (int i = tid; i < ncell * CELL; i += kBlock)
-    {
-      const int lc = i / CELL;
-      R[lc * CELLP + (i - lc * CELL)] = 0;
-    }
-  __syncthreads();
-
-  // nodal -> quadrature values
-  if (!collocation)
-    for (int d = 0; d < D; ++d)
-      {
-        quad::contract_lines<D, NQ, ND, NQ, T>(U, CELLP, d, tb.S, ND, 1,
-                                               ncell);
-        __syncthreads();
-      }
 
   // the integrand: -skew (a . grad u) JxW tested with the values plus
   // (1 - skew) u a_d JxW tested with the gradients (D_d^T)
@@ -157,27 +192,54 @@ __global__ __launch_bounds__(kBlock) void cell_kernel(
                   s += tb.Dq[j * NQ + i] * ((1 - skew) * uq[j] * jw[j]);
                 acc[i] += s;
               }
+          // direction 0 writes R, the others add
 #pragma unroll
           for (int i = 0; i < NQ; ++i)
-            r[i * sd] += acc[i];
+            r[i * sd] = d == 0 ? acc[i] : r[i * sd] + acc[i];
         }
       __syncthreads();
     }
 
-  // test with the nodal basis: S^T along every direction (NQ -> ND)
+  // test with the nodal basis: S^T along every direction (NQ -> ND), the
+  // last one (direction 0) with the store of the line's ND nodal values
   if (!collocation)
-    for (int d = D - 1; d >= 0; --d)
-      {
-        quad::contract_lines<D, NQ, NQ, ND, T>(R, CELLP, d, tb.S, 1, ND,
-                                               ncell);
-        __syncthreads();
-      }
-  for (int i = tid; i < ncell * CELLD; i += kBlock)
     {
-      const int lc = i / CELLD;
-      asmv[cell0 * CELLD + i] =
-        R[lc * CELLP + quad::qpos<D, ND, NQ>(i - lc * CELLD)];
+      for (int d = D - 1; d >= 1; --d)
+        {
+          quad::contract_lines<D, NQ, NQ, ND, T>(R, CELLP, d, tb.S, 1, ND,
+                                                 ncell);
+          __syncthreads();
+        }
+      for (int l = tid; l < ncell * FACE; l += kBlock)
+        {
+          const int lc = l / FACE, fi = l - lc * FACE;
+          int nb;
+          bool node;
+          line0(fi, nb, node);
+          if (!node)
+            continue;
+          const T *pl = R + lc * CELLP + fi * NQ;
+          T v[NQ];
+#pragma unroll
+          for (int j = 0; j < NQ; ++j)
+            v[j] = pl[j];
+#pragma unroll
+          for (int i = 0; i < ND; ++i)
+            {
+              T acc = 0;
+#pragma unroll
+              for (int j = 0; j < NQ; ++j)
+                acc += tb.S[j * ND + i] * v[j];
+              asmv[(cell0 + lc) * CELLD + nb + i] = acc;
+            }
+        }
     }
+  else
+    for (int i = tid; i < ncell * CELLD; i += kBlock)
+      {
+        const int lc = i / CELLD;
+        asmv[cell0 * CELLD + i] = R[lc * CELLP + (i - lc * CELLD)];
+      }
 }
 
 // ---------------------------------------------------------------------------
