@@ -596,20 +596,48 @@ __global__ __launch_bounds__(kBlock) void mass_kernel(const T *asmv, T *dst,
   const long long cell0 = (long long)blockIdx.x * CPB;
   const int ncell = (int)min((long long)CPB, mesh.n_cells - cell0);
   const int tid = threadIdx.x;
-  for (int i = tid; i < ncell * CELLD; i += kBlock)
-    {
-      const int lc = i / CELLD;
-      U[lc * CELLP + (i - lc * CELLD)] = asmv[cell0 * CELLD + i];
-    }
-  __syncthreads();
+  constexpr int NL = CELLD / ND; // lines along a direction per cell
   if (!md.collocation)
-    for (int d = 0; d < D; ++d)
-      {
-        // Sinv^T: A(r, c) = Sinv[c][r]
-        quad::contract_lines<D, ND, ND, ND, T>(U, CELLP, d, sinv, 1, ND,
-                                               ncell);
-        __syncthreads();
-      }
+    {
+      __syncthreads(); // the table is visible
+      // Sinv^T along direction 0 with the load (a thread reads a line of
+      // ND contiguous values), then the other directions; A(r, c) =
+      // Sinv[c][r]
+      for (int l = tid; l < ncell * NL; l += kBlock)
+        {
+          const int lc = l / NL, fi = l - lc * NL;
+          T in[ND];
+#pragma unroll
+          for (int t = 0; t < ND; ++t)
+            in[t] = asmv[(cell0 + lc) * CELLD + fi * ND + t];
+          T *pl = U + lc * CELLP + fi * ND;
+#pragma unroll
+          for (int i = 0; i < ND; ++i)
+            {
+              T acc = 0;
+#pragma unroll
+              for (int t = 0; t < ND; ++t)
+                acc += sinv[t * ND + i] * in[t];
+              pl[i] = acc;
+            }
+        }
+      __syncthreads();
+      for (int d = 1; d < D; ++d)
+        {
+          quad::contract_lines<D, ND, ND, ND, T>(U, CELLP, d, sinv, 1, ND,
+                                                 ncell);
+          __syncthreads();
+        }
+    }
+  else
+    {
+      for (int i = tid; i < ncell * CELLD; i += kBlock)
+        {
+          const int lc = i / CELLD;
+          U[lc * CELLP + (i - lc * CELLD)] = asmv[cell0 * CELLD + i];
+        }
+      __syncthreads();
+    }
   for (int i = tid; i < ncell * CELLD; i += kBlock)
     {
       const int lc = i / CELLD;
@@ -627,28 +655,51 @@ __global__ __launch_bounds__(kBlock) void mass_kernel(const T *asmv, T *dst,
       U[lc * CELLP + (i - lc * CELLD)] /= (T)jxw;
     }
   __syncthreads();
-  if (!md.collocation)
-    for (int d = D - 1; d >= 0; --d)
+  auto store = [&](long long g, T k) {
+    if constexpr (MODE == 0)
+      dst[g] = k;
+    else
       {
-        quad::contract_lines<D, ND, ND, ND, T>(U, CELLP, d, sinv, ND, 1,
-                                               ncell);
-        __syncthreads();
+        const double s = stage.sol[g];
+        stage.sol[g] = s + stage.b_dt * (double)k;
+        if (stage.ti_out)
+          static_cast<T *>(stage.ti_out)[g] = (T)(s + stage.a_dt * (double)k);
       }
-  for (int i = tid; i < ncell * CELLD; i += kBlock)
+  };
+  if (!md.collocation)
     {
-      const int lc = i / CELLD;
-      const T k = U[lc * CELLP + (i - lc * CELLD)];
-      const long long g = cell0 * CELLD + i;
-      if constexpr (MODE == 0)
-        dst[g] = k;
-      else
+      for (int d = D - 1; d >= 1; --d)
         {
-          const double s = stage.sol[g];
-          stage.sol[g] = s + stage.b_dt * (double)k;
-          if (stage.ti_out)
-            static_cast<T *>(stage.ti_out)[g] = (T)(s + stage.a_dt * (double)k);
+          quad::contract_lines<D, ND, ND, ND, T>(U, CELLP, d, sinv, ND, 1,
+                                                 ncell);
+          __syncthreads();
+        }
+      // Sinv along direction 0 with the store of the line's values
+      for (int l = tid; l < ncell * NL; l += kBlock)
+        {
+          const int lc = l / NL, fi = l - lc * NL;
+          const T *pl = U + lc * CELLP + fi * ND;
+          T v[ND];
+#pragma unroll
+          for (int t = 0; t < ND; ++t)
+            v[t] = pl[t];
+#pragma unroll
+          for (int i = 0; i < ND; ++i)
+            {
+              T acc = 0;
+#pragma unroll
+              for (int t = 0; t < ND; ++t)
+                acc += sinv[i * ND + t] * v[t];
+              store((cell0 + lc) * CELLD + fi * ND + i, acc);
+            }
         }
     }
+  else
+    for (int i = tid; i < ncell * CELLD; i += kBlock)
+      {
+        const int lc = i / CELLD;
+        store(cell0 * CELLD + i, U[lc * CELLP + (i - lc * CELLD)]);
+      }
 }
 
 // the colours of direction d: (start, step, count) of face positions
